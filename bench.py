@@ -1,0 +1,156 @@
+"""Benchmark of the per-step particle loop on one MI355X.
+
+Workload (default): the beam-plasma wakefield case -- electron-proton plasma
+n = 1e17 m^-3 on a 512 (r) x 4096 (z) grid, dr = dz = 1 mm, 64 macro-particles
+per cell per species (268,435,456 plasma particles), Te = 1 eV, Ti = 0.1 eV,
+plus a Gaussian electron bunch (gamma = 5, sigma_z = 5 mm, sigma_r = 2 mm,
+peak density 0.1 n, 1,048,576 macro-particles), ZigZag deposition, Boris
+pusher, dt = 0.5 dz / c, conducting walls.  Inputs are generated on the
+device from a fixed seed and are resident in HBM before timing starts.
+
+One step = Yee E + stitch, Yee H + stitch, push + move + deposit of every
+particle, migration keys, J stitch.
+
+Prints one JSON line (see the repository documentation for the fields).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+C_LIGHT = 3e8
+
+
+def workload(name):
+    if name == "wakefield":
+        return dict(name="beam-plasma wakefield 512x4096, 64 ppc/species + Gaussian bunch (gamma 5)",
+                    grid=(512, 4096), domains=(1, 1), ppc=64, density=1e17, ccs="zigzag", pusher="boris-adaptive",
+                    bunch=dict(n=1 << 20, n_peak=1e16, sigma_r=2e-3, sigma_z=5e-3, z0=0.1, gamma=5.0))
+    if name == "uniform":
+        return dict(name="uniform e-p plasma 128x512, 16 ppc/species, VB", grid=(128, 512), domains=(1, 1), ppc=16,
+                    density=1e17, ccs="vb", pusher="boris-adaptive", bunch=None)
+    raise SystemExit("unknown workload " + name)
+
+
+def run_gpu(w, steps, warmup, prec):
+    from picopic_amd import capi
+    dr = dz = 1e-3
+    grid = w["grid"]
+    size = (grid[0] * dr, grid[1] * dz)
+    dt = 0.5 * dz / C_LIGHT
+    n_plasma = w["ppc"] * grid[0] * grid[1]
+    species = [(-1, 1, False, n_plasma + n_plasma // 50), (1, 1836, False, n_plasma + n_plasma // 50)]
+    if w["bunch"]:
+        species.append((-1, 1, True, w["bunch"]["n"]))
+    ctx = capi.Context(size, grid, w["domains"], dt, species, pusher=w["pusher"], ccs=w["ccs"], precision=prec)
+    ctx.init_plasma(0, w["ppc"], w["density"], w["density"], 1.0, 11)
+    ctx.init_plasma(1, w["ppc"], w["density"], w["density"], 0.1, 12)
+    if w["bunch"]:
+        b = w["bunch"]
+        ctx.init_bunch(2, b["n"], b["n_peak"], b["sigma_r"], b["sigma_z"], b["z0"], b["gamma"], 13)
+    n_part = sum(ctx.count(s) for s in range(len(species)))
+    ctx.step(warmup)
+    L = capi.load()
+    L.pcp_set_timing(ctx.h, 1)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(steps)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    phases = {k: L.pcp_kernel_ms(ctx.h, k.encode()) for k in ("fields", "particles", "fixup", "stitch_j", "step")}
+    n_end = sum(ctx.count(s) for s in range(len(species)))
+    ctx.close()
+    return dict(n_particles=n_part, n_particles_end=n_end, wall=wall, phases=phases, cells=grid[0] * grid[1])
+
+
+def cpu_baseline(w, threads):
+    """The oracle (a C restatement of the reference CPU path, AoS particles,
+    OpenMP over domains) on a slab of the same plasma: all 512 radial cells x
+    256 axial cells, 16 domains, same ppc, 1 warm-up + 3 timed steps."""
+    import numpy as np
+
+    from oracle import oracle as O
+    dr = dz = 1e-3
+    grid = (w["grid"][0], 256)
+    size = (grid[0] * dr, grid[1] * dz)
+    dt = 0.5 * dz / C_LIGHT
+    species = [(-1, 1, False), (1, 1836, False)]
+    smb = O.SMB(size, grid, (8, 2), dt, species, pusher=w["pusher"], ccs=w["ccs"])
+    rng = np.random.default_rng(5)
+    n = w["ppc"] * grid[0] * grid[1]
+    for sp, (q, m, b) in enumerate(species):
+        r = np.sqrt(rng.uniform(0.0, size[0] ** 2, n))
+        r = np.clip(r, 0.5 * dr, size[0] - 0.5 * dr)
+        z = rng.uniform(0.5 * dz, size[1] - 0.5 * dz, n)
+        vth = np.sqrt((1.0 if m == 1 else 0.1) * 1.6e-19 / (m * 9.1e-31))
+        v = rng.normal(0, vth, (3, n))
+        wgt = np.full(n, w["density"] * np.pi * size[0] ** 2 * size[1] / n)
+        smb.add_particles(sp, r, z, v[0], v[1], v[2], wgt)
+        del r, z, v, wgt
+    O.set_threads(threads)
+    smb.step(1)
+    t0 = time.perf_counter()
+    nsteps = 3
+    smb.step(nsteps)
+    dt_wall = time.perf_counter() - t0
+    total = 2 * n
+    return dict(value=total * nsteps / dt_wall, unit="particle-steps/s", cores=threads, kind="port",
+                sample="oracle on a 512x256-cell slab of the same plasma (%d particles, 16 domains), "
+                       "%d timed steps after 1 warm-up, %.1f s" % (total, nsteps, dt_wall))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="wakefield")
+    ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every box in this pool has a single MI355X "
+              "(the sharding plan is described in the documentation)", file=sys.stderr)
+        sys.exit(2)
+    w = workload(args.workload)
+    g = run_gpu(w, args.steps, args.warmup, args.prec)
+    n = g["n_particles"]
+    value = n * args.steps / g["wall"]
+    ms_step = g["wall"] * 1e3 / args.steps
+    t_part = g["phases"]["particles"]   # ms per step of the fused push+move+deposit kernels
+    bytes_pp = 88 if args.prec == "fp64" else 52
+    alg_bytes = bytes_pp * n
+    achieved = alg_bytes / (t_part * 1e-3) / 1e9 if t_part > 0 else None
+    line = {
+        "metric": "particle-steps/s per GPU",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.prec == "fp64" else "f32",
+        "data": "synthetic (seeded, generated on device)",
+        "config": {"workload": w["name"], "particles": n, "cells": g["cells"], "precision": args.prec,
+                   "deposition": w["ccs"], "pusher": w["pusher"], "phases_ms": g["phases"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                     "kernel": "k_step_fused (push+move+deposit)", "alg_bytes_per_particle": bytes_pp},
+    }
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        line["cpu_baseline"] = cpu_baseline(w, threads)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,157 @@
+/*
+ * picopic_amd -- MI355X (gfx950) implementation of the PiCoPiC per-step
+ * particle loop behind a C ABI.
+ *
+ * One context (`pcp_ctx`) holds one shared-memory block (SMB) of the
+ * reference: an r x z grid of equal domains with halo-2 field grids, and the
+ * particles of every species, all resident in device memory.  The caller owns
+ * host buffers; the context owns all device memory.  Calls are ordered on the
+ * context's stream; a context is driven by one host thread.
+ *
+ * Errors: every entry point returns 0 on success and a negative PCP_E* code
+ * otherwise; pcp_last_error() gives the message of the calling thread's last
+ * failure.  Nothing throws across this boundary.
+ *
+ * Each entry point names the reference interface it replaces (file:line in
+ * cosmonaut-ok/PiCoPiC).
+ */
+#ifndef PICOPIC_H
+#define PICOPIC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PCP_OK 0
+#define PCP_E_ARG -1      /* bad argument */
+#define PCP_E_HIP -2      /* HIP runtime error */
+#define PCP_E_NOMEM -3    /* capacity exceeded or device allocation failed */
+#define PCP_E_STATE -4    /* a kernel reported a FATAL condition (NaN, beta > 1, ...) */
+
+/* pushers: configure.ac:96-98,226-239 (--with-pusher) */
+enum { PCP_PUSHER_BORIS = 0, PCP_PUSHER_BORIS_RELATIVISTIC = 1, PCP_PUSHER_BORIS_ADAPTIVE = 2,
+       PCP_PUSHER_VAY = 3, PCP_PUSHER_HC = 4 };
+/* charge conservation schemes: configure.ac:108-110,268-275 (--with-ccs) */
+enum { PCP_CCS_VB = 0, PCP_CCS_ZIGZAG = 1 };
+/* grids: MaxwellSolver::field_e/field_h (include/maxwellSolver.hpp:39-46),
+ * MaxwellSolverYee::field_h_at_et, Current::current, epsilon, sigma */
+enum { PCP_GRID_E = 0, PCP_GRID_H = 1, PCP_GRID_H_AT_ET = 2, PCP_GRID_J = 3,
+       PCP_GRID_EPS = 4, PCP_GRID_SIGMA = 5 };
+/* precision of the particle/field storage, fixed at creation */
+enum { PCP_PREC_FP64 = 0, PCP_PREC_FP32_SPLIT = 1 };
+
+typedef struct {
+  double charge;        /* SI [C]; SpecieP::charge = Z * EL_CHARGE (src/specieP.cpp:40) */
+  double mass;          /* SI [kg]; SpecieP::mass = A * EL_MASS (src/specieP.cpp:41) */
+  int is_beam;          /* reflect like BeamP::reflect (src/beamP.cpp:159-177) */
+  long capacity;        /* maximum number of macro-particles held */
+} pcp_species_desc;
+
+/* The JSON geometry/time blocks (src/cfg.cpp:379-439) plus the compile-time
+ * switches of configure.ac as run-time fields. */
+typedef struct {
+  double size_r, size_z;         /* geometry.size.radius / longitude [m] */
+  long grid_r, grid_z;           /* geometry.grid.radius / longitude [cells] */
+  long domains_r, domains_z;     /* geometry.domains_amount; must divide the grid */
+  double dt;                     /* time.step [s] */
+  int pusher;                    /* PCP_PUSHER_* */
+  int ccs;                       /* PCP_CCS_* */
+  int pml_on;                    /* ENABLE_PML */
+  double pml_left, pml_right, pml_outer;   /* geometry.pml.*_wall (fractions) */
+  double pml_sigma1, pml_sigma2;           /* geometry.pml.sigma_1/2 */
+  int precision;                 /* PCP_PREC_* */
+  int n_species;
+  pcp_species_desc species[8];
+  long max_migrants;             /* per-step capacity of the migration list (0: default) */
+} pcp_config;
+
+typedef struct pcp_ctx pcp_ctx;
+
+const char *pcp_last_error(void);
+
+/* SMB::SMB + Domain::Domain + solver constructors (src/SMB.cpp:28-192,
+ * src/domain.cpp:26-84, src/maxwellSolver/maxwellSolverYee.cpp:22-109,
+ * include/current.hpp:45-53). Fields start at zero, epsilon = EPSILON0,
+ * sigma from set_pml. */
+int pcp_create(const pcp_config *cfg, pcp_ctx **out);
+int pcp_destroy(pcp_ctx *ctx);
+int pcp_sync(pcp_ctx *ctx);
+
+/* Domain::distribute + bind (src/domain.cpp:86-94): append particles to
+ * species `sp`.  Each particle joins the domain owning its cell; within a
+ * domain the input order is the reference's vector order.  `id` (optional)
+ * is carried through unchanged. */
+int pcp_upload_species(pcp_ctx *ctx, int sp, long n, const double *r, const double *z,
+                       const double *vr, const double *vphi, const double *vz,
+                       const double *w, const uint64_t *id);
+/* number of live particles of species sp */
+int pcp_count(pcp_ctx *ctx, int sp, long *n);
+/* live particles, in device order; any pointer may be NULL. `order` receives
+ * (domain << 40 | vector position key) -- sorting by it gives the reference's
+ * domain-major vector order.  r_old/z_old are the positions before the last
+ * move. */
+int pcp_download_species(pcp_ctx *ctx, int sp, double *r, double *z, double *vr, double *vphi,
+                         double *vz, double *w, uint64_t *id, uint64_t *order, double *r_old,
+                         double *z_old);
+/* Synthetic seeded inputs generated on the device (counter-based RNG), in
+ * cell order: n_per_cell particles per cell with r^2 and z uniform inside the
+ * cell, weight density * cell volume / n_per_cell, Maxwellian velocities of
+ * temperature_ev.  density_right != density_left gives the linear z ramp of
+ * SMB::SMB (src/SMB.cpp:134-136), rounding the per-cell count. */
+int pcp_init_plasma(pcp_ctx *ctx, int sp, int n_per_cell, double density_left,
+                    double density_right, double temperature_ev, uint64_t seed);
+/* Gaussian bunch (sigma_r, sigma_z) centred at z0 on the axis, moving along z
+ * with Lorentz factor gamma, peak density n_peak, n macro-particles. */
+int pcp_init_bunch(pcp_ctx *ctx, int sp, long n, double n_peak, double sigma_r, double sigma_z,
+                   double z0, double gamma, uint64_t seed);
+
+/* Whole halo grids, domain-major [domain][component][Nr_d+4][Nz_d+4]
+ * (one component for EPS and SIGMA), as Grid<double> stores them
+ * (include/algo/grid.hpp:24-58). */
+int pcp_download_grid(pcp_ctx *ctx, int which, double *host);
+int pcp_upload_grid(pcp_ctx *ctx, int which, const double *host);
+long pcp_grid_points(pcp_ctx *ctx);
+
+/* MaxwellSolverYee::calc_field_e / calc_field_h for every domain
+ * (src/maxwellSolver/maxwellSolverYee.cpp:111-257) */
+int pcp_yee_e(pcp_ctx *ctx);
+int pcp_yee_h(pcp_ctx *ctx);
+/* SMB::field_e_overlay / field_h_overlay / current_overlay
+ * (src/SMB.cpp:566-596, 784-823, 1010-1044); which = E, H (stitches H and
+ * H_at_et) or J.  After the J stitch only interior J values are defined. */
+int pcp_stitch(pcp_ctx *ctx, int which);
+/* Pusher::operator() for every domain (include/pusher.hpp:58) */
+int pcp_push(pcp_ctx *ctx);
+/* SMB::advance_particles after the push: dump_position_to_old, mover,
+ * back_position_to_rz, reflect, back_velocity_to_rz, bind_cell_numbers
+ * (src/SMB.cpp:1290-1303, src/specieP.cpp:482-663) */
+int pcp_move(pcp_ctx *ctx);
+/* SMB::particles_runaway_collector (src/SMB.cpp:194-344): removal and the
+ * reference's vector order of migrating particles */
+int pcp_migrate(pcp_ctx *ctx);
+/* Domain::reset_current + Current::current_distribution
+ * (src/domain.cpp:135-139, include/current.hpp:53) */
+int pcp_deposit(pcp_ctx *ctx);
+/* n whole steps: solve_maxvell, advance_particles, solve_current
+ * (src/PiCoPiC.cpp:299-345) with push + move + deposit fused */
+int pcp_step(pcp_ctx *ctx, int n_steps);
+
+/* field gather at one point of domain d (include/maxwellSolver.hpp:101-316),
+ * evaluated on the device; out[3] = (r, phi, z) components */
+int pcp_get_field_e(pcp_ctx *ctx, int d, double r, double z, double *out);
+int pcp_get_field_h(pcp_ctx *ctx, int d, double r, double z, double *out);
+
+/* hipStream_t the context enqueues on (as void*) */
+void *pcp_stream(pcp_ctx *ctx);
+/* average device time [ms] of the named kernel over the last pcp_step call
+ * when timing is enabled with pcp_set_timing(ctx, 1); -1 if unknown */
+int pcp_set_timing(pcp_ctx *ctx, int on);
+double pcp_kernel_ms(pcp_ctx *ctx, const char *name);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,166 @@
+/*
+ * CPU oracle for the PiCoPiC per-step particle loop (test infrastructure only).
+ *
+ * A plain-C restatement of the reference's FP64 algorithm: field gather,
+ * Boris / Vay / Higuera-Cary pushers, cylindrical mover + reflection + rz
+ * rotation + cell binding, Villasenor-Buneman and ZigZag current deposition,
+ * the Yee E/H update with PML, the domain-edge overlay stitching, the
+ * runaway collector (particle migration between domains) and the P12 binary
+ * collision operator.  Every function cites the reference file:line it
+ * follows; arithmetic is written in the same order as the reference so that
+ * an IEEE build (-ffp-contract=off, no fast-math) reproduces it.
+ *
+ * This library is the checker: only tests, the smoke check and the CPU
+ * baseline leg of the benchmark load it.  It is never the product path.
+ *
+ * The parity status of each function (which known-answer vector, closed
+ * form, invariant or brute-force check pins it, or "parity unpinned") is
+ * listed at the top of picopic_oracle.c.
+ */
+#ifndef PICOPIC_ORACLE_H
+#define PICOPIC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* reference include/constant.hpp:28-58 */
+#define ORC_PI 3.1415926535897932
+#define ORC_EPSILON0 8.85E-12
+#define ORC_EL_MASS 9.1E-31
+#define ORC_EL_CHARGE 1.6E-19
+#define ORC_LIGHT_VEL 3.0E8
+#define ORC_MAGN_CONST 1.26E-6
+#define ORC_MNZL 1e-15
+#define ORC_PLANK_CONST 6.6261e-34
+/* configure.ac:224 */
+#define ORC_REL_LIMIT 5e7
+
+enum { ORC_PUSHER_BORIS = 0, ORC_PUSHER_BORIS_REL = 1, ORC_PUSHER_BORIS_ADAPTIVE = 2,
+       ORC_PUSHER_VAY = 3, ORC_PUSHER_HC = 4 };
+enum { ORC_CCS_VB = 0, ORC_CCS_ZIGZAG = 1 };
+enum { ORC_GRID_E = 0, ORC_GRID_H = 1, ORC_GRID_H_AT_ET = 2, ORC_GRID_J = 3,
+       ORC_GRID_EPS = 4, ORC_GRID_SIGMA = 5 };
+
+/* reference include/geometry.hpp:27-100 (per-domain Geometry) */
+typedef struct {
+  double size[2];
+  double global_size[2];
+  double cell_size[2];
+  long cell_dims[4];          /* bot_r, left_z, top_r, right_z */
+  long cell_amount[2];
+  long global_cell_amount[2];
+  int walls[4];               /* r=0, z=0, r=R, z=L */
+  long pml_size[4];
+  double pml_sigma[2];
+} orc_geometry;
+
+/* reference include/algo/grid.hpp:24-58: halo grid, x = r rows, y = z contiguous */
+typedef struct {
+  int o_s;
+  int x_real, y_real;
+  int x_size, y_size;
+  double *g;                  /* g[i * y_real + j] */
+} orc_grid;
+
+/* reference include/specieP.hpp:74-119 (Particle), plus a stable id */
+typedef struct {
+  double pos_r, pos_phi, pos_z;
+  double pos_old_r, pos_old_phi, pos_old_z;
+  double vel_r, vel_phi, vel_z;
+  double weight;
+  double sin;
+  long cell_r, cell_z;
+  uint64_t id;
+} orc_particle;
+
+/* global configuration (the JSON geometry/time/particles blocks) */
+typedef struct {
+  double size[2];             /* radius, longitude [m] */
+  long grid[2];               /* cells r, z */
+  long domains[2];            /* domains r, z */
+  double dt;
+  int pusher;                 /* ORC_PUSHER_* */
+  int ccs;                    /* ORC_CCS_* */
+  int pml_on;                 /* ENABLE_PML */
+  double pml_left, pml_right, pml_outer;   /* fractions of the grid */
+  double pml_sigma1, pml_sigma2;
+  int n_species;
+  double charge[8];           /* SI (already multiplied by EL_CHARGE) */
+  double mass[8];             /* SI */
+  int is_beam[8];             /* BeamP::reflect instead of SpecieP::reflect */
+} orc_config;
+
+typedef struct orc_smb orc_smb;
+
+/* ---- SMB-level API (reference src/SMB.cpp) ---- */
+orc_smb *orc_smb_create(const orc_config *cfg);
+void orc_smb_destroy(orc_smb *s);
+int orc_smb_n_domains(const orc_smb *s);
+void orc_smb_domain_geometry(const orc_smb *s, int d, orc_geometry *out);
+/* cells per domain including halo: (Nr_d + 4) x (Nz_d + 4) */
+long orc_smb_grid_points(const orc_smb *s);
+
+/* Place particles: each goes to the domain that owns its cell; order within a
+ * domain follows input order (reference Domain::distribute + bind). */
+int orc_smb_add_particles(orc_smb *s, int sp, long n, const double *r, const double *z,
+                          const double *vr, const double *vphi, const double *vz,
+                          const double *w, const uint64_t *id);
+long orc_smb_count(const orc_smb *s, int sp);
+/* Domain-major, vector order within a domain. dom may be NULL. */
+void orc_smb_get_particles(const orc_smb *s, int sp, double *r, double *z, double *vr,
+                           double *vphi, double *vz, double *w, uint64_t *id, int *dom,
+                           double *r_old, double *z_old);
+
+/* whole halo grids, domain-major: [domain][comp][x_real][y_real] */
+void orc_smb_get_grid(const orc_smb *s, int which, double *out);
+void orc_smb_set_grid(orc_smb *s, int which, const double *in);
+
+/* step phases (reference src/SMB.cpp:1231-1306, src/domain.cpp:118-204) */
+enum { ORC_PH_YEE_E = 0, ORC_PH_OVERLAY_E, ORC_PH_YEE_H, ORC_PH_OVERLAY_H,
+       ORC_PH_PUSH, ORC_PH_MOVE, ORC_PH_MIGRATE, ORC_PH_RESET_J, ORC_PH_DEPOSIT,
+       ORC_PH_OVERLAY_J, ORC_PH_COUNT };
+void orc_smb_phase(orc_smb *s, int phase);
+/* solve_maxvell + advance_particles + solve_current, n times (no beam injection) */
+void orc_smb_step(orc_smb *s, int n);
+void orc_set_threads(int n);
+
+/* ---- unit-level entry points used by the known-answer tests ---- */
+void orc_grid_overlay_x(orc_grid *a, orc_grid *b);
+void orc_grid_overlay_y(orc_grid *a, orc_grid *b);
+void orc_grid_overlay_xy(orc_grid *a, orc_grid *b);
+void orc_grid_overlay_set(orc_grid *a, double v);
+
+/* gather on domain d of an SMB */
+void orc_smb_get_field_e(const orc_smb *s, int d, double r, double z, double out[3]);
+void orc_smb_get_field_h(const orc_smb *s, int d, double r, double z, double out[3]);
+/* one particle velocity update with given (already gathered) fields */
+void orc_push_velocity(int pusher, double charge, double mass, double dt,
+                       const double e[3], const double h[3], double v[3]);
+double orc_lorenz_factor(double sq_velocity);
+double orc_lorenz_factor_inv(double sq_velocity);
+
+/* ---- collisions (reference src/collisions.cpp, src/collisions/collisionsP12.cpp) ----
+ * Random draws are taken from a caller-supplied stream so that a GPU
+ * implementation fed the same stream can be compared pair by pair. */
+typedef struct {
+  const double *u;  /* uniform [0,1) draws, consumed in order */
+  long n, pos;
+} orc_rng_stream;
+/* P12 on one domain for the electron species se and ion species si, cells
+ * visited r-major exactly like CollisionsP12::collide; the per-cell particle
+ * order is the given vector order (the random shuffle is the caller's). */
+void orc_smb_collide_p12(orc_smb *s, int d, int se, int si, orc_rng_stream *rng);
+void orc_p12_collide_single(double m_a, double m_b, double q_a, double q_b,
+                            double *va, double wa, double *vb, double wb,
+                            double dens_a, double dens_b, double debye, double dt,
+                            orc_rng_stream *rng);
+double orc_debye_length(double n_e, double n_i, double t_e, double t_i);
+double orc_coulomb_logarithm(double m_a, double m_b, double debye, double v_rel);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

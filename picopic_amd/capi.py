@@ -1,0 +1,229 @@
+"""ctypes binding of the picopic_amd C ABI (include/picopic.h).
+
+The HIP library must be built (``python -m picopic_amd.build``); importing
+this module on a machine without it raises instead of falling back to any CPU
+path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libpicopic_hip.so")
+
+PUSHERS = {"boris": 0, "boris-relativistic": 1, "boris-adaptive": 2, "vay": 3, "hc": 4}
+CCS = {"vb": 0, "zigzag": 1}
+GRIDS = {"E": 0, "H": 1, "H_at_et": 2, "J": 3, "eps": 4, "sigma": 5}
+PREC = {"fp64": 0, "fp32": 1}
+
+EL_CHARGE = 1.6e-19
+EL_MASS = 9.1e-31
+
+EXPORTS = [
+    "pcp_last_error", "pcp_create", "pcp_destroy", "pcp_sync", "pcp_upload_species", "pcp_count",
+    "pcp_download_species", "pcp_init_plasma", "pcp_init_bunch", "pcp_download_grid", "pcp_upload_grid",
+    "pcp_grid_points", "pcp_yee_e", "pcp_yee_h", "pcp_stitch", "pcp_push", "pcp_move", "pcp_migrate",
+    "pcp_deposit", "pcp_step", "pcp_get_field_e", "pcp_get_field_h", "pcp_stream", "pcp_set_timing",
+    "pcp_kernel_ms",
+]
+
+
+class SpeciesDesc(C.Structure):
+    _fields_ = [("charge", C.c_double), ("mass", C.c_double), ("is_beam", C.c_int), ("capacity", C.c_long)]
+
+
+class Config(C.Structure):
+    _fields_ = [("size_r", C.c_double), ("size_z", C.c_double), ("grid_r", C.c_long), ("grid_z", C.c_long),
+                ("domains_r", C.c_long), ("domains_z", C.c_long), ("dt", C.c_double), ("pusher", C.c_int),
+                ("ccs", C.c_int), ("pml_on", C.c_int), ("pml_left", C.c_double), ("pml_right", C.c_double),
+                ("pml_outer", C.c_double), ("pml_sigma1", C.c_double), ("pml_sigma2", C.c_double),
+                ("precision", C.c_int), ("n_species", C.c_int), ("species", SpeciesDesc * 8),
+                ("max_migrants", C.c_long)]
+
+
+class PcpError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise PcpError("picopic_amd HIP library not built: %s (run python -m picopic_amd.build)" % path)
+    L = C.CDLL(path)
+    vp, dp, u64p = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)
+    L.pcp_last_error.restype = C.c_char_p
+    L.pcp_create.argtypes = [C.POINTER(Config), C.POINTER(vp)]
+    L.pcp_destroy.argtypes = [vp]
+    L.pcp_sync.argtypes = [vp]
+    L.pcp_upload_species.argtypes = [vp, C.c_int, C.c_long] + [dp] * 6 + [u64p]
+    L.pcp_count.argtypes = [vp, C.c_int, C.POINTER(C.c_long)]
+    L.pcp_download_species.argtypes = [vp, C.c_int] + [dp] * 6 + [u64p, u64p, dp, dp]
+    L.pcp_init_plasma.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64]
+    L.pcp_init_bunch.argtypes = [vp, C.c_int, C.c_long] + [C.c_double] * 5 + [C.c_uint64]
+    L.pcp_download_grid.argtypes = [vp, C.c_int, dp]
+    L.pcp_upload_grid.argtypes = [vp, C.c_int, dp]
+    L.pcp_grid_points.argtypes = [vp]
+    L.pcp_grid_points.restype = C.c_long
+    for f in ("pcp_yee_e", "pcp_yee_h", "pcp_push", "pcp_move", "pcp_migrate", "pcp_deposit"):
+        getattr(L, f).argtypes = [vp]
+    L.pcp_stitch.argtypes = [vp, C.c_int]
+    L.pcp_step.argtypes = [vp, C.c_int]
+    L.pcp_get_field_e.argtypes = [vp, C.c_int, C.c_double, C.c_double, dp]
+    L.pcp_get_field_h.argtypes = [vp, C.c_int, C.c_double, C.c_double, dp]
+    L.pcp_stream.argtypes = [vp]
+    L.pcp_stream.restype = vp
+    L.pcp_set_timing.argtypes = [vp, C.c_int]
+    L.pcp_kernel_ms.argtypes = [vp, C.c_char_p]
+    L.pcp_kernel_ms.restype = C.c_double
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise PcpError("picopic_amd error %d: %s" % (rc, _lib.pcp_last_error().decode()))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _u64p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint64))
+
+
+class Context:
+    """One shared-memory block of domains resident on the GPU."""
+
+    def __init__(self, size, grid, domains, dt, species, pusher="boris-adaptive", ccs="vb", pml=None,
+                 capacity=None, precision="fp64", max_migrants=0):
+        """species: list of (charge_in_e, mass_in_me, is_beam[, capacity])."""
+        L = load()
+        cfg = Config()
+        cfg.size_r, cfg.size_z = size
+        cfg.grid_r, cfg.grid_z = grid
+        cfg.domains_r, cfg.domains_z = domains
+        cfg.dt = dt
+        cfg.pusher = PUSHERS[pusher]
+        cfg.ccs = CCS[ccs]
+        if pml is None:
+            cfg.pml_on = 0
+        else:
+            cfg.pml_on = 1
+            cfg.pml_left, cfg.pml_right, cfg.pml_outer, cfg.pml_sigma1, cfg.pml_sigma2 = pml
+        cfg.precision = PREC[precision]
+        cfg.n_species = len(species)
+        for i, spc in enumerate(species):
+            q, m, beam = spc[:3]
+            cap = spc[3] if len(spc) > 3 else (capacity if capacity is not None else 1 << 20)
+            cfg.species[i] = SpeciesDesc(q * EL_CHARGE, m * EL_MASS, int(beam), int(cap))
+        cfg.max_migrants = max_migrants
+        self.cfg = cfg
+        h = C.c_void_p()
+        _check(L.pcp_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.n_species = len(species)
+        self.D = domains[0] * domains[1]
+        self.nr_d = grid[0] // domains[0]
+        self.nz_d = grid[1] // domains[1]
+        self.P = L.pcp_grid_points(h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.pcp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def upload(self, sp, r, z, vr, vphi, vz, w, ids=None):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (r, z, vr, vphi, vz, w)]
+        n = len(arrs[0])
+        idp = None
+        if ids is not None:
+            ids = np.ascontiguousarray(ids, dtype=np.uint64)
+            idp = _u64p(ids)
+        _check(_lib.pcp_upload_species(self.h, sp, n, *[_dp(a) for a in arrs], idp))
+
+    def count(self, sp):
+        n = C.c_long()
+        _check(_lib.pcp_count(self.h, sp, C.byref(n)))
+        return n.value
+
+    def particles(self, sp):
+        n = self.count(sp)
+        out = {k: np.zeros(n) for k in ("r", "z", "vr", "vphi", "vz", "w", "r_old", "z_old")}
+        ids = np.zeros(n, dtype=np.uint64)
+        order = np.zeros(n, dtype=np.uint64)
+        _check(_lib.pcp_download_species(self.h, sp, *[_dp(out[k]) for k in ("r", "z", "vr", "vphi", "vz", "w")],
+                                         _u64p(ids), _u64p(order), _dp(out["r_old"]), _dp(out["z_old"])))
+        out["id"] = ids
+        out["order"] = order
+        return out
+
+    def init_plasma(self, sp, ppc, density_left, density_right, temperature_ev, seed):
+        _check(_lib.pcp_init_plasma(self.h, sp, ppc, density_left, density_right, temperature_ev, seed))
+
+    def init_bunch(self, sp, n, n_peak, sigma_r, sigma_z, z0, gamma, seed):
+        _check(_lib.pcp_init_bunch(self.h, sp, n, n_peak, sigma_r, sigma_z, z0, gamma, seed))
+
+    def _ncomp(self, which):
+        return 1 if which in ("eps", "sigma") else 3
+
+    def get_grid(self, which):
+        out = np.zeros((self.D, self._ncomp(which), self.nr_d + 4, self.nz_d + 4))
+        _check(_lib.pcp_download_grid(self.h, GRIDS[which], _dp(out)))
+        return out
+
+    def set_grid(self, which, arr):
+        a = np.ascontiguousarray(np.broadcast_to(arr, (self.D, self._ncomp(which), self.nr_d + 4, self.nz_d + 4)),
+                                 dtype=np.float64)
+        _check(_lib.pcp_upload_grid(self.h, GRIDS[which], _dp(a)))
+
+    def phase(self, name):
+        L = _lib
+        if name == "yee_e":
+            _check(L.pcp_yee_e(self.h))
+        elif name == "overlay_e":
+            _check(L.pcp_stitch(self.h, GRIDS["E"]))
+        elif name == "yee_h":
+            _check(L.pcp_yee_h(self.h))
+        elif name == "overlay_h":
+            _check(L.pcp_stitch(self.h, GRIDS["H"]))
+        elif name == "push":
+            _check(L.pcp_push(self.h))
+        elif name == "move":
+            _check(L.pcp_move(self.h))
+        elif name == "migrate":
+            _check(L.pcp_migrate(self.h))
+        elif name == "reset_j":
+            pass   # folded into deposit
+        elif name == "deposit":
+            _check(L.pcp_deposit(self.h))
+        elif name == "overlay_j":
+            _check(L.pcp_stitch(self.h, GRIDS["J"]))
+        else:
+            raise ValueError(name)
+
+    def step(self, n=1):
+        _check(_lib.pcp_step(self.h, n))
+
+    def sync(self):
+        _check(_lib.pcp_sync(self.h))
+
+    def get_field_e(self, d, r, z):
+        out = np.zeros(3)
+        _check(_lib.pcp_get_field_e(self.h, d, r, z, _dp(out)))
+        return out
+
+    def get_field_h(self, d, r, z):
+        out = np.zeros(3)
+        _check(_lib.pcp_get_field_h(self.h, d, r, z, _dp(out)))
+        return out

@@ -1,0 +1,626 @@
+// C ABI of picopic_amd (include/picopic.h): context life cycle, host <-> device
+// transfers, and the step phases.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pcp_init.hpp"
+#include "pcp_internal.hpp"
+#include "pcp_kernels.hpp"
+
+using namespace pcp;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(PCP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
+  } while (0)
+
+template <class T>
+int dalloc(T **p, size_t n) {
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess) return fail(PCP_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  e = hipMemset(*p, 0, n * sizeof(T));
+  if (e != hipSuccess) return fail(PCP_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+  return 0;
+}
+
+// MaxwellSolverYee::set_pml for one domain (maxwellSolverYee.cpp:57-109),
+// evaluated on the host once at creation.
+void set_pml_host(const pcp_ctx *c, int d, const long pml_size[4], double *sigma) {
+  const Geom &g = c->g;
+  Dom dm = make_dom(g, d);
+  double cell_size_r = g.dr, cell_size_z = g.dz;
+  double lenght_sigma_left = cell_size_z * pml_size[1];
+  double lenght_sigma_right = cell_size_z * pml_size[3];
+  double lenght_sigma_extern = cell_size_r * pml_size[2];
+  double radial_cell_offset = dm.bot_r * g.dr;
+  double longitudinal_cell_offset = dm.left_z * g.dz;
+  double system_radius = g.gsize_r;
+  double system_longitude = g.gsize_z;
+  double delta_sigma = c->cfg.pml_sigma2 - c->cfg.pml_sigma1;
+  auto S = [&](int i, int k) -> double & { return sigma[(long)(i + 2) * g.yr + (k + 2)]; };
+  if (pml_size[2] != 0)
+    for (int i = 0; i < g.nr; i++)
+      for (int k = 0; k < g.nz; k++) {
+        unsigned long rel_end = (unsigned long)(g.gca_r - dm.bot_r);
+        double dist = (double)(rel_end - (unsigned long)i) * g.dr;
+        if (dist < lenght_sigma_extern)
+          S(i, k) += c->cfg.pml_sigma1 + delta_sigma / std::pow(lenght_sigma_extern, 2) *
+                     std::pow(cell_size_r * (i + 1) + radial_cell_offset - system_radius + lenght_sigma_extern, 2);
+      }
+  if (pml_size[1] != 0)
+    for (int k = 0; k < g.nz; k++)
+      for (int i = 0; i < g.nr; i++)
+        if (cell_size_z * k + longitudinal_cell_offset < lenght_sigma_left)
+          S(i, k) += c->cfg.pml_sigma1 + delta_sigma / std::pow(lenght_sigma_left, 2) *
+                     std::pow(lenght_sigma_left - cell_size_z * k - longitudinal_cell_offset, 2);
+  if (pml_size[3] != 0)
+    for (int k = 0; k < g.nz; k++)
+      for (int i = 0; i < g.nr; i++) {
+        unsigned long rel_end = (unsigned long)(g.gca_z - dm.left_z);
+        double dist = (double)(rel_end - (unsigned long)k) * g.dz;
+        if (dist < lenght_sigma_right)
+          S(i, k) += c->cfg.pml_sigma1 + delta_sigma / std::pow(lenght_sigma_right, 2) *
+                     std::pow(cell_size_z * (k + 1) + longitudinal_cell_offset - system_longitude + lenght_sigma_right, 2);
+      }
+}
+
+int check_state(pcp_ctx *c) {
+  StepState h;
+  HIP_TRY(hipMemcpyAsync(&h, c->st, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  int fatal = h.err & (ERR_NAN_POSITION | ERR_BETA_GT_1 | ERR_NAN_FIELD | ERR_MIGRANT_OVERFLOW);
+  if (fatal) {
+    char buf[160];
+    snprintf(buf, sizeof buf, "device reported error bits 0x%x (1 NaN position, 2 beta>1, 4 NaN field, 8 migrant overflow)",
+             h.err);
+    int zero = 0;
+    hipMemcpy(&c->st->err, &zero, sizeof zero, hipMemcpyHostToDevice);
+    return fail(PCP_E_STATE, buf);
+  }
+  return 0;
+}
+
+double *grid_ptr(pcp_ctx *c, int which, int *ncomp) {
+  *ncomp = 3;
+  switch (which) {
+    case PCP_GRID_E: return c->f.E;
+    case PCP_GRID_H: return c->f.H;
+    case PCP_GRID_H_AT_ET: return c->f.Hat;
+    case PCP_GRID_J: return c->f.J;
+    case PCP_GRID_EPS: *ncomp = 1; return c->f.eps;
+    case PCP_GRID_SIGMA: *ncomp = 1; return c->f.sigma;
+    default: return nullptr;
+  }
+}
+
+int ensure_capacity(pcp_ctx *c, int sp, long add) {
+  if (c->sp[sp].n + add > c->sp[sp].cap) {
+    char buf[128];
+    snprintf(buf, sizeof buf, "species %d capacity %ld exceeded (%ld + %ld)", sp, c->sp[sp].cap, c->sp[sp].n, add);
+    return fail(PCP_E_NOMEM, buf);
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *pcp_last_error(void) { return g_last_error.c_str(); }
+
+int pcp_create(const pcp_config *cfg, pcp_ctx **out) {
+  if (!cfg || !out) return fail(PCP_E_ARG, "null argument");
+  *out = nullptr;
+  if (cfg->grid_r <= 0 || cfg->grid_z <= 0 || cfg->domains_r <= 0 || cfg->domains_z <= 0)
+    return fail(PCP_E_ARG, "grid and domains must be positive");
+  if (cfg->grid_r % cfg->domains_r || cfg->grid_z % cfg->domains_z)
+    return fail(PCP_E_ARG, "the grid must divide evenly into domains (overlay needs equal domain sizes)");
+  if (cfg->n_species < 1 || cfg->n_species > MAX_SPECIES) return fail(PCP_E_ARG, "n_species must be 1..8");
+  if (cfg->pusher < 0 || cfg->pusher > PCP_PUSHER_HC) return fail(PCP_E_ARG, "unknown pusher");
+  if (cfg->ccs != PCP_CCS_VB && cfg->ccs != PCP_CCS_ZIGZAG) return fail(PCP_E_ARG, "unknown ccs");
+  if (cfg->precision != PCP_PREC_FP64 && cfg->precision != PCP_PREC_FP32_SPLIT)
+    return fail(PCP_E_ARG, "unknown precision");
+  if (cfg->precision == PCP_PREC_FP32_SPLIT) return fail(PCP_E_ARG, "FP32 split contexts are created by pcp32_create");
+  if (cfg->domains_r * cfg->domains_z > 4096) return fail(PCP_E_ARG, "at most 4096 domains");
+  if (!(cfg->dt > 0)) return fail(PCP_E_ARG, "dt must be positive");
+
+  pcp_ctx *c = new pcp_ctx();
+  c->cfg = *cfg;
+  Geom &g = c->g;
+  g.Dr = (int)cfg->domains_r;
+  g.Dz = (int)cfg->domains_z;
+  g.nr = (int)(cfg->grid_r / cfg->domains_r);
+  g.nz = (int)(cfg->grid_z / cfg->domains_z);
+  g.xr = g.nr + 4;
+  g.yr = g.nz + 4;
+  g.P = (long)g.xr * g.yr;
+  g.Nr = cfg->grid_r;
+  g.Nz = cfg->grid_z;
+  // domain geometry exactly as SMB::SMB builds it (src/SMB.cpp:96-103,
+  // include/geometry.hpp:52-75)
+  g.size_r = cfg->size_r / g.Dr;
+  g.size_z = cfg->size_z / g.Dz;
+  g.dr = g.size_r / g.nr;
+  g.dz = g.size_z / g.nz;
+  g.gsize_r = cfg->size_r;
+  g.gsize_z = cfg->size_z;
+  g.gca_r = (long)(g.gsize_r / g.dr);
+  g.gca_z = (long)(g.gsize_z / g.dz);
+  g.dt = cfg->dt;
+  g.pml_on = cfg->pml_on;
+  g.ccs = cfg->ccs;
+  g.pusher = cfg->pusher;
+  c->D = g.Dr * g.Dz;
+  c->cur = 0;
+
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) != hipSuccess) {
+    delete c;
+    return fail(PCP_E_HIP, "no HIP device");
+  }
+  c->n_cu = prop.multiProcessorCount;
+  if (hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return fail(PCP_E_HIP, "hipStreamCreate failed");
+  }
+  long gridn = (long)c->D * 3 * g.P;
+  int rc = 0;
+  if ((rc = dalloc(&c->f.E, gridn)) || (rc = dalloc(&c->f.H, gridn)) || (rc = dalloc(&c->f.Hat, gridn)) ||
+      (rc = dalloc(&c->f.J, gridn)) || (rc = dalloc(&c->f.eps, (size_t)c->D * g.P)) ||
+      (rc = dalloc(&c->f.sigma, (size_t)c->D * g.P)) || (rc = dalloc(&c->st, 1)) ||
+      (rc = dalloc(&c->marker, (size_t)c->D * g.P))) {
+    pcp_destroy(c);
+    return rc;
+  }
+  // epsilon = EPSILON0 everywhere (maxwellSolver.hpp:60-62); sigma per PML
+  {
+    std::vector<double> eps((size_t)c->D * g.P, EPSILON0);
+    std::vector<double> sig((size_t)c->D * g.P, 0.0);
+    if (cfg->pml_on) {
+      long gpml[4] = {0, (long)std::floor(cfg->pml_left * cfg->grid_z), (long)std::floor(cfg->pml_outer * cfg->grid_r),
+                      (long)std::floor(cfg->pml_right * cfg->grid_z)};
+      for (int d = 0; d < c->D; ++d) {
+        Dom dm = make_dom(g, d);
+        long top_r = dm.bot_r + g.nr, right_z = dm.left_z + g.nz;
+        long psz[4] = {0, 0, 0, 0};
+        // SMB.cpp:84-93 (size_t comparisons)
+        long gca_r_glob = (long)(cfg->size_r / (cfg->size_r / cfg->grid_r));
+        long gca_z_glob = (long)(cfg->size_z / (cfg->size_z / cfg->grid_z));
+        if ((unsigned long)(gca_r_glob - top_r) < (unsigned long)gpml[2]) psz[2] = gpml[2];
+        if ((unsigned long)(gca_z_glob - right_z) < (unsigned long)gpml[3]) psz[3] = gpml[3];
+        if (dm.left_z < gpml[1]) psz[1] = gpml[1];
+        set_pml_host(c, d, psz, sig.data() + (size_t)d * g.P);
+      }
+    }
+    if (hipMemcpy(c->f.eps, eps.data(), eps.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->f.sigma, sig.data(), sig.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+      pcp_destroy(c);
+      return fail(PCP_E_HIP, "upload of epsilon/sigma failed");
+    }
+  }
+  // species
+  long total_cap = 0;
+  for (int s = 0; s < cfg->n_species; ++s) {
+    SpeciesDev &sp = c->sp[s];
+    sp.cap = cfg->species[s].capacity;
+    sp.q = cfg->species[s].charge;
+    sp.m = cfg->species[s].mass;
+    sp.beam = cfg->species[s].is_beam;
+    sp.n = 0;
+    if (sp.cap < 0 || !(sp.m > 0)) {
+      pcp_destroy(c);
+      return fail(PCP_E_ARG, "species capacity must be >= 0 and mass > 0");
+    }
+    total_cap += sp.cap;
+    if ((rc = dalloc(&sp.r[0], sp.cap)) || (rc = dalloc(&sp.r[1], sp.cap)) || (rc = dalloc(&sp.z[0], sp.cap)) ||
+        (rc = dalloc(&sp.z[1], sp.cap)) || (rc = dalloc(&sp.vr, sp.cap)) || (rc = dalloc(&sp.vphi, sp.cap)) ||
+        (rc = dalloc(&sp.vz, sp.cap)) || (rc = dalloc(&sp.w, sp.cap)) || (rc = dalloc(&sp.seq, sp.cap)) ||
+        (rc = dalloc(&sp.id, sp.cap))) {
+      pcp_destroy(c);
+      return rc;
+    }
+  }
+  // migration bookkeeping
+  Migrants &m = c->mig;
+  m.cap = cfg->max_migrants > 0 ? cfg->max_migrants : std::max<long>(1 << 16, total_cap / 64);
+  m.ngroups = c->D * MAX_SPECIES;
+  if ((rc = dalloc(&m.key, m.cap)) || (rc = dalloc(&m.ref, m.cap)) || (rc = dalloc(&m.group, m.cap)) ||
+      (rc = dalloc(&m.bucket, m.cap)) || (rc = dalloc(&m.gcount, m.ngroups)) || (rc = dalloc(&m.gstart, m.ngroups)) ||
+      (rc = dalloc(&m.gfill, m.ngroups)) || (rc = dalloc(&m.gbase, m.ngroups))) {
+    pcp_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return 0;
+}
+
+int pcp_destroy(pcp_ctx *c) {
+  if (!c) return 0;
+  hipFree(c->f.E); hipFree(c->f.H); hipFree(c->f.Hat); hipFree(c->f.J);
+  hipFree(c->f.eps); hipFree(c->f.sigma); hipFree(c->st); hipFree(c->marker);
+  for (int s = 0; s < MAX_SPECIES; ++s) {
+    SpeciesDev &sp = c->sp[s];
+    hipFree(sp.r[0]); hipFree(sp.r[1]); hipFree(sp.z[0]); hipFree(sp.z[1]);
+    hipFree(sp.vr); hipFree(sp.vphi); hipFree(sp.vz); hipFree(sp.w); hipFree(sp.seq); hipFree(sp.id);
+  }
+  Migrants &m = c->mig;
+  hipFree(m.key); hipFree(m.ref); hipFree(m.group); hipFree(m.bucket);
+  hipFree(m.gcount); hipFree(m.gstart); hipFree(m.gfill); hipFree(m.gbase);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+int pcp_sync(pcp_ctx *c) {
+  if (!c) return fail(PCP_E_ARG, "null context");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return check_state(c);
+}
+
+void *pcp_stream(pcp_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+long pcp_grid_points(pcp_ctx *c) { return c ? c->g.P : -1; }
+
+int pcp_upload_species(pcp_ctx *c, int sp, long n, const double *r, const double *z, const double *vr,
+                       const double *vphi, const double *vz, const double *w, const uint64_t *id) {
+  if (!c || sp < 0 || sp >= c->cfg.n_species || n < 0) return fail(PCP_E_ARG, "bad argument");
+  if (n == 0) return 0;
+  if (!r || !z || !vr || !vphi || !vz || !w) return fail(PCP_E_ARG, "null particle array");
+  int rc = ensure_capacity(c, sp, n);
+  if (rc) return rc;
+  const Geom &g = c->g;
+  std::vector<uint64_t> gbase(c->mig.ngroups);
+  HIP_TRY(hipMemcpy(gbase.data(), c->mig.gbase, gbase.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  std::vector<uint64_t> seq(n), ids(n);
+  for (long p = 0; p < n; ++p) {
+    long cr = (long)std::floor(r[p] / g.dr), cz = (long)std::floor(z[p] / g.dz);
+    if (!(cr >= 0 && cz >= 0 && cr < g.Nr && cz < g.Nz)) {
+      char buf[128];
+      snprintf(buf, sizeof buf, "particle %ld at (%g, %g) is outside the simulation area", p, r[p], z[p]);
+      return fail(PCP_E_ARG, buf);
+    }
+    int d = (int)((cr / g.nr) * g.Dz + cz / g.nz);
+    seq[p] = gbase[(size_t)d * MAX_SPECIES + sp]++;
+    ids[p] = id ? id[p] : (uint64_t)(c->sp[sp].n + p);
+  }
+  SpeciesDev &s = c->sp[sp];
+  size_t off = (size_t)s.n, bytes = (size_t)n * sizeof(double);
+  HIP_TRY(hipMemcpy(s.r[0] + off, r, bytes, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s.r[1] + off, r, bytes, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s.z[0] + off, z, bytes, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s.z[1] + off, z, bytes, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s.vr + off, vr, bytes, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s.vphi + off, vphi, bytes, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s.vz + off, vz, bytes, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s.w + off, w, bytes, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s.seq + off, seq.data(), (size_t)n * sizeof(uint64_t), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s.id + off, ids.data(), (size_t)n * sizeof(uint64_t), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->mig.gbase, gbase.data(), gbase.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+  s.n += n;
+  return 0;
+}
+
+int pcp_count(pcp_ctx *c, int sp, long *n) {
+  if (!c || !n || sp < 0 || sp >= c->cfg.n_species) return fail(PCP_E_ARG, "bad argument");
+  if (c->sp[sp].n == 0) { *n = 0; return 0; }
+  unsigned long long *d = nullptr, h = 0;
+  HIP_TRY(hipMalloc(&d, sizeof h));
+  HIP_TRY(hipMemsetAsync(d, 0, sizeof h, c->stream));
+  launch_count_live(c, sp, d);
+  HIP_TRY(hipMemcpyAsync(&h, d, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  hipFree(d);
+  *n = (long)h;
+  return 0;
+}
+
+int pcp_download_species(pcp_ctx *c, int sp, double *r, double *z, double *vr, double *vphi, double *vz, double *w,
+                         uint64_t *id, uint64_t *order, double *r_old, double *z_old) {
+  if (!c || sp < 0 || sp >= c->cfg.n_species) return fail(PCP_E_ARG, "bad argument");
+  SpeciesDev &s = c->sp[sp];
+  long n = s.n;
+  if (n == 0) return 0;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  std::vector<double> hr(n), hz(n), tmp(n);
+  std::vector<uint64_t> hu(n);
+  HIP_TRY(hipMemcpy(hr.data(), s.r[c->cur], n * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(hz.data(), s.z[c->cur], n * sizeof(double), hipMemcpyDeviceToHost));
+  auto emit = [&](double *dst, const std::vector<double> &src) {
+    if (!dst) return;
+    long k = 0;
+    for (long p = 0; p < n; ++p)
+      if (hr[p] >= 0) dst[k++] = src[p];
+  };
+  emit(r, hr);
+  emit(z, hz);
+  const double *srcs[6] = {s.vr, s.vphi, s.vz, s.w, s.r[c->cur ^ 1], s.z[c->cur ^ 1]};
+  double *dsts[6] = {vr, vphi, vz, w, r_old, z_old};
+  for (int a = 0; a < 6; ++a) {
+    if (!dsts[a]) continue;
+    HIP_TRY(hipMemcpy(tmp.data(), srcs[a], n * sizeof(double), hipMemcpyDeviceToHost));
+    emit(dsts[a], tmp);
+  }
+  if (id) {
+    HIP_TRY(hipMemcpy(hu.data(), s.id, n * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    long k = 0;
+    for (long p = 0; p < n; ++p)
+      if (hr[p] >= 0) id[k++] = hu[p];
+  }
+  if (order) {
+    HIP_TRY(hipMemcpy(hu.data(), s.seq, n * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    const Geom &g = c->g;
+    long k = 0;
+    for (long p = 0; p < n; ++p)
+      if (hr[p] >= 0) {
+        long cr = (long)std::floor(hr[p] / g.dr), cz = (long)std::floor(hz[p] / g.dz);
+        uint64_t d = (uint64_t)((cr / g.nr) * g.Dz + cz / g.nz);
+        order[k++] = (d << 40) | hu[p];
+      }
+  }
+  return 0;
+}
+
+int pcp_init_plasma(pcp_ctx *c, int sp, int n_per_cell, double density_left, double density_right,
+                    double temperature_ev, uint64_t seed) {
+  if (!c || sp < 0 || sp >= c->cfg.n_species || n_per_cell <= 0 || !(density_left >= 0) || !(density_right >= 0))
+    return fail(PCP_E_ARG, "bad argument");
+  const Geom &g = c->g;
+  double n_peak = std::max(density_left, density_right);
+  if (!(n_peak > 0)) return fail(PCP_E_ARG, "density must be positive");
+  // columns: N_col(k) = ppc * Nr * n(z_k) / n_peak, spread over the radial
+  // cells in proportion to the ring area (uniform in r^2, equal weights)
+  long ncell = (long)c->D * g.nr * g.nz;
+  std::vector<long> cnt(ncell), off(ncell + 1), doff(c->D);
+  std::vector<uint64_t> gbase(c->mig.ngroups);
+  HIP_TRY(hipMemcpy(gbase.data(), c->mig.gbase, gbase.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  long total = 0;
+  for (int d = 0; d < c->D; ++d) {
+    Dom dm = make_dom(g, d);
+    long dstart = total;
+    for (int li = 0; li < g.nr; ++li)
+      for (int lk = 0; lk < g.nz; ++lk) {
+        long gi = dm.bot_r + li, gk = dm.left_z + lk;
+        double zc = (gk + 0.5) * g.dz;
+        double dens = density_left + (density_right - density_left) * zc / g.gsize_z;
+        double ncol = std::floor((double)n_per_cell * g.Nr * dens / n_peak + 0.5);
+        double a0 = std::floor(ncol * (double)gi * gi / ((double)g.Nr * g.Nr));
+        double a1 = std::floor(ncol * (double)(gi + 1) * (gi + 1) / ((double)g.Nr * g.Nr));
+        long ci = (long)(li * g.nz + lk) + (long)d * g.nr * g.nz;
+        cnt[ci] = (long)(a1 - a0);
+        total += cnt[ci];
+      }
+    doff[d] = dstart - (long)gbase[(size_t)d * MAX_SPECIES + sp];
+    gbase[(size_t)d * MAX_SPECIES + sp] += total - dstart;
+  }
+  int rc = ensure_capacity(c, sp, total);
+  if (rc) return rc;
+  off[0] = 0;
+  for (long i = 0; i < ncell; ++i) off[i + 1] = off[i] + cnt[i];
+  double weight = n_peak * PI * g.gsize_r * g.gsize_r * g.gsize_z / ((double)n_per_cell * g.Nr * g.Nz);
+  double m = c->sp[sp].m;
+  double vth = std::sqrt(temperature_ev * EL_CHARGE / m);
+  long *d_off = nullptr, *d_doff = nullptr;
+  HIP_TRY(hipMalloc(&d_off, sizeof(long) * (ncell + 1)));
+  HIP_TRY(hipMalloc(&d_doff, sizeof(long) * c->D));
+  HIP_TRY(hipMemcpy(d_off, off.data(), sizeof(long) * (ncell + 1), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d_doff, doff.data(), sizeof(long) * c->D, hipMemcpyHostToDevice));
+  launch_init_plasma(c, sp, c->sp[sp].n, total, d_off, ncell, weight, vth, vth, vth, seed, d_doff);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  hipFree(d_off);
+  hipFree(d_doff);
+  HIP_TRY(hipMemcpy(c->mig.gbase, gbase.data(), gbase.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+  // the kernel wrote both position buffers; keep the current one current
+  c->sp[sp].n += total;
+  return 0;
+}
+
+int pcp_init_bunch(pcp_ctx *c, int sp, long n, double n_peak, double sigma_r, double sigma_z, double z0, double gamma,
+                   uint64_t seed) {
+  if (!c || sp < 0 || sp >= c->cfg.n_species || n <= 0 || !(gamma >= 1) || !(sigma_r > 0) || !(sigma_z > 0))
+    return fail(PCP_E_ARG, "bad argument");
+  int rc = ensure_capacity(c, sp, n);
+  if (rc) return rc;
+  // real particles of a Gaussian bunch: n_peak (2 pi)^(3/2) sigma_r^2 sigma_z
+  double real = n_peak * std::pow(2 * PI, 1.5) * sigma_r * sigma_r * sigma_z;
+  double weight = real / n;
+  double vz = LIGHT_VEL * std::sqrt(1.0 - 1.0 / (gamma * gamma));
+  launch_init_bunch(c, sp, c->sp[sp].n, n, sigma_r, sigma_z, z0, vz, weight, seed);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  // order keys of the bunch are gbase + index: advance every domain's base
+  std::vector<uint64_t> gbase(c->mig.ngroups);
+  HIP_TRY(hipMemcpy(gbase.data(), c->mig.gbase, gbase.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  for (int d = 0; d < c->D; ++d) gbase[(size_t)d * MAX_SPECIES + sp] += (uint64_t)n;
+  HIP_TRY(hipMemcpy(c->mig.gbase, gbase.data(), gbase.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+  c->sp[sp].n += n;
+  return 0;
+}
+
+int pcp_download_grid(pcp_ctx *c, int which, double *host) {
+  if (!c || !host) return fail(PCP_E_ARG, "bad argument");
+  int nc;
+  double *p = grid_ptr(c, which, &nc);
+  if (!p) return fail(PCP_E_ARG, "unknown grid");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(host, p, sizeof(double) * (size_t)c->D * nc * c->g.P, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pcp_upload_grid(pcp_ctx *c, int which, const double *host) {
+  if (!c || !host) return fail(PCP_E_ARG, "bad argument");
+  int nc;
+  double *p = grid_ptr(c, which, &nc);
+  if (!p) return fail(PCP_E_ARG, "unknown grid");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(p, host, sizeof(double) * (size_t)c->D * nc * c->g.P, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int pcp_yee_e(pcp_ctx *c) {
+  if (!c) return fail(PCP_E_ARG, "null context");
+  launch_yee_e(c, 0);
+  HIP_TRY(hipGetLastError());
+  return pcp_sync(c);
+}
+
+int pcp_yee_h(pcp_ctx *c) {
+  if (!c) return fail(PCP_E_ARG, "null context");
+  launch_yee_h(c);
+  HIP_TRY(hipGetLastError());
+  return pcp_sync(c);
+}
+
+int pcp_stitch(pcp_ctx *c, int which) {
+  if (!c) return fail(PCP_E_ARG, "null context");
+  if (which != PCP_GRID_E && which != PCP_GRID_H && which != PCP_GRID_H_AT_ET && which != PCP_GRID_J)
+    return fail(PCP_E_ARG, "stitch: which must be E, H or J");
+  launch_stitch(c, which);
+  HIP_TRY(hipGetLastError());
+  return pcp_sync(c);
+}
+
+int pcp_push(pcp_ctx *c) {
+  if (!c) return fail(PCP_E_ARG, "null context");
+  for (int s = 0; s < c->cfg.n_species; ++s) launch_push(c, s);
+  HIP_TRY(hipGetLastError());
+  return pcp_sync(c);
+}
+
+int pcp_move(pcp_ctx *c) {
+  if (!c) return fail(PCP_E_ARG, "null context");
+  for (int s = 0; s < c->cfg.n_species; ++s) launch_move(c, s);
+  HIP_TRY(hipGetLastError());
+  c->cur ^= 1;
+  return pcp_sync(c);
+}
+
+int pcp_migrate(pcp_ctx *c) {
+  if (!c) return fail(PCP_E_ARG, "null context");
+  launch_step_begin(c);   // resets the migrant counter (and advances the stamp)
+  for (int s = 0; s < c->cfg.n_species; ++s) launch_migrate_mark(c, s);
+  launch_migrant_keys(c);
+  HIP_TRY(hipGetLastError());
+  return pcp_sync(c);
+}
+
+int pcp_deposit(pcp_ctx *c) {
+  if (!c) return fail(PCP_E_ARG, "null context");
+  launch_stamp(c);
+  launch_fill(c, c->f.J, (long)c->D * 3 * c->g.P, 0.0);   // Domain::reset_current
+  for (int s = 0; s < c->cfg.n_species; ++s) launch_mark(c, s, 0);
+  for (int s = 0; s < c->cfg.n_species; ++s) launch_deposit_exact(c, s, 0);
+  HIP_TRY(hipGetLastError());
+  return pcp_sync(c);
+}
+
+int pcp_step(pcp_ctx *c, int n_steps) {
+  if (!c || n_steps < 0) return fail(PCP_E_ARG, "bad argument");
+  // optional per-phase device timing: events on the context stream
+  static const char *names[4] = {"fields", "particles", "fixup", "stitch_j"};
+  std::vector<hipEvent_t> ev;
+  if (c->timing) {
+    ev.resize((size_t)n_steps * 5);
+    for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+  }
+  auto mark = [&](int t, int k) {
+    if (c->timing) (void)hipEventRecord(ev[(size_t)t * 5 + k], c->stream);
+  };
+  for (int t = 0; t < n_steps; ++t) {
+    mark(t, 0);
+    launch_step_begin(c);
+    launch_yee_e(c, 1);
+    launch_stitch(c, PCP_GRID_E);
+    launch_yee_h(c);
+    launch_stitch(c, PCP_GRID_H);
+    mark(t, 1);
+    for (int s = 0; s < c->cfg.n_species; ++s) launch_step_fused(c, s);
+    c->cur ^= 1;
+    mark(t, 2);
+    launch_migrant_keys(c);
+    // exact re-deposition when a strict axial motion reset J_r nodes
+    if (c->g.ccs == PCP_CCS_VB) {
+      launch_zero_j_if_exact(c);
+      for (int s = 0; s < c->cfg.n_species; ++s) launch_mark(c, s, 1);
+      for (int s = 0; s < c->cfg.n_species; ++s) launch_deposit_exact(c, s, 1);
+    }
+    mark(t, 3);
+    launch_stitch(c, PCP_GRID_J);
+    mark(t, 4);
+  }
+  HIP_TRY(hipGetLastError());
+  int rc = pcp_sync(c);
+  if (c->timing && n_steps > 0) {
+    double acc[4] = {0, 0, 0, 0};
+    for (int t = 0; t < n_steps; ++t)
+      for (int k = 0; k < 4; ++k) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, ev[(size_t)t * 5 + k], ev[(size_t)t * 5 + k + 1]);
+        acc[k] += ms;
+      }
+    c->ktimes.clear();
+    double tot = 0;
+    for (int k = 0; k < 4; ++k) {
+      c->ktimes.emplace_back(names[k], acc[k] / n_steps);
+      tot += acc[k];
+    }
+    c->ktimes.emplace_back("step", tot / n_steps);
+    for (auto &e : ev) (void)hipEventDestroy(e);
+  }
+  return rc;
+}
+
+int pcp_get_field_e(pcp_ctx *c, int d, double r, double z, double *out) {
+  if (!c || !out || d < 0 || d >= c->D) return fail(PCP_E_ARG, "bad argument");
+  double *dev;
+  HIP_TRY(hipMalloc(&dev, 3 * sizeof(double)));
+  launch_gather_point(c, d, r, z, 0, dev);
+  HIP_TRY(hipMemcpyAsync(out, dev, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  hipFree(dev);
+  return 0;
+}
+
+int pcp_get_field_h(pcp_ctx *c, int d, double r, double z, double *out) {
+  if (!c || !out || d < 0 || d >= c->D) return fail(PCP_E_ARG, "bad argument");
+  double *dev;
+  HIP_TRY(hipMalloc(&dev, 3 * sizeof(double)));
+  launch_gather_point(c, d, r, z, 1, dev);
+  HIP_TRY(hipMemcpyAsync(out, dev, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  hipFree(dev);
+  return 0;
+}
+
+int pcp_set_timing(pcp_ctx *c, int on) {
+  if (!c) return fail(PCP_E_ARG, "null context");
+  c->timing = on;
+  return 0;
+}
+
+double pcp_kernel_ms(pcp_ctx *c, const char *name) {
+  if (!c || !name) return -1;
+  for (auto &kv : c->ktimes)
+    if (kv.first == name) return kv.second;
+  return -1;
+}
+
+}  // extern "C"

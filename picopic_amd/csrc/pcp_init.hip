@@ -1,0 +1,150 @@
+// Seeded synthetic inputs generated on the device.
+//
+// Random numbers come from Philox-4x32-10 (Salmon, Moraes, Dror, Shaw,
+// "Parallel random numbers: as easy as 1, 2, 3", SC11), keyed by the seed and
+// counted by (particle index, draw), so every particle's draws are
+// independent of the launch shape.
+#include <hip/hip_runtime.h>
+
+#include "pcp_init.hpp"
+#include "pcp_physics.hpp"
+
+namespace pcp {
+
+__device__ __forceinline__ void philox_round(uint32_t c[4], uint32_t k[2]) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  uint64_t p0 = (uint64_t)M0 * c[0];
+  uint64_t p1 = (uint64_t)M1 * c[2];
+  uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+  uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+  uint32_t n0 = hi1 ^ c[1] ^ k[0];
+  uint32_t n2 = hi0 ^ c[3] ^ k[1];
+  c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+}
+
+__device__ __forceinline__ void philox4x32(uint64_t ctr, uint32_t draw, uint64_t seed, uint32_t out[4]) {
+  uint32_t c[4] = {(uint32_t)ctr, (uint32_t)(ctr >> 32), draw, 0x5eedu};
+  uint32_t k[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  for (int r = 0; r < 10; ++r) {
+    philox_round(c, k);
+    k[0] += 0x9E3779B9u;
+    k[1] += 0xBB67AE85u;
+  }
+  out[0] = c[0]; out[1] = c[1]; out[2] = c[2]; out[3] = c[3];
+}
+
+// two doubles in [0, 1) from one Philox block
+__device__ __forceinline__ void uniform2(uint64_t ctr, uint32_t draw, uint64_t seed, double &u0, double &u1) {
+  uint32_t o[4];
+  philox4x32(ctr, draw, seed, o);
+  const double s = 1.0 / 9007199254740992.0;   // 2^-53
+  u0 = (double)(((uint64_t)(o[0] >> 5) << 26) | (o[1] >> 6)) * s;
+  u1 = (double)(((uint64_t)(o[2] >> 5) << 26) | (o[3] >> 6)) * s;
+}
+
+// Box-Muller on (0,1] x [0,1)
+__device__ __forceinline__ void normal2(uint64_t ctr, uint32_t draw, uint64_t seed, double &n0, double &n1) {
+  double u0, u1;
+  uniform2(ctr, draw, seed, u0, u1);
+  double rad = sqrt(-2.0 * log(1.0 - u0));
+  n0 = rad * cos(2.0 * PI * u1);
+  n1 = rad * sin(2.0 * PI * u1);
+}
+
+// One thread per particle; cell_of[p] found by binary search over the
+// per-cell exclusive offsets (cells in domain-major order).
+__global__ void k_init_plasma(Geom g, SpeciesDev s, long base, long n_new, const long *cell_off, long n_cells,
+                              double weight, double vth_r, double vth_phi, double vth_z, uint64_t seed,
+                              const long *dom_off) {
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n_new; p += (long)gridDim.x * blockDim.x) {
+    long lo = 0, hi = n_cells - 1;
+    while (lo < hi) {   // last cell with cell_off[c] <= p
+      long mid = (lo + hi + 1) >> 1;
+      if (cell_off[mid] <= p) lo = mid; else hi = mid - 1;
+    }
+    long c = lo;
+    long cells_per_dom = (long)g.nr * g.nz;
+    int d = (int)(c / cells_per_dom);
+    long lc = c - (long)d * cells_per_dom;
+    int li = (int)(lc / g.nz), lk = (int)(lc % g.nz);
+    Dom dm = make_dom(g, d);
+    long gi = dm.bot_r + li, gk = dm.left_z + lk;
+    double u0, u1, n0, n1, n2, n3;
+    uniform2((uint64_t)(base + p), 0, seed, u0, u1);
+    normal2((uint64_t)(base + p), 1, seed, n0, n1);
+    normal2((uint64_t)(base + p), 2, seed, n2, n3);
+    double r0 = gi * g.dr, r1 = (gi + 1) * g.dr;
+    double r = sqrt(r0 * r0 + u0 * (r1 * r1 - r0 * r0));
+    double z = (gk + u1) * g.dz;
+    // keep the particle inside its cell
+    if (r < r0) r = r0;
+    if (r >= r1) r = nextafter(r1, 0.0);
+    if (z >= (gk + 1) * g.dz) z = nextafter((gk + 1) * g.dz, 0.0);
+    long q = base + p;
+    s.r[0][q] = r; s.r[1][q] = r;
+    s.z[0][q] = z; s.z[1][q] = z;
+    s.vr[q] = n0 * vth_r;
+    s.vphi[q] = n1 * vth_phi;
+    s.vz[q] = n2 * vth_z;
+    s.w[q] = weight;
+    s.seq[q] = (uint64_t)(p - dom_off[d]);
+    s.id[q] = (uint64_t)q;
+  }
+}
+
+__global__ void k_init_bunch(Geom g, SpeciesDev s, long base, long n_new, double sigma_r, double sigma_z, double z0,
+                             double vz, double weight, uint64_t seed, const uint64_t *gbase, int sp) {
+  double R = g.gsize_r, L = g.gsize_z;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n_new; p += (long)gridDim.x * blockDim.x) {
+    double n0, n1, n2, n3;
+    normal2((uint64_t)(base + p), 7, seed, n0, n1);
+    normal2((uint64_t)(base + p), 8, seed, n2, n3);
+    double x = n0 * sigma_r, y = n1 * sigma_r;
+    double r = sqrt(x * x + y * y);
+    double z = z0 + n2 * sigma_z;
+    double rmin = 0.5 * g.dr, rmax = R - 0.5 * g.dr, zmin = 0.5 * g.dz, zmax = L - 0.5 * g.dz;
+    if (r < rmin) r = rmin;
+    if (r > rmax) r = rmax;
+    if (z < zmin) z = zmin;
+    if (z > zmax) z = zmax;
+    long q = base + p;
+    s.r[0][q] = r; s.r[1][q] = r;
+    s.z[0][q] = z; s.z[1][q] = z;
+    s.vr[q] = 0;
+    s.vphi[q] = 0;
+    s.vz[q] = vz;
+    s.w[q] = weight;
+    int d = (int)((long)floor(r / g.dr) / g.nr) * g.Dz + (int)((long)floor(z / g.dz) / g.nz);
+    s.seq[q] = gbase[(long)d * MAX_SPECIES + sp] + (uint64_t)p;
+    s.id[q] = (uint64_t)q;
+  }
+}
+
+__global__ void k_count_live(const double *r, long n, unsigned long long *out) {
+  unsigned long long c = 0;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long)gridDim.x * blockDim.x) c += r[p] >= 0;
+  atomicAdd(out, c);
+}
+
+void launch_init_plasma(pcp_ctx *c, int sp, long base, long n_new, const long *dev_cell_off, long n_cells,
+                        double weight, double vth_r, double vth_phi, double vth_z, uint64_t seed,
+                        const long *dev_dom_off) {
+  unsigned grid = (unsigned)std::min<long>((n_new + 255) / 256, (long)c->n_cu * 32);
+  hipLaunchKernelGGL(k_init_plasma, dim3(grid ? grid : 1), dim3(256), 0, c->stream, c->g, c->sp[sp], base, n_new,
+                     dev_cell_off, n_cells, weight, vth_r, vth_phi, vth_z, seed, dev_dom_off);
+}
+
+void launch_init_bunch(pcp_ctx *c, int sp, long base, long n_new, double sigma_r, double sigma_z, double z0,
+                       double vz, double weight, uint64_t seed) {
+  unsigned grid = (unsigned)std::min<long>((n_new + 255) / 256, (long)c->n_cu * 32);
+  hipLaunchKernelGGL(k_init_bunch, dim3(grid ? grid : 1), dim3(256), 0, c->stream, c->g, c->sp[sp], base, n_new,
+                     sigma_r, sigma_z, z0, vz, weight, seed, c->mig.gbase, sp);
+}
+
+void launch_count_live(pcp_ctx *c, int sp, unsigned long long *dev_out) {
+  SpeciesDev &s = c->sp[sp];
+  unsigned grid = (unsigned)std::min<long>((s.n + 255) / 256, (long)c->n_cu * 8);
+  hipLaunchKernelGGL(k_count_live, dim3(grid ? grid : 1), dim3(256), 0, c->stream, s.r[c->cur], s.n, dev_out);
+}
+
+}  // namespace pcp

@@ -1,0 +1,138 @@
+// Internal state and device helpers of the picopic_amd HIP library.
+//
+// Device layout (HBM):
+//   field grids  : one array per kind, [domain][component][Nr_d+4][Nz_d+4],
+//                  z contiguous -- the reference Grid<double> layout with the
+//                  row pointers flattened (include/algo/grid.hpp:24-58)
+//   particles    : per species SoA. Positions are double-buffered (pos[cur]
+//                  is the current position, pos[cur^1] the position before
+//                  the last move), velocities/weights/keys single-buffered.
+//   order keys   : per particle, the position in the reference's per-domain
+//                  vector (seq); removed particles carry r < 0.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/picopic.h"
+
+namespace pcp {
+
+// reference include/constant.hpp:28-58, configure.ac:224
+constexpr double PI = 3.1415926535897932;
+constexpr double EPSILON0 = 8.85E-12;
+constexpr double EL_CHARGE = 1.6E-19;
+constexpr double EL_MASS = 9.1E-31;
+constexpr double LIGHT_VEL = 3.0E8;
+constexpr double LIGHT_VEL_POW_2 = LIGHT_VEL * LIGHT_VEL;
+constexpr double MAGN_CONST = 1.26E-6;
+constexpr double MNZL = 1e-15;
+constexpr double REL_LIMIT = 5e7;
+
+constexpr int MAX_SPECIES = 8;
+
+// device-side error bits (reported as PCP_E_STATE)
+enum : int {
+  ERR_NAN_POSITION = 1,   // pusherBoris.cpp:51-57
+  ERR_BETA_GT_1 = 2,      // rel.cpp:32-34
+  ERR_NAN_FIELD = 4,      // maxwellSolverYee.cpp:253-255
+  ERR_MIGRANT_OVERFLOW = 8,
+  ERR_MOVED_TOO_FAR = 16, // currentVB.cpp:199-204 (LOG_S(ERROR), not fatal)
+};
+
+// Geometry shared by every domain of the SMB (domains are equal).
+struct Geom {
+  int Dr, Dz;        // domains
+  int nr, nz;        // cells per domain
+  int xr, yr;        // nr + 4, nz + 4
+  long P;            // xr * yr
+  long Nr, Nz;       // global cells
+  double dr, dz;     // domain cell size (size / domains / cells)
+  double dt;
+  double size_r, size_z;             // domain size
+  double gsize_r, gsize_z;           // global size
+  long gca_r, gca_z;                 // domain Geometry::global_cell_amount
+  int pml_on;
+  int ccs, pusher;
+};
+
+// Per-domain values derived from the domain index (src/SMB.cpp:49-121).
+struct Dom {
+  int I, J;          // domain indices (r, z)
+  long bot_r, left_z;
+  int w0, w1, w2, w3; // walls r=0, z=0, r=R, z=L
+};
+
+__host__ __device__ inline Dom make_dom(const Geom &g, int d) {
+  Dom o;
+  o.I = d / g.Dz;
+  o.J = d % g.Dz;
+  o.bot_r = (long)g.nr * o.I;
+  o.left_z = (long)g.nz * o.J;
+  o.w0 = o.I == 0;
+  o.w1 = o.J == 0;
+  o.w2 = o.I == g.Dr - 1;
+  o.w3 = o.J == g.Dz - 1;
+  return o;
+}
+
+// halo-grid index of local (i, k) (interior-relative, may be -2..n+1)
+__host__ __device__ inline long gidx(const Geom &g, int d, int comp, int ncomp, int i, int k) {
+  return ((long)d * ncomp + comp) * g.P + (long)(i + 2) * g.yr + (k + 2);
+}
+
+struct Fields {
+  double *E, *H, *Hat, *J;   // [D][3][P]
+  double *eps, *sigma;       // [D][P]
+};
+
+struct SpeciesDev {
+  double *r[2], *z[2];
+  double *vr, *vphi, *vz, *w;
+  uint64_t *seq;     // order key within (domain, species)
+  uint64_t *id;
+  long n;            // slots in use (live + removed)
+  long cap;
+  double q, m;
+  int beam;
+};
+
+// one step's bookkeeping kept in device memory
+struct StepState {
+  unsigned long long step;     // step counter (stamps the strict-motion markers)
+  int err;
+  int need_exact;              // a strict-axis J_r reset happened in this step
+  unsigned int n_mig;          // migrants appended in this step
+};
+
+struct Migrants {
+  uint64_t *key;      // colour << 47 | srcdom << 35 | old seq
+  uint64_t *ref;      // species << 40 | particle index
+  uint32_t *group;    // dst domain * MAX_SPECIES + species
+  uint32_t *bucket;   // migrant indices grouped by group
+  uint32_t *gcount, *gstart, *gfill;
+  uint64_t *gbase;    // next free seq per (domain, species)
+  long cap;
+  int ngroups;
+};
+
+}  // namespace pcp
+
+struct pcp_ctx {
+  pcp_config cfg;
+  pcp::Geom g;
+  int D;
+  hipStream_t stream;
+  pcp::Fields f;
+  pcp::SpeciesDev sp[pcp::MAX_SPECIES];
+  int cur;                         // current position buffer
+  pcp::StepState *st;              // device
+  pcp::Migrants mig;
+  uint64_t *marker;                // [D][P] strict-motion J_r reset markers
+  int n_cu;
+  int timing;
+  std::vector<std::pair<std::string, double>> ktimes;
+};

@@ -1,0 +1,604 @@
+// Kernels of the FP64 per-step particle loop on gfx950.
+//
+//  k_yee_e / k_yee_h    Yee stencils, one thread per node of the halo-extended
+//                       grid of every domain (coalesced along z)
+//  k_stitch_copy        E/H overlay: halo <- neighbour interior
+//  k_stitch_sum_j       J overlay: interior += neighbour halo contributions
+//  k_step_fused         push + move + VB/ZigZag deposit, one thread per particle
+//  k_push, k_move, k_migrate, k_mark, k_deposit_exact
+//                       the same work split into the reference's phases; the
+//                       exact deposit honours the order-dependent J_r reset of
+//                       strict axial motion through per-node markers
+//  k_mig_*              order keys of particles that changed domain
+#include <hip/hip_runtime.h>
+
+#include "pcp_kernels.hpp"
+#include "pcp_physics.hpp"
+
+namespace pcp {
+
+// ---------------------------------------------------------------------------
+// sinks for the deposition templates
+// ---------------------------------------------------------------------------
+struct AtomicSink {
+  double *J;   // component 0 of the destination domain
+  long P;
+  int yr, nr, nz;
+  int *err;
+  int *need_exact;   // set when a strict-axis J_r reset happens (fused mode)
+  __device__ __forceinline__ bool inb(int i, int k) const { return i >= -2 && i < nr + 2 && k >= -2 && k < nz + 2; }
+  __device__ __forceinline__ void add(int c, int i, int k, double v) {
+    if (!inb(i, k)) { *err |= ERR_MOVED_TOO_FAR; return; }
+    unsafeAtomicAdd(&J[c * P + (long)(i + 2) * yr + (k + 2)], v);
+  }
+  __device__ __forceinline__ void strict_reset(int, int) {
+    if (need_exact) *need_exact = 1;
+  }
+};
+
+struct ExactSink {
+  double *J;
+  const uint64_t *marker;   // J_r reset markers of the destination domain
+  long P;
+  int yr, nr, nz;
+  uint64_t stamp, key;      // this step's stamp, this particle's order key
+  int *err;
+  __device__ __forceinline__ bool inb(int i, int k) const { return i >= -2 && i < nr + 2 && k >= -2 && k < nz + 2; }
+  __device__ __forceinline__ void add(int c, int i, int k, double v) {
+    if (!inb(i, k)) { *err |= ERR_MOVED_TOO_FAR; return; }
+    long o = (long)(i + 2) * yr + (k + 2);
+    if (c == 0) {
+      uint64_t m = marker[o];
+      // a particle later in the reference order reset this node: drop
+      if ((m >> 40) == stamp && (m & ((1ull << 40) - 1)) > key) return;
+    }
+    unsafeAtomicAdd(&J[c * P + o], v);
+  }
+  __device__ __forceinline__ void strict_reset(int, int) {}
+};
+
+struct MarkSink {
+  uint64_t *marker;
+  int yr, nr, nz;
+  uint64_t val;   // stamp << 40 | key
+  __device__ __forceinline__ void add(int, int, int, double) {}
+  __device__ __forceinline__ void strict_reset(int i, int k) {
+    if (i < -2 || i >= nr + 2 || k < -2 || k >= nz + 2) return;
+    atomicMax((unsigned long long *)&marker[(long)(i + 2) * yr + (k + 2)], (unsigned long long)val);
+  }
+};
+
+// domain owning global cell (cr, cz)
+__device__ __forceinline__ int domain_of(const Geom &g, int cr, int cz) { return (cr / g.nr) * g.Dz + (cz / g.nz); }
+
+__device__ __forceinline__ uint64_t order_key(int sp, uint64_t seq) { return ((uint64_t)sp << 36) | seq; }
+
+// ---------------------------------------------------------------------------
+// Yee E: maxwellSolverYee.cpp:161-257. Halo nodes are zeroed
+// (field_e.overlay_set(0)); with clear_j the J node is zeroed after it is
+// read (Domain::reset_current folded into the only reader of J).
+// ---------------------------------------------------------------------------
+__global__ void k_yee_e(Geom g, Fields f, int clear_j, StepState *st) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long total = (long)g.Dr * g.Dz * g.P;
+  if (t >= total) return;
+  int d = (int)(t / g.P);
+  long o = t - (long)d * g.P;
+  int i = (int)(o / g.yr) - 2, k = (int)(o % g.yr) - 2;
+  double *E = f.E + (long)d * 3 * g.P;
+  double *J = f.J + (long)d * 3 * g.P;
+  const double *Hat = f.Hat + (long)d * 3 * g.P;
+  const long P = g.P;
+  const int yr = g.yr;
+  if (i < 0 || k < 0 || i >= g.nr || k >= g.nz) {
+    E[o] = 0; E[P + o] = 0; E[2 * P + o] = 0;
+    if (clear_j) { J[o] = 0; J[P + o] = 0; J[2 * P + o] = 0; }
+    return;
+  }
+  Dom dm = make_dom(g, d);
+  const int r_begin = dm.w0 ? 1 : 0, z_begin = dm.w1 ? 1 : 0;
+  const int r_end = dm.w2 ? g.nr - 1 : g.nr, z_end = dm.w3 ? g.nz - 1 : g.nz;
+  const double dr = g.dr, dz = g.dz, dt = g.dt;
+  double epsilonx2 = 2 * f.eps[(long)d * P + o];
+  double sigma_t = g.pml_on ? f.sigma[(long)d * P + o] * dt : 0;
+  double koef_e = (epsilonx2 - sigma_t) / (epsilonx2 + sigma_t);
+  double koef_h = 2 * dt / (epsilonx2 + sigma_t);
+  const long om = o - 1;    // k - 1
+  const long oi = o - yr;   // i - 1
+  double jr = J[o], jp = J[P + o], jz = J[2 * P + o];
+  if (i == 0 && dm.w0) {
+    if (k >= z_begin && k < z_end) {
+      double er = E[o] * koef_e;
+      er -= (jr + (Hat[P + o] - Hat[P + om]) / dz) * koef_h;
+      E[o] = er;
+      double ez = E[2 * P + o] * koef_e;
+      ez -= (jz - Hat[P + o] * 4. / dr) * koef_h;
+      E[2 * P + o] = ez;
+    }
+  } else if (k == 0 && dm.w1) {
+    if (i >= r_begin && i < r_end) {
+      double ez = E[2 * P + o] * koef_e;
+      ez -= (jz - (Hat[P + o] - Hat[P + oi]) / dr
+             - (Hat[P + o] + Hat[P + oi]) / (2. * dr * (double)(i + dm.bot_r))) * koef_h;
+      E[2 * P + o] = ez;
+    }
+  } else if (i >= r_begin && i < r_end && k >= z_begin && k < z_end) {
+    double er = E[o] * koef_e;
+    er -= (jr + (Hat[P + o] - Hat[P + om]) / dz) * koef_h;
+    double ep = E[P + o] * koef_e;
+    ep -= (jp - (Hat[o] - Hat[om]) / dz + (Hat[2 * P + o] - Hat[2 * P + oi]) / dr) * koef_h;
+    double ez = E[2 * P + o] * koef_e;
+    ez -= (jz - (Hat[P + o] - Hat[P + oi]) / dr
+           - (Hat[P + o] + Hat[P + oi]) / (2. * dr * (double)(i + dm.bot_r))) * koef_h;
+    E[o] = er; E[P + o] = ep; E[2 * P + o] = ez;
+    if (isnan(er) || isnan(ep) || isnan(ez)) atomicOr(&st->err, ERR_NAN_FIELD);
+  }
+  if (clear_j) { J[o] = 0; J[P + o] = 0; J[2 * P + o] = 0; }
+}
+
+// Yee H: maxwellSolverYee.cpp:111-159 (the outer-wall H_r block runs before
+// the regular loop, so H_r at i = Nr-1 is advanced twice, quirk Q12)
+__global__ void k_yee_h(Geom g, Fields f) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long total = (long)g.Dr * g.Dz * g.P;
+  if (t >= total) return;
+  int d = (int)(t / g.P);
+  long o = t - (long)d * g.P;
+  int i = (int)(o / g.yr) - 2, k = (int)(o % g.yr) - 2;
+  double *H = f.H + (long)d * 3 * g.P;
+  double *Hat = f.Hat + (long)d * 3 * g.P;
+  const double *E = f.E + (long)d * 3 * g.P;
+  const long P = g.P;
+  const int yr = g.yr;
+  if (i < 0 || k < 0 || i >= g.nr || k >= g.nz) {
+    H[o] = 0; H[P + o] = 0; H[2 * P + o] = 0;
+    Hat[o] = 0; Hat[P + o] = 0; Hat[2 * P + o] = 0;
+    return;
+  }
+  Dom dm = make_dom(g, d);
+  const double dr = g.dr, dz = g.dz, dt = g.dt;
+  const long op = o + 1;    // k + 1
+  const long oi = o + yr;   // i + 1
+  double hat_r = Hat[o];
+  if (dm.w2 && i == g.nr - 1) {
+    double alpha_t = dt * (E[P + op] - E[P + o]) / (dz * MAGN_CONST);
+    hat_r += alpha_t;
+  }
+  double alpha_t_r = dt * (E[P + op] - E[P + o]) / (dz * MAGN_CONST);
+  H[o] = hat_r + alpha_t_r / 2;
+  Hat[o] = hat_r + alpha_t_r;
+  double alpha_t_phi = dt * ((E[2 * P + oi] - E[2 * P + o]) / dr - (E[op] - E[o]) / dz) / MAGN_CONST;
+  double hp = Hat[P + o];
+  H[P + o] = hp + alpha_t_phi / 2;
+  Hat[P + o] = hp + alpha_t_phi;
+  double alpha_t_z = dt * ((E[P + oi] + E[P + o]) / (2. * dr * (i + 0.5 + (double)dm.bot_r))
+                           + (E[P + oi] - E[P + o]) / dr) / MAGN_CONST;
+  double hz = Hat[2 * P + o];
+  H[2 * P + o] = hz - alpha_t_z / 2;
+  Hat[2 * P + o] = hz - alpha_t_z;
+}
+
+// E/H overlay (SMB.cpp:784-823, 1010-1044): halos hold zeros before the
+// stitch, so every stitched halo node is a copy of the neighbour's interior
+// node; the anti-diagonal corners are never stitched (quirk Q14).
+__global__ void k_stitch_copy(Geom g, double *G, int ncomp) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long total = (long)g.Dr * g.Dz * g.P;
+  if (t >= total) return;
+  int d = (int)(t / g.P);
+  long o = t - (long)d * g.P;
+  int i = (int)(o / g.yr) - 2, k = (int)(o % g.yr) - 2;
+  int di = i < 0 ? -1 : (i >= g.nr ? 1 : 0);
+  int dk = k < 0 ? -1 : (k >= g.nz ? 1 : 0);
+  if ((di == 0 && dk == 0) || di * dk == -1) return;
+  int I = d / g.Dz + di, Jd = d % g.Dz + dk;
+  if (I < 0 || I >= g.Dr || Jd < 0 || Jd >= g.Dz) return;
+  int dn = I * g.Dz + Jd;
+  int in = i - di * g.nr, kn = k - dk * g.nz;
+  long on = (long)(in + 2) * g.yr + (kn + 2);
+  for (int c = 0; c < ncomp; ++c) G[((long)d * ncomp + c) * g.P + o] = G[((long)dn * ncomp + c) * g.P + on];
+}
+
+// J overlay (SMB.cpp:566-596): every interior node within two cells of a
+// domain edge gains the halo deposits of the r, z and main-diagonal
+// neighbours; anti-diagonal deposits are lost (quirk Q14).
+__global__ void k_stitch_sum_j(Geom g, double *J) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long total = (long)g.Dr * g.Dz * g.nr * g.nz;
+  if (t >= total) return;
+  long cells = (long)g.nr * g.nz;
+  int d = (int)(t / cells);
+  long c = t - (long)d * cells;
+  int i = (int)(c / g.nz), k = (int)(c % g.nz);
+  int lo_i = i < 2, hi_i = i >= g.nr - 2, lo_k = k < 2, hi_k = k >= g.nz - 2;
+  if (!(lo_i || hi_i || lo_k || hi_k)) return;
+  int I0 = d / g.Dz, J0 = d % g.Dz;
+  long o = (long)(i + 2) * g.yr + (k + 2);
+  double acc[3] = {0, 0, 0};
+  bool any = false;
+  for (int di = -1; di <= 1; ++di) {
+    if (di == -1 && !lo_i) continue;
+    if (di == 1 && !hi_i) continue;
+    for (int dk = -1; dk <= 1; ++dk) {
+      if (di == 0 && dk == 0) continue;
+      if (di * dk == -1) continue;
+      if (dk == -1 && !lo_k) continue;
+      if (dk == 1 && !hi_k) continue;
+      int I = I0 + di, Jd = J0 + dk;
+      if (I < 0 || I >= g.Dr || Jd < 0 || Jd >= g.Dz) continue;
+      int dn = I * g.Dz + Jd;
+      int in = i - di * g.nr, kn = k - dk * g.nz;
+      long on = (long)(in + 2) * g.yr + (kn + 2);
+      for (int cc = 0; cc < 3; ++cc) acc[cc] += J[((long)dn * 3 + cc) * g.P + on];
+      any = true;
+    }
+  }
+  if (any)
+    for (int cc = 0; cc < 3; ++cc) J[((long)d * 3 + cc) * g.P + o] += acc[cc];
+}
+
+__global__ void k_fill(double *p, long n, double v) {
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) p[t] = v;
+}
+
+__global__ void k_fill_if_exact(double *p, long n, const StepState *st) {
+  if (!st->need_exact) return;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) p[t] = 0;
+}
+
+__global__ void k_step_begin(StepState *st) {
+  st->step += 1;
+  st->need_exact = 0;
+  st->n_mig = 0;
+}
+
+// ---------------------------------------------------------------------------
+// particle kernels
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ GridView view(const double *G, const Geom &g, int d) {
+  GridView v;
+  v.G = G + (long)d * 3 * g.P;
+  v.P = g.P;
+  v.yr = g.yr;
+  return v;
+}
+
+// gather + push for one particle (Pusher::operator() body)
+__device__ __forceinline__ void push_one(const Geom &g, const Fields &f, double q, double m, double r, double z,
+                                         double v[3], int *err) {
+  int d = domain_of(g, cell_number(r, g.dr), cell_number(z, g.dz));
+  Dom dm = make_dom(g, d);
+  double e[3], h[3];
+  gather_e(view(f.E, g, d), g.dr, g.dz, dm.bot_r, dm.left_z, r, z, e);
+  gather_h(view(f.H, g, d), g.dr, g.dz, dm.bot_r, dm.left_z, r, z, h);
+  push_velocity(g.pusher, q, m, g.dt, e, h, v, err);
+}
+
+__device__ __forceinline__ bool bad_position(double r, double z) {
+  return isnan(r) || isinf(r) || isnan(z) || isinf(z);
+}
+
+__global__ void __launch_bounds__(256) k_push(Geom g, Fields f, SpeciesDev s, int cur, StepState *st) {
+  int err = 0;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < s.n; p += (long)gridDim.x * blockDim.x) {
+    double r = s.r[cur][p], z = s.z[cur][p];
+    if (r < 0) continue;
+    if (bad_position(r, z)) { err |= ERR_NAN_POSITION; continue; }
+    double v[3] = {s.vr[p], s.vphi[p], s.vz[p]};
+    push_one(g, f, s.q, s.m, r, z, v, &err);
+    s.vr[p] = v[0]; s.vphi[p] = v[1]; s.vz[p] = v[2];
+  }
+  if (err) atomicOr(&st->err, err);
+}
+
+// move one particle; returns false if it left the simulation area (removed
+// by the runaway collector, SMB.cpp:243-320)
+__device__ __forceinline__ bool move_one(const Geom &g, int beam, double &r, double &z, double &vr, double &vphi,
+                                         double &vz, int &d_old, int &d_new) {
+  d_old = domain_of(g, cell_number(r, g.dr), cell_number(z, g.dz));
+  Dom dm = make_dom(g, d_old);
+  double vzo;
+  move_particle(g, dm, beam, r, z, vr, vphi, vz, vzo);
+  vz = vzo;
+  if (bad_position(r, z)) return false;
+  long cr = (long)floor(r / g.dr), cz = (long)floor(z / g.dz);
+  if (cr < 0 || cz < 0 || cr >= g.Nr || cz >= g.Nz) return false;
+  d_new = domain_of(g, (int)cr, (int)cz);
+  return true;
+}
+
+__device__ __forceinline__ int colour_rank(const Geom &g, int d) {
+  int I = d / g.Dz, J = d % g.Dz;
+  return (I % 2) * 2 + (J % 2);
+}
+
+__device__ __forceinline__ void append_migrant(Migrants mg, StepState *st, int sp, long p, int d_old, int d_new,
+                                               uint64_t seq, const Geom &g, int *err) {
+  unsigned int slot = atomicAdd(&st->n_mig, 1u);
+  if (slot >= (unsigned long)mg.cap) { *err |= ERR_MIGRANT_OVERFLOW; return; }
+  mg.key[slot] = ((uint64_t)colour_rank(g, d_old) << 47) | ((uint64_t)d_old << 35) | (seq & ((1ull << 35) - 1));
+  mg.ref[slot] = ((uint64_t)sp << 40) | (uint64_t)p;
+  mg.group[slot] = (uint32_t)(d_new * MAX_SPECIES + sp);
+}
+
+// phase "move": pos[cur] -> pos[cur^1]; velocities in place.  Removal and
+// migration are left to k_migrate (the reference collects after binding).
+__global__ void __launch_bounds__(256) k_move(Geom g, SpeciesDev s, int cur) {
+  const int nxt = cur ^ 1;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < s.n; p += (long)gridDim.x * blockDim.x) {
+    double r = s.r[cur][p], z = s.z[cur][p];
+    if (r < 0 || bad_position(r, z)) { s.r[nxt][p] = -1; s.z[nxt][p] = z; continue; }
+    double vr = s.vr[p], vphi = s.vphi[p], vz = s.vz[p];
+    Dom dm = make_dom(g, domain_of(g, cell_number(r, g.dr), cell_number(z, g.dz)));
+    double vzo;
+    move_particle(g, dm, s.beam, r, z, vr, vphi, vz, vzo);
+    s.r[nxt][p] = r; s.z[nxt][p] = z;
+    s.vr[p] = vr; s.vphi[p] = vphi; s.vz[p] = vzo;
+  }
+}
+
+// phase "migrate": pos[cur] is the new position, pos[cur^1] the old one
+__global__ void k_migrate(Geom g, SpeciesDev s, int sp, int cur, StepState *st, Migrants mg) {
+  int err = 0;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < s.n; p += (long)gridDim.x * blockDim.x) {
+    double r = s.r[cur][p], z = s.z[cur][p];
+    double ro = s.r[cur ^ 1][p], zo = s.z[cur ^ 1][p];
+    if (r < 0 || ro < 0) { s.r[cur][p] = -1; continue; }
+    long cr = bad_position(r, z) ? -1 : (long)floor(r / g.dr);
+    long cz = bad_position(r, z) ? -1 : (long)floor(z / g.dz);
+    if (cr < 0 || cz < 0 || cr >= g.Nr || cz >= g.Nz) { s.r[cur][p] = -1; continue; }
+    int d_new = domain_of(g, (int)cr, (int)cz);
+    int d_old = domain_of(g, cell_number(ro, g.dr), cell_number(zo, g.dz));
+    if (d_new != d_old) append_migrant(mg, st, sp, p, d_old, d_new, s.seq[p], g, &err);
+  }
+  if (err) atomicOr(&st->err, err);
+}
+
+// fused push + move + deposit (the hot kernel)
+__global__ void __launch_bounds__(256) k_step_fused(Geom g, Fields f, SpeciesDev s, int sp, int cur, StepState *st,
+                                                    Migrants mg) {
+  int err = 0;
+  const int nxt = cur ^ 1;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < s.n; p += (long)gridDim.x * blockDim.x) {
+    double r = s.r[cur][p], z = s.z[cur][p];
+    if (r < 0) { s.r[nxt][p] = -1; s.z[nxt][p] = z; continue; }
+    if (bad_position(r, z)) { err |= ERR_NAN_POSITION; s.r[nxt][p] = -1; s.z[nxt][p] = z; continue; }
+    double v[3] = {s.vr[p], s.vphi[p], s.vz[p]};
+    push_one(g, f, s.q, s.m, r, z, v, &err);
+    const double ro = r, zo = z;
+    double vr = v[0], vphi = v[1], vz = v[2];
+    int d_old, d_new = 0;
+    bool alive = move_one(g, s.beam, r, z, vr, vphi, vz, d_old, d_new);
+    s.vr[p] = vr; s.vphi[p] = vphi; s.vz[p] = vz;
+    if (!alive) { s.r[nxt][p] = -1; s.z[nxt][p] = z; continue; }
+    s.r[nxt][p] = r; s.z[nxt][p] = z;
+    if (d_new != d_old) append_migrant(mg, st, sp, p, d_old, d_new, s.seq[p], g, &err);
+    Dom dn = make_dom(g, d_new);
+    double w = s.w[p];
+    AtomicSink S{f.J + (long)d_new * 3 * g.P, g.P, g.yr, g.nr, g.nz, &err, &st->need_exact};
+    if (g.ccs == PCP_CCS_VB) {
+      double p_charge = s.q * w;
+      vb_azimuthal(S, g.nr, g.nz, g.dr, g.dz, dn.bot_r, dn.left_z, r, z, ro, zo, vphi, p_charge);
+      vb_rz(S, g.dr, g.dz, g.dt, dn.bot_r, dn.left_z, r, z, ro, zo, p_charge, &err);
+    } else {
+      zigzag(S, g.dr, g.dz, g.dt, dn.bot_r, dn.left_z, r, z, ro, zo, vphi, s.q, w);
+    }
+  }
+  if (err) atomicOr(&st->err, err);
+}
+
+// strict-motion J_r reset markers (exact path)
+__global__ void k_mark(Geom g, SpeciesDev s, int sp, int cur, StepState *st, uint64_t *marker, int only_if_exact) {
+  if (only_if_exact && !st->need_exact) return;
+  const uint64_t stamp = st->step & 0xFFFFFF;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < s.n; p += (long)gridDim.x * blockDim.x) {
+    double r = s.r[cur][p], z = s.z[cur][p];
+    double ro = s.r[cur ^ 1][p], zo = s.z[cur ^ 1][p];
+    if (r < 0 || ro < 0) continue;
+    if (!vb_is_strict_reset(g.dr, g.dz, r, z, ro, zo)) continue;
+    int d = domain_of(g, cell_number(r, g.dr), cell_number(z, g.dz));
+    Dom dm = make_dom(g, d);
+    MarkSink S{marker + (long)d * g.P, g.yr, g.nr, g.nz, (stamp << 40) | order_key(sp, s.seq[p])};
+    int i_n = cell_number(r, g.dr) - (int)dm.bot_r, k_n = cell_number(z, g.dz) - (int)dm.left_z;
+    S.strict_reset(i_n, k_n);
+    S.strict_reset(i_n, k_n + 1);
+  }
+}
+
+// exact deposition of one species: Current::current_distribution semantics
+__global__ void __launch_bounds__(256) k_deposit_exact(Geom g, Fields f, SpeciesDev s, int sp, int cur, StepState *st,
+                                const uint64_t *marker, int only_if_exact) {
+  if (only_if_exact && !st->need_exact) return;
+  int err = 0;
+  const uint64_t stamp = st->step & 0xFFFFFF;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < s.n; p += (long)gridDim.x * blockDim.x) {
+    double r = s.r[cur][p], z = s.z[cur][p];
+    double ro = s.r[cur ^ 1][p], zo = s.z[cur ^ 1][p];
+    if (r < 0 || ro < 0) continue;
+    int d = domain_of(g, cell_number(r, g.dr), cell_number(z, g.dz));
+    Dom dm = make_dom(g, d);
+    double w = s.w[p];
+    ExactSink S{f.J + (long)d * 3 * g.P, marker + (long)d * g.P, g.P, g.yr, g.nr, g.nz, stamp,
+                order_key(sp, s.seq[p]), &err};
+    if (g.ccs == PCP_CCS_VB) {
+      double p_charge = s.q * w;
+      vb_azimuthal(S, g.nr, g.nz, g.dr, g.dz, dm.bot_r, dm.left_z, r, z, ro, zo, s.vphi[p], p_charge);
+      vb_rz(S, g.dr, g.dz, g.dt, dm.bot_r, dm.left_z, r, z, ro, zo, p_charge, &err);
+    } else {
+      zigzag(S, g.dr, g.dz, g.dt, dm.bot_r, dm.left_z, r, z, ro, zo, s.vphi[p], s.q, w);
+    }
+  }
+  if (err) atomicOr(&st->err, err);
+}
+
+__global__ void k_stamp(StepState *st) { st->step += 1; }
+
+// ---------------------------------------------------------------------------
+// migrant order keys: arrivals follow every particle already in the
+// destination, ordered by (colour pass of the source, source domain, source
+// vector position) -- the sequential push_back order of
+// particles_runaway_collector (SMB.cpp:211-340)
+// ---------------------------------------------------------------------------
+__global__ void k_mig_count(StepState *st, Migrants mg) {
+  unsigned int n = min(st->n_mig, (unsigned int)mg.cap);
+  for (unsigned int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
+    atomicAdd(&mg.gcount[mg.group[t]], 1u);
+}
+
+__global__ void k_mig_scan(StepState *st, Migrants mg) {
+  // one block: exclusive scan over groups (small), then reset fill counters
+  if (st->n_mig == 0) return;
+  __shared__ uint32_t part[1024];
+  int tid = threadIdx.x, nt = blockDim.x;
+  int per = (mg.ngroups + nt - 1) / nt;
+  int b = tid * per, e = min(mg.ngroups, b + per);
+  uint32_t s = 0;
+  for (int i = b; i < e; ++i) s += mg.gcount[i];
+  part[tid] = s;
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t acc = 0;
+    for (int i = 0; i < nt; ++i) { uint32_t v = part[i]; part[i] = acc; acc += v; }
+  }
+  __syncthreads();
+  uint32_t acc = part[tid];
+  for (int i = b; i < e; ++i) { mg.gstart[i] = acc; acc += mg.gcount[i]; mg.gfill[i] = 0; }
+}
+
+__global__ void k_mig_scatter(StepState *st, Migrants mg) {
+  unsigned int n = min(st->n_mig, (unsigned int)mg.cap);
+  for (unsigned int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    uint32_t gi = mg.group[t];
+    uint32_t pos = mg.gstart[gi] + atomicAdd(&mg.gfill[gi], 1u);
+    mg.bucket[pos] = t;
+  }
+}
+
+__global__ void k_mig_rank(StepState *st, Migrants mg, SeqTable seqs) {
+  unsigned int n = min(st->n_mig, (unsigned int)mg.cap);
+  for (unsigned int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    uint32_t gi = mg.group[t];
+    uint64_t k = mg.key[t];
+    uint32_t b = mg.gstart[gi], e = b + mg.gcount[gi];
+    uint64_t rank = 0;
+    for (uint32_t u = b; u < e; ++u) rank += mg.key[mg.bucket[u]] < k;
+    uint64_t ref = mg.ref[t];
+    int sp = (int)(ref >> 40);
+    long p = (long)(ref & ((1ull << 40) - 1));
+    seqs.p[sp][p] = mg.gbase[gi] + rank;
+  }
+}
+
+__global__ void k_mig_finish(StepState *st, Migrants mg) {
+  if (st->n_mig == 0) return;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < mg.ngroups; t += gridDim.x * blockDim.x) {
+    mg.gbase[t] += mg.gcount[t];
+    mg.gcount[t] = 0;
+  }
+}
+
+// point gathers for the host-side get_field_e/h
+__global__ void k_gather_point(Geom g, Fields f, int d, double r, double z, int which, double *out) {
+  Dom dm = make_dom(g, d);
+  double c[3];
+  if (which == 0) gather_e(view(f.E, g, d), g.dr, g.dz, dm.bot_r, dm.left_z, r, z, c);
+  else gather_h(view(f.H, g, d), g.dr, g.dz, dm.bot_r, dm.left_z, r, z, c);
+  out[0] = c[0]; out[1] = c[1]; out[2] = c[2];
+}
+
+}  // namespace pcp
+
+// ---------------------------------------------------------------------------
+// launch wrappers
+// ---------------------------------------------------------------------------
+namespace pcp {
+
+static inline unsigned grid_for(long n, int block) { return (unsigned)((n + block - 1) / block); }
+// grid-stride particle kernels: enough waves to fill every CU several times
+static inline unsigned pgrid(pcp_ctx *c, long n, int block) {
+  long want = (n + block - 1) / block;
+  long cap = (long)c->n_cu * 16;
+  if (want > cap) want = cap;
+  return (unsigned)(want < 1 ? 1 : want);
+}
+
+void launch_yee_e(pcp_ctx *c, int clear_j) {
+  long total = (long)c->D * c->g.P;
+  hipLaunchKernelGGL(k_yee_e, dim3(grid_for(total, 256)), dim3(256), 0, c->stream, c->g, c->f, clear_j, c->st);
+}
+void launch_yee_h(pcp_ctx *c) {
+  long total = (long)c->D * c->g.P;
+  hipLaunchKernelGGL(k_yee_h, dim3(grid_for(total, 256)), dim3(256), 0, c->stream, c->g, c->f);
+}
+void launch_stitch(pcp_ctx *c, int which) {
+  if (c->D == 1) return;
+  long total = (long)c->D * c->g.P;
+  if (which == PCP_GRID_E) {
+    hipLaunchKernelGGL(k_stitch_copy, dim3(grid_for(total, 256)), dim3(256), 0, c->stream, c->g, c->f.E, 3);
+  } else if (which == PCP_GRID_H || which == PCP_GRID_H_AT_ET) {
+    hipLaunchKernelGGL(k_stitch_copy, dim3(grid_for(total, 256)), dim3(256), 0, c->stream, c->g, c->f.H, 3);
+    hipLaunchKernelGGL(k_stitch_copy, dim3(grid_for(total, 256)), dim3(256), 0, c->stream, c->g, c->f.Hat, 3);
+  } else {
+    long cells = (long)c->D * c->g.nr * c->g.nz;
+    hipLaunchKernelGGL(k_stitch_sum_j, dim3(grid_for(cells, 256)), dim3(256), 0, c->stream, c->g, c->f.J);
+  }
+}
+void launch_fill(pcp_ctx *c, double *p, long n, double v) {
+  hipLaunchKernelGGL(k_fill, dim3(pgrid(c, n, 256)), dim3(256), 0, c->stream, p, n, v);
+}
+void launch_step_begin(pcp_ctx *c) { hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, c->stream, c->st); }
+void launch_stamp(pcp_ctx *c) { hipLaunchKernelGGL(k_stamp, dim3(1), dim3(1), 0, c->stream, c->st); }
+void launch_push(pcp_ctx *c, int sp) {
+  SpeciesDev &s = c->sp[sp];
+  if (!s.n) return;
+  hipLaunchKernelGGL(k_push, dim3(pgrid(c, s.n, 256)), dim3(256), 0, c->stream, c->g, c->f, s, c->cur, c->st);
+}
+void launch_move(pcp_ctx *c, int sp) {
+  SpeciesDev &s = c->sp[sp];
+  if (!s.n) return;
+  hipLaunchKernelGGL(k_move, dim3(pgrid(c, s.n, 256)), dim3(256), 0, c->stream, c->g, s, c->cur);
+}
+void launch_migrate_mark(pcp_ctx *c, int sp) {
+  SpeciesDev &s = c->sp[sp];
+  if (!s.n) return;
+  hipLaunchKernelGGL(k_migrate, dim3(pgrid(c, s.n, 256)), dim3(256), 0, c->stream, c->g, s, sp, c->cur, c->st,
+                     c->mig);
+}
+void launch_step_fused(pcp_ctx *c, int sp) {
+  SpeciesDev &s = c->sp[sp];
+  if (!s.n) return;
+  hipLaunchKernelGGL(k_step_fused, dim3(pgrid(c, s.n, 256)), dim3(256), 0, c->stream, c->g, c->f, s, sp, c->cur,
+                     c->st, c->mig);
+}
+void launch_mark(pcp_ctx *c, int sp, int only_if_exact) {
+  SpeciesDev &s = c->sp[sp];
+  if (!s.n || c->g.ccs != PCP_CCS_VB) return;
+  hipLaunchKernelGGL(k_mark, dim3(pgrid(c, s.n, 256)), dim3(256), 0, c->stream, c->g, s, sp, c->cur, c->st,
+                     c->marker, only_if_exact);
+}
+void launch_deposit_exact(pcp_ctx *c, int sp, int only_if_exact) {
+  SpeciesDev &s = c->sp[sp];
+  if (!s.n) return;
+  hipLaunchKernelGGL(k_deposit_exact, dim3(pgrid(c, s.n, 256)), dim3(256), 0, c->stream, c->g, c->f, s, sp, c->cur,
+                     c->st, c->marker, only_if_exact);
+}
+void launch_zero_j_if_exact(pcp_ctx *c) {
+  long n = (long)c->D * 3 * c->g.P;
+  hipLaunchKernelGGL(k_fill_if_exact, dim3(pgrid(c, n, 256)), dim3(256), 0, c->stream, c->f.J, n, c->st);
+}
+void launch_migrant_keys(pcp_ctx *c) {
+  if (c->D == 1) return;
+  SeqTable t;
+  for (int i = 0; i < MAX_SPECIES; ++i) t.p[i] = c->sp[i].seq;
+  unsigned gm = (unsigned)(c->n_cu * 4);
+  hipLaunchKernelGGL(k_mig_count, dim3(gm), dim3(256), 0, c->stream, c->st, c->mig);
+  hipLaunchKernelGGL(k_mig_scan, dim3(1), dim3(1024), 0, c->stream, c->st, c->mig);
+  hipLaunchKernelGGL(k_mig_scatter, dim3(gm), dim3(256), 0, c->stream, c->st, c->mig);
+  hipLaunchKernelGGL(k_mig_rank, dim3(gm), dim3(256), 0, c->stream, c->st, c->mig, t);
+  hipLaunchKernelGGL(k_mig_finish, dim3(grid_for(c->mig.ngroups, 256)), dim3(256), 0, c->stream, c->st, c->mig);
+}
+void launch_gather_point(pcp_ctx *c, int d, double r, double z, int which, double *dev_out) {
+  hipLaunchKernelGGL(k_gather_point, dim3(1), dim3(1), 0, c->stream, c->g, c->f, d, r, z, which, dev_out);
+}
+
+}  // namespace pcp
