@@ -590,6 +590,12 @@ int pcp_step(pcp_ctx *c, int n_steps) {
 
 int pcp_get_field_e(pcp_ctx *c, int d, double r, double z, double *out) {
   if (!c || !out || d < 0 || d >= c->D) return fail(PCP_E_ARG, "bad argument");
+  {
+    Dom dm = make_dom(c->g, d);
+    double r0 = dm.bot_r * c->g.dr, z0 = dm.left_z * c->g.dz;
+    if (!(r >= r0 && r < r0 + c->g.nr * c->g.dr && z >= z0 && z < z0 + c->g.nz * c->g.dz))
+      return fail(PCP_E_ARG, "point outside domain");
+  }
   double *dev;
   HIP_TRY(hipMalloc(&dev, 3 * sizeof(double)));
   launch_gather_point(c, d, r, z, 0, dev);
@@ -601,6 +607,12 @@ int pcp_get_field_e(pcp_ctx *c, int d, double r, double z, double *out) {
 
 int pcp_get_field_h(pcp_ctx *c, int d, double r, double z, double *out) {
   if (!c || !out || d < 0 || d >= c->D) return fail(PCP_E_ARG, "bad argument");
+  {
+    Dom dm = make_dom(c->g, d);
+    double r0 = dm.bot_r * c->g.dr, z0 = dm.left_z * c->g.dz;
+    if (!(r >= r0 && r < r0 + c->g.nr * c->g.dr && z >= z0 && z < z0 + c->g.nz * c->g.dz))
+      return fail(PCP_E_ARG, "point outside domain");
+  }
   double *dev;
   HIP_TRY(hipMalloc(&dev, 3 * sizeof(double)));
   launch_gather_point(c, d, r, z, 1, dev);
