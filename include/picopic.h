@@ -139,6 +139,14 @@ int pcp_deposit(pcp_ctx *ctx);
  * (src/PiCoPiC.cpp:299-345) with push + move + deposit fused */
 int pcp_step(pcp_ctx *ctx, int n_steps);
 
+/* Counting sort of species sp by (domain, cell), dropping removed particles
+ * (no reference counterpart: the reference keeps per-domain vectors).  The
+ * reference order keys travel with the particles, so results do not depend
+ * on the storage order.  pcp_step sorts species sp before every `every`-th
+ * step (0: never; defaults: beams 1, plasma 50). */
+int pcp_sort(pcp_ctx *ctx, int sp);
+int pcp_set_sort_interval(pcp_ctx *ctx, int sp, int every);
+
 /* field gather at one point of domain d (include/maxwellSolver.hpp:101-316),
  * evaluated on the device; out[3] = (r, phi, z) components */
 int pcp_get_field_e(pcp_ctx *ctx, int d, double r, double z, double *out);

@@ -25,7 +25,7 @@ EXPORTS = [
     "pcp_download_species", "pcp_init_plasma", "pcp_init_bunch", "pcp_download_grid", "pcp_upload_grid",
     "pcp_grid_points", "pcp_yee_e", "pcp_yee_h", "pcp_stitch", "pcp_push", "pcp_move", "pcp_migrate",
     "pcp_deposit", "pcp_step", "pcp_get_field_e", "pcp_get_field_h", "pcp_stream", "pcp_set_timing",
-    "pcp_kernel_ms",
+    "pcp_kernel_ms", "pcp_sort", "pcp_set_sort_interval",
 ]
 
 
@@ -82,6 +82,8 @@ def load(path=LIB_PATH):
     L.pcp_set_timing.argtypes = [vp, C.c_int]
     L.pcp_kernel_ms.argtypes = [vp, C.c_char_p]
     L.pcp_kernel_ms.restype = C.c_double
+    L.pcp_sort.argtypes = [vp, C.c_int]
+    L.pcp_set_sort_interval.argtypes = [vp, C.c_int, C.c_int]
     _lib = L
     return L
 
@@ -214,6 +216,12 @@ class Context:
 
     def step(self, n=1):
         _check(_lib.pcp_step(self.h, n))
+
+    def sort(self, sp):
+        _check(_lib.pcp_sort(self.h, sp))
+
+    def set_sort_interval(self, sp, every):
+        _check(_lib.pcp_set_sort_interval(self.h, sp, every))
 
     def sync(self):
         _check(_lib.pcp_sync(self.h))

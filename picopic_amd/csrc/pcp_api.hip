@@ -12,6 +12,7 @@
 #include "pcp_init.hpp"
 #include "pcp_internal.hpp"
 #include "pcp_kernels.hpp"
+#include "pcp_sort.hpp"
 
 using namespace pcp;
 
@@ -238,6 +239,30 @@ int pcp_create(const pcp_config *cfg, pcp_ctx **out) {
       return rc;
     }
   }
+  // sort scratch: alternate particle arrays and per-cell counters
+  long max_cap = 0;
+  for (int s = 0; s < cfg->n_species; ++s) {
+    SpeciesDev &sp = c->sp[s];
+    SortBuffers &b = c->sortb[s];
+    max_cap = std::max(max_cap, sp.cap);
+    if ((rc = dalloc(&b.vr, sp.cap)) || (rc = dalloc(&b.vphi, sp.cap)) || (rc = dalloc(&b.vz, sp.cap)) ||
+        (rc = dalloc(&b.w, sp.cap)) || (rc = dalloc(&b.seq, sp.cap)) || (rc = dalloc(&b.id, sp.cap))) {
+      pcp_destroy(c);
+      return rc;
+    }
+    c->sort_every[s] = sp.beam ? 1 : 50;
+  }
+  {
+    long ncells = (long)c->D * g.nr * g.nz;
+    c->sort_key_cap = max_cap;
+    c->sort_temp_bytes = sort_temp_bytes(ncells);
+    if ((rc = dalloc(&c->sort_key, max_cap)) || (rc = dalloc(&c->sort_count, ncells + 1)) ||
+        (rc = dalloc(&c->sort_offset, ncells + 1)) || (rc = dalloc(&c->sort_fill, ncells + 1)) ||
+        (rc = dalloc((char **)&c->sort_temp, c->sort_temp_bytes))) {
+      pcp_destroy(c);
+      return rc;
+    }
+  }
   // migration bookkeeping
   Migrants &m = c->mig;
   m.cap = cfg->max_migrants > 0 ? cfg->max_migrants : std::max<long>(1 << 16, total_cap / 64);
@@ -261,6 +286,12 @@ int pcp_destroy(pcp_ctx *c) {
     hipFree(sp.r[0]); hipFree(sp.r[1]); hipFree(sp.z[0]); hipFree(sp.z[1]);
     hipFree(sp.vr); hipFree(sp.vphi); hipFree(sp.vz); hipFree(sp.w); hipFree(sp.seq); hipFree(sp.id);
   }
+  for (int s = 0; s < MAX_SPECIES; ++s) {
+    SortBuffers &b = c->sortb[s];
+    hipFree(b.vr); hipFree(b.vphi); hipFree(b.vz); hipFree(b.w); hipFree(b.seq); hipFree(b.id);
+  }
+  hipFree(c->sort_key); hipFree(c->sort_count); hipFree(c->sort_offset); hipFree(c->sort_fill);
+  hipFree(c->sort_temp);
   Migrants &m = c->mig;
   hipFree(m.key); hipFree(m.ref); hipFree(m.group); hipFree(m.bucket);
   hipFree(m.gcount); hipFree(m.gstart); hipFree(m.gfill); hipFree(m.gbase);
@@ -551,6 +582,9 @@ int pcp_step(pcp_ctx *c, int n_steps) {
     launch_stitch(c, PCP_GRID_E);
     launch_yee_h(c);
     launch_stitch(c, PCP_GRID_H);
+    for (int s = 0; s < c->cfg.n_species; ++s)
+      if (c->sort_every[s] > 0 && c->steps_done % c->sort_every[s] == 0 && launch_sort(c, s) < 0)
+        return fail(PCP_E_HIP, "sort failed");
     mark(t, 1);
     for (int s = 0; s < c->cfg.n_species; ++s) launch_step_fused(c, s);
     c->cur ^= 1;
@@ -565,6 +599,7 @@ int pcp_step(pcp_ctx *c, int n_steps) {
     mark(t, 3);
     launch_stitch(c, PCP_GRID_J);
     mark(t, 4);
+    ++c->steps_done;
   }
   HIP_TRY(hipGetLastError());
   int rc = pcp_sync(c);
@@ -586,6 +621,19 @@ int pcp_step(pcp_ctx *c, int n_steps) {
     for (auto &e : ev) (void)hipEventDestroy(e);
   }
   return rc;
+}
+
+int pcp_sort(pcp_ctx *c, int sp) {
+  if (!c || sp < 0 || sp >= c->cfg.n_species) return fail(PCP_E_ARG, "bad argument");
+  if (launch_sort(c, sp) < 0) return fail(PCP_E_HIP, "sort failed");
+  HIP_TRY(hipGetLastError());
+  return pcp_sync(c);
+}
+
+int pcp_set_sort_interval(pcp_ctx *c, int sp, int every) {
+  if (!c || sp < 0 || sp >= c->cfg.n_species || every < 0) return fail(PCP_E_ARG, "bad argument");
+  c->sort_every[sp] = every;
+  return 0;
 }
 
 int pcp_get_field_e(pcp_ctx *c, int d, double r, double z, double *out) {

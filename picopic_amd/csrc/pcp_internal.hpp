@@ -108,6 +108,12 @@ struct StepState {
   unsigned int n_mig;          // migrants appended in this step
 };
 
+// second set of per-particle arrays used as the sort's scatter target
+struct SortBuffers {
+  double *vr, *vphi, *vz, *w;
+  uint64_t *seq, *id;
+};
+
 struct Migrants {
   uint64_t *key;      // colour << 47 | srcdom << 35 | old seq
   uint64_t *ref;      // species << 40 | particle index
@@ -132,6 +138,13 @@ struct pcp_ctx {
   pcp::StepState *st;              // device
   pcp::Migrants mig;
   uint64_t *marker;                // [D][P] strict-motion J_r reset markers
+  pcp::SortBuffers sortb[pcp::MAX_SPECIES];
+  uint32_t *sort_key, *sort_count, *sort_offset, *sort_fill;
+  void *sort_temp;
+  size_t sort_temp_bytes;
+  long sort_key_cap;
+  int sort_every[pcp::MAX_SPECIES]; // 0: never; k: before every k-th step
+  long steps_done;
   int n_cu;
   int timing;
   std::vector<std::pair<std::string, double>> ktimes;

@@ -14,6 +14,7 @@
 
 #include "pcp_kernels.hpp"
 #include "pcp_physics.hpp"
+#include "pcp_wave.hpp"
 
 namespace pcp {
 
@@ -66,6 +67,22 @@ struct MarkSink {
     if (i < -2 || i >= nr + 2 || k < -2 || k >= nz + 2) return;
     atomicMax((unsigned long long *)&marker[(long)(i + 2) * yr + (k + 2)], (unsigned long long)val);
   }
+};
+
+// wave-aggregated sink (call sites must be converged across the wavefront)
+struct WaveSink {
+  double *J;      // f.J, every domain
+  int base;       // offset of component 0 of the destination domain
+  int P, yr, nr, nz;
+  bool active;
+  int *err;
+  __device__ __forceinline__ void add_if(int c, int i, int k, double v, bool a) {
+    bool in = i >= -2 && i < nr + 2 && k >= -2 && k < nz + 2;
+    if (a && !in) *err |= ERR_MOVED_TOO_FAR;
+    wave_add(J, base + c * P + (i + 2) * yr + (k + 2), v, a && in);
+  }
+  __device__ __forceinline__ void add(int c, int i, int k, double v) { add_if(c, i, k, v, active); }
+  __device__ __forceinline__ void strict_reset(int, int) {}
 };
 
 // domain owning global cell (cr, cz)
@@ -354,34 +371,87 @@ __global__ void k_migrate(Geom g, SpeciesDev s, int sp, int cur, StepState *st, 
   if (err) atomicOr(&st->err, err);
 }
 
-// fused push + move + deposit (the hot kernel)
-__global__ void __launch_bounds__(256) k_step_fused(Geom g, Fields f, SpeciesDev s, int sp, int cur, StepState *st,
+// fused push + move + deposit (the hot kernel).  Every thread of a block runs
+// the same number of loop trips so that the ZigZag deposit, which has no
+// data-dependent branches, executes converged and aggregates its atomics per
+// wavefront.
+template <int CCS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CCS == PCP_CCS_ZIGZAG ? 3 : 2))) k_step_fused(Geom g, Fields f, SpeciesDev s, int sp, int cur, StepState *st,
                                                     Migrants mg) {
   int err = 0;
   const int nxt = cur ^ 1;
-  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < s.n; p += (long)gridDim.x * blockDim.x) {
-    double r = s.r[cur][p], z = s.z[cur][p];
-    if (r < 0) { s.r[nxt][p] = -1; s.z[nxt][p] = z; continue; }
-    if (bad_position(r, z)) { err |= ERR_NAN_POSITION; s.r[nxt][p] = -1; s.z[nxt][p] = z; continue; }
-    double v[3] = {s.vr[p], s.vphi[p], s.vz[p]};
-    push_one(g, f, s.q, s.m, r, z, v, &err);
-    const double ro = r, zo = z;
-    double vr = v[0], vphi = v[1], vz = v[2];
-    int d_old, d_new = 0;
-    bool alive = move_one(g, s.beam, r, z, vr, vphi, vz, d_old, d_new);
-    s.vr[p] = vr; s.vphi[p] = vphi; s.vz[p] = vz;
-    if (!alive) { s.r[nxt][p] = -1; s.z[nxt][p] = z; continue; }
-    s.r[nxt][p] = r; s.z[nxt][p] = z;
-    if (d_new != d_old) append_migrant(mg, st, sp, p, d_old, d_new, s.seq[p], g, &err);
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long b0 = (long)blockIdx.x * blockDim.x; b0 < s.n; b0 += stride) {
+    const long p = b0 + threadIdx.x;
+    bool live = false;
+    double r = g.dr, z = g.dz, ro = g.dr, zo = g.dz, vphi = 0, w = 0;
+    int d_new = 0;
+    if (p < s.n) {
+      r = s.r[cur][p];
+      z = s.z[cur][p];
+      if (r < 0) {
+        s.r[nxt][p] = -1; s.z[nxt][p] = z;
+      } else if (bad_position(r, z)) {
+        err |= ERR_NAN_POSITION;
+        s.r[nxt][p] = -1; s.z[nxt][p] = z;
+      } else {
+        double v[3] = {s.vr[p], s.vphi[p], s.vz[p]};
+        push_one(g, f, s.q, s.m, r, z, v, &err);
+        ro = r; zo = z;
+        double vr = v[0], vz = v[2];
+        vphi = v[1];
+        int d_old;
+        bool alive = move_one(g, s.beam, r, z, vr, vphi, vz, d_old, d_new);
+        s.vr[p] = vr; s.vphi[p] = vphi; s.vz[p] = vz;
+        if (!alive) {
+          s.r[nxt][p] = -1; s.z[nxt][p] = z;
+        } else {
+          s.r[nxt][p] = r; s.z[nxt][p] = z;
+          if (d_new != d_old) append_migrant(mg, st, sp, p, d_old, d_new, s.seq[p], g, &err);
+          w = s.w[p];
+          live = true;
+        }
+      }
+    }
+    if (!live) { r = ro = g.dr; z = zo = g.dz; d_new = 0; vphi = 0; w = 0; }
     Dom dn = make_dom(g, d_new);
-    double w = s.w[p];
-    AtomicSink S{f.J + (long)d_new * 3 * g.P, g.P, g.yr, g.nr, g.nz, &err, &st->need_exact};
-    if (g.ccs == PCP_CCS_VB) {
-      double p_charge = s.q * w;
-      vb_azimuthal(S, g.nr, g.nz, g.dr, g.dz, dn.bot_r, dn.left_z, r, z, ro, zo, vphi, p_charge);
-      vb_rz(S, g.dr, g.dz, g.dt, dn.bot_r, dn.left_z, r, z, ro, zo, p_charge, &err);
-    } else {
+    WaveSink S{f.J, (int)((long)d_new * 3 * g.P), (int)g.P, g.yr, g.nr, g.nz, live, &err};
+    if (CCS == PCP_CCS_ZIGZAG) {
       zigzag(S, g.dr, g.dz, g.dt, dn.bot_r, dn.left_z, r, z, ro, zo, vphi, s.q, w);
+    } else {
+      const double p_charge = s.q * w;
+      // azimuthal part (4 interior nodes)
+      AzTerms A = vb_azimuthal_terms(g.nr, g.nz, g.dr, g.dz, dn.bot_r, dn.left_z, r, z, ro, zo, vphi, p_charge);
+      S.add_if(1, A.i, A.k, A.w[0], live && A.in[0]);
+      S.add_if(1, A.i + 1, A.k, A.w[1], live && A.in[1]);
+      S.add_if(1, A.i, A.k + 1, A.w[2], live && A.in[2]);
+      S.add_if(1, A.i + 1, A.k + 1, A.w[3], live && A.in[3]);
+      // r-z part: branch selection per lane, arithmetic and atomics converged
+      SegRecorder R;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        R.rn[q] = 1.5 * g.dr; R.zn[q] = 1.25 * g.dz; R.ro[q] = 1.25 * g.dr; R.zo[q] = 1.5 * g.dz;
+        R.i[q] = 1; R.k[q] = 1;
+      }
+      if (live) vb_rz_dispatch(R, g.dr, g.dz, r, z, ro, zo, &err);
+      if (live && R.is_strict) {
+        // strict motion is rare: plain atomics; a J_r reset flags the
+        // exact re-deposition of this step
+        AtomicSink A1{f.J + (long)d_new * 3 * g.P, g.P, g.yr, g.nr, g.nz, &err, &st->need_exact};
+        vb_strict(A1, g.dr, g.dz, g.dt, dn.bot_r, dn.left_z, r, z, ro, zo, p_charge);
+      }
+#pragma unroll
+      for (int sg = 0; sg < 3; ++sg) {
+        const bool a = live && !R.is_strict && sg < R.n;
+        if (__ballot(a) == 0) break;
+        VBTerms T = vb_simple_terms(g.dr, g.dz, g.dt, dn.bot_r, dn.left_z, R.rn[sg], R.zn[sg], R.ro[sg], R.zo[sg],
+                                    R.i[sg], R.k[sg], p_charge);
+        const bool v = a && T.valid;
+        S.add_if(2, T.i, T.k, T.jz0, v);
+        S.add_if(2, T.i + 1, T.k, T.jz1, v);
+        S.add_if(0, T.i, T.k, T.jr0, v);
+        S.add_if(0, T.i, T.k + 1, T.jr1, v);
+      }
     }
   }
   if (err) atomicOr(&st->err, err);
@@ -567,8 +637,12 @@ void launch_migrate_mark(pcp_ctx *c, int sp) {
 void launch_step_fused(pcp_ctx *c, int sp) {
   SpeciesDev &s = c->sp[sp];
   if (!s.n) return;
-  hipLaunchKernelGGL(k_step_fused, dim3(pgrid(c, s.n, 256)), dim3(256), 0, c->stream, c->g, c->f, s, sp, c->cur,
-                     c->st, c->mig);
+  if (c->g.ccs == PCP_CCS_ZIGZAG)
+    hipLaunchKernelGGL(k_step_fused<PCP_CCS_ZIGZAG>, dim3(pgrid(c, s.n, 256)), dim3(256), 0, c->stream, c->g, c->f, s,
+                       sp, c->cur, c->st, c->mig);
+  else
+    hipLaunchKernelGGL(k_step_fused<PCP_CCS_VB>, dim3(pgrid(c, s.n, 256)), dim3(256), 0, c->stream, c->g, c->f, s, sp,
+                       c->cur, c->st, c->mig);
 }
 void launch_mark(pcp_ctx *c, int sp, int only_if_exact) {
   SpeciesDev &s = c->sp[sp];

@@ -1,0 +1,129 @@
+// Counting sort of one species by (domain, cell), with compaction.
+//
+// Cell order makes the 64 particles of a wavefront share their field nodes
+// (the gather becomes broadcast loads) and their current nodes (the
+// wave-aggregated atomics issue one add per node).  Removed particles
+// (r < 0) are dropped; the tail [n_live, n) is marked removed so that the
+// host-side slot count stays valid without a synchronisation.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "pcp_sort.hpp"
+
+namespace pcp {
+
+__device__ __forceinline__ uint32_t wave_reserve(uint32_t *ctr, uint32_t key, bool active) {
+  // one atomic per distinct key in the wavefront; lanes get consecutive slots
+  uint32_t slot = 0;
+  unsigned long long pending = __ballot(active);
+  const int lane = __lane_id();
+  while (pending) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const uint32_t lk = __shfl(key, leader, 64);
+    const bool m = active && key == lk;
+    const unsigned long long mask = __ballot(m);
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(ctr + lk, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader, 64);
+    if (m) slot = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    pending &= ~mask;
+    active = active && !m;
+  }
+  return slot;
+}
+
+__global__ void __launch_bounds__(256) k_sort_count(Geom g, SpeciesDev s, int cur, uint32_t *key, uint32_t *count,
+                                                    uint32_t sentinel) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long b0 = (long)blockIdx.x * blockDim.x; b0 < s.n; b0 += stride) {
+    const long p = b0 + threadIdx.x;
+    uint32_t k = sentinel;
+    if (p < s.n) {
+      double r = s.r[cur][p], z = s.z[cur][p];
+      if (r >= 0) {
+        long cr = (long)floor(r / g.dr), cz = (long)floor(z / g.dz);
+        int d = (int)((cr / g.nr) * g.Dz + cz / g.nz);
+        long lr = cr - (long)(d / g.Dz) * g.nr, lz = cz - (long)(d % g.Dz) * g.nz;
+        k = (uint32_t)((long)d * g.nr * g.nz + lr * g.nz + lz);
+      }
+      key[p] = k;
+    }
+    (void)wave_reserve(count, k, p < s.n && k != sentinel);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_sort_scatter(SpeciesDev s, SortBuffers sb, int cur, const uint32_t *key,
+                                                      const uint32_t *offset, uint32_t *fill, uint32_t sentinel) {
+  const int nxt = cur ^ 1;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long b0 = (long)blockIdx.x * blockDim.x; b0 < s.n; b0 += stride) {
+    const long p = b0 + threadIdx.x;
+    uint32_t k = p < s.n ? key[p] : sentinel;
+    const bool a = p < s.n && k != sentinel;
+    uint32_t slot = wave_reserve(fill, a ? k : 0, a);
+    if (a) {
+      long q = (long)offset[k] + slot;
+      s.r[nxt][q] = s.r[cur][p];
+      s.z[nxt][q] = s.z[cur][p];
+      sb.vr[q] = s.vr[p];
+      sb.vphi[q] = s.vphi[p];
+      sb.vz[q] = s.vz[p];
+      sb.w[q] = s.w[p];
+      sb.seq[q] = s.seq[p];
+      sb.id[q] = s.id[p];
+    }
+  }
+}
+
+// mark [n_live, n) removed in both position buffers
+__global__ void k_sort_tail(SpeciesDev s, const uint32_t *n_live_ptr) {
+  long n_live = (long)*n_live_ptr;
+  for (long p = n_live + (long)blockIdx.x * blockDim.x + threadIdx.x; p < s.n; p += (long)gridDim.x * blockDim.x) {
+    s.r[0][p] = -1;
+    s.r[1][p] = -1;
+  }
+}
+
+__global__ void k_zero_u32(uint32_t *p, long n) {
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) p[t] = 0;
+}
+
+size_t sort_temp_bytes(long ncells) {
+  size_t bytes = 0;
+  hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (uint32_t *)nullptr, (uint32_t *)nullptr, (int)(ncells + 1));
+  return bytes;
+}
+
+int launch_sort(pcp_ctx *c, int sp) {
+  SpeciesDev &s = c->sp[sp];
+  if (s.n == 0) return 0;
+  SortBuffers &sb = c->sortb[sp];
+  const long ncells = (long)c->D * c->g.nr * c->g.nz;
+  const uint32_t sentinel = (uint32_t)ncells;
+  unsigned grid = (unsigned)std::min<long>((s.n + 255) / 256, (long)c->n_cu * 16);
+  unsigned gz = (unsigned)std::min<long>((ncells + 256) / 256, (long)c->n_cu * 16);
+  hipLaunchKernelGGL(k_zero_u32, dim3(gz), dim3(256), 0, c->stream, c->sort_count, ncells + 1);
+  hipLaunchKernelGGL(k_zero_u32, dim3(gz), dim3(256), 0, c->stream, c->sort_fill, ncells + 1);
+  hipLaunchKernelGGL(k_sort_count, dim3(grid), dim3(256), 0, c->stream, c->g, s, c->cur, c->sort_key, c->sort_count,
+                     sentinel);
+  size_t bytes = c->sort_temp_bytes;
+  if (hipcub::DeviceScan::ExclusiveSum(c->sort_temp, bytes, c->sort_count, c->sort_offset, (int)(ncells + 1),
+                                       c->stream) != hipSuccess)
+    return -1;
+  hipLaunchKernelGGL(k_sort_scatter, dim3(grid), dim3(256), 0, c->stream, s, sb, c->cur, c->sort_key, c->sort_offset,
+                     c->sort_fill, sentinel);
+  // swap the sorted buffers in (this species only; the shared `cur` index
+  // keeps its meaning because the two position pointers are exchanged)
+  std::swap(s.r[c->cur], s.r[c->cur ^ 1]);
+  std::swap(s.z[c->cur], s.z[c->cur ^ 1]);
+  std::swap(s.vr, sb.vr);
+  std::swap(s.vphi, sb.vphi);
+  std::swap(s.vz, sb.vz);
+  std::swap(s.w, sb.w);
+  std::swap(s.seq, sb.seq);
+  std::swap(s.id, sb.id);
+  hipLaunchKernelGGL(k_sort_tail, dim3(grid), dim3(256), 0, c->stream, s, c->sort_offset + ncells);
+  return 1;
+}
+
+}  // namespace pcp
