@@ -64,8 +64,10 @@ def run_gpu(w, steps, warmup, prec):
     wall = time.perf_counter() - t0
     phases = {k: L.pcp_kernel_ms(ctx.h, k.encode()) for k in ("fields", "particles", "fixup", "stitch_j", "step")}
     n_end = sum(ctx.count(s) for s in range(len(species)))
+    copy_gbs = ctx.copy_bandwidth(1 << 30, 20)
     ctx.close()
-    return dict(n_particles=n_part, n_particles_end=n_end, wall=wall, phases=phases, cells=grid[0] * grid[1])
+    return dict(n_particles=n_part, n_particles_end=n_end, wall=wall, phases=phases, cells=grid[0] * grid[1],
+                copy_gbs=copy_gbs)
 
 
 def cpu_baseline(w, threads):
@@ -144,7 +146,8 @@ def main():
                    "deposition": w["ccs"], "pusher": w["pusher"], "phases_ms": g["phases"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-                     "kernel": "k_step_fused (push+move+deposit)", "alg_bytes_per_particle": bytes_pp},
+                     "kernel": "k_step_fused (push+move+deposit)", "alg_bytes_per_particle": bytes_pp,
+                     "copy_kernel_gbs": g["copy_gbs"]},
     }
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

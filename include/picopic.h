@@ -152,6 +152,11 @@ int pcp_set_sort_interval(pcp_ctx *ctx, int sp, int every);
 int pcp_get_field_e(pcp_ctx *ctx, int d, double r, double z, double *out);
 int pcp_get_field_h(pcp_ctx *ctx, int d, double r, double z, double *out);
 
+/* achievable HBM bandwidth of a plain device copy of `bytes` bytes (read +
+ * write counted), averaged over iters launches [GB/s]; a calibration aid for
+ * the roofline figures, no reference counterpart */
+int pcp_copy_bandwidth(pcp_ctx *ctx, long bytes, int iters, double *gbps);
+
 /* hipStream_t the context enqueues on (as void*) */
 void *pcp_stream(pcp_ctx *ctx);
 /* average device time [ms] of the named kernel over the last pcp_step call

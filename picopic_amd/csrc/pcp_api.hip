@@ -670,6 +670,12 @@ int pcp_get_field_h(pcp_ctx *c, int d, double r, double z, double *out) {
   return 0;
 }
 
+int pcp_copy_bandwidth(pcp_ctx *c, long bytes, int iters, double *gbps) {
+  if (!c || !gbps || bytes < 16 || iters < 1) return fail(PCP_E_ARG, "bad argument");
+  if (bench_copy(c, bytes, iters, gbps) != 0) return fail(PCP_E_NOMEM, "copy buffers");
+  return 0;
+}
+
 int pcp_set_timing(pcp_ctx *c, int on) {
   if (!c) return fail(PCP_E_ARG, "null context");
   c->timing = on;

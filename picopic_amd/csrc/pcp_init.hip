@@ -126,6 +126,35 @@ __global__ void k_count_live(const double *r, long n, unsigned long long *out) {
   atomicAdd(out, c);
 }
 
+// streaming copy used to calibrate the achievable HBM bandwidth (16 B/lane)
+__global__ void __launch_bounds__(256) k_copy(const double2 *__restrict__ a, double2 *__restrict__ b, long n) {
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) b[t] = a[t];
+}
+
+int bench_copy(pcp_ctx *c, long bytes, int iters, double *gbps) {
+  long n = bytes / 16;
+  double2 *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, n * 16) != hipSuccess || hipMalloc(&b, n * 16) != hipSuccess) return -1;
+  (void)hipMemsetAsync(a, 0, n * 16, c->stream);
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  unsigned grid = (unsigned)std::min<long>((n + 255) / 256, (long)c->n_cu * 64);
+  hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, c->stream, a, b, n);
+  (void)hipEventRecord(e0, c->stream);
+  for (int i = 0; i < iters; ++i) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, c->stream, a, b, n);
+  (void)hipEventRecord(e1, c->stream);
+  (void)hipEventSynchronize(e1);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  *gbps = 2.0 * n * 16 * iters / (ms * 1e-3) / 1e9;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return 0;
+}
+
 void launch_init_plasma(pcp_ctx *c, int sp, long base, long n_new, const long *dev_cell_off, long n_cells,
                         double weight, double vth_r, double vth_phi, double vth_z, uint64_t seed,
                         const long *dev_dom_off) {

@@ -13,5 +13,7 @@ void launch_init_plasma(pcp_ctx *c, int sp, long base, long n_new, const long *d
 void launch_init_bunch(pcp_ctx *c, int sp, long base, long n_new, double sigma_r, double sigma_z, double z0,
                        double vz, double weight, uint64_t seed);
 void launch_count_live(pcp_ctx *c, int sp, unsigned long long *dev_out);
+// device-to-device copy of `bytes` (read + write counted), iters launches
+int bench_copy(pcp_ctx *c, long bytes, int iters, double *gbps);
 
 }  // namespace pcp
