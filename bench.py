@@ -112,7 +112,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="wakefield")
-    ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--prec", default="both", choices=["fp64", "fp32", "both"],
+                    help="'both' times the FP32 split path (the reported value) and the plain FP64 path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
@@ -121,14 +122,27 @@ def main():
               "(the sharding plan is described in the documentation)", file=sys.stderr)
         sys.exit(2)
     w = workload(args.workload)
-    g = run_gpu(w, args.steps, args.warmup, args.prec)
+    runs = {}
+    for prec in (["fp32", "fp64"] if args.prec == "both" else [args.prec]):
+        runs[prec] = run_gpu(w, args.steps, args.warmup, prec)
+    main_prec = "fp32" if "fp32" in runs else "fp64"
+    g = runs[main_prec]
     n = g["n_particles"]
     value = n * args.steps / g["wall"]
     ms_step = g["wall"] * 1e3 / args.steps
     t_part = g["phases"]["particles"]   # ms per step of the fused push+move+deposit kernels
-    bytes_pp = 88 if args.prec == "fp64" else 52
+    bytes_pp = 88 if main_prec == "fp64" else 52
     alg_bytes = bytes_pp * n
     achieved = alg_bytes / (t_part * 1e-3) / 1e9 if t_part > 0 else None
+    config = {"workload": w["name"], "particles": n, "cells": g["cells"], "precision": main_prec,
+              "deposition": w["ccs"], "pusher": w["pusher"], "phases_ms": g["phases"]}
+    if "fp64" in runs and main_prec != "fp64":
+        g64 = runs["fp64"]
+        v64 = g64["n_particles"] * args.steps / g64["wall"]
+        config["fp64_plain"] = {"value": v64, "ms_per_step": g64["wall"] * 1e3 / args.steps,
+                                "phases_ms": g64["phases"],
+                                "achieved_gbs": 88 * g64["n_particles"] / (g64["phases"]["particles"] * 1e-3) / 1e9}
+        config["speedup_fp32_vs_fp64"] = value / v64
     line = {
         "metric": "particle-steps/s per GPU",
         "value": value,
@@ -140,14 +154,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.prec == "fp64" else "f32",
+        "dtype": "f64" if main_prec == "fp64" else "f32",
         "data": "synthetic (seeded, generated on device)",
-        "config": {"workload": w["name"], "particles": n, "cells": g["cells"], "precision": args.prec,
-                   "deposition": w["ccs"], "pusher": w["pusher"], "phases_ms": g["phases"]},
+        "config": config,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-                     "kernel": "k_step_fused (push+move+deposit)", "alg_bytes_per_particle": bytes_pp,
-                     "copy_kernel_gbs": g["copy_gbs"]},
+                     "kernel": "k%s_step_fused (push+move+deposit)" % ("32" if main_prec == "fp32" else ""),
+                     "alg_bytes_per_particle": bytes_pp, "copy_kernel_gbs": g["copy_gbs"]},
     }
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

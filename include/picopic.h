@@ -40,7 +40,11 @@ enum { PCP_CCS_VB = 0, PCP_CCS_ZIGZAG = 1 };
  * MaxwellSolverYee::field_h_at_et, Current::current, epsilon, sigma */
 enum { PCP_GRID_E = 0, PCP_GRID_H = 1, PCP_GRID_H_AT_ET = 2, PCP_GRID_J = 3,
        PCP_GRID_EPS = 4, PCP_GRID_SIGMA = 5 };
-/* precision of the particle/field storage, fixed at creation */
+/* precision of the particle/field storage, fixed at creation.  FP32_SPLIT
+ * stores each position as a packed cell index plus FP32 in-cell offsets and
+ * keeps fields, velocities and J in FP32; such contexts run pcp_step and the
+ * field entry points (the push/move/migrate/deposit phase entry points are
+ * FP64 only). */
 enum { PCP_PREC_FP64 = 0, PCP_PREC_FP32_SPLIT = 1 };
 
 typedef struct {
@@ -151,6 +155,14 @@ int pcp_set_sort_interval(pcp_ctx *ctx, int sp, int every);
  * evaluated on the device; out[3] = (r, phi, z) components */
 int pcp_get_field_e(pcp_ctx *ctx, int d, double r, double z, double *out);
 int pcp_get_field_h(pcp_ctx *ctx, int d, double r, double z, double *out);
+
+/* FP64 reductions over the device state (diagnostics; the reference computes
+ * such sums on the host from its maps, src/specieP.cpp:665-839):
+ * field energy  sum over interior nodes of (EPSILON0 E^2 + MAGN_CONST H^2)/2
+ *               times the ring volume PI dz dr^2 (2 i + 1)   [J]
+ * kinetic energy of species sp: sum of w m c^2 (gamma - 1)    [J] */
+int pcp_field_energy(pcp_ctx *ctx, double *out);
+int pcp_kinetic_energy(pcp_ctx *ctx, int sp, double *out);
 
 /* achievable HBM bandwidth of a plain device copy of `bytes` bytes (read +
  * write counted), averaged over iters launches [GB/s]; a calibration aid for

@@ -25,7 +25,7 @@ EXPORTS = [
     "pcp_download_species", "pcp_init_plasma", "pcp_init_bunch", "pcp_download_grid", "pcp_upload_grid",
     "pcp_grid_points", "pcp_yee_e", "pcp_yee_h", "pcp_stitch", "pcp_push", "pcp_move", "pcp_migrate",
     "pcp_deposit", "pcp_step", "pcp_get_field_e", "pcp_get_field_h", "pcp_stream", "pcp_set_timing",
-    "pcp_kernel_ms", "pcp_sort", "pcp_set_sort_interval", "pcp_copy_bandwidth",
+    "pcp_kernel_ms", "pcp_sort", "pcp_set_sort_interval", "pcp_copy_bandwidth", "pcp_field_energy", "pcp_kinetic_energy",
 ]
 
 
@@ -85,6 +85,8 @@ def load(path=LIB_PATH):
     L.pcp_sort.argtypes = [vp, C.c_int]
     L.pcp_set_sort_interval.argtypes = [vp, C.c_int, C.c_int]
     L.pcp_copy_bandwidth.argtypes = [vp, C.c_long, C.c_int, C.POINTER(C.c_double)]
+    L.pcp_field_energy.argtypes = [vp, C.POINTER(C.c_double)]
+    L.pcp_kinetic_energy.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
     _lib = L
     return L
 
@@ -227,6 +229,16 @@ class Context:
     def copy_bandwidth(self, nbytes=1 << 30, iters=20):
         g = C.c_double()
         _check(_lib.pcp_copy_bandwidth(self.h, nbytes, iters, C.byref(g)))
+        return g.value
+
+    def field_energy(self):
+        g = C.c_double()
+        _check(_lib.pcp_field_energy(self.h, C.byref(g)))
+        return g.value
+
+    def kinetic_energy(self, sp):
+        g = C.c_double()
+        _check(_lib.pcp_kinetic_energy(self.h, sp, C.byref(g)))
         return g.value
 
     def sync(self):

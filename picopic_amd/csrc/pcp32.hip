@@ -1,0 +1,1015 @@
+// FP32 split-precision particle loop (see pcp32.hpp for the storage format).
+//
+// The algorithm is the reference's (same staggering, the same CELL_VOLUME
+// factors and the same quirks), rewritten in cell units so that FP32 keeps
+// its precision: r = (i + f) dr with integer i and FP32 f, every ring-volume
+// weight becomes a product of small exact factors, and the radial move is
+// tracked as an increment of f.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "pcp32.hpp"
+#include "pcp_kernels.hpp"
+#include "pcp_physics.hpp"
+#include "pcp_wave.hpp"
+
+namespace pcp {
+
+// ---------------------------------------------------------------------------
+// gather in cell units
+//
+// For a component whose radial node is i = floor(u - 1/2) (E_r, H_z, H_phi)
+// the reference weights are PI dz1 (r2^2 - r1^2) / CELL_VOLUME(i + 1) and
+// PI dz1 (r3^2 - r2^2) / CELL_VOLUME(i + 3) with r1 = r - dr/2, r2 = (i+1) dr,
+// r3 = r + dr/2 and CELL_VOLUME(i + m) = PI dz dr^2 (2 i + m + 1).  With
+// e = u - 1/2 - i in [0, 1) they become
+//   a1 = (1 - e)(2i + 1 + e) / (2i + 2),   a2 = e (2i + 2 + e) / (2i + 4).
+// For node i = floor(u) (E_z, E_phi, H_r), e = u - i, r2 = (i + 1/2) dr and
+// the volumes CELL_VOLUME(i), CELL_VOLUME(i + 2):
+//   a1 = (1 - e)(2i + e) / (2i + 1),       a2 = e (2i + 1 + e) / (2i + 3).
+// Axial weights are 1 - e_z and e_z for the node below and above.
+// Near the axis (i <= 1) the radial factors are evaluated in FP64.
+// ---------------------------------------------------------------------------
+struct W2 {
+  int i;
+  float a1, a2;
+};
+
+__device__ __forceinline__ W2 rad_half(int cr, float fr) {
+  W2 w;
+  const bool lo = fr < 0.5f;
+  const float e = lo ? fr + 0.5f : fr - 0.5f;
+  w.i = cr - (lo ? 1 : 0);
+  if (w.i <= 1) {
+    double ed = e, id = w.i;
+    w.a1 = (float)((1.0 - ed) * (2 * id + 1 + ed) / (2 * id + 2));
+    w.a2 = (float)(ed * (2 * id + 2 + ed) / (2 * id + 4));
+  } else {
+    float fi = (float)w.i;
+    w.a1 = (1.f - e) * (2.f * fi + 1.f + e) / (2.f * fi + 2.f);
+    w.a2 = e * (2.f * fi + 2.f + e) / (2.f * fi + 4.f);
+  }
+  return w;
+}
+
+__device__ __forceinline__ W2 rad_int(int cr, float fr) {
+  W2 w;
+  const float e = fr;
+  w.i = cr;
+  if (w.i <= 1) {
+    double ed = e, id = w.i;
+    w.a1 = (float)((1.0 - ed) * (2 * id + ed) / (2 * id + 1));
+    w.a2 = (float)(ed * (2 * id + 1 + ed) / (2 * id + 3));
+  } else {
+    float fi = (float)w.i;
+    w.a1 = (1.f - e) * (2.f * fi + e) / (2.f * fi + 1.f);
+    w.a2 = e * (2.f * fi + 1.f + e) / (2.f * fi + 3.f);
+  }
+  return w;
+}
+
+struct Z2 {
+  int k;
+  float w1, w2;
+};
+__device__ __forceinline__ Z2 z_int(int cz, float fz) { return Z2{cz, 1.f - fz, fz}; }
+__device__ __forceinline__ Z2 z_half(int cz, float fz) {
+  const bool lo = fz < 0.5f;
+  const float e = lo ? fz + 0.5f : fz - 0.5f;
+  return Z2{cz - (lo ? 1 : 0), 1.f - e, e};
+}
+
+struct GridView32 {
+  const float *G;   // component 0 of the domain
+  long P;
+  int yr;
+  long bot_r, left_z;
+  __device__ __forceinline__ float at(int c, int i, int k) const {
+    return G[c * P + (long)(i - bot_r + 2) * yr + (k - left_z + 2)];
+  }
+};
+
+// E (maxwellSolver.hpp:211-316) in cell units
+__device__ __forceinline__ void gather32_e(const GridView32 &E, int cr, int cz, float fr, float fz, float out[3]) {
+  W2 ra = rad_half(cr, fr), ri = rad_int(cr, fr);
+  Z2 zi = z_int(cz, fz), zh = z_half(cz, fz);
+  float c0 = 0.f, c1 = 0.f, c2 = 0.f;
+  // E_r: node (floor(u - 1/2), floor(w))
+  c0 += E.at(0, ra.i, zi.k) * (ra.a1 * zi.w1);
+  c0 += E.at(0, ra.i + 1, zi.k) * (ra.a2 * zi.w1);
+  c0 += E.at(0, ra.i, zi.k + 1) * (ra.a1 * zi.w2);
+  c0 += E.at(0, ra.i + 1, zi.k + 1) * (ra.a2 * zi.w2);
+  // E_z: node (floor(u), floor(w - 1/2))
+  c2 += E.at(2, ri.i, zh.k) * (ri.a1 * zh.w1);
+  c2 += E.at(2, ri.i + 1, zh.k) * (ri.a2 * zh.w1);
+  c2 += E.at(2, ri.i, zh.k + 1) * (ri.a1 * zh.w2);
+  c2 += E.at(2, ri.i + 1, zh.k + 1) * (ri.a2 * zh.w2);
+  // E_phi: node (floor(u), floor(w)); the 4th term goes to E_z (quirk Q2)
+  c1 += E.at(1, ri.i, zi.k) * (ri.a1 * zi.w1);
+  c1 += E.at(1, ri.i + 1, zi.k) * (ri.a2 * zi.w1);
+  c1 += E.at(1, ri.i, zi.k + 1) * (ri.a1 * zi.w2);
+  c2 += E.at(1, ri.i + 1, zi.k + 1) * (ri.a2 * zi.w2);
+  out[0] = c0; out[1] = c1; out[2] = c2;
+}
+
+// H (maxwellSolver.hpp:101-209) in cell units
+__device__ __forceinline__ void gather32_h(const GridView32 &H, int cr, int cz, float fr, float fz, float out[3]) {
+  W2 ra = rad_half(cr, fr), ri = rad_int(cr, fr);
+  Z2 zi = z_int(cz, fz), zh = z_half(cz, fz);
+  float c0 = 0.f, c1 = 0.f, c2 = 0.f;
+  // H_z: node (floor(u - 1/2), floor(w))
+  c2 += H.at(2, ra.i, zi.k) * (ra.a1 * zi.w1);
+  c2 += H.at(2, ra.i + 1, zi.k) * (ra.a2 * zi.w1);
+  c2 += H.at(2, ra.i, zi.k + 1) * (ra.a1 * zi.w2);
+  c2 += H.at(2, ra.i + 1, zi.k + 1) * (ra.a2 * zi.w2);
+  // H_r: node (floor(u), floor(w - 1/2))
+  c0 += H.at(0, ri.i, zh.k) * (ri.a1 * zh.w1);
+  c0 += H.at(0, ri.i + 1, zh.k) * (ri.a2 * zh.w1);
+  c0 += H.at(0, ri.i, zh.k + 1) * (ri.a1 * zh.w2);
+  c0 += H.at(0, ri.i + 1, zh.k + 1) * (ri.a2 * zh.w2);
+  // H_phi: node (floor(u - 1/2), floor(w - 1/2))
+  c1 += H.at(1, ra.i, zh.k) * (ra.a1 * zh.w1);
+  c1 += H.at(1, ra.i + 1, zh.k) * (ra.a2 * zh.w1);
+  c1 += H.at(1, ra.i, zh.k + 1) * (ra.a1 * zh.w2);
+  c1 += H.at(1, ra.i + 1, zh.k + 1) * (ra.a2 * zh.w2);
+  out[0] = c0; out[1] = c1; out[2] = c2;
+}
+
+// ---------------------------------------------------------------------------
+// pushers in FP32 (same algebra as pcp_physics.hpp::push_velocity)
+// ---------------------------------------------------------------------------
+struct PushConst {
+  float c2m;     // q dt / (2 m)
+  float c2m_mu;  // q dt / (2 m) * MAGN_CONST
+  float inv_c2;  // 1 / c^2
+};
+
+__device__ __forceinline__ float len2f(const float v[3]) { return v[0] * v[0] + v[1] * v[1] + v[2] * v[2]; }
+
+__device__ __forceinline__ void push32(int pusher, const PushConst &K, float e[3], float b[3], float v[3], int *err) {
+  if (pusher <= PCP_PUSHER_BORIS_ADAPTIVE) {
+    const bool rel = pusher != PCP_PUSHER_BORIS;
+    for (int i = 0; i < 3; ++i) { e[i] *= K.c2m; b[i] *= K.c2m_mu; }
+    if (rel) {
+      float beta = len2f(v) * K.inv_c2;
+      if (beta > 1.f) *err |= ERR_BETA_GT_1;
+      float gamma = 1.f / sqrtf(1.f - beta);
+      for (int i = 0; i < 3; ++i) v[i] *= gamma;
+    }
+    for (int i = 0; i < 3; ++i) v[i] += e[i];
+    if (rel) {
+      float gi = 1.f / sqrtf(1.f + len2f(v) * K.inv_c2);
+      for (int i = 0; i < 3; ++i) b[i] *= gi;
+    }
+    float const2 = 2.f / (1.f + len2f(b));
+    float t0 = v[0], t1 = v[1], t2 = v[2];
+    v[0] = t0 + const2 * ((t1 - t0 * b[2] + t2 * b[0]) * b[2] - (t2 + t0 * b[1] - t1 * b[0]) * b[1]);
+    v[1] = t1 + const2 * (-(t0 + t1 * b[2] - t2 * b[1]) * b[2] + (t2 + t0 * b[1] - t1 * b[0]) * b[0]);
+    v[2] = t2 + const2 * ((t0 + t1 * b[2] - t2 * b[1]) * b[1] - (t1 - t0 * b[2] + t2 * b[0]) * b[0]);
+    for (int i = 0; i < 3; ++i) v[i] += e[i];
+    if (rel) {
+      float gi = 1.f / sqrtf(1.f + len2f(v) * K.inv_c2);
+      for (int i = 0; i < 3; ++i) v[i] *= gi;
+    }
+  } else if (pusher == PCP_PUSHER_VAY) {
+    float beta = len2f(v) * K.inv_c2;
+    if (beta > 1.f) *err |= ERR_BETA_GT_1;
+    float gamma = 1.f / sqrtf(1.f - beta);
+    for (int i = 0; i < 3; ++i) v[i] *= gamma;
+    float up[3];
+    for (int i = 0; i < 3; ++i) { up[i] = v[i] + e[i] * (2.f * K.c2m); b[i] *= K.c2m_mu; }
+    float gi = 1.f / sqrtf(1.f + len2f(v) * K.inv_c2);
+    up[0] += gi * (v[1] * b[2] - v[2] * b[1]);
+    up[1] += gi * (v[2] * b[0] - v[0] * b[2]);
+    up[2] += gi * (v[0] * b[1] - v[1] * b[0]);
+    float alpha = 1.f + len2f(up);   // unit mix of the reference (quirk Q4)
+    float B2 = len2f(b);
+    float s = alpha - B2;
+    float ub = up[0] * b[0] + up[1] * b[1] + up[2] * b[2];
+    alpha = 1.f / sqrtf(0.5f * (s + sqrtf(s * s + 4.f * (B2 + ub * ub))));
+    for (int i = 0; i < 3; ++i) b[i] *= alpha;
+    s = 1.f / (1.f + len2f(b));
+    alpha = up[0] * b[0] + up[1] * b[1] + up[2] * b[2];
+    float psm[3];
+    psm[0] = s * (up[0] + alpha * b[0] + b[2] * up[1] - b[1] * up[2]);
+    psm[1] = s * (up[1] + alpha * b[1] + b[0] * up[2] - b[2] * up[0]);
+    psm[2] = s * (up[2] + alpha * b[2] + b[1] * up[0] - b[0] * up[1]);
+    gi = 1.f / sqrtf(1.f + len2f(psm) * K.inv_c2);
+    for (int i = 0; i < 3; ++i) v[i] = psm[i] * gi;
+  } else {
+    float beta = len2f(v) * K.inv_c2;
+    if (beta > 1.f) *err |= ERR_BETA_GT_1;
+    float gamma = 1.f / sqrtf(1.f - beta);
+    float um[3], psm[3], b2[3], bc[3], up[3];
+    for (int i = 0; i < 3; ++i) {
+      v[i] *= gamma;
+      psm[i] = e[i] * K.c2m;
+      um[i] = v[i] + psm[i];
+      b[i] *= K.c2m_mu;
+    }
+    float gfm2 = 1.f + len2f(um);   // quirk Q5
+    float B2 = len2f(b);
+    float bu = b[0] * um[0] + b[1] * um[1] + b[2] * um[2];
+    float gb = gfm2 - B2;
+    gamma = 1.f / sqrtf(0.5f * (gb + sqrtf(gb * gb + 4.f * (B2 + bu * bu))));
+    for (int i = 0; i < 3; ++i) { b[i] *= gamma; b2[i] = b[i] * b[i]; }
+    bc[0] = b[0] * b[1]; bc[1] = b[1] * b[2]; bc[2] = b[2] * b[0];
+    float inv_det_B = 1.f / (1.f + b2[0] + b2[1] + b2[0]);   // b2[0] twice (pusherHC.cpp:94)
+    up[0] = ((1.f + b2[0] - b2[1] - b2[2]) * um[0] + 2.f * (bc[0] + b[2]) * um[1] + 2.f * (bc[2] - b[2]) * um[2]) * inv_det_B;
+    up[1] = (2.f * (bc[0] - b[2]) * um[0] + (1.f - b2[0] + b2[1] - b2[2]) * um[1] + 2.f * (bc[1] + b[0]) * um[2]) * inv_det_B;
+    up[2] = (2.f * (bc[2] + b[1]) * um[0] + 2.f * (bc[1] - b[0]) * um[1] + (1.f - b2[0] - b2[1] + b2[2]) * um[2]) * inv_det_B;
+    for (int i = 0; i < 3; ++i) psm[i] += up[i];
+    float gi = 1.f / sqrtf(1.f + len2f(psm) * K.inv_c2);
+    for (int i = 0; i < 3; ++i) v[i] = psm[i] * gi;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// move in split form: mover_cylindrical + back_position_to_rz track the new
+// radius as r_old + dr_x + Y^2 / (X + sqrt(X^2 + Y^2)) (X = r + v_r dt,
+// Y = v_phi dt), an increment of the in-cell offset; particles that cross a
+// wall are reflected in FP64 (reflect_plasma / reflect_beam).
+// ---------------------------------------------------------------------------
+struct Split {
+  int cr, cz;
+  float fr, fz;
+};
+
+__device__ __forceinline__ void renorm(int &c, float &f) {
+  float fl = floorf(f);
+  c += (int)fl;
+  f -= fl;
+  if (f >= 1.f) { f -= 1.f; c += 1; }
+}
+
+// returns false when the particle left the simulation area
+__device__ __forceinline__ bool move32(const Geom &g, const Dom &o, int beam, float dtr, float dtz, Split &p, float v[3]) {
+  const Split old = p;
+  float du = v[0] * dtr, dph = v[1] * dtr, dw = v[2] * dtz;
+  float xoff = p.fr + du;
+  float X = (float)p.cr + xoff;
+  float Y = dph;
+  float sn = 0.f;
+  if (Y != 0.f) {
+    float R = sqrtf(X * X + Y * Y);
+    if (X > 0.f) xoff += Y * Y / (X + R);
+    else xoff = R - (float)p.cr;
+    sn = Y / R;
+  } else if (X < 0.f) {
+    xoff = -X - (float)p.cr;   // |X|: the cylinder radius of a particle that crossed the axis
+  }
+  p.fr = xoff;
+  p.fz = p.fz + dw;
+  // walls (local cell units): r < 1/2 or > nr - 1/2, z < 1/2 or > nz - 1/2
+  const float lr = (float)(p.cr - (int)o.bot_r) + p.fr, lz = (float)(p.cz - (int)o.left_z) + p.fz;
+  const bool hit = beam ? (o.w0 && lr < 0.5f)
+                        : ((o.w0 && lr < 0.5f) || (o.w2 && lr > g.nr - 0.5f) || (o.w1 && lz < 0.5f) ||
+                           (o.w3 && lz > g.nz - 0.5f));
+  if (hit) {
+    double pr = ((double)p.cr + (double)p.fr) * g.dr, pz = ((double)p.cz + (double)p.fz) * g.dz;
+    double por = ((double)old.cr + (double)old.fr) * g.dr, poz = ((double)old.cz + (double)old.fz) * g.dz;
+    double vr = v[0], vz = v[2];
+    if (beam) reflect_beam(g, o, pr, vr);
+    else reflect_plasma(g, o, pr, pz, por, poz, vr, vz);
+    v[0] = (float)vr;
+    v[2] = (float)vz;
+    double ur = pr / g.dr, uz = pz / g.dz;
+    double fcr = floor(ur), fcz = floor(uz);
+    p.cr = (int)fcr; p.fr = (float)(ur - fcr);
+    p.cz = (int)fcz; p.fz = (float)(uz - fcz);
+  }
+  renorm(p.cr, p.fr);
+  renorm(p.cz, p.fz);
+  // back_velocity_to_rz
+  float cs = sqrtf(1.f - sn * sn);
+  float vr = v[0], vp = v[1];
+  v[0] = cs * vr - sn * vp;
+  v[1] = sn * vr + cs * vp;
+  return p.cr >= 0 && p.cz >= 0 && p.cr < g.Nr && p.cz < g.Nz;
+}
+
+// ---------------------------------------------------------------------------
+// deposition sinks on FP32 J
+// ---------------------------------------------------------------------------
+struct WaveSink32 {
+  float *J;
+  int base, P, yr, nr, nz;
+  int *err;
+  __device__ __forceinline__ void add_if(int c, int i, int k, float v, bool a) {
+    bool in = i >= -2 && i < nr + 2 && k >= -2 && k < nz + 2;
+    if (a && !in) *err |= ERR_MOVED_TOO_FAR;
+    wave_add(J, base + c * P + (i + 2) * yr + (k + 2), v, a && in);
+  }
+};
+
+struct AtomicSink32 {
+  float *J;
+  long P;
+  int yr, nr, nz;
+  int *err;
+  int *need_exact;
+  __device__ __forceinline__ void add(int c, int i, int k, double v) {
+    if (i < -2 || i >= nr + 2 || k < -2 || k >= nz + 2) { *err |= ERR_MOVED_TOO_FAR; return; }
+    unsafeAtomicAdd(&J[c * P + (long)(i + 2) * yr + (k + 2)], (float)v);
+  }
+  __device__ __forceinline__ void strict_reset(int, int) {
+    if (need_exact) *need_exact = 1;
+  }
+};
+
+struct ExactSink32 {
+  float *J;
+  const uint64_t *marker;
+  long P;
+  int yr, nr, nz;
+  uint64_t stamp, key;
+  int *err;
+  __device__ __forceinline__ void add(int c, int i, int k, double v) {
+    if (i < -2 || i >= nr + 2 || k < -2 || k >= nz + 2) { *err |= ERR_MOVED_TOO_FAR; return; }
+    long o = (long)(i + 2) * yr + (k + 2);
+    if (c == 0) {
+      uint64_t m = marker[o];
+      if ((m >> 40) == stamp && (m & ((1ull << 40) - 1)) > key) return;
+    }
+    unsafeAtomicAdd(&J[c * P + o], (float)v);
+  }
+  __device__ __forceinline__ void strict_reset(int, int) {}
+};
+
+struct MarkSink32 {
+  uint64_t *marker;
+  int yr, nr, nz;
+  uint64_t val;
+  __device__ __forceinline__ void add(int, int, int, double) {}
+  __device__ __forceinline__ void strict_reset(int i, int k) {
+    if (i < -2 || i >= nr + 2 || k < -2 || k >= nz + 2) return;
+    atomicMax((unsigned long long *)&marker[(long)(i + 2) * yr + (k + 2)], (unsigned long long)val);
+  }
+};
+
+// ZigZag (currentZigZag.cpp:22-136) in FP32, positions relative to the old
+// cell: uo = f_old, un = (i_n - i_o) + f_new (likewise in z)
+__device__ __forceinline__ void zigzag32(WaveSink32 &S, bool live, const Geom &g, const Dom &dn, const Split &o,
+                                         const Split &n, float vphi, float qw, float inv_dt) {
+  const float dr = (float)g.dr, dz = (float)g.dz;
+  const int di = n.cr - o.cr, dk = n.cz - o.cz;
+  const float uo = o.fr, un = (float)di + n.fr;
+  const float wo = o.fz, wn = (float)dk + n.fz;
+  // relay points (currentZigZag.hpp:28-30) relative to the old cell
+  const float rr = fminf(fminf(0.f, (float)di) + 1.f, fmaxf(fmaxf(0.f, (float)di), 0.5f * (uo + un)));
+  const float rz = fminf(fminf(0.f, (float)dk) + 1.f, fmaxf(fmaxf(0.f, (float)dk), 0.5f * (wo + wn)));
+  const int i_middle = o.cr + (int)floorf(0.5f * (uo + un));
+  const float one_over_volume =
+      (float)(1.0 / (PI * g.dz * g.dr * g.dr * (2.0 * i_middle + 1)));
+  const float charge_over_dt = qw * inv_dt;
+  const float F_phi = qw * vphi;
+  const float W_phi = vphi * (float)g.dt / (2.f * (float)PI * dr * ((float)(2 * o.cr) + uo + un));
+  const float F_r1 = charge_over_dt * dr * (rr - uo);
+  const float F_r2 = charge_over_dt * dr * (un - rr);
+  const float F_z1 = charge_over_dt * dz * (rz - wo);
+  const float F_z2 = charge_over_dt * dz * (wn - rz);
+  // weights in metres (quirk Q11)
+  const float W_r1 = dr * (0.5f * (uo + rr));
+  const float W_r2 = dr * (0.5f * (un + rr) - (float)di);
+  const float W_z1 = dz * (0.5f * (wo + rz));
+  const float W_z2 = dz * (0.5f * (wn + rz) - (float)dk);
+  const float mWr1 = 1.f - W_r1, mWr2 = 1.f - W_r2, mWz1 = 1.f - W_z1, mWz2 = 1.f - W_z2, mWphi = 1.f - W_phi;
+  const int io = o.cr - (int)dn.bot_r, ko = o.cz - (int)dn.left_z;
+  const int in = n.cr - (int)dn.bot_r, kn = n.cz - (int)dn.left_z;
+  const float A = one_over_volume;
+  S.add_if(0, io, ko, A * F_r1 * mWphi * mWz1 + A * F_r1 * W_phi * mWz1, live);
+  S.add_if(0, io, ko + 1, A * F_r1 * mWphi * W_z1 + A * F_r1 * W_phi * W_z1, live);
+  S.add_if(0, in, kn, A * F_r2 * mWphi * mWz2 + A * F_r2 * W_phi * mWz2, live);
+  S.add_if(0, in, kn + 1, A * F_r2 * mWphi * W_z2 + A * F_r2 * W_phi * W_z2, live);
+  S.add_if(1, in, kn, A * F_phi * mWr2 * mWz2, live);
+  S.add_if(1, in, kn + 1, A * F_phi * mWr2 * W_z1, live);   // uses W_z1 (currentZigZag.cpp:104)
+  S.add_if(1, in + 1, kn, A * F_phi * W_r2 * mWz2, live);
+  S.add_if(1, in + 1, kn + 1, A * F_phi * W_r2 * W_z2, live);
+  S.add_if(2, io, ko, A * F_z1 * mWr1 * mWphi + A * F_z1 * mWr1 * W_phi, live);
+  S.add_if(2, io + 1, ko, A * F_z1 * W_r1 * mWphi + A * F_z1 * W_r1 * W_phi, live);
+  S.add_if(2, in, kn, A * F_z2 * mWr2 * mWphi + A * F_z2 * mWr2 * W_phi, live);
+  S.add_if(2, in + 1, kn, A * F_z2 * W_r2 * mWphi + A * F_z2 * W_r2 * W_phi, live);
+}
+
+__device__ __forceinline__ int domain_of32(const Geom &g, int cr, int cz) { return (cr / g.nr) * g.Dz + (cz / g.nz); }
+
+__device__ __forceinline__ Split unpack(uint32_t cell, float fr, float fz, long Nz) {
+  Split s;
+  s.cr = (int)(cell / (uint32_t)Nz);
+  s.cz = (int)(cell - (uint32_t)s.cr * (uint32_t)Nz);
+  s.fr = fr;
+  s.fz = fz;
+  return s;
+}
+
+__device__ __forceinline__ uint64_t order_key32(int sp, uint64_t seq) { return ((uint64_t)sp << 36) | seq; }
+
+__device__ __forceinline__ void append_migrant32(Migrants mg, StepState *st, int sp, long p, int d_old, int d_new,
+                                                 uint64_t seq, const Geom &g, int *err) {
+  unsigned int slot = atomicAdd(&st->n_mig, 1u);
+  if (slot >= (unsigned long)mg.cap) { *err |= ERR_MIGRANT_OVERFLOW; return; }
+  int I = d_old / g.Dz, J = d_old % g.Dz;
+  uint64_t colour = (uint64_t)((I % 2) * 2 + (J % 2));
+  mg.key[slot] = (colour << 47) | ((uint64_t)d_old << 35) | (seq & ((1ull << 35) - 1));
+  mg.ref[slot] = ((uint64_t)sp << 40) | (uint64_t)p;
+  mg.group[slot] = (uint32_t)(d_new * MAX_SPECIES + sp);
+}
+
+// ---------------------------------------------------------------------------
+// the fused FP32 step kernel
+// ---------------------------------------------------------------------------
+template <int CCS>
+__global__ void __launch_bounds__(256) k32_step_fused(Geom g, FieldsT<float> f, Species32 s, int sp, int cur,
+                                                      StepState *st, Migrants mg, PushConst K) {
+  int err = 0;
+  const int nxt = cur ^ 1;
+  const float dtr = (float)(g.dt / g.dr), dtz = (float)(g.dt / g.dz);
+  const float inv_dt = (float)(1.0 / g.dt);
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long b0 = (long)blockIdx.x * blockDim.x; b0 < s.n; b0 += stride) {
+    const long p = b0 + threadIdx.x;
+    bool live = false;
+    Split o{1, 1, 0.5f, 0.5f}, n{1, 1, 0.5f, 0.5f};
+    float v[3] = {0.f, 0.f, 0.f};
+    float w = 0.f;
+    int d_new = 0;
+    if (p < s.n) {
+      uint32_t cell = s.cell[cur][p];
+      if (cell == DEAD_CELL) {
+        s.cell[nxt][p] = DEAD_CELL;
+      } else {
+        o = unpack(cell, s.fr[cur][p], s.fz[cur][p], g.Nz);
+        const int d_old = domain_of32(g, o.cr, o.cz);
+        const Dom dm = make_dom(g, d_old);
+        GridView32 EV{f.E + (long)d_old * 3 * g.P, g.P, g.yr, dm.bot_r, dm.left_z};
+        GridView32 HV{f.H + (long)d_old * 3 * g.P, g.P, g.yr, dm.bot_r, dm.left_z};
+        float e[3], h[3];
+        gather32_e(EV, o.cr, o.cz, o.fr, o.fz, e);
+        gather32_h(HV, o.cr, o.cz, o.fr, o.fz, h);
+        v[0] = s.vr[p]; v[1] = s.vphi[p]; v[2] = s.vz[p];
+        push32(g.pusher, K, e, h, v, &err);
+        n = o;
+        bool alive = move32(g, dm, s.beam, dtr, dtz, n, v);
+        s.vr[p] = v[0]; s.vphi[p] = v[1]; s.vz[p] = v[2];
+        if (!alive) {
+          s.cell[nxt][p] = DEAD_CELL;
+        } else {
+          s.cell[nxt][p] = (uint32_t)n.cr * (uint32_t)g.Nz + (uint32_t)n.cz;
+          s.fr[nxt][p] = n.fr;
+          s.fz[nxt][p] = n.fz;
+          d_new = domain_of32(g, n.cr, n.cz);
+          if (d_new != d_old) append_migrant32(mg, st, sp, p, d_old, d_new, s.seq[p], g, &err);
+          w = s.w[p];
+          live = true;
+        }
+      }
+    }
+    if (!live) { o = n = Split{1, 1, 0.5f, 0.5f}; d_new = 0; v[1] = 0.f; w = 0.f; }
+    const Dom dn = make_dom(g, d_new);
+    WaveSink32 S{f.J, (int)((long)d_new * 3 * g.P), (int)g.P, g.yr, g.nr, g.nz, &err};
+    const float qw = (float)s.q * w;
+    if (CCS == PCP_CCS_ZIGZAG) {
+      zigzag32(S, live, g, dn, o, n, v[1], qw, inv_dt);
+    } else {
+      // Villasenor-Buneman terms in FP64 from the exact split positions
+      const double r = ((double)n.cr + (double)n.fr) * g.dr, z = ((double)n.cz + (double)n.fz) * g.dz;
+      const double ro = ((double)o.cr + (double)o.fr) * g.dr, zo = ((double)o.cz + (double)o.fz) * g.dz;
+      const double p_charge = s.q * (double)w;
+      AzTerms A = vb_azimuthal_terms(g.nr, g.nz, g.dr, g.dz, dn.bot_r, dn.left_z, r, z, ro, zo, (double)v[1],
+                                     p_charge);
+      S.add_if(1, A.i, A.k, (float)A.w[0], live && A.in[0]);
+      S.add_if(1, A.i + 1, A.k, (float)A.w[1], live && A.in[1]);
+      S.add_if(1, A.i, A.k + 1, (float)A.w[2], live && A.in[2]);
+      S.add_if(1, A.i + 1, A.k + 1, (float)A.w[3], live && A.in[3]);
+      SegRecorder R;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        R.rn[q] = 1.5 * g.dr; R.zn[q] = 1.25 * g.dz; R.ro[q] = 1.25 * g.dr; R.zo[q] = 1.5 * g.dz;
+        R.i[q] = 1; R.k[q] = 1;
+      }
+      if (live) vb_rz_dispatch(R, g.dr, g.dz, r, z, ro, zo, &err);
+      if (live && R.is_strict) {
+        AtomicSink32 A1{f.J + (long)d_new * 3 * g.P, g.P, g.yr, g.nr, g.nz, &err, &st->need_exact};
+        vb_strict(A1, g.dr, g.dz, g.dt, dn.bot_r, dn.left_z, r, z, ro, zo, p_charge);
+      }
+#pragma unroll
+      for (int sg = 0; sg < 3; ++sg) {
+        const bool a = live && !R.is_strict && sg < R.n;
+        if (__ballot(a) == 0) break;
+        VBTerms T = vb_simple_terms(g.dr, g.dz, g.dt, dn.bot_r, dn.left_z, R.rn[sg], R.zn[sg], R.ro[sg], R.zo[sg],
+                                    R.i[sg], R.k[sg], p_charge);
+        const bool va = a && T.valid;
+        S.add_if(2, T.i, T.k, (float)T.jz0, va);
+        S.add_if(2, T.i + 1, T.k, (float)T.jz1, va);
+        S.add_if(0, T.i, T.k, (float)T.jr0, va);
+        S.add_if(0, T.i, T.k + 1, (float)T.jr1, va);
+      }
+    }
+  }
+  if (err) atomicOr(&st->err, err);
+}
+
+// exact path (rare): markers and ordered re-deposition in FP64 arithmetic
+__global__ void __launch_bounds__(256) k32_mark(Geom g, Species32 s, int sp, int cur, StepState *st, uint64_t *marker,
+                                                int only_if_exact) {
+  if (only_if_exact && !st->need_exact) return;
+  const uint64_t stamp = st->step & 0xFFFFFF;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < s.n; p += (long)gridDim.x * blockDim.x) {
+    uint32_t c = s.cell[cur][p], co = s.cell[cur ^ 1][p];
+    if (c == DEAD_CELL || co == DEAD_CELL) continue;
+    Split n = unpack(c, s.fr[cur][p], s.fz[cur][p], g.Nz), o = unpack(co, s.fr[cur ^ 1][p], s.fz[cur ^ 1][p], g.Nz);
+    double r = ((double)n.cr + (double)n.fr) * g.dr, z = ((double)n.cz + (double)n.fz) * g.dz;
+    double ro = ((double)o.cr + (double)o.fr) * g.dr, zo = ((double)o.cz + (double)o.fz) * g.dz;
+    if (!vb_is_strict_reset(g.dr, g.dz, r, z, ro, zo)) continue;
+    int d = domain_of32(g, n.cr, n.cz);
+    Dom dm = make_dom(g, d);
+    MarkSink32 S{marker + (long)d * g.P, g.yr, g.nr, g.nz, (stamp << 40) | order_key32(sp, s.seq[p])};
+    int i_n = cell_number(r, g.dr) - (int)dm.bot_r, k_n = cell_number(z, g.dz) - (int)dm.left_z;
+    S.strict_reset(i_n, k_n);
+    S.strict_reset(i_n, k_n + 1);
+  }
+}
+
+__global__ void __launch_bounds__(256) k32_deposit_exact(Geom g, FieldsT<float> f, Species32 s, int sp, int cur,
+                                                         StepState *st, const uint64_t *marker, int only_if_exact) {
+  if (only_if_exact && !st->need_exact) return;
+  int err = 0;
+  const uint64_t stamp = st->step & 0xFFFFFF;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < s.n; p += (long)gridDim.x * blockDim.x) {
+    uint32_t c = s.cell[cur][p], co = s.cell[cur ^ 1][p];
+    if (c == DEAD_CELL || co == DEAD_CELL) continue;
+    Split n = unpack(c, s.fr[cur][p], s.fz[cur][p], g.Nz), o = unpack(co, s.fr[cur ^ 1][p], s.fz[cur ^ 1][p], g.Nz);
+    double r = ((double)n.cr + (double)n.fr) * g.dr, z = ((double)n.cz + (double)n.fz) * g.dz;
+    double ro = ((double)o.cr + (double)o.fr) * g.dr, zo = ((double)o.cz + (double)o.fz) * g.dz;
+    int d = domain_of32(g, n.cr, n.cz);
+    Dom dm = make_dom(g, d);
+    double w = s.w[p];
+    ExactSink32 S{f.J + (long)d * 3 * g.P, marker + (long)d * g.P, g.P, g.yr, g.nr, g.nz, stamp,
+                  order_key32(sp, s.seq[p]), &err};
+    if (g.ccs == PCP_CCS_VB) {
+      double p_charge = s.q * w;
+      vb_azimuthal(S, g.nr, g.nz, g.dr, g.dz, dm.bot_r, dm.left_z, r, z, ro, zo, (double)s.vphi[p], p_charge);
+      vb_rz(S, g.dr, g.dz, g.dt, dm.bot_r, dm.left_z, r, z, ro, zo, p_charge, &err);
+    } else {
+      zigzag(S, g.dr, g.dz, g.dt, dm.bot_r, dm.left_z, r, z, ro, zo, (double)s.vphi[p], s.q, w);
+    }
+  }
+  if (err) atomicOr(&st->err, err);
+}
+
+__global__ void k32_fill_if_exact(float *p, long n, const StepState *st) {
+  if (!st->need_exact) return;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) p[t] = 0.f;
+}
+
+// ---------------------------------------------------------------------------
+// sort by (domain, cell) and compaction
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t sort_key32(const Geom &g, uint32_t cell) {
+  int cr = (int)(cell / (uint32_t)g.Nz), cz = (int)(cell - (uint32_t)cr * (uint32_t)g.Nz);
+  int d = domain_of32(g, cr, cz);
+  long lr = cr - (long)(d / g.Dz) * g.nr, lz = cz - (long)(d % g.Dz) * g.nz;
+  return (uint32_t)((long)d * g.nr * g.nz + lr * g.nz + lz);
+}
+
+__global__ void __launch_bounds__(256) k32_sort_count(Geom g, Species32 s, int cur, uint32_t *key, uint32_t *count,
+                                                      uint32_t sentinel) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long b0 = (long)blockIdx.x * blockDim.x; b0 < s.n; b0 += stride) {
+    const long p = b0 + threadIdx.x;
+    uint32_t k = sentinel;
+    if (p < s.n) {
+      uint32_t c = s.cell[cur][p];
+      if (c != DEAD_CELL) k = sort_key32(g, c);
+      key[p] = k;
+    }
+    (void)wave_reserve(count, k, p < s.n && k != sentinel);
+  }
+}
+
+__global__ void __launch_bounds__(256) k32_sort_scatter(Species32 s, SortBuffers32 sb, int cur, const uint32_t *key,
+                                                        const uint32_t *offset, uint32_t *fill, uint32_t sentinel) {
+  const int nxt = cur ^ 1;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long b0 = (long)blockIdx.x * blockDim.x; b0 < s.n; b0 += stride) {
+    const long p = b0 + threadIdx.x;
+    uint32_t k = p < s.n ? key[p] : sentinel;
+    const bool a = p < s.n && k != sentinel;
+    uint32_t slot = wave_reserve(fill, a ? k : 0, a);
+    if (a) {
+      long q = (long)offset[k] + slot;
+      s.cell[nxt][q] = s.cell[cur][p];
+      s.fr[nxt][q] = s.fr[cur][p];
+      s.fz[nxt][q] = s.fz[cur][p];
+      sb.vr[q] = s.vr[p];
+      sb.vphi[q] = s.vphi[p];
+      sb.vz[q] = s.vz[p];
+      sb.w[q] = s.w[p];
+      sb.seq[q] = s.seq[p];
+      sb.id[q] = s.id[p];
+    }
+  }
+}
+
+__global__ void k32_sort_tail(Species32 s, const uint32_t *n_live_ptr) {
+  long n_live = (long)*n_live_ptr;
+  for (long p = n_live + (long)blockIdx.x * blockDim.x + threadIdx.x; p < s.n; p += (long)gridDim.x * blockDim.x) {
+    s.cell[0][p] = DEAD_CELL;
+    s.cell[1][p] = DEAD_CELL;
+  }
+}
+
+__global__ void k32_zero_u32(uint32_t *p, long n) {
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) p[t] = 0;
+}
+
+__global__ void k32_count_live(const uint32_t *cell, long n, unsigned long long *out) {
+  unsigned long long c = 0;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long)gridDim.x * blockDim.x)
+    c += cell[p] != DEAD_CELL;
+  atomicAdd(out, c);
+}
+
+// ---------------------------------------------------------------------------
+// synthetic inputs in split form (same sampling as pcp_init.hip)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void philox_round32(uint32_t c[4], uint32_t k[2]) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  uint64_t p0 = (uint64_t)M0 * c[0];
+  uint64_t p1 = (uint64_t)M1 * c[2];
+  uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k[0];
+  uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k[1];
+  c[0] = n0; c[1] = (uint32_t)p1; c[2] = n2; c[3] = (uint32_t)p0;
+}
+
+__device__ __forceinline__ void uniform2_32(uint64_t ctr, uint32_t draw, uint64_t seed, double &u0, double &u1) {
+  uint32_t c[4] = {(uint32_t)ctr, (uint32_t)(ctr >> 32), draw, 0x5eedu};
+  uint32_t k[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  for (int r = 0; r < 10; ++r) {
+    philox_round32(c, k);
+    k[0] += 0x9E3779B9u;
+    k[1] += 0xBB67AE85u;
+  }
+  const double s = 1.0 / 9007199254740992.0;
+  u0 = (double)(((uint64_t)(c[0] >> 5) << 26) | (c[1] >> 6)) * s;
+  u1 = (double)(((uint64_t)(c[2] >> 5) << 26) | (c[3] >> 6)) * s;
+}
+
+__device__ __forceinline__ void normal2_32(uint64_t ctr, uint32_t draw, uint64_t seed, double &n0, double &n1) {
+  double u0, u1;
+  uniform2_32(ctr, draw, seed, u0, u1);
+  double rad = sqrt(-2.0 * log(1.0 - u0));
+  n0 = rad * cos(2.0 * PI * u1);
+  n1 = rad * sin(2.0 * PI * u1);
+}
+
+__device__ __forceinline__ void store_split(Species32 &s, long q, const Geom &g, double r, double z) {
+  double ur = r / g.dr, uz = z / g.dz;
+  double fcr = floor(ur), fcz = floor(uz);
+  float fr = (float)(ur - fcr), fz = (float)(uz - fcz);
+  int cr = (int)fcr, cz = (int)fcz;
+  if (fr >= 1.f) { fr = 0.f; cr += 1; }
+  if (fz >= 1.f) { fz = 0.f; cz += 1; }
+  uint32_t cell = (uint32_t)cr * (uint32_t)g.Nz + (uint32_t)cz;
+  s.cell[0][q] = cell; s.cell[1][q] = cell;
+  s.fr[0][q] = fr; s.fr[1][q] = fr;
+  s.fz[0][q] = fz; s.fz[1][q] = fz;
+}
+
+__global__ void k32_init_plasma(Geom g, Species32 s, long base, long n_new, const long *cell_off, long n_cells,
+                                double weight, double vth, uint64_t seed, const long *dom_off) {
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n_new; p += (long)gridDim.x * blockDim.x) {
+    long lo = 0, hi = n_cells - 1;
+    while (lo < hi) {
+      long mid = (lo + hi + 1) >> 1;
+      if (cell_off[mid] <= p) lo = mid; else hi = mid - 1;
+    }
+    long c = lo;
+    long cells_per_dom = (long)g.nr * g.nz;
+    int d = (int)(c / cells_per_dom);
+    long lc = c - (long)d * cells_per_dom;
+    int li = (int)(lc / g.nz), lk = (int)(lc % g.nz);
+    Dom dm = make_dom(g, d);
+    long gi = dm.bot_r + li, gk = dm.left_z + lk;
+    double u0, u1, n0, n1, n2, n3;
+    uniform2_32((uint64_t)(base + p), 0, seed, u0, u1);
+    normal2_32((uint64_t)(base + p), 1, seed, n0, n1);
+    normal2_32((uint64_t)(base + p), 2, seed, n2, n3);
+    double r0 = gi * g.dr, r1 = (gi + 1) * g.dr;
+    double r = sqrt(r0 * r0 + u0 * (r1 * r1 - r0 * r0));
+    double z = (gk + u1) * g.dz;
+    if (r < r0) r = r0;
+    if (r >= r1) r = nextafter(r1, 0.0);
+    if (z >= (gk + 1) * g.dz) z = nextafter((gk + 1) * g.dz, 0.0);
+    long q = base + p;
+    store_split(s, q, g, r, z);
+    s.vr[q] = (float)(n0 * vth);
+    s.vphi[q] = (float)(n1 * vth);
+    s.vz[q] = (float)(n2 * vth);
+    s.w[q] = (float)weight;
+    s.seq[q] = (uint64_t)(p - dom_off[d]);
+    s.id[q] = (uint64_t)q;
+  }
+}
+
+__global__ void k32_init_bunch(Geom g, Species32 s, long base, long n_new, double sigma_r, double sigma_z, double z0,
+                               double vz, double weight, uint64_t seed, const uint64_t *gbase, int sp) {
+  double R = g.gsize_r, L = g.gsize_z;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n_new; p += (long)gridDim.x * blockDim.x) {
+    double n0, n1, n2, n3;
+    normal2_32((uint64_t)(base + p), 7, seed, n0, n1);
+    normal2_32((uint64_t)(base + p), 8, seed, n2, n3);
+    double x = n0 * sigma_r, y = n1 * sigma_r;
+    double r = sqrt(x * x + y * y);
+    double z = z0 + n2 * sigma_z;
+    double rmin = 0.5 * g.dr, rmax = R - 0.5 * g.dr, zmin = 0.5 * g.dz, zmax = L - 0.5 * g.dz;
+    if (r < rmin) r = rmin;
+    if (r > rmax) r = rmax;
+    if (z < zmin) z = zmin;
+    if (z > zmax) z = zmax;
+    long q = base + p;
+    store_split(s, q, g, r, z);
+    s.vr[q] = 0.f;
+    s.vphi[q] = 0.f;
+    s.vz[q] = (float)vz;
+    s.w[q] = (float)weight;
+    int d = (int)((long)floor(r / g.dr) / g.nr) * g.Dz + (int)((long)floor(z / g.dz) / g.nz);
+    s.seq[q] = gbase[(long)d * MAX_SPECIES + sp] + (uint64_t)p;
+    s.id[q] = (uint64_t)q;
+  }
+}
+
+__global__ void k32_gather_point(Geom g, FieldsT<float> f, int d, double r, double z, int which, double *out) {
+  Dom dm = make_dom(g, d);
+  double ur = r / g.dr, uz = z / g.dz;
+  int cr = (int)floor(ur), cz = (int)floor(uz);
+  float fr = (float)(ur - cr), fz = (float)(uz - cz);
+  GridView32 V{(which == 0 ? f.E : f.H) + (long)d * 3 * g.P, g.P, g.yr, dm.bot_r, dm.left_z};
+  float c[3];
+  if (which == 0) gather32_e(V, cr, cz, fr, fz, c);
+  else gather32_h(V, cr, cz, fr, fz, c);
+  out[0] = c[0]; out[1] = c[1]; out[2] = c[2];
+}
+
+// ---------------------------------------------------------------------------
+// FP64 energy reductions (diagnostics)
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void k_field_energy(Geom g, FieldsT<T> f, double *out) {
+  double acc = 0;
+  long total = (long)g.Dr * g.Dz * g.nr * g.nz;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+    long cells = (long)g.nr * g.nz;
+    int d = (int)(t / cells);
+    long c = t - (long)d * cells;
+    int i = (int)(c / g.nz), k = (int)(c % g.nz);
+    long o = (long)(i + 2) * g.yr + (k + 2);
+    Dom dm = make_dom(g, d);
+    double vol = PI * g.dz * g.dr * g.dr * (2.0 * (double)(i + dm.bot_r) + 1.0);
+    double e2 = 0, h2 = 0;
+    for (int cc = 0; cc < 3; ++cc) {
+      double e = (double)f.E[((long)d * 3 + cc) * g.P + o];
+      double h = (double)f.H[((long)d * 3 + cc) * g.P + o];
+      e2 += e * e;
+      h2 += h * h;
+    }
+    acc += 0.5 * (EPSILON0 * e2 + MAGN_CONST * h2) * vol;
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s2 = blockDim.x / 2; s2 > 0; s2 >>= 1) {
+    if ((int)threadIdx.x < s2) red[threadIdx.x] += red[threadIdx.x + s2];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(out, red[0]);
+}
+
+template <class V>
+__device__ __forceinline__ double kin(double m, double w, V vr, V vp, V vz) {
+  double v2 = (double)vr * vr + (double)vp * vp + (double)vz * vz;
+  double b2 = v2 / LIGHT_VEL_POW_2;
+  // m c^2 (gamma - 1) = m v^2 / (sqrt(1 - b2) (1 + sqrt(1 - b2))), cancellation free
+  double s = sqrt(1.0 - b2);
+  return w * m * v2 / (s * (1.0 + s));
+}
+
+__global__ void k_kinetic_energy64(SpeciesDev s, int cur, double *out) {
+  double acc = 0;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < s.n; p += (long)gridDim.x * blockDim.x)
+    if (s.r[cur][p] >= 0) acc += kin(s.m, s.w[p], s.vr[p], s.vphi[p], s.vz[p]);
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s2 = blockDim.x / 2; s2 > 0; s2 >>= 1) {
+    if ((int)threadIdx.x < s2) red[threadIdx.x] += red[threadIdx.x + s2];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(out, red[0]);
+}
+
+__global__ void k_kinetic_energy32(Species32 s, int cur, double *out) {
+  double acc = 0;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < s.n; p += (long)gridDim.x * blockDim.x)
+    if (s.cell[cur][p] != DEAD_CELL) acc += kin(s.m, (double)s.w[p], s.vr[p], s.vphi[p], s.vz[p]);
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s2 = blockDim.x / 2; s2 > 0; s2 >>= 1) {
+    if ((int)threadIdx.x < s2) red[threadIdx.x] += red[threadIdx.x + s2];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(out, red[0]);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline unsigned grid_for32(long n, int block) { return (unsigned)((n + block - 1) / block); }
+static inline unsigned pgrid32(pcp_ctx *c, long n, int block) {
+  long want = (n + block - 1) / block;
+  long cap = (long)c->n_cu * 16;
+  if (want > cap) want = cap;
+  return (unsigned)(want < 1 ? 1 : want);
+}
+
+template <class T>
+int dalloc32(T **p, size_t n) {
+  if (n == 0) n = 1;
+  if (hipMalloc((void **)p, n * sizeof(T)) != hipSuccess) return -1;
+  if (hipMemset(*p, 0, n * sizeof(T)) != hipSuccess) return -1;
+  return 0;
+}
+
+int alloc32(pcp_ctx *c) {
+  State32 *S = new State32();
+  c->s32 = S;
+  const Geom &g = c->g;
+  long gridn = (long)c->D * 3 * g.P;
+  if (dalloc32(&S->f.E, gridn) || dalloc32(&S->f.H, gridn) || dalloc32(&S->f.Hat, gridn) ||
+      dalloc32(&S->f.J, gridn) || dalloc32(&S->f.eps, (size_t)c->D * g.P) || dalloc32(&S->f.sigma, (size_t)c->D * g.P))
+    return -1;
+  for (int s = 0; s < c->cfg.n_species; ++s) {
+    Species32 &sp = S->sp[s];
+    SortBuffers32 &b = S->sb[s];
+    sp.cap = c->cfg.species[s].capacity;
+    sp.q = c->cfg.species[s].charge;
+    sp.m = c->cfg.species[s].mass;
+    sp.beam = c->cfg.species[s].is_beam;
+    sp.n = 0;
+    if (dalloc32(&sp.cell[0], sp.cap) || dalloc32(&sp.cell[1], sp.cap) || dalloc32(&sp.fr[0], sp.cap) ||
+        dalloc32(&sp.fr[1], sp.cap) || dalloc32(&sp.fz[0], sp.cap) || dalloc32(&sp.fz[1], sp.cap) ||
+        dalloc32(&sp.vr, sp.cap) || dalloc32(&sp.vphi, sp.cap) || dalloc32(&sp.vz, sp.cap) ||
+        dalloc32(&sp.w, sp.cap) || dalloc32(&sp.seq, sp.cap) || dalloc32(&sp.id, sp.cap) ||
+        dalloc32(&b.vr, sp.cap) || dalloc32(&b.vphi, sp.cap) || dalloc32(&b.vz, sp.cap) || dalloc32(&b.w, sp.cap) ||
+        dalloc32(&b.seq, sp.cap) || dalloc32(&b.id, sp.cap))
+      return -1;
+  }
+  return 0;
+}
+
+void free32(pcp_ctx *c) {
+  State32 *S = c->s32;
+  if (!S) return;
+  (void)hipFree(S->f.E); (void)hipFree(S->f.H); (void)hipFree(S->f.Hat); (void)hipFree(S->f.J);
+  (void)hipFree(S->f.eps); (void)hipFree(S->f.sigma);
+  for (int s = 0; s < MAX_SPECIES; ++s) {
+    Species32 &sp = S->sp[s];
+    SortBuffers32 &b = S->sb[s];
+    (void)hipFree(sp.cell[0]); (void)hipFree(sp.cell[1]); (void)hipFree(sp.fr[0]); (void)hipFree(sp.fr[1]);
+    (void)hipFree(sp.fz[0]); (void)hipFree(sp.fz[1]); (void)hipFree(sp.vr); (void)hipFree(sp.vphi);
+    (void)hipFree(sp.vz); (void)hipFree(sp.w); (void)hipFree(sp.seq); (void)hipFree(sp.id);
+    (void)hipFree(b.vr); (void)hipFree(b.vphi); (void)hipFree(b.vz); (void)hipFree(b.w);
+    (void)hipFree(b.seq); (void)hipFree(b.id);
+  }
+  delete S;
+  c->s32 = nullptr;
+}
+
+void launch32_yee_e(pcp_ctx *c, int clear_j) { launch_yee_e_t<float>(c, c->s32->f, clear_j); }
+void launch32_yee_h(pcp_ctx *c) { launch_yee_h_t<float>(c, c->s32->f); }
+void launch32_stitch(pcp_ctx *c, int which) { launch_stitch_t<float>(c, c->s32->f, which); }
+
+static PushConst push_const(const Species32 &s, double dt) {
+  PushConst K;
+  double c2m = s.q * dt / (2 * s.m);
+  K.c2m = (float)c2m;
+  K.c2m_mu = (float)(c2m * MAGN_CONST);
+  K.inv_c2 = (float)(1.0 / LIGHT_VEL_POW_2);
+  return K;
+}
+
+void launch32_step_fused(pcp_ctx *c, int sp) {
+  Species32 &s = c->s32->sp[sp];
+  if (!s.n) return;
+  PushConst K = push_const(s, c->g.dt);
+  if (c->g.ccs == PCP_CCS_ZIGZAG)
+    hipLaunchKernelGGL(k32_step_fused<PCP_CCS_ZIGZAG>, dim3(pgrid32(c, s.n, 256)), dim3(256), 0, c->stream, c->g,
+                       c->s32->f, s, sp, c->cur, c->st, c->mig, K);
+  else
+    hipLaunchKernelGGL(k32_step_fused<PCP_CCS_VB>, dim3(pgrid32(c, s.n, 256)), dim3(256), 0, c->stream, c->g,
+                       c->s32->f, s, sp, c->cur, c->st, c->mig, K);
+}
+
+void launch32_mark(pcp_ctx *c, int sp, int only_if_exact) {
+  Species32 &s = c->s32->sp[sp];
+  if (!s.n || c->g.ccs != PCP_CCS_VB) return;
+  hipLaunchKernelGGL(k32_mark, dim3(pgrid32(c, s.n, 256)), dim3(256), 0, c->stream, c->g, s, sp, c->cur, c->st,
+                     c->marker, only_if_exact);
+}
+
+void launch32_deposit_exact(pcp_ctx *c, int sp, int only_if_exact) {
+  Species32 &s = c->s32->sp[sp];
+  if (!s.n) return;
+  hipLaunchKernelGGL(k32_deposit_exact, dim3(pgrid32(c, s.n, 256)), dim3(256), 0, c->stream, c->g, c->s32->f, s, sp,
+                     c->cur, c->st, c->marker, only_if_exact);
+}
+
+void launch32_zero_j_if_exact(pcp_ctx *c) {
+  long n = (long)c->D * 3 * c->g.P;
+  hipLaunchKernelGGL(k32_fill_if_exact, dim3(pgrid32(c, n, 256)), dim3(256), 0, c->stream, c->s32->f.J, n, c->st);
+}
+
+void launch32_migrant_keys(pcp_ctx *c) {
+  if (c->D == 1) return;
+  SeqTable t;
+  for (int i = 0; i < MAX_SPECIES; ++i) t.p[i] = c->s32->sp[i].seq;
+  launch_migrant_keys_t(c, t);
+}
+
+int launch32_sort(pcp_ctx *c, int sp) {
+  Species32 &s = c->s32->sp[sp];
+  if (s.n == 0) return 0;
+  SortBuffers32 &sb = c->s32->sb[sp];
+  const long ncells = (long)c->D * c->g.nr * c->g.nz;
+  const uint32_t sentinel = (uint32_t)ncells;
+  unsigned grid = pgrid32(c, s.n, 256);
+  unsigned gz = (unsigned)std::min<long>((ncells + 256) / 256, (long)c->n_cu * 16);
+  hipLaunchKernelGGL(k32_zero_u32, dim3(gz), dim3(256), 0, c->stream, c->sort_count, ncells + 1);
+  hipLaunchKernelGGL(k32_zero_u32, dim3(gz), dim3(256), 0, c->stream, c->sort_fill, ncells + 1);
+  hipLaunchKernelGGL(k32_sort_count, dim3(grid), dim3(256), 0, c->stream, c->g, s, c->cur, c->sort_key, c->sort_count,
+                     sentinel);
+  size_t bytes = c->sort_temp_bytes;
+  if (hipcub::DeviceScan::ExclusiveSum(c->sort_temp, bytes, c->sort_count, c->sort_offset, (int)(ncells + 1),
+                                       c->stream) != hipSuccess)
+    return -1;
+  hipLaunchKernelGGL(k32_sort_scatter, dim3(grid), dim3(256), 0, c->stream, s, sb, c->cur, c->sort_key,
+                     c->sort_offset, c->sort_fill, sentinel);
+  std::swap(s.cell[c->cur], s.cell[c->cur ^ 1]);
+  std::swap(s.fr[c->cur], s.fr[c->cur ^ 1]);
+  std::swap(s.fz[c->cur], s.fz[c->cur ^ 1]);
+  std::swap(s.vr, sb.vr);
+  std::swap(s.vphi, sb.vphi);
+  std::swap(s.vz, sb.vz);
+  std::swap(s.w, sb.w);
+  std::swap(s.seq, sb.seq);
+  std::swap(s.id, sb.id);
+  hipLaunchKernelGGL(k32_sort_tail, dim3(grid), dim3(256), 0, c->stream, s, c->sort_offset + ncells);
+  return 1;
+}
+
+void launch32_count_live(pcp_ctx *c, int sp, unsigned long long *dev_out) {
+  Species32 &s = c->s32->sp[sp];
+  hipLaunchKernelGGL(k32_count_live, dim3(pgrid32(c, s.n, 256)), dim3(256), 0, c->stream, s.cell[c->cur], s.n, dev_out);
+}
+
+void launch32_init_plasma(pcp_ctx *c, int sp, long base, long n_new, const long *dev_cell_off, long n_cells,
+                          double weight, double vth, uint64_t seed, const long *dev_dom_off) {
+  unsigned grid = (unsigned)std::min<long>((n_new + 255) / 256, (long)c->n_cu * 32);
+  hipLaunchKernelGGL(k32_init_plasma, dim3(grid ? grid : 1), dim3(256), 0, c->stream, c->g, c->s32->sp[sp], base,
+                     n_new, dev_cell_off, n_cells, weight, vth, seed, dev_dom_off);
+}
+
+void launch32_init_bunch(pcp_ctx *c, int sp, long base, long n_new, double sigma_r, double sigma_z, double z0,
+                         double vz, double weight, uint64_t seed) {
+  unsigned grid = (unsigned)std::min<long>((n_new + 255) / 256, (long)c->n_cu * 32);
+  hipLaunchKernelGGL(k32_init_bunch, dim3(grid ? grid : 1), dim3(256), 0, c->stream, c->g, c->s32->sp[sp], base,
+                     n_new, sigma_r, sigma_z, z0, vz, weight, seed, c->mig.gbase, sp);
+}
+
+void launch32_gather_point(pcp_ctx *c, int d, double r, double z, int which, double *dev_out) {
+  hipLaunchKernelGGL(k32_gather_point, dim3(1), dim3(1), 0, c->stream, c->g, c->s32->f, d, r, z, which, dev_out);
+}
+
+void launch_field_energy(pcp_ctx *c, double *dev_out) {
+  long total = (long)c->D * c->g.nr * c->g.nz;
+  unsigned grid = (unsigned)std::min<long>((total + 255) / 256, (long)c->n_cu * 8);
+  if (c->s32)
+    hipLaunchKernelGGL(k_field_energy<float>, dim3(grid), dim3(256), 0, c->stream, c->g, c->s32->f, dev_out);
+  else
+    hipLaunchKernelGGL(k_field_energy<double>, dim3(grid), dim3(256), 0, c->stream, c->g, c->f, dev_out);
+}
+
+void launch_kinetic_energy(pcp_ctx *c, int sp, double *dev_out) {
+  if (c->s32) {
+    Species32 &s = c->s32->sp[sp];
+    if (!s.n) return;
+    hipLaunchKernelGGL(k_kinetic_energy32, dim3(pgrid32(c, s.n, 256)), dim3(256), 0, c->stream, s, c->cur, dev_out);
+  } else {
+    SpeciesDev &s = c->sp[sp];
+    if (!s.n) return;
+    hipLaunchKernelGGL(k_kinetic_energy64, dim3(pgrid32(c, s.n, 256)), dim3(256), 0, c->stream, s, c->cur, dev_out);
+  }
+}
+
+}  // namespace pcp

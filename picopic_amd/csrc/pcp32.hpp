@@ -1,0 +1,64 @@
+// FP32 split-precision mode (defined in pcp32.hip).
+//
+// Particles: a packed global cell index (u32, r_cell * Nz + z_cell) plus FP32
+// in-cell offsets (0 <= f < 1, in units of the cell size) -- 12 B of position
+// with the absolute resolution of a 2^-24 fraction of a cell everywhere in
+// the grid -- and FP32 velocities and weight: 28 B read, 24 B written per
+// particle-step.  Fields and J are FP32.  FP64 is used for the near-axis
+// radial weights, for wall reflections, for the Villasenor-Buneman terms
+// (their logarithms cancel catastrophically in FP32) and for the energy
+// reductions.
+#pragma once
+
+#include "pcp_internal.hpp"
+
+namespace pcp {
+
+constexpr uint32_t DEAD_CELL = 0xFFFFFFFFu;
+
+struct Species32 {
+  uint32_t *cell[2];
+  float *fr[2], *fz[2];
+  float *vr, *vphi, *vz, *w;
+  uint64_t *seq, *id;
+  long n, cap;
+  double q, m;
+  int beam;
+};
+
+struct SortBuffers32 {
+  float *vr, *vphi, *vz, *w;
+  uint64_t *seq, *id;
+};
+
+// state of an FP32 context (owned by pcp_ctx)
+struct State32 {
+  FieldsT<float> f;
+  Species32 sp[MAX_SPECIES];
+  SortBuffers32 sb[MAX_SPECIES];
+};
+
+int alloc32(pcp_ctx *c);
+void free32(pcp_ctx *c);
+void launch32_yee_e(pcp_ctx *c, int clear_j);
+void launch32_yee_h(pcp_ctx *c);
+void launch32_stitch(pcp_ctx *c, int which);
+void launch32_step_fused(pcp_ctx *c, int sp);
+void launch32_mark(pcp_ctx *c, int sp, int only_if_exact);
+void launch32_deposit_exact(pcp_ctx *c, int sp, int only_if_exact);
+void launch32_zero_j_if_exact(pcp_ctx *c);
+void launch32_migrant_keys(pcp_ctx *c);
+int launch32_sort(pcp_ctx *c, int sp);
+void launch32_count_live(pcp_ctx *c, int sp, unsigned long long *dev_out);
+void launch32_init_plasma(pcp_ctx *c, int sp, long base, long n_new, const long *dev_cell_off, long n_cells,
+                          double weight, double vth, uint64_t seed, const long *dev_dom_off);
+void launch32_init_bunch(pcp_ctx *c, int sp, long base, long n_new, double sigma_r, double sigma_z, double z0,
+                         double vz, double weight, uint64_t seed);
+void launch32_gather_point(pcp_ctx *c, int d, double r, double z, int which, double *dev_out);
+// FP64 reductions: field energy sum over interior nodes of
+// eps0 E^2 / 2 + mu0 H^2 / 2 weighted by the cell volume; kinetic energy
+// sum of w m c^2 (gamma - 1) of a species
+void launch_field_energy(pcp_ctx *c, double *dev_out);
+void launch_kinetic_energy(pcp_ctx *c, int sp, double *dev_out);
+
+}  // namespace pcp

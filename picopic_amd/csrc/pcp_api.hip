@@ -13,6 +13,7 @@
 #include "pcp_internal.hpp"
 #include "pcp_kernels.hpp"
 #include "pcp_sort.hpp"
+#include "pcp32.hpp"
 
 using namespace pcp;
 
@@ -139,7 +140,8 @@ int pcp_create(const pcp_config *cfg, pcp_ctx **out) {
   if (cfg->ccs != PCP_CCS_VB && cfg->ccs != PCP_CCS_ZIGZAG) return fail(PCP_E_ARG, "unknown ccs");
   if (cfg->precision != PCP_PREC_FP64 && cfg->precision != PCP_PREC_FP32_SPLIT)
     return fail(PCP_E_ARG, "unknown precision");
-  if (cfg->precision == PCP_PREC_FP32_SPLIT) return fail(PCP_E_ARG, "FP32 split contexts are created by pcp32_create");
+  if (cfg->precision == PCP_PREC_FP32_SPLIT && (long)cfg->grid_r * cfg->grid_z >= 0xFFFFFFFFL)
+    return fail(PCP_E_ARG, "FP32 split mode packs the cell index in 32 bits");
   if (cfg->domains_r * cfg->domains_z > 4096) return fail(PCP_E_ARG, "at most 4096 domains");
   if (!(cfg->dt > 0)) return fail(PCP_E_ARG, "dt must be positive");
 
@@ -182,14 +184,19 @@ int pcp_create(const pcp_config *cfg, pcp_ctx **out) {
     delete c;
     return fail(PCP_E_HIP, "hipStreamCreate failed");
   }
-  long gridn = (long)c->D * 3 * g.P;
+  const bool f32 = cfg->precision == PCP_PREC_FP32_SPLIT;
+  long gridn = f32 ? 0 : (long)c->D * 3 * g.P;
+  long gp = f32 ? 0 : (long)c->D * g.P;
   int rc = 0;
   if ((rc = dalloc(&c->f.E, gridn)) || (rc = dalloc(&c->f.H, gridn)) || (rc = dalloc(&c->f.Hat, gridn)) ||
-      (rc = dalloc(&c->f.J, gridn)) || (rc = dalloc(&c->f.eps, (size_t)c->D * g.P)) ||
-      (rc = dalloc(&c->f.sigma, (size_t)c->D * g.P)) || (rc = dalloc(&c->st, 1)) ||
-      (rc = dalloc(&c->marker, (size_t)c->D * g.P))) {
+      (rc = dalloc(&c->f.J, gridn)) || (rc = dalloc(&c->f.eps, gp)) || (rc = dalloc(&c->f.sigma, gp)) ||
+      (rc = dalloc(&c->st, 1)) || (rc = dalloc(&c->marker, (size_t)c->D * g.P))) {
     pcp_destroy(c);
     return rc;
+  }
+  if (f32 && alloc32(c) != 0) {
+    pcp_destroy(c);
+    return fail(PCP_E_NOMEM, "FP32 split buffers");
   }
   // epsilon = EPSILON0 everywhere (maxwellSolver.hpp:60-62); sigma per PML
   {
@@ -211,8 +218,16 @@ int pcp_create(const pcp_config *cfg, pcp_ctx **out) {
         set_pml_host(c, d, psz, sig.data() + (size_t)d * g.P);
       }
     }
-    if (hipMemcpy(c->f.eps, eps.data(), eps.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(c->f.sigma, sig.data(), sig.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+    bool ok;
+    if (f32) {
+      std::vector<float> e32(eps.begin(), eps.end()), s32(sig.begin(), sig.end());
+      ok = hipMemcpy(c->s32->f.eps, e32.data(), e32.size() * sizeof(float), hipMemcpyHostToDevice) == hipSuccess &&
+           hipMemcpy(c->s32->f.sigma, s32.data(), s32.size() * sizeof(float), hipMemcpyHostToDevice) == hipSuccess;
+    } else {
+      ok = hipMemcpy(c->f.eps, eps.data(), eps.size() * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+           hipMemcpy(c->f.sigma, sig.data(), sig.size() * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
+    }
+    if (!ok) {
       pcp_destroy(c);
       return fail(PCP_E_HIP, "upload of epsilon/sigma failed");
     }
@@ -231,6 +246,7 @@ int pcp_create(const pcp_config *cfg, pcp_ctx **out) {
       return fail(PCP_E_ARG, "species capacity must be >= 0 and mass > 0");
     }
     total_cap += sp.cap;
+    if (f32) continue;
     if ((rc = dalloc(&sp.r[0], sp.cap)) || (rc = dalloc(&sp.r[1], sp.cap)) || (rc = dalloc(&sp.z[0], sp.cap)) ||
         (rc = dalloc(&sp.z[1], sp.cap)) || (rc = dalloc(&sp.vr, sp.cap)) || (rc = dalloc(&sp.vphi, sp.cap)) ||
         (rc = dalloc(&sp.vz, sp.cap)) || (rc = dalloc(&sp.w, sp.cap)) || (rc = dalloc(&sp.seq, sp.cap)) ||
@@ -245,12 +261,13 @@ int pcp_create(const pcp_config *cfg, pcp_ctx **out) {
     SpeciesDev &sp = c->sp[s];
     SortBuffers &b = c->sortb[s];
     max_cap = std::max(max_cap, sp.cap);
+    c->sort_every[s] = sp.beam ? 1 : 50;
+    if (f32) continue;
     if ((rc = dalloc(&b.vr, sp.cap)) || (rc = dalloc(&b.vphi, sp.cap)) || (rc = dalloc(&b.vz, sp.cap)) ||
         (rc = dalloc(&b.w, sp.cap)) || (rc = dalloc(&b.seq, sp.cap)) || (rc = dalloc(&b.id, sp.cap))) {
       pcp_destroy(c);
       return rc;
     }
-    c->sort_every[s] = sp.beam ? 1 : 50;
   }
   {
     long ncells = (long)c->D * g.nr * g.nz;
@@ -295,6 +312,7 @@ int pcp_destroy(pcp_ctx *c) {
   Migrants &m = c->mig;
   hipFree(m.key); hipFree(m.ref); hipFree(m.group); hipFree(m.bucket);
   hipFree(m.gcount); hipFree(m.gstart); hipFree(m.gfill); hipFree(m.gbase);
+  free32(c);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -332,6 +350,37 @@ int pcp_upload_species(pcp_ctx *c, int sp, long n, const double *r, const double
     seq[p] = gbase[(size_t)d * MAX_SPECIES + sp]++;
     ids[p] = id ? id[p] : (uint64_t)(c->sp[sp].n + p);
   }
+  if (c->s32) {
+    Species32 &s = c->s32->sp[sp];
+    std::vector<uint32_t> cell(n);
+    std::vector<float> fr(n), fz(n), fvr(n), fvp(n), fvz(n), fw(n);
+    for (long p = 0; p < n; ++p) {
+      double ur = r[p] / g.dr, uz = z[p] / g.dz;
+      double ir = std::floor(ur), iz = std::floor(uz);
+      float a = (float)(ur - ir), b = (float)(uz - iz);
+      if (a >= 1.f) { a = 0.f; ir += 1; }
+      if (b >= 1.f) { b = 0.f; iz += 1; }
+      cell[p] = (uint32_t)((long)ir * g.Nz + (long)iz);
+      fr[p] = a; fz[p] = b;
+      fvr[p] = (float)vr[p]; fvp[p] = (float)vphi[p]; fvz[p] = (float)vz[p]; fw[p] = (float)w[p];
+    }
+    size_t off = (size_t)s.n, fb = (size_t)n * sizeof(float);
+    for (int k = 0; k < 2; ++k) {
+      HIP_TRY(hipMemcpy(s.cell[k] + off, cell.data(), (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(s.fr[k] + off, fr.data(), fb, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(s.fz[k] + off, fz.data(), fb, hipMemcpyHostToDevice));
+    }
+    HIP_TRY(hipMemcpy(s.vr + off, fvr.data(), fb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s.vphi + off, fvp.data(), fb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s.vz + off, fvz.data(), fb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s.w + off, fw.data(), fb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s.seq + off, seq.data(), (size_t)n * sizeof(uint64_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s.id + off, ids.data(), (size_t)n * sizeof(uint64_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->mig.gbase, gbase.data(), gbase.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    s.n += n;
+    c->sp[sp].n = s.n;
+    return 0;
+  }
   SpeciesDev &s = c->sp[sp];
   size_t off = (size_t)s.n, bytes = (size_t)n * sizeof(double);
   HIP_TRY(hipMemcpy(s.r[0] + off, r, bytes, hipMemcpyHostToDevice));
@@ -355,7 +404,8 @@ int pcp_count(pcp_ctx *c, int sp, long *n) {
   unsigned long long *d = nullptr, h = 0;
   HIP_TRY(hipMalloc(&d, sizeof h));
   HIP_TRY(hipMemsetAsync(d, 0, sizeof h, c->stream));
-  launch_count_live(c, sp, d);
+  if (c->s32) launch32_count_live(c, sp, d);
+  else launch_count_live(c, sp, d);
   HIP_TRY(hipMemcpyAsync(&h, d, sizeof h, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   hipFree(d);
@@ -366,6 +416,65 @@ int pcp_count(pcp_ctx *c, int sp, long *n) {
 int pcp_download_species(pcp_ctx *c, int sp, double *r, double *z, double *vr, double *vphi, double *vz, double *w,
                          uint64_t *id, uint64_t *order, double *r_old, double *z_old) {
   if (!c || sp < 0 || sp >= c->cfg.n_species) return fail(PCP_E_ARG, "bad argument");
+  if (c->s32) {
+    Species32 &s = c->s32->sp[sp];
+    long n = s.n;
+    if (n == 0) return 0;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const Geom &g = c->g;
+    std::vector<uint32_t> cell(n), cello(n);
+    std::vector<float> a(n), b(n), ao(n), bo(n), t(n);
+    std::vector<uint64_t> hu(n);
+    HIP_TRY(hipMemcpy(cell.data(), s.cell[c->cur], n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(a.data(), s.fr[c->cur], n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(b.data(), s.fz[c->cur], n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(cello.data(), s.cell[c->cur ^ 1], n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(ao.data(), s.fr[c->cur ^ 1], n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(bo.data(), s.fz[c->cur ^ 1], n * 4, hipMemcpyDeviceToHost));
+    auto pos = [&](uint32_t cl, float fa, float fb, double &pr, double &pz) {
+      long cr = (long)(cl / (uint32_t)g.Nz), cz = (long)(cl % (uint32_t)g.Nz);
+      pr = ((double)cr + (double)fa) * g.dr;
+      pz = ((double)cz + (double)fb) * g.dz;
+    };
+    long k = 0;
+    for (long p = 0; p < n; ++p) {
+      if (cell[p] == DEAD_CELL) continue;
+      double pr, pz, por = 0, poz = 0;
+      pos(cell[p], a[p], b[p], pr, pz);
+      if (cello[p] != DEAD_CELL) pos(cello[p], ao[p], bo[p], por, poz);
+      if (r) r[k] = pr;
+      if (z) z[k] = pz;
+      if (r_old) r_old[k] = por;
+      if (z_old) z_old[k] = poz;
+      ++k;
+    }
+    const float *srcs[4] = {s.vr, s.vphi, s.vz, s.w};
+    double *dsts[4] = {vr, vphi, vz, w};
+    for (int q = 0; q < 4; ++q) {
+      if (!dsts[q]) continue;
+      HIP_TRY(hipMemcpy(t.data(), srcs[q], n * 4, hipMemcpyDeviceToHost));
+      long kk = 0;
+      for (long p = 0; p < n; ++p)
+        if (cell[p] != DEAD_CELL) dsts[q][kk++] = (double)t[p];
+    }
+    if (id) {
+      HIP_TRY(hipMemcpy(hu.data(), s.id, n * 8, hipMemcpyDeviceToHost));
+      long kk = 0;
+      for (long p = 0; p < n; ++p)
+        if (cell[p] != DEAD_CELL) id[kk++] = hu[p];
+    }
+    if (order) {
+      HIP_TRY(hipMemcpy(hu.data(), s.seq, n * 8, hipMemcpyDeviceToHost));
+      long kk = 0;
+      for (long p = 0; p < n; ++p)
+        if (cell[p] != DEAD_CELL) {
+          long cr = (long)(cell[p] / (uint32_t)g.Nz), cz = (long)(cell[p] % (uint32_t)g.Nz);
+          uint64_t d = (uint64_t)((cr / g.nr) * g.Dz + cz / g.nz);
+          order[kk++] = (d << 40) | hu[p];
+        }
+    }
+    return 0;
+  }
   SpeciesDev &s = c->sp[sp];
   long n = s.n;
   if (n == 0) return 0;
@@ -453,14 +562,16 @@ int pcp_init_plasma(pcp_ctx *c, int sp, int n_per_cell, double density_left, dou
   HIP_TRY(hipMalloc(&d_doff, sizeof(long) * c->D));
   HIP_TRY(hipMemcpy(d_off, off.data(), sizeof(long) * (ncell + 1), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(d_doff, doff.data(), sizeof(long) * c->D, hipMemcpyHostToDevice));
-  launch_init_plasma(c, sp, c->sp[sp].n, total, d_off, ncell, weight, vth, vth, vth, seed, d_doff);
+  if (c->s32) launch32_init_plasma(c, sp, c->s32->sp[sp].n, total, d_off, ncell, weight, vth, seed, d_doff);
+  else launch_init_plasma(c, sp, c->sp[sp].n, total, d_off, ncell, weight, vth, vth, vth, seed, d_doff);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(c->stream));
   hipFree(d_off);
   hipFree(d_doff);
   HIP_TRY(hipMemcpy(c->mig.gbase, gbase.data(), gbase.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
-  // the kernel wrote both position buffers; keep the current one current
+  // the kernel wrote both position buffers
   c->sp[sp].n += total;
+  if (c->s32) c->s32->sp[sp].n = c->sp[sp].n;
   return 0;
 }
 
@@ -474,7 +585,8 @@ int pcp_init_bunch(pcp_ctx *c, int sp, long n, double n_peak, double sigma_r, do
   double real = n_peak * std::pow(2 * PI, 1.5) * sigma_r * sigma_r * sigma_z;
   double weight = real / n;
   double vz = LIGHT_VEL * std::sqrt(1.0 - 1.0 / (gamma * gamma));
-  launch_init_bunch(c, sp, c->sp[sp].n, n, sigma_r, sigma_z, z0, vz, weight, seed);
+  if (c->s32) launch32_init_bunch(c, sp, c->s32->sp[sp].n, n, sigma_r, sigma_z, z0, vz, weight, seed);
+  else launch_init_bunch(c, sp, c->sp[sp].n, n, sigma_r, sigma_z, z0, vz, weight, seed);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(c->stream));
   // order keys of the bunch are gbase + index: advance every domain's base
@@ -483,12 +595,37 @@ int pcp_init_bunch(pcp_ctx *c, int sp, long n, double n_peak, double sigma_r, do
   for (int d = 0; d < c->D; ++d) gbase[(size_t)d * MAX_SPECIES + sp] += (uint64_t)n;
   HIP_TRY(hipMemcpy(c->mig.gbase, gbase.data(), gbase.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
   c->sp[sp].n += n;
+  if (c->s32) c->s32->sp[sp].n = c->sp[sp].n;
   return 0;
+}
+
+static float *grid_ptr32(pcp_ctx *c, int which, int *ncomp) {
+  *ncomp = 3;
+  FieldsT<float> &f = c->s32->f;
+  switch (which) {
+    case PCP_GRID_E: return f.E;
+    case PCP_GRID_H: return f.H;
+    case PCP_GRID_H_AT_ET: return f.Hat;
+    case PCP_GRID_J: return f.J;
+    case PCP_GRID_EPS: *ncomp = 1; return f.eps;
+    case PCP_GRID_SIGMA: *ncomp = 1; return f.sigma;
+    default: return nullptr;
+  }
 }
 
 int pcp_download_grid(pcp_ctx *c, int which, double *host) {
   if (!c || !host) return fail(PCP_E_ARG, "bad argument");
   int nc;
+  if (c->s32) {
+    float *p = grid_ptr32(c, which, &nc);
+    if (!p) return fail(PCP_E_ARG, "unknown grid");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    size_t n = (size_t)c->D * nc * c->g.P;
+    std::vector<float> t(n);
+    HIP_TRY(hipMemcpy(t.data(), p, n * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) host[i] = t[i];
+    return 0;
+  }
   double *p = grid_ptr(c, which, &nc);
   if (!p) return fail(PCP_E_ARG, "unknown grid");
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -499,6 +636,15 @@ int pcp_download_grid(pcp_ctx *c, int which, double *host) {
 int pcp_upload_grid(pcp_ctx *c, int which, const double *host) {
   if (!c || !host) return fail(PCP_E_ARG, "bad argument");
   int nc;
+  if (c->s32) {
+    float *p = grid_ptr32(c, which, &nc);
+    if (!p) return fail(PCP_E_ARG, "unknown grid");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    size_t n = (size_t)c->D * nc * c->g.P;
+    std::vector<float> t(host, host + n);
+    HIP_TRY(hipMemcpy(p, t.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    return 0;
+  }
   double *p = grid_ptr(c, which, &nc);
   if (!p) return fail(PCP_E_ARG, "unknown grid");
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -508,14 +654,16 @@ int pcp_upload_grid(pcp_ctx *c, int which, const double *host) {
 
 int pcp_yee_e(pcp_ctx *c) {
   if (!c) return fail(PCP_E_ARG, "null context");
-  launch_yee_e(c, 0);
+  if (c->s32) launch32_yee_e(c, 0);
+  else launch_yee_e(c, 0);
   HIP_TRY(hipGetLastError());
   return pcp_sync(c);
 }
 
 int pcp_yee_h(pcp_ctx *c) {
   if (!c) return fail(PCP_E_ARG, "null context");
-  launch_yee_h(c);
+  if (c->s32) launch32_yee_h(c);
+  else launch_yee_h(c);
   HIP_TRY(hipGetLastError());
   return pcp_sync(c);
 }
@@ -524,13 +672,15 @@ int pcp_stitch(pcp_ctx *c, int which) {
   if (!c) return fail(PCP_E_ARG, "null context");
   if (which != PCP_GRID_E && which != PCP_GRID_H && which != PCP_GRID_H_AT_ET && which != PCP_GRID_J)
     return fail(PCP_E_ARG, "stitch: which must be E, H or J");
-  launch_stitch(c, which);
+  if (c->s32) launch32_stitch(c, which);
+  else launch_stitch(c, which);
   HIP_TRY(hipGetLastError());
   return pcp_sync(c);
 }
 
 int pcp_push(pcp_ctx *c) {
   if (!c) return fail(PCP_E_ARG, "null context");
+  if (c->s32) return fail(PCP_E_ARG, "the phase entry points run FP64 contexts; FP32 split contexts advance with pcp_step");
   for (int s = 0; s < c->cfg.n_species; ++s) launch_push(c, s);
   HIP_TRY(hipGetLastError());
   return pcp_sync(c);
@@ -538,6 +688,7 @@ int pcp_push(pcp_ctx *c) {
 
 int pcp_move(pcp_ctx *c) {
   if (!c) return fail(PCP_E_ARG, "null context");
+  if (c->s32) return fail(PCP_E_ARG, "the phase entry points run FP64 contexts; FP32 split contexts advance with pcp_step");
   for (int s = 0; s < c->cfg.n_species; ++s) launch_move(c, s);
   HIP_TRY(hipGetLastError());
   c->cur ^= 1;
@@ -546,6 +697,7 @@ int pcp_move(pcp_ctx *c) {
 
 int pcp_migrate(pcp_ctx *c) {
   if (!c) return fail(PCP_E_ARG, "null context");
+  if (c->s32) return fail(PCP_E_ARG, "the phase entry points run FP64 contexts; FP32 split contexts advance with pcp_step");
   launch_step_begin(c);   // resets the migrant counter (and advances the stamp)
   for (int s = 0; s < c->cfg.n_species; ++s) launch_migrate_mark(c, s);
   launch_migrant_keys(c);
@@ -555,6 +707,7 @@ int pcp_migrate(pcp_ctx *c) {
 
 int pcp_deposit(pcp_ctx *c) {
   if (!c) return fail(PCP_E_ARG, "null context");
+  if (c->s32) return fail(PCP_E_ARG, "the phase entry points run FP64 contexts; FP32 split contexts advance with pcp_step");
   launch_stamp(c);
   launch_fill(c, c->f.J, (long)c->D * 3 * c->g.P, 0.0);   // Domain::reset_current
   for (int s = 0; s < c->cfg.n_species; ++s) launch_mark(c, s, 0);
@@ -577,27 +730,47 @@ int pcp_step(pcp_ctx *c, int n_steps) {
   };
   for (int t = 0; t < n_steps; ++t) {
     mark(t, 0);
+    const bool f32 = c->s32 != nullptr;
     launch_step_begin(c);
-    launch_yee_e(c, 1);
-    launch_stitch(c, PCP_GRID_E);
-    launch_yee_h(c);
-    launch_stitch(c, PCP_GRID_H);
+    if (f32) {
+      launch32_yee_e(c, 1);
+      launch32_stitch(c, PCP_GRID_E);
+      launch32_yee_h(c);
+      launch32_stitch(c, PCP_GRID_H);
+    } else {
+      launch_yee_e(c, 1);
+      launch_stitch(c, PCP_GRID_E);
+      launch_yee_h(c);
+      launch_stitch(c, PCP_GRID_H);
+    }
     for (int s = 0; s < c->cfg.n_species; ++s)
-      if (c->sort_every[s] > 0 && c->steps_done % c->sort_every[s] == 0 && launch_sort(c, s) < 0)
+      if (c->sort_every[s] > 0 && c->steps_done % c->sort_every[s] == 0 &&
+          (f32 ? launch32_sort(c, s) : launch_sort(c, s)) < 0)
         return fail(PCP_E_HIP, "sort failed");
     mark(t, 1);
-    for (int s = 0; s < c->cfg.n_species; ++s) launch_step_fused(c, s);
+    for (int s = 0; s < c->cfg.n_species; ++s) {
+      if (f32) launch32_step_fused(c, s);
+      else launch_step_fused(c, s);
+    }
     c->cur ^= 1;
     mark(t, 2);
-    launch_migrant_keys(c);
+    if (f32) launch32_migrant_keys(c);
+    else launch_migrant_keys(c);
     // exact re-deposition when a strict axial motion reset J_r nodes
     if (c->g.ccs == PCP_CCS_VB) {
-      launch_zero_j_if_exact(c);
-      for (int s = 0; s < c->cfg.n_species; ++s) launch_mark(c, s, 1);
-      for (int s = 0; s < c->cfg.n_species; ++s) launch_deposit_exact(c, s, 1);
+      if (f32) {
+        launch32_zero_j_if_exact(c);
+        for (int s = 0; s < c->cfg.n_species; ++s) launch32_mark(c, s, 1);
+        for (int s = 0; s < c->cfg.n_species; ++s) launch32_deposit_exact(c, s, 1);
+      } else {
+        launch_zero_j_if_exact(c);
+        for (int s = 0; s < c->cfg.n_species; ++s) launch_mark(c, s, 1);
+        for (int s = 0; s < c->cfg.n_species; ++s) launch_deposit_exact(c, s, 1);
+      }
     }
     mark(t, 3);
-    launch_stitch(c, PCP_GRID_J);
+    if (f32) launch32_stitch(c, PCP_GRID_J);
+    else launch_stitch(c, PCP_GRID_J);
     mark(t, 4);
     ++c->steps_done;
   }
@@ -625,7 +798,7 @@ int pcp_step(pcp_ctx *c, int n_steps) {
 
 int pcp_sort(pcp_ctx *c, int sp) {
   if (!c || sp < 0 || sp >= c->cfg.n_species) return fail(PCP_E_ARG, "bad argument");
-  if (launch_sort(c, sp) < 0) return fail(PCP_E_HIP, "sort failed");
+  if ((c->s32 ? launch32_sort(c, sp) : launch_sort(c, sp)) < 0) return fail(PCP_E_HIP, "sort failed");
   HIP_TRY(hipGetLastError());
   return pcp_sync(c);
 }
@@ -646,7 +819,8 @@ int pcp_get_field_e(pcp_ctx *c, int d, double r, double z, double *out) {
   }
   double *dev;
   HIP_TRY(hipMalloc(&dev, 3 * sizeof(double)));
-  launch_gather_point(c, d, r, z, 0, dev);
+  if (c->s32) launch32_gather_point(c, d, r, z, 0, dev);
+  else launch_gather_point(c, d, r, z, 0, dev);
   HIP_TRY(hipMemcpyAsync(out, dev, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   hipFree(dev);
@@ -663,10 +837,35 @@ int pcp_get_field_h(pcp_ctx *c, int d, double r, double z, double *out) {
   }
   double *dev;
   HIP_TRY(hipMalloc(&dev, 3 * sizeof(double)));
-  launch_gather_point(c, d, r, z, 1, dev);
+  if (c->s32) launch32_gather_point(c, d, r, z, 1, dev);
+  else launch_gather_point(c, d, r, z, 1, dev);
   HIP_TRY(hipMemcpyAsync(out, dev, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   hipFree(dev);
+  return 0;
+}
+
+int pcp_field_energy(pcp_ctx *c, double *out) {
+  if (!c || !out) return fail(PCP_E_ARG, "bad argument");
+  double *d;
+  HIP_TRY(hipMalloc(&d, sizeof(double)));
+  HIP_TRY(hipMemsetAsync(d, 0, sizeof(double), c->stream));
+  launch_field_energy(c, d);
+  HIP_TRY(hipMemcpyAsync(out, d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  (void)hipFree(d);
+  return 0;
+}
+
+int pcp_kinetic_energy(pcp_ctx *c, int sp, double *out) {
+  if (!c || !out || sp < 0 || sp >= c->cfg.n_species) return fail(PCP_E_ARG, "bad argument");
+  double *d;
+  HIP_TRY(hipMalloc(&d, sizeof(double)));
+  HIP_TRY(hipMemsetAsync(d, 0, sizeof(double), c->stream));
+  launch_kinetic_energy(c, sp, d);
+  HIP_TRY(hipMemcpyAsync(out, d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  (void)hipFree(d);
   return 0;
 }
 

@@ -84,10 +84,12 @@ __host__ __device__ inline long gidx(const Geom &g, int d, int comp, int ncomp, 
   return ((long)d * ncomp + comp) * g.P + (long)(i + 2) * g.yr + (k + 2);
 }
 
-struct Fields {
-  double *E, *H, *Hat, *J;   // [D][3][P]
-  double *eps, *sigma;       // [D][P]
+template <class T>
+struct FieldsT {
+  T *E, *H, *Hat, *J;   // [D][3][P]
+  T *eps, *sigma;       // [D][P]
 };
+using Fields = FieldsT<double>;
 
 struct SpeciesDev {
   double *r[2], *z[2];
@@ -125,6 +127,8 @@ struct Migrants {
   int ngroups;
 };
 
+struct State32;   // FP32 split-precision state (pcp32.hpp)
+
 }  // namespace pcp
 
 struct pcp_ctx {
@@ -145,6 +149,7 @@ struct pcp_ctx {
   long sort_key_cap;
   int sort_every[pcp::MAX_SPECIES]; // 0: never; k: before every k-th step
   long steps_done;
+  pcp::State32 *s32 = nullptr;    // non-null in FP32 split mode
   int n_cu;
   int timing;
   std::vector<std::pair<std::string, double>> ktimes;

@@ -93,18 +93,21 @@ __device__ __forceinline__ uint64_t order_key(int sp, uint64_t seq) { return ((u
 // ---------------------------------------------------------------------------
 // Yee E: maxwellSolverYee.cpp:161-257. Halo nodes are zeroed
 // (field_e.overlay_set(0)); with clear_j the J node is zeroed after it is
-// read (Domain::reset_current folded into the only reader of J).
+// read (Domain::reset_current folded into the only reader of J).  T is the
+// storage/arithmetic type: double reproduces the reference operation for
+// operation, float is the FP32 split-precision mode.
 // ---------------------------------------------------------------------------
-__global__ void k_yee_e(Geom g, Fields f, int clear_j, StepState *st) {
+template <class T>
+__global__ void k_yee_e(Geom g, FieldsT<T> f, int clear_j, StepState *st) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long total = (long)g.Dr * g.Dz * g.P;
   if (t >= total) return;
   int d = (int)(t / g.P);
   long o = t - (long)d * g.P;
   int i = (int)(o / g.yr) - 2, k = (int)(o % g.yr) - 2;
-  double *E = f.E + (long)d * 3 * g.P;
-  double *J = f.J + (long)d * 3 * g.P;
-  const double *Hat = f.Hat + (long)d * 3 * g.P;
+  T *E = f.E + (long)d * 3 * g.P;
+  T *J = f.J + (long)d * 3 * g.P;
+  const T *Hat = f.Hat + (long)d * 3 * g.P;
   const long P = g.P;
   const int yr = g.yr;
   if (i < 0 || k < 0 || i >= g.nr || k >= g.nz) {
@@ -115,38 +118,38 @@ __global__ void k_yee_e(Geom g, Fields f, int clear_j, StepState *st) {
   Dom dm = make_dom(g, d);
   const int r_begin = dm.w0 ? 1 : 0, z_begin = dm.w1 ? 1 : 0;
   const int r_end = dm.w2 ? g.nr - 1 : g.nr, z_end = dm.w3 ? g.nz - 1 : g.nz;
-  const double dr = g.dr, dz = g.dz, dt = g.dt;
-  double epsilonx2 = 2 * f.eps[(long)d * P + o];
-  double sigma_t = g.pml_on ? f.sigma[(long)d * P + o] * dt : 0;
-  double koef_e = (epsilonx2 - sigma_t) / (epsilonx2 + sigma_t);
-  double koef_h = 2 * dt / (epsilonx2 + sigma_t);
+  const T dr = (T)g.dr, dz = (T)g.dz, dt = (T)g.dt;
+  T epsilonx2 = 2 * f.eps[(long)d * P + o];
+  T sigma_t = g.pml_on ? f.sigma[(long)d * P + o] * dt : (T)0;
+  T koef_e = (epsilonx2 - sigma_t) / (epsilonx2 + sigma_t);
+  T koef_h = 2 * dt / (epsilonx2 + sigma_t);
   const long om = o - 1;    // k - 1
   const long oi = o - yr;   // i - 1
-  double jr = J[o], jp = J[P + o], jz = J[2 * P + o];
+  T jr = J[o], jp = J[P + o], jz = J[2 * P + o];
   if (i == 0 && dm.w0) {
     if (k >= z_begin && k < z_end) {
-      double er = E[o] * koef_e;
+      T er = E[o] * koef_e;
       er -= (jr + (Hat[P + o] - Hat[P + om]) / dz) * koef_h;
       E[o] = er;
-      double ez = E[2 * P + o] * koef_e;
-      ez -= (jz - Hat[P + o] * 4. / dr) * koef_h;
+      T ez = E[2 * P + o] * koef_e;
+      ez -= (jz - Hat[P + o] * (T)4. / dr) * koef_h;
       E[2 * P + o] = ez;
     }
   } else if (k == 0 && dm.w1) {
     if (i >= r_begin && i < r_end) {
-      double ez = E[2 * P + o] * koef_e;
+      T ez = E[2 * P + o] * koef_e;
       ez -= (jz - (Hat[P + o] - Hat[P + oi]) / dr
-             - (Hat[P + o] + Hat[P + oi]) / (2. * dr * (double)(i + dm.bot_r))) * koef_h;
+             - (Hat[P + o] + Hat[P + oi]) / ((T)2. * dr * (T)(double)(i + dm.bot_r))) * koef_h;
       E[2 * P + o] = ez;
     }
   } else if (i >= r_begin && i < r_end && k >= z_begin && k < z_end) {
-    double er = E[o] * koef_e;
+    T er = E[o] * koef_e;
     er -= (jr + (Hat[P + o] - Hat[P + om]) / dz) * koef_h;
-    double ep = E[P + o] * koef_e;
+    T ep = E[P + o] * koef_e;
     ep -= (jp - (Hat[o] - Hat[om]) / dz + (Hat[2 * P + o] - Hat[2 * P + oi]) / dr) * koef_h;
-    double ez = E[2 * P + o] * koef_e;
+    T ez = E[2 * P + o] * koef_e;
     ez -= (jz - (Hat[P + o] - Hat[P + oi]) / dr
-           - (Hat[P + o] + Hat[P + oi]) / (2. * dr * (double)(i + dm.bot_r))) * koef_h;
+           - (Hat[P + o] + Hat[P + oi]) / ((T)2. * dr * (T)(double)(i + dm.bot_r))) * koef_h;
     E[o] = er; E[P + o] = ep; E[2 * P + o] = ez;
     if (isnan(er) || isnan(ep) || isnan(ez)) atomicOr(&st->err, ERR_NAN_FIELD);
   }
@@ -155,16 +158,17 @@ __global__ void k_yee_e(Geom g, Fields f, int clear_j, StepState *st) {
 
 // Yee H: maxwellSolverYee.cpp:111-159 (the outer-wall H_r block runs before
 // the regular loop, so H_r at i = Nr-1 is advanced twice, quirk Q12)
-__global__ void k_yee_h(Geom g, Fields f) {
+template <class T>
+__global__ void k_yee_h(Geom g, FieldsT<T> f) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long total = (long)g.Dr * g.Dz * g.P;
   if (t >= total) return;
   int d = (int)(t / g.P);
   long o = t - (long)d * g.P;
   int i = (int)(o / g.yr) - 2, k = (int)(o % g.yr) - 2;
-  double *H = f.H + (long)d * 3 * g.P;
-  double *Hat = f.Hat + (long)d * 3 * g.P;
-  const double *E = f.E + (long)d * 3 * g.P;
+  T *H = f.H + (long)d * 3 * g.P;
+  T *Hat = f.Hat + (long)d * 3 * g.P;
+  const T *E = f.E + (long)d * 3 * g.P;
   const long P = g.P;
   const int yr = g.yr;
   if (i < 0 || k < 0 || i >= g.nr || k >= g.nz) {
@@ -173,24 +177,24 @@ __global__ void k_yee_h(Geom g, Fields f) {
     return;
   }
   Dom dm = make_dom(g, d);
-  const double dr = g.dr, dz = g.dz, dt = g.dt;
+  const T dr = (T)g.dr, dz = (T)g.dz, dt = (T)g.dt, mu0 = (T)MAGN_CONST;
   const long op = o + 1;    // k + 1
   const long oi = o + yr;   // i + 1
-  double hat_r = Hat[o];
+  T hat_r = Hat[o];
   if (dm.w2 && i == g.nr - 1) {
-    double alpha_t = dt * (E[P + op] - E[P + o]) / (dz * MAGN_CONST);
+    T alpha_t = dt * (E[P + op] - E[P + o]) / (dz * mu0);
     hat_r += alpha_t;
   }
-  double alpha_t_r = dt * (E[P + op] - E[P + o]) / (dz * MAGN_CONST);
+  T alpha_t_r = dt * (E[P + op] - E[P + o]) / (dz * mu0);
   H[o] = hat_r + alpha_t_r / 2;
   Hat[o] = hat_r + alpha_t_r;
-  double alpha_t_phi = dt * ((E[2 * P + oi] - E[2 * P + o]) / dr - (E[op] - E[o]) / dz) / MAGN_CONST;
-  double hp = Hat[P + o];
+  T alpha_t_phi = dt * ((E[2 * P + oi] - E[2 * P + o]) / dr - (E[op] - E[o]) / dz) / mu0;
+  T hp = Hat[P + o];
   H[P + o] = hp + alpha_t_phi / 2;
   Hat[P + o] = hp + alpha_t_phi;
-  double alpha_t_z = dt * ((E[P + oi] + E[P + o]) / (2. * dr * (i + 0.5 + (double)dm.bot_r))
-                           + (E[P + oi] - E[P + o]) / dr) / MAGN_CONST;
-  double hz = Hat[2 * P + o];
+  T alpha_t_z = dt * ((E[P + oi] + E[P + o]) / ((T)2. * dr * (T)(i + 0.5 + (double)dm.bot_r))
+                      + (E[P + oi] - E[P + o]) / dr) / mu0;
+  T hz = Hat[2 * P + o];
   H[2 * P + o] = hz - alpha_t_z / 2;
   Hat[2 * P + o] = hz - alpha_t_z;
 }
@@ -198,7 +202,8 @@ __global__ void k_yee_h(Geom g, Fields f) {
 // E/H overlay (SMB.cpp:784-823, 1010-1044): halos hold zeros before the
 // stitch, so every stitched halo node is a copy of the neighbour's interior
 // node; the anti-diagonal corners are never stitched (quirk Q14).
-__global__ void k_stitch_copy(Geom g, double *G, int ncomp) {
+template <class T>
+__global__ void k_stitch_copy(Geom g, T *G, int ncomp) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long total = (long)g.Dr * g.Dz * g.P;
   if (t >= total) return;
@@ -219,7 +224,8 @@ __global__ void k_stitch_copy(Geom g, double *G, int ncomp) {
 // J overlay (SMB.cpp:566-596): every interior node within two cells of a
 // domain edge gains the halo deposits of the r, z and main-diagonal
 // neighbours; anti-diagonal deposits are lost (quirk Q14).
-__global__ void k_stitch_sum_j(Geom g, double *J) {
+template <class T>
+__global__ void k_stitch_sum_j(Geom g, T *J) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long total = (long)g.Dr * g.Dz * g.nr * g.nz;
   if (t >= total) return;
@@ -231,7 +237,7 @@ __global__ void k_stitch_sum_j(Geom g, double *J) {
   if (!(lo_i || hi_i || lo_k || hi_k)) return;
   int I0 = d / g.Dz, J0 = d % g.Dz;
   long o = (long)(i + 2) * g.yr + (k + 2);
-  double acc[3] = {0, 0, 0};
+  T acc[3] = {0, 0, 0};
   bool any = false;
   for (int di = -1; di <= 1; ++di) {
     if (di == -1 && !lo_i) continue;
@@ -592,27 +598,40 @@ static inline unsigned pgrid(pcp_ctx *c, long n, int block) {
   return (unsigned)(want < 1 ? 1 : want);
 }
 
-void launch_yee_e(pcp_ctx *c, int clear_j) {
+template <class T>
+void launch_yee_e_t(pcp_ctx *c, FieldsT<T> f, int clear_j) {
   long total = (long)c->D * c->g.P;
-  hipLaunchKernelGGL(k_yee_e, dim3(grid_for(total, 256)), dim3(256), 0, c->stream, c->g, c->f, clear_j, c->st);
+  hipLaunchKernelGGL(k_yee_e<T>, dim3(grid_for(total, 256)), dim3(256), 0, c->stream, c->g, f, clear_j, c->st);
 }
-void launch_yee_h(pcp_ctx *c) {
+template <class T>
+void launch_yee_h_t(pcp_ctx *c, FieldsT<T> f) {
   long total = (long)c->D * c->g.P;
-  hipLaunchKernelGGL(k_yee_h, dim3(grid_for(total, 256)), dim3(256), 0, c->stream, c->g, c->f);
+  hipLaunchKernelGGL(k_yee_h<T>, dim3(grid_for(total, 256)), dim3(256), 0, c->stream, c->g, f);
 }
-void launch_stitch(pcp_ctx *c, int which) {
+template <class T>
+void launch_stitch_t(pcp_ctx *c, FieldsT<T> f, int which) {
   if (c->D == 1) return;
   long total = (long)c->D * c->g.P;
   if (which == PCP_GRID_E) {
-    hipLaunchKernelGGL(k_stitch_copy, dim3(grid_for(total, 256)), dim3(256), 0, c->stream, c->g, c->f.E, 3);
+    hipLaunchKernelGGL(k_stitch_copy<T>, dim3(grid_for(total, 256)), dim3(256), 0, c->stream, c->g, f.E, 3);
   } else if (which == PCP_GRID_H || which == PCP_GRID_H_AT_ET) {
-    hipLaunchKernelGGL(k_stitch_copy, dim3(grid_for(total, 256)), dim3(256), 0, c->stream, c->g, c->f.H, 3);
-    hipLaunchKernelGGL(k_stitch_copy, dim3(grid_for(total, 256)), dim3(256), 0, c->stream, c->g, c->f.Hat, 3);
+    hipLaunchKernelGGL(k_stitch_copy<T>, dim3(grid_for(total, 256)), dim3(256), 0, c->stream, c->g, f.H, 3);
+    hipLaunchKernelGGL(k_stitch_copy<T>, dim3(grid_for(total, 256)), dim3(256), 0, c->stream, c->g, f.Hat, 3);
   } else {
     long cells = (long)c->D * c->g.nr * c->g.nz;
-    hipLaunchKernelGGL(k_stitch_sum_j, dim3(grid_for(cells, 256)), dim3(256), 0, c->stream, c->g, c->f.J);
+    hipLaunchKernelGGL(k_stitch_sum_j<T>, dim3(grid_for(cells, 256)), dim3(256), 0, c->stream, c->g, f.J);
   }
 }
+template void launch_yee_e_t<double>(pcp_ctx *, FieldsT<double>, int);
+template void launch_yee_e_t<float>(pcp_ctx *, FieldsT<float>, int);
+template void launch_yee_h_t<double>(pcp_ctx *, FieldsT<double>);
+template void launch_yee_h_t<float>(pcp_ctx *, FieldsT<float>);
+template void launch_stitch_t<double>(pcp_ctx *, FieldsT<double>, int);
+template void launch_stitch_t<float>(pcp_ctx *, FieldsT<float>, int);
+
+void launch_yee_e(pcp_ctx *c, int clear_j) { launch_yee_e_t<double>(c, c->f, clear_j); }
+void launch_yee_h(pcp_ctx *c) { launch_yee_h_t<double>(c, c->f); }
+void launch_stitch(pcp_ctx *c, int which) { launch_stitch_t<double>(c, c->f, which); }
 void launch_fill(pcp_ctx *c, double *p, long n, double v) {
   hipLaunchKernelGGL(k_fill, dim3(pgrid(c, n, 256)), dim3(256), 0, c->stream, p, n, v);
 }
@@ -660,16 +679,19 @@ void launch_zero_j_if_exact(pcp_ctx *c) {
   long n = (long)c->D * 3 * c->g.P;
   hipLaunchKernelGGL(k_fill_if_exact, dim3(pgrid(c, n, 256)), dim3(256), 0, c->stream, c->f.J, n, c->st);
 }
-void launch_migrant_keys(pcp_ctx *c) {
+void launch_migrant_keys_t(pcp_ctx *c, SeqTable t) {
   if (c->D == 1) return;
-  SeqTable t;
-  for (int i = 0; i < MAX_SPECIES; ++i) t.p[i] = c->sp[i].seq;
   unsigned gm = (unsigned)(c->n_cu * 4);
   hipLaunchKernelGGL(k_mig_count, dim3(gm), dim3(256), 0, c->stream, c->st, c->mig);
   hipLaunchKernelGGL(k_mig_scan, dim3(1), dim3(1024), 0, c->stream, c->st, c->mig);
   hipLaunchKernelGGL(k_mig_scatter, dim3(gm), dim3(256), 0, c->stream, c->st, c->mig);
   hipLaunchKernelGGL(k_mig_rank, dim3(gm), dim3(256), 0, c->stream, c->st, c->mig, t);
   hipLaunchKernelGGL(k_mig_finish, dim3(grid_for(c->mig.ngroups, 256)), dim3(256), 0, c->stream, c->st, c->mig);
+}
+void launch_migrant_keys(pcp_ctx *c) {
+  SeqTable t;
+  for (int i = 0; i < MAX_SPECIES; ++i) t.p[i] = c->sp[i].seq;
+  launch_migrant_keys_t(c, t);
 }
 void launch_gather_point(pcp_ctx *c, int d, double r, double z, int which, double *dev_out) {
   hipLaunchKernelGGL(k_gather_point, dim3(1), dim3(1), 0, c->stream, c->g, c->f, d, r, z, which, dev_out);

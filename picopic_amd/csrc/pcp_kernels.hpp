@@ -9,6 +9,14 @@ struct SeqTable {
   uint64_t *p[MAX_SPECIES];
 };
 
+template <class T>
+void launch_yee_e_t(pcp_ctx *c, FieldsT<T> f, int clear_j);
+template <class T>
+void launch_yee_h_t(pcp_ctx *c, FieldsT<T> f);
+template <class T>
+void launch_stitch_t(pcp_ctx *c, FieldsT<T> f, int which);
+void launch_migrant_keys_t(pcp_ctx *c, SeqTable t);
+
 void launch_yee_e(pcp_ctx *c, int clear_j);
 void launch_yee_h(pcp_ctx *c);
 void launch_stitch(pcp_ctx *c, int which);

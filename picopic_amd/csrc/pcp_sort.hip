@@ -9,28 +9,9 @@
 #include <hipcub/hipcub.hpp>
 
 #include "pcp_sort.hpp"
+#include "pcp_wave.hpp"
 
 namespace pcp {
-
-__device__ __forceinline__ uint32_t wave_reserve(uint32_t *ctr, uint32_t key, bool active) {
-  // one atomic per distinct key in the wavefront; lanes get consecutive slots
-  uint32_t slot = 0;
-  unsigned long long pending = __ballot(active);
-  const int lane = __lane_id();
-  while (pending) {
-    const int leader = __ffsll((long long)pending) - 1;
-    const uint32_t lk = __shfl(key, leader, 64);
-    const bool m = active && key == lk;
-    const unsigned long long mask = __ballot(m);
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(ctr + lk, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader, 64);
-    if (m) slot = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-    pending &= ~mask;
-    active = active && !m;
-  }
-  return slot;
-}
 
 __global__ void __launch_bounds__(256) k_sort_count(Geom g, SpeciesDev s, int cur, uint32_t *key, uint32_t *count,
                                                     uint32_t sentinel) {

@@ -36,7 +36,8 @@ def make_particles(rng, size, grid, n, mass_me, fast_fraction=0.3, vmax=2.5e8, t
         v[:, :nf] = dirs * speed
     if special and n >= 40:
         k = n // 20
-        v[0, nf:nf + k] = 0.0            # strict axial motion
+        v[0, nf:nf + k] = 0.0            # strict axial motion (no radial displacement:
+        v[1, nf:nf + k] = 0.0            # the rz rotation moves r unless v_phi = 0 too)
         v[2, nf + k:nf + 2 * k] = 0.0    # strict radial motion
         # next to the walls, heading out
         m = nf + 2 * k

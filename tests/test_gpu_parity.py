@@ -177,6 +177,7 @@ def test_strict_axial_reset_order():
         # a third of the particles move strictly along z, crowded in few cells
         k = n // 3
         vr[::3] = 0.0
+        vp[::3] = 0.0
         r[::3] = rng.choice([2.5e-3, 3.5e-3, 4.5e-3], size=len(r[::3])) + rng.uniform(-4e-4, 4e-4, len(r[::3]))
         ids = np.arange(n, dtype=np.uint64) + np.uint64(sp * 10 ** 6)
         orc.add_particles(sp, r, z, vr, vp, vz, w, ids)
