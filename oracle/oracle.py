@@ -245,3 +245,14 @@ def lorenz_factor(sq):
 
 def lorenz_factor_inv(sq):
     return lib().orc_lorenz_factor_inv(sq)
+
+
+def p12_collide_single(m_a, m_b, q_a, q_b, va, wa, vb, wb, dens_a, dens_b, debye, dt, uniforms):
+    """One P12 pair collision with the given uniform draws (3 per call)."""
+    va = np.array(va, dtype=np.float64)
+    vb = np.array(vb, dtype=np.float64)
+    u = np.ascontiguousarray(uniforms, dtype=np.float64)
+    rng = OrcRng(_dp(u), len(u), 0)
+    lib().orc_p12_collide_single(m_a, m_b, q_a, q_b, _dp(va), wa, _dp(vb), wb, dens_a, dens_b, debye, dt,
+                                 C.byref(rng))
+    return va, vb, rng.pos
