@@ -28,12 +28,13 @@ C_LIGHT = 3e8
 
 def workload(name):
     if name == "wakefield":
-        return dict(name="beam-plasma wakefield 512x4096, 64 ppc/species + Gaussian bunch (gamma 5)",
+        return dict(key="wakefield",
+                    name="beam-plasma wakefield 512x4096, 64 ppc/species + Gaussian bunch (gamma 5)",
                     grid=(512, 4096), domains=(1, 1), ppc=64, density=1e17, ccs="zigzag", pusher="boris-adaptive",
                     bunch=dict(n=1 << 20, n_peak=1e16, sigma_r=2e-3, sigma_z=5e-3, z0=0.1, gamma=5.0))
     if name == "uniform":
-        return dict(name="uniform e-p plasma 128x512, 16 ppc/species, VB", grid=(128, 512), domains=(1, 1), ppc=16,
-                    density=1e17, ccs="vb", pusher="boris-adaptive", bunch=None)
+        return dict(key="uniform", name="uniform e-p plasma 128x512, 16 ppc/species, VB", grid=(128, 512),
+                    domains=(1, 1), ppc=16, density=1e17, ccs="vb", pusher="boris-adaptive", bunch=None)
     raise SystemExit("unknown workload " + name)
 
 
@@ -67,22 +68,23 @@ def run_gpu(w, steps, warmup, prec):
     copy_gbs = ctx.copy_bandwidth(1 << 30, 20)
     ctx.close()
     return dict(n_particles=n_part, n_particles_end=n_end, wall=wall, phases=phases, cells=grid[0] * grid[1],
-                copy_gbs=copy_gbs)
+                copy_gbs=copy_gbs, n_species=len(species))
 
 
 def cpu_baseline(w, threads):
     """The oracle (a C restatement of the reference CPU path, AoS particles,
     OpenMP over domains) on a slab of the same plasma: all 512 radial cells x
-    256 axial cells, 16 domains, same ppc, 1 warm-up + 3 timed steps."""
+    512 axial cells, 32 domains, same ppc, 1 warm-up + 12 timed steps (about
+    10-15 s on 16 cores)."""
     import numpy as np
 
     from oracle import oracle as O
     dr = dz = 1e-3
-    grid = (w["grid"][0], 256)
+    grid = (w["grid"][0], 512)
     size = (grid[0] * dr, grid[1] * dz)
     dt = 0.5 * dz / C_LIGHT
     species = [(-1, 1, False), (1, 1836, False)]
-    smb = O.SMB(size, grid, (8, 2), dt, species, pusher=w["pusher"], ccs=w["ccs"])
+    smb = O.SMB(size, grid, (8, 4), dt, species, pusher=w["pusher"], ccs=w["ccs"])
     rng = np.random.default_rng(5)
     n = w["ppc"] * grid[0] * grid[1]
     for sp, (q, m, b) in enumerate(species):
@@ -97,13 +99,36 @@ def cpu_baseline(w, threads):
     O.set_threads(threads)
     smb.step(1)
     t0 = time.perf_counter()
-    nsteps = 3
+    nsteps = 12
     smb.step(nsteps)
     dt_wall = time.perf_counter() - t0
     total = 2 * n
     return dict(value=total * nsteps / dt_wall, unit="particle-steps/s", cores=threads, kind="port",
-                sample="oracle on a 512x256-cell slab of the same plasma (%d particles, 16 domains), "
+                sample="oracle on a 512x512-cell slab of the same plasma (%d particles, 32 domains), "
                        "%d timed steps after 1 warm-up, %.1f s" % (total, nsteps, dt_wall))
+
+
+def traffic_from_profile(w, prec, n_particles, n_launches_per_step):
+    """HBM bytes per launch of the fused particle kernel, from the committed
+    rocprofv3 PMC summary of this workload (FETCH_SIZE and WRITE_SIZE per
+    dispatch, each scaled by the same counter's reading for a copy kernel of
+    known size); None when no summary for this workload and precision exists."""
+    path = os.path.join(ROOT, "profiles", "%s_%s_traffic.txt" % (KERNEL[prec], w["key"]))
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if ":" in line and not line.startswith("#"):
+            k, v = line.split(":", 1)
+            vals[k.strip()] = v.strip()
+    try:
+        per_particle = float(vals["hbm_bytes_per_particle"])
+    except (KeyError, ValueError):
+        return None
+    return per_particle * n_particles / n_launches_per_step
+
+
+KERNEL = {"fp32": "k32_step_fused", "fp64": "k_step_fused"}
 
 
 def main():
@@ -130,10 +155,15 @@ def main():
     n = g["n_particles"]
     value = n * args.steps / g["wall"]
     ms_step = g["wall"] * 1e3 / args.steps
-    t_part = g["phases"]["particles"]   # ms per step of the fused push+move+deposit kernels
+    # the fused push+move+deposit kernel is launched once per species and
+    # step; the "particles" phase is exactly those launches, timed with HIP
+    # events on the context stream
+    t_part = g["phases"]["particles"]
+    launches = g["n_species"]
     bytes_pp = 88 if main_prec == "fp64" else 52
     alg_bytes = bytes_pp * n
     achieved = alg_bytes / (t_part * 1e-3) / 1e9 if t_part > 0 else None
+    traffic = traffic_from_profile(w, main_prec, n, launches)
     config = {"workload": w["name"], "particles": n, "cells": g["cells"], "precision": main_prec,
               "deposition": w["ccs"], "pusher": w["pusher"], "phases_ms": g["phases"]}
     if "fp64" in runs and main_prec != "fp64":
@@ -158,8 +188,10 @@ def main():
         "data": "synthetic (seeded, generated on device)",
         "config": config,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-                     "kernel": "k%s_step_fused (push+move+deposit)" % ("32" if main_prec == "fp32" else ""),
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
+                     "kernel": "%s (push+move+deposit)" % KERNEL[main_prec],
+                     "launches_per_step": launches, "avg_launch_ms": t_part / launches,
+                     "alg_bytes_per_launch": alg_bytes / launches,
                      "alg_bytes_per_particle": bytes_pp, "copy_kernel_gbs": g["copy_gbs"]},
     }
     if not args.no_cpu:

@@ -2,7 +2,9 @@
 
 hipcc compiles every ``csrc/*.hip`` into ``_lib/libpicopic_hip.so``.  FP64
 kernels are compiled without FMA contraction so their rounding follows the
-reference's IEEE build; packed FP32 instructions are disabled.
+reference's IEEE build; packed FP32 instructions are disabled.  g++ then
+builds ``_lib/smb_driver``, an SMB-style C++ time loop over the drop-in
+solver objects of ``include/picopic_solvers.hpp``.
 """
 import os
 import subprocess
@@ -12,6 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libpicopic_hip.so")
+DRIVER = os.path.join(OUT_DIR, "smb_driver")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PCP_OFFLOAD_ARCH", "gfx950")
 
@@ -54,7 +58,15 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
+    build_driver()
     return LIB
+
+
+def build_driver():
+    src = os.path.join(HERE, "host", "smb_driver.cpp")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fopenmp", "-Wall", "-I", INCLUDE, src, "-o", DRIVER,
+                    "-L", OUT_DIR, "-lpicopic_hip", "-Wl,-rpath,$ORIGIN"], check=True)
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -242,41 +242,56 @@ __device__ __forceinline__ void renorm(int &c, float &f) {
   if (f >= 1.f) { f -= 1.f; c += 1; }
 }
 
-// returns false when the particle left the simulation area
-__device__ __forceinline__ bool move32(const Geom &g, const Dom &o, int beam, float dtr, float dtz, Split &p, float v[3]) {
+// Returns false when the particle left the simulation area.  (rd, zd) is the
+// unrounded end point in metres -- the old split position plus the FP32
+// displacement, summed in FP64 -- from which the current is deposited: the
+// stored offset rounds displacements below 2^-24 of a cell away, and a
+// rounded-away radial step would turn an ordinary move into the reference's
+// strict axial motion (which resets J_r nodes).
+__device__ __forceinline__ bool move32(const Geom &g, const Dom &o, int beam, float dtr, float dtz, Split &p, float v[3],
+                                       double &rd, double &zd) {
   const Split old = p;
   float du = v[0] * dtr, dph = v[1] * dtr, dw = v[2] * dtz;
   float xoff = p.fr + du;
   float X = (float)p.cr + xoff;
   float Y = dph;
   float sn = 0.f;
+  double ur = (double)p.cr + (double)p.fr + (double)du;   // unrounded radius, cell units
   if (Y != 0.f) {
     float R = sqrtf(X * X + Y * Y);
-    if (X > 0.f) xoff += Y * Y / (X + R);
-    else xoff = R - (float)p.cr;
+    if (X > 0.f) {
+      float inc = Y * Y / (X + R);
+      xoff += inc;
+      ur += (double)inc;
+    } else {
+      xoff = R - (float)p.cr;
+      ur = (double)R;
+    }
     sn = Y / R;
   } else if (X < 0.f) {
     xoff = -X - (float)p.cr;   // |X|: the cylinder radius of a particle that crossed the axis
+    ur = -ur;
   }
   p.fr = xoff;
   p.fz = p.fz + dw;
+  rd = ur * g.dr;
+  zd = ((double)old.cz + (double)old.fz + (double)dw) * g.dz;
   // walls (local cell units): r < 1/2 or > nr - 1/2, z < 1/2 or > nz - 1/2
   const float lr = (float)(p.cr - (int)o.bot_r) + p.fr, lz = (float)(p.cz - (int)o.left_z) + p.fz;
   const bool hit = beam ? (o.w0 && lr < 0.5f)
                         : ((o.w0 && lr < 0.5f) || (o.w2 && lr > g.nr - 0.5f) || (o.w1 && lz < 0.5f) ||
                            (o.w3 && lz > g.nz - 0.5f));
   if (hit) {
-    double pr = ((double)p.cr + (double)p.fr) * g.dr, pz = ((double)p.cz + (double)p.fz) * g.dz;
     double por = ((double)old.cr + (double)old.fr) * g.dr, poz = ((double)old.cz + (double)old.fz) * g.dz;
     double vr = v[0], vz = v[2];
-    if (beam) reflect_beam(g, o, pr, vr);
-    else reflect_plasma(g, o, pr, pz, por, poz, vr, vz);
+    if (beam) reflect_beam(g, o, rd, vr);
+    else reflect_plasma(g, o, rd, zd, por, poz, vr, vz);
     v[0] = (float)vr;
     v[2] = (float)vz;
-    double ur = pr / g.dr, uz = pz / g.dz;
-    double fcr = floor(ur), fcz = floor(uz);
-    p.cr = (int)fcr; p.fr = (float)(ur - fcr);
-    p.cz = (int)fcz; p.fz = (float)(uz - fcz);
+    double qr = rd / g.dr, qz = zd / g.dz;
+    double fcr = floor(qr), fcz = floor(qz);
+    p.cr = (int)fcr; p.fr = (float)(qr - fcr);
+    p.cz = (int)fcz; p.fz = (float)(qz - fcz);
   }
   renorm(p.cr, p.fr);
   renorm(p.cz, p.fz);
@@ -418,6 +433,33 @@ __device__ __forceinline__ void append_migrant32(Migrants mg, StepState *st, int
 // ---------------------------------------------------------------------------
 // the fused FP32 step kernel
 // ---------------------------------------------------------------------------
+// Gather, push and move of live particle p from slot `from` (positions and
+// velocities).  The exact re-deposition calls it again on the pre-step slot
+// and gets bit-identical (n, v, rd, zd).
+__device__ __forceinline__ bool advance32(const Geom &g, const FieldsT<float> &f, const Species32 &s, long p,
+                                          uint32_t cell, int from, const PushConst &K, float dtr, float dtz, Split &o,
+                                          Split &n, float v[3], double &rd, double &zd, int &d_old, int *err) {
+  o = unpack(cell, s.fr[from][p], s.fz[from][p], g.Nz);
+  d_old = domain_of32(g, o.cr, o.cz);
+  const Dom dm = make_dom(g, d_old);
+  GridView32 EV{f.E + (long)d_old * 3 * g.P, g.P, g.yr, dm.bot_r, dm.left_z};
+  GridView32 HV{f.H + (long)d_old * 3 * g.P, g.P, g.yr, dm.bot_r, dm.left_z};
+  float e[3], h[3];
+  gather32_e(EV, o.cr, o.cz, o.fr, o.fz, e);
+  gather32_h(HV, o.cr, o.cz, o.fr, o.fz, h);
+  v[0] = s.vr[from][p]; v[1] = s.vphi[from][p]; v[2] = s.vz[from][p];
+  push32(g.pusher, K, e, h, v, err);
+  n = o;
+  return move32(g, dm, s.beam, dtr, dtz, n, v, rd, zd);
+}
+
+// domain whose grid receives the current of a particle ending at (rd, zd)
+__device__ __forceinline__ int deposit_domain32(const Geom &g, double rd, double zd) {
+  int cr = min(max(cell_number(rd, g.dr), 0), (int)g.Nr - 1);
+  int cz = min(max(cell_number(zd, g.dz), 0), (int)g.Nz - 1);
+  return domain_of32(g, cr, cz);
+}
+
 template <int CCS>
 __global__ void __launch_bounds__(256) k32_step_fused(Geom g, FieldsT<float> f, Species32 s, int sp, int cur,
                                                       StepState *st, Migrants mg, PushConst K) {
@@ -431,48 +473,38 @@ __global__ void __launch_bounds__(256) k32_step_fused(Geom g, FieldsT<float> f, 
     bool live = false;
     Split o{1, 1, 0.5f, 0.5f}, n{1, 1, 0.5f, 0.5f};
     float v[3] = {0.f, 0.f, 0.f};
+    double rd = 1.5 * g.dr, zd = 1.5 * g.dz;
     float w = 0.f;
     int d_new = 0;
     if (p < s.n) {
       uint32_t cell = s.cell[cur][p];
-      if (cell == DEAD_CELL) {
-        s.cell[nxt][p] = DEAD_CELL;
+      int d_old = 0;
+      if (cell != DEAD_CELL && advance32(g, f, s, p, cell, cur, K, dtr, dtz, o, n, v, rd, zd, d_old, &err)) {
+        s.vr[nxt][p] = v[0]; s.vphi[nxt][p] = v[1]; s.vz[nxt][p] = v[2];
+        s.cell[nxt][p] = (uint32_t)n.cr * (uint32_t)g.Nz + (uint32_t)n.cz;
+        s.fr[nxt][p] = n.fr;
+        s.fz[nxt][p] = n.fz;
+        d_new = domain_of32(g, n.cr, n.cz);
+        if (d_new != d_old) append_migrant32(mg, st, sp, p, d_old, d_new, s.seq[p], g, &err);
+        w = s.w[p];
+        live = true;
       } else {
-        o = unpack(cell, s.fr[cur][p], s.fz[cur][p], g.Nz);
-        const int d_old = domain_of32(g, o.cr, o.cz);
-        const Dom dm = make_dom(g, d_old);
-        GridView32 EV{f.E + (long)d_old * 3 * g.P, g.P, g.yr, dm.bot_r, dm.left_z};
-        GridView32 HV{f.H + (long)d_old * 3 * g.P, g.P, g.yr, dm.bot_r, dm.left_z};
-        float e[3], h[3];
-        gather32_e(EV, o.cr, o.cz, o.fr, o.fz, e);
-        gather32_h(HV, o.cr, o.cz, o.fr, o.fz, h);
-        v[0] = s.vr[p]; v[1] = s.vphi[p]; v[2] = s.vz[p];
-        push32(g.pusher, K, e, h, v, &err);
-        n = o;
-        bool alive = move32(g, dm, s.beam, dtr, dtz, n, v);
-        s.vr[p] = v[0]; s.vphi[p] = v[1]; s.vz[p] = v[2];
-        if (!alive) {
-          s.cell[nxt][p] = DEAD_CELL;
-        } else {
-          s.cell[nxt][p] = (uint32_t)n.cr * (uint32_t)g.Nz + (uint32_t)n.cz;
-          s.fr[nxt][p] = n.fr;
-          s.fz[nxt][p] = n.fz;
-          d_new = domain_of32(g, n.cr, n.cz);
-          if (d_new != d_old) append_migrant32(mg, st, sp, p, d_old, d_new, s.seq[p], g, &err);
-          w = s.w[p];
-          live = true;
-        }
+        s.cell[nxt][p] = DEAD_CELL;
       }
     }
-    if (!live) { o = n = Split{1, 1, 0.5f, 0.5f}; d_new = 0; v[1] = 0.f; w = 0.f; }
-    const Dom dn = make_dom(g, d_new);
-    WaveSink32 S{f.J, (int)((long)d_new * 3 * g.P), (int)g.P, g.yr, g.nr, g.nz, &err};
+    if (!live) { o = n = Split{1, 1, 0.5f, 0.5f}; d_new = 0; v[1] = 0.f; w = 0.f; rd = 1.5 * g.dr; zd = 1.5 * g.dz; }
     const float qw = (float)s.q * w;
     if (CCS == PCP_CCS_ZIGZAG) {
+      const Dom dn = make_dom(g, d_new);
+      WaveSink32 S{f.J, (int)((long)d_new * 3 * g.P), (int)g.P, g.yr, g.nr, g.nz, &err};
       zigzag32(S, live, g, dn, o, n, v[1], qw, inv_dt);
     } else {
-      // Villasenor-Buneman terms in FP64 from the exact split positions
-      const double r = ((double)n.cr + (double)n.fr) * g.dr, z = ((double)n.cz + (double)n.fz) * g.dz;
+      // Villasenor-Buneman terms in FP64 from the old split position and the
+      // unrounded end point
+      const int d_dep = live ? deposit_domain32(g, rd, zd) : 0;
+      const Dom dn = make_dom(g, d_dep);
+      WaveSink32 S{f.J, (int)((long)d_dep * 3 * g.P), (int)g.P, g.yr, g.nr, g.nz, &err};
+      const double r = rd, z = zd;
       const double ro = ((double)o.cr + (double)o.fr) * g.dr, zo = ((double)o.cz + (double)o.fz) * g.dz;
       const double p_charge = s.q * (double)w;
       AzTerms A = vb_azimuthal_terms(g.nr, g.nz, g.dr, g.dz, dn.bot_r, dn.left_z, r, z, ro, zo, (double)v[1],
@@ -489,7 +521,7 @@ __global__ void __launch_bounds__(256) k32_step_fused(Geom g, FieldsT<float> f, 
       }
       if (live) vb_rz_dispatch(R, g.dr, g.dz, r, z, ro, zo, &err);
       if (live && R.is_strict) {
-        AtomicSink32 A1{f.J + (long)d_new * 3 * g.P, g.P, g.yr, g.nr, g.nz, &err, &st->need_exact};
+        AtomicSink32 A1{f.J + (long)d_dep * 3 * g.P, g.P, g.yr, g.nr, g.nz, &err, &st->need_exact};
         vb_strict(A1, g.dr, g.dz, g.dt, dn.bot_r, dn.left_z, r, z, ro, zo, p_charge);
       }
 #pragma unroll
@@ -509,19 +541,28 @@ __global__ void __launch_bounds__(256) k32_step_fused(Geom g, FieldsT<float> f, 
   if (err) atomicOr(&st->err, err);
 }
 
-// exact path (rare): markers and ordered re-deposition in FP64 arithmetic
-__global__ void __launch_bounds__(256) k32_mark(Geom g, Species32 s, int sp, int cur, StepState *st, uint64_t *marker,
-                                                int only_if_exact) {
+// Exact path of the Villasenor-Buneman deposit (rare: a strict axial motion
+// reset J_r nodes this step).  Both kernels run after the step flipped `cur`:
+// they redo the particle's gather/push/move from the pre-step slot cur ^ 1 to
+// recover the deposit end point, then mark the reset nodes and re-deposit in
+// the reference's particle order, with the FP64 terms of the fused kernel.
+__global__ void __launch_bounds__(256) k32_mark(Geom g, FieldsT<float> f, Species32 s, int sp, int cur, StepState *st,
+                                                uint64_t *marker, PushConst K, int only_if_exact) {
   if (only_if_exact && !st->need_exact) return;
   const uint64_t stamp = st->step & 0xFFFFFF;
+  const float dtr = (float)(g.dt / g.dr), dtz = (float)(g.dt / g.dz);
+  int dummy = 0;
   for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < s.n; p += (long)gridDim.x * blockDim.x) {
     uint32_t c = s.cell[cur][p], co = s.cell[cur ^ 1][p];
     if (c == DEAD_CELL || co == DEAD_CELL) continue;
-    Split n = unpack(c, s.fr[cur][p], s.fz[cur][p], g.Nz), o = unpack(co, s.fr[cur ^ 1][p], s.fz[cur ^ 1][p], g.Nz);
-    double r = ((double)n.cr + (double)n.fr) * g.dr, z = ((double)n.cz + (double)n.fz) * g.dz;
-    double ro = ((double)o.cr + (double)o.fr) * g.dr, zo = ((double)o.cz + (double)o.fz) * g.dz;
+    Split o, n;
+    float v[3];
+    double r, z;
+    int d_old;
+    if (!advance32(g, f, s, p, co, cur ^ 1, K, dtr, dtz, o, n, v, r, z, d_old, &dummy)) continue;
+    const double ro = ((double)o.cr + (double)o.fr) * g.dr, zo = ((double)o.cz + (double)o.fz) * g.dz;
     if (!vb_is_strict_reset(g.dr, g.dz, r, z, ro, zo)) continue;
-    int d = domain_of32(g, n.cr, n.cz);
+    int d = deposit_domain32(g, r, z);
     Dom dm = make_dom(g, d);
     MarkSink32 S{marker + (long)d * g.P, g.yr, g.nr, g.nz, (stamp << 40) | order_key32(sp, s.seq[p])};
     int i_n = cell_number(r, g.dr) - (int)dm.bot_r, k_n = cell_number(z, g.dz) - (int)dm.left_z;
@@ -531,28 +572,28 @@ __global__ void __launch_bounds__(256) k32_mark(Geom g, Species32 s, int sp, int
 }
 
 __global__ void __launch_bounds__(256) k32_deposit_exact(Geom g, FieldsT<float> f, Species32 s, int sp, int cur,
-                                                         StepState *st, const uint64_t *marker, int only_if_exact) {
+                                                         StepState *st, const uint64_t *marker, PushConst K,
+                                                         int only_if_exact) {
   if (only_if_exact && !st->need_exact) return;
-  int err = 0;
+  int err = 0, dummy = 0;
   const uint64_t stamp = st->step & 0xFFFFFF;
+  const float dtr = (float)(g.dt / g.dr), dtz = (float)(g.dt / g.dz);
   for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < s.n; p += (long)gridDim.x * blockDim.x) {
     uint32_t c = s.cell[cur][p], co = s.cell[cur ^ 1][p];
     if (c == DEAD_CELL || co == DEAD_CELL) continue;
-    Split n = unpack(c, s.fr[cur][p], s.fz[cur][p], g.Nz), o = unpack(co, s.fr[cur ^ 1][p], s.fz[cur ^ 1][p], g.Nz);
-    double r = ((double)n.cr + (double)n.fr) * g.dr, z = ((double)n.cz + (double)n.fz) * g.dz;
-    double ro = ((double)o.cr + (double)o.fr) * g.dr, zo = ((double)o.cz + (double)o.fz) * g.dz;
-    int d = domain_of32(g, n.cr, n.cz);
+    Split o, n;
+    float v[3];
+    double r, z;
+    int d_old;
+    if (!advance32(g, f, s, p, co, cur ^ 1, K, dtr, dtz, o, n, v, r, z, d_old, &dummy)) continue;
+    const double ro = ((double)o.cr + (double)o.fr) * g.dr, zo = ((double)o.cz + (double)o.fz) * g.dz;
+    int d = deposit_domain32(g, r, z);
     Dom dm = make_dom(g, d);
-    double w = s.w[p];
     ExactSink32 S{f.J + (long)d * 3 * g.P, marker + (long)d * g.P, g.P, g.yr, g.nr, g.nz, stamp,
                   order_key32(sp, s.seq[p]), &err};
-    if (g.ccs == PCP_CCS_VB) {
-      double p_charge = s.q * w;
-      vb_azimuthal(S, g.nr, g.nz, g.dr, g.dz, dm.bot_r, dm.left_z, r, z, ro, zo, (double)s.vphi[p], p_charge);
-      vb_rz(S, g.dr, g.dz, g.dt, dm.bot_r, dm.left_z, r, z, ro, zo, p_charge, &err);
-    } else {
-      zigzag(S, g.dr, g.dz, g.dt, dm.bot_r, dm.left_z, r, z, ro, zo, (double)s.vphi[p], s.q, w);
-    }
+    const double p_charge = s.q * (double)s.w[p];
+    vb_azimuthal(S, g.nr, g.nz, g.dr, g.dz, dm.bot_r, dm.left_z, r, z, ro, zo, (double)v[1], p_charge);
+    vb_rz(S, g.dr, g.dz, g.dt, dm.bot_r, dm.left_z, r, z, ro, zo, p_charge, &err);
   }
   if (err) atomicOr(&st->err, err);
 }
@@ -601,9 +642,9 @@ __global__ void __launch_bounds__(256) k32_sort_scatter(Species32 s, SortBuffers
       s.cell[nxt][q] = s.cell[cur][p];
       s.fr[nxt][q] = s.fr[cur][p];
       s.fz[nxt][q] = s.fz[cur][p];
-      sb.vr[q] = s.vr[p];
-      sb.vphi[q] = s.vphi[p];
-      sb.vz[q] = s.vz[p];
+      s.vr[nxt][q] = s.vr[cur][p];
+      s.vphi[nxt][q] = s.vphi[cur][p];
+      s.vz[nxt][q] = s.vz[cur][p];
       sb.w[q] = s.w[p];
       sb.seq[q] = s.seq[p];
       sb.id[q] = s.id[p];
@@ -703,9 +744,9 @@ __global__ void k32_init_plasma(Geom g, Species32 s, long base, long n_new, cons
     if (z >= (gk + 1) * g.dz) z = nextafter((gk + 1) * g.dz, 0.0);
     long q = base + p;
     store_split(s, q, g, r, z);
-    s.vr[q] = (float)(n0 * vth);
-    s.vphi[q] = (float)(n1 * vth);
-    s.vz[q] = (float)(n2 * vth);
+    s.vr[0][q] = s.vr[1][q] = (float)(n0 * vth);
+    s.vphi[0][q] = s.vphi[1][q] = (float)(n1 * vth);
+    s.vz[0][q] = s.vz[1][q] = (float)(n2 * vth);
     s.w[q] = (float)weight;
     s.seq[q] = (uint64_t)(p - dom_off[d]);
     s.id[q] = (uint64_t)q;
@@ -729,9 +770,9 @@ __global__ void k32_init_bunch(Geom g, Species32 s, long base, long n_new, doubl
     if (z > zmax) z = zmax;
     long q = base + p;
     store_split(s, q, g, r, z);
-    s.vr[q] = 0.f;
-    s.vphi[q] = 0.f;
-    s.vz[q] = (float)vz;
+    s.vr[0][q] = s.vr[1][q] = 0.f;
+    s.vphi[0][q] = s.vphi[1][q] = 0.f;
+    s.vz[0][q] = s.vz[1][q] = (float)vz;
     s.w[q] = (float)weight;
     int d = (int)((long)floor(r / g.dr) / g.nr) * g.Dz + (int)((long)floor(z / g.dz) / g.nz);
     s.seq[q] = gbase[(long)d * MAX_SPECIES + sp] + (uint64_t)p;
@@ -811,7 +852,7 @@ __global__ void k_kinetic_energy64(SpeciesDev s, int cur, double *out) {
 __global__ void k_kinetic_energy32(Species32 s, int cur, double *out) {
   double acc = 0;
   for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < s.n; p += (long)gridDim.x * blockDim.x)
-    if (s.cell[cur][p] != DEAD_CELL) acc += kin(s.m, (double)s.w[p], s.vr[p], s.vphi[p], s.vz[p]);
+    if (s.cell[cur][p] != DEAD_CELL) acc += kin(s.m, (double)s.w[p], s.vr[cur][p], s.vphi[cur][p], s.vz[cur][p]);
   __shared__ double red[256];
   red[threadIdx.x] = acc;
   __syncthreads();
@@ -859,9 +900,10 @@ int alloc32(pcp_ctx *c) {
     sp.n = 0;
     if (dalloc32(&sp.cell[0], sp.cap) || dalloc32(&sp.cell[1], sp.cap) || dalloc32(&sp.fr[0], sp.cap) ||
         dalloc32(&sp.fr[1], sp.cap) || dalloc32(&sp.fz[0], sp.cap) || dalloc32(&sp.fz[1], sp.cap) ||
-        dalloc32(&sp.vr, sp.cap) || dalloc32(&sp.vphi, sp.cap) || dalloc32(&sp.vz, sp.cap) ||
+        dalloc32(&sp.vr[0], sp.cap) || dalloc32(&sp.vphi[0], sp.cap) || dalloc32(&sp.vz[0], sp.cap) ||
+        dalloc32(&sp.vr[1], sp.cap) || dalloc32(&sp.vphi[1], sp.cap) || dalloc32(&sp.vz[1], sp.cap) ||
         dalloc32(&sp.w, sp.cap) || dalloc32(&sp.seq, sp.cap) || dalloc32(&sp.id, sp.cap) ||
-        dalloc32(&b.vr, sp.cap) || dalloc32(&b.vphi, sp.cap) || dalloc32(&b.vz, sp.cap) || dalloc32(&b.w, sp.cap) ||
+        dalloc32(&b.w, sp.cap) ||
         dalloc32(&b.seq, sp.cap) || dalloc32(&b.id, sp.cap))
       return -1;
   }
@@ -877,9 +919,10 @@ void free32(pcp_ctx *c) {
     Species32 &sp = S->sp[s];
     SortBuffers32 &b = S->sb[s];
     (void)hipFree(sp.cell[0]); (void)hipFree(sp.cell[1]); (void)hipFree(sp.fr[0]); (void)hipFree(sp.fr[1]);
-    (void)hipFree(sp.fz[0]); (void)hipFree(sp.fz[1]); (void)hipFree(sp.vr); (void)hipFree(sp.vphi);
-    (void)hipFree(sp.vz); (void)hipFree(sp.w); (void)hipFree(sp.seq); (void)hipFree(sp.id);
-    (void)hipFree(b.vr); (void)hipFree(b.vphi); (void)hipFree(b.vz); (void)hipFree(b.w);
+    (void)hipFree(sp.fz[0]); (void)hipFree(sp.fz[1]); (void)hipFree(sp.vr[0]); (void)hipFree(sp.vphi[0]);
+    (void)hipFree(sp.vz[0]); (void)hipFree(sp.vr[1]); (void)hipFree(sp.vphi[1]); (void)hipFree(sp.vz[1]);
+    (void)hipFree(sp.w); (void)hipFree(sp.seq); (void)hipFree(sp.id);
+    (void)hipFree(b.w);
     (void)hipFree(b.seq); (void)hipFree(b.id);
   }
   delete S;
@@ -914,15 +957,15 @@ void launch32_step_fused(pcp_ctx *c, int sp) {
 void launch32_mark(pcp_ctx *c, int sp, int only_if_exact) {
   Species32 &s = c->s32->sp[sp];
   if (!s.n || c->g.ccs != PCP_CCS_VB) return;
-  hipLaunchKernelGGL(k32_mark, dim3(pgrid32(c, s.n, 256)), dim3(256), 0, c->stream, c->g, s, sp, c->cur, c->st,
-                     c->marker, only_if_exact);
+  hipLaunchKernelGGL(k32_mark, dim3(pgrid32(c, s.n, 256)), dim3(256), 0, c->stream, c->g, c->s32->f, s, sp, c->cur,
+                     c->st, c->marker, push_const(s, c->g.dt), only_if_exact);
 }
 
 void launch32_deposit_exact(pcp_ctx *c, int sp, int only_if_exact) {
   Species32 &s = c->s32->sp[sp];
-  if (!s.n) return;
+  if (!s.n || c->g.ccs != PCP_CCS_VB) return;
   hipLaunchKernelGGL(k32_deposit_exact, dim3(pgrid32(c, s.n, 256)), dim3(256), 0, c->stream, c->g, c->s32->f, s, sp,
-                     c->cur, c->st, c->marker, only_if_exact);
+                     c->cur, c->st, c->marker, push_const(s, c->g.dt), only_if_exact);
 }
 
 void launch32_zero_j_if_exact(pcp_ctx *c) {
@@ -958,9 +1001,9 @@ int launch32_sort(pcp_ctx *c, int sp) {
   std::swap(s.cell[c->cur], s.cell[c->cur ^ 1]);
   std::swap(s.fr[c->cur], s.fr[c->cur ^ 1]);
   std::swap(s.fz[c->cur], s.fz[c->cur ^ 1]);
-  std::swap(s.vr, sb.vr);
-  std::swap(s.vphi, sb.vphi);
-  std::swap(s.vz, sb.vz);
+  std::swap(s.vr[c->cur], s.vr[c->cur ^ 1]);
+  std::swap(s.vphi[c->cur], s.vphi[c->cur ^ 1]);
+  std::swap(s.vz[c->cur], s.vz[c->cur ^ 1]);
   std::swap(s.w, sb.w);
   std::swap(s.seq, sb.seq);
   std::swap(s.id, sb.id);

@@ -4,7 +4,10 @@
 // in-cell offsets (0 <= f < 1, in units of the cell size) -- 12 B of position
 // with the absolute resolution of a 2^-24 fraction of a cell everywhere in
 // the grid -- and FP32 velocities and weight: 28 B read, 24 B written per
-// particle-step.  Fields and J are FP32.  FP64 is used for the near-axis
+// particle-step.  Fields and J are FP32.  The current is deposited from the
+// unrounded FP64 end point of the move (old split position + FP32
+// displacement), so that a displacement below the offset's resolution still
+// deposits, as in FP64.  FP64 is used for the near-axis
 // radial weights, for wall reflections, for the Villasenor-Buneman terms
 // (their logarithms cancel catastrophically in FP32) and for the energy
 // reductions.
@@ -19,7 +22,11 @@ constexpr uint32_t DEAD_CELL = 0xFFFFFFFFu;
 struct Species32 {
   uint32_t *cell[2];
   float *fr[2], *fz[2];
-  float *vr, *vphi, *vz, *w;
+  // velocities are double-buffered like the positions: a step reads slot
+  // cur and writes slot cur ^ 1, so the pre-push state stays available to
+  // the ordered re-deposition of the same step
+  float *vr[2], *vphi[2], *vz[2];
+  float *w;
   uint64_t *seq, *id;
   long n, cap;
   double q, m;
@@ -27,7 +34,7 @@ struct Species32 {
 };
 
 struct SortBuffers32 {
-  float *vr, *vphi, *vz, *w;
+  float *w;
   uint64_t *seq, *id;
 };
 

@@ -369,10 +369,10 @@ int pcp_upload_species(pcp_ctx *c, int sp, long n, const double *r, const double
       HIP_TRY(hipMemcpy(s.cell[k] + off, cell.data(), (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice));
       HIP_TRY(hipMemcpy(s.fr[k] + off, fr.data(), fb, hipMemcpyHostToDevice));
       HIP_TRY(hipMemcpy(s.fz[k] + off, fz.data(), fb, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(s.vr[k] + off, fvr.data(), fb, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(s.vphi[k] + off, fvp.data(), fb, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(s.vz[k] + off, fvz.data(), fb, hipMemcpyHostToDevice));
     }
-    HIP_TRY(hipMemcpy(s.vr + off, fvr.data(), fb, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(s.vphi + off, fvp.data(), fb, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(s.vz + off, fvz.data(), fb, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(s.w + off, fw.data(), fb, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(s.seq + off, seq.data(), (size_t)n * sizeof(uint64_t), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(s.id + off, ids.data(), (size_t)n * sizeof(uint64_t), hipMemcpyHostToDevice));
@@ -448,7 +448,7 @@ int pcp_download_species(pcp_ctx *c, int sp, double *r, double *z, double *vr, d
       if (z_old) z_old[k] = poz;
       ++k;
     }
-    const float *srcs[4] = {s.vr, s.vphi, s.vz, s.w};
+    const float *srcs[4] = {s.vr[c->cur], s.vphi[c->cur], s.vz[c->cur], s.w};
     double *dsts[4] = {vr, vphi, vz, w};
     for (int q = 0; q < 4; ++q) {
       if (!dsts[q]) continue;

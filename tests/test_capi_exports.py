@@ -45,3 +45,14 @@ def test_create_rejects_bad_config_without_gpu():
     rc = lib.pcp_create(ctypes.byref(cfg), ctypes.byref(h))
     assert rc == -1
     assert b"divide" in lib.pcp_last_error()
+
+
+def test_cpp_driver_links_against_the_library():
+    """The SMB-style C++ driver over include/picopic_solvers.hpp is built and
+    resolves libpicopic_hip.so from its own directory."""
+    import subprocess
+    drv = os.path.join(ROOT, "picopic_amd", "_lib", "smb_driver")
+    assert os.path.exists(drv), "run python -m picopic_amd.build"
+    out = subprocess.run(["ldd", drv], capture_output=True, text=True).stdout
+    line = [l for l in out.splitlines() if "libpicopic_hip.so" in l]
+    assert line and "not found" not in line[0], out

@@ -144,3 +144,41 @@ def test_fp32_field_energy_error(ccs):
     print("field energy", ccs, w_ref, w_gpu, rel)
     assert w_ref > 0
     assert rel <= 1e-5
+
+
+def test_fp32_vb_sub_resolution_radial_step():
+    """Villasenor-Buneman with radial steps far below the FP32 offset's
+    resolution (|v_r| dt ~ 1e-14 m, a 1e-11 fraction of a cell): the FP64
+    reference treats them as ordinary moves (they are above its 1e-15 m
+    strict-motion threshold), so the FP32 path must deposit them from the
+    unrounded end point and must not take the strict branch that resets J_r
+    nodes."""
+    capi = _cap()
+    size, grid = Hh.geometry(16, 32)
+    dt = 0.5 * (size[1] / grid[1]) / 3e8
+    species = [(-1, 1, False), (1, 1836, False)]
+    orc = O.SMB(size, grid, (2, 2), dt, species, ccs="vb")
+    gpu = capi.Context(size, grid, (2, 2), dt, [s + (8000,) for s in species], ccs="vb", precision="fp32")
+    rng = np.random.default_rng(9)
+    dr, dz = size[0] / grid[0], size[1] / grid[1]
+    for sp, (q, m, b) in enumerate(species):
+        n = 2000
+        r, z, vr, vp, vz, w = Hh.make_particles(rng, size, grid, n, m, fast_fraction=0.0, special=False,
+                                                temp_ev=1.0 if m == 1 else 0.1)
+        if sp == 1:
+            # a quarter of the ions: radial speed of a few mm/s, no azimuthal speed
+            k = n // 4
+            vr[:k] = rng.choice([-1.0, 1.0], k) * rng.uniform(2e-3, 5e-3, k)
+            vp[:k] = 0.0
+        ir, iz = np.floor(r / dr), np.floor(z / dz)
+        r = (ir + (r / dr - ir).astype(np.float32).astype(np.float64)) * dr
+        z = (iz + (z / dz - iz).astype(np.float32).astype(np.float64)) * dz
+        vr, vp, vz, w = (a.astype(np.float32).astype(np.float64) for a in (vr, vp, vz, w))
+        ids = np.arange(n, dtype=np.uint64) + np.uint64(sp * 10 ** 6)
+        orc.add_particles(sp, r, z, vr, vp, vz, w, ids)
+        gpu.upload(sp, r, z, vr, vp, vz, w, ids)
+    orc.step(1)
+    gpu.step(1)
+    e = Hh.relerr(interior(gpu.get_grid("J")), interior(orc.get_grid("J")))
+    print("J", e)
+    assert e <= 1e-5
