@@ -15,8 +15,9 @@ LIB_PATH = os.path.join(HERE, "_build", "libpicopic_oracle.so")
 PUSHERS = {"boris": 0, "boris-relativistic": 1, "boris-adaptive": 2, "vay": 3, "hc": 4}
 CCS = {"vb": 0, "zigzag": 1}
 GRIDS = {"E": 0, "H": 1, "H_at_et": 2, "J": 3, "eps": 4, "sigma": 5}
-PHASES = ["yee_e", "overlay_e", "yee_h", "overlay_h", "push", "move", "migrate",
+PHASES = ["yee_e", "overlay_e", "yee_h", "overlay_h", "push", "collide", "move", "migrate",
           "reset_j", "deposit", "overlay_j"]
+COLL = {"p12": 0, "ta77s": 1, "sk98": 2}
 
 EL_CHARGE = 1.6e-19
 EL_MASS = 9.1e-31
@@ -94,6 +95,15 @@ def lib():
         L.orc_p12_collide_single.argtypes = [C.c_double] * 4 + [dp, C.c_double, dp, C.c_double] + [
             C.c_double] * 4 + [C.POINTER(OrcRng)]
         L.orc_smb_collide_p12.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(OrcRng)]
+        for f in ("orc_ta77s_collide_single", "orc_sk98_collide_single"):
+            getattr(L, f).argtypes = L.orc_p12_collide_single.argtypes
+        L.orc_smb_collide.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_uint64]
+        L.orc_smb_set_collisions.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_uint64]
+        L.orc_coll_uniforms.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint64, dp]
+        L.orc_coll_key.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
+        L.orc_coll_key.restype = C.c_uint32
+        L.orc_philox4x32.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.POINTER(C.c_uint32)]
+        L.orc_philox4x32_raw.argtypes = [C.POINTER(C.c_uint32)] * 3
         L.orc_debye_length.argtypes = [C.c_double] * 4
         L.orc_debye_length.restype = C.c_double
         L.orc_coulomb_logarithm.argtypes = [C.c_double] * 4
@@ -216,6 +226,15 @@ class SMB:
     def step(self, n=1):
         lib().orc_smb_step(self.h, n)
 
+    def collide(self, scheme, se, si, seed, step):
+        """One collision pass over every domain with the counter-based draws."""
+        for d in range(self.n_domains):
+            lib().orc_smb_collide(self.h, d, COLL[scheme], se, si, seed, step)
+
+    def set_collisions(self, scheme, se=0, si=1, seed=0):
+        """Collision phase inside step() (scheme None: off)."""
+        lib().orc_smb_set_collisions(self.h, -1 if scheme is None else COLL[scheme], se, si, seed)
+
     def get_field_e(self, d, r, z):
         out = np.zeros(3)
         lib().orc_smb_get_field_e(self.h, d, r, z, _dp(out))
@@ -247,12 +266,37 @@ def lorenz_factor_inv(sq):
     return lib().orc_lorenz_factor_inv(sq)
 
 
-def p12_collide_single(m_a, m_b, q_a, q_b, va, wa, vb, wb, dens_a, dens_b, debye, dt, uniforms):
-    """One P12 pair collision with the given uniform draws (3 per call)."""
+def collide_single(scheme, m_a, m_b, q_a, q_b, va, wa, vb, wb, dens_a, dens_b, debye, dt, uniforms):
+    """One pair collision of `scheme` with the given uniform draws."""
     va = np.array(va, dtype=np.float64)
     vb = np.array(vb, dtype=np.float64)
     u = np.ascontiguousarray(uniforms, dtype=np.float64)
     rng = OrcRng(_dp(u), len(u), 0)
-    lib().orc_p12_collide_single(m_a, m_b, q_a, q_b, _dp(va), wa, _dp(vb), wb, dens_a, dens_b, debye, dt,
-                                 C.byref(rng))
+    getattr(lib(), "orc_%s_collide_single" % scheme)(m_a, m_b, q_a, q_b, _dp(va), wa, _dp(vb), wb, dens_a, dens_b,
+                                                       debye, dt, C.byref(rng))
     return va, vb, rng.pos
+
+
+def p12_collide_single(m_a, m_b, q_a, q_b, va, wa, vb, wb, dens_a, dens_b, debye, dt, uniforms):
+    """One P12 pair collision with the given uniform draws (3 per call)."""
+    return collide_single("p12", m_a, m_b, q_a, q_b, va, wa, vb, wb, dens_a, dens_b, debye, dt, uniforms)
+
+
+def philox4x32(ctr, draw, seed):
+    out = (C.c_uint32 * 4)()
+    lib().orc_philox4x32(ctr, draw, seed, out)
+    return list(out)
+
+
+def philox4x32_raw(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    out = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_raw(c, k, out)
+    return list(out)
+
+
+def coll_uniforms(seed, cell, q, step):
+    u = np.zeros(4)
+    lib().orc_coll_uniforms(seed, cell, q, step, _dp(u))
+    return u

@@ -120,8 +120,8 @@ void orc_smb_set_grid(orc_smb *s, int which, const double *in);
 
 /* step phases (reference src/SMB.cpp:1231-1306, src/domain.cpp:118-204) */
 enum { ORC_PH_YEE_E = 0, ORC_PH_OVERLAY_E, ORC_PH_YEE_H, ORC_PH_OVERLAY_H,
-       ORC_PH_PUSH, ORC_PH_MOVE, ORC_PH_MIGRATE, ORC_PH_RESET_J, ORC_PH_DEPOSIT,
-       ORC_PH_OVERLAY_J, ORC_PH_COUNT };
+       ORC_PH_PUSH, ORC_PH_COLLIDE, ORC_PH_MOVE, ORC_PH_MIGRATE, ORC_PH_RESET_J,
+       ORC_PH_DEPOSIT, ORC_PH_OVERLAY_J, ORC_PH_COUNT };
 void orc_smb_phase(orc_smb *s, int phase);
 /* solve_maxvell + advance_particles + solve_current, n times (no beam injection) */
 void orc_smb_step(orc_smb *s, int n);
@@ -158,6 +158,34 @@ void orc_p12_collide_single(double m_a, double m_b, double q_a, double q_b,
                             double dens_a, double dens_b, double debye, double dt,
                             orc_rng_stream *rng);
 double orc_debye_length(double n_e, double n_i, double t_e, double t_i);
+
+/* collision schemes: configure.ac --with-coulomb-collisions (P12 default) */
+enum { ORC_COLL_P12 = 0, ORC_COLL_TA77S = 1, ORC_COLL_SK98 = 2 };
+/* TA77S and SK98 collide_single (collisionsTA77S.cpp:31-219,
+ * collisionsSK98.cpp:29-244); stream draws: normal deviate from two
+ * uniforms (Box-Muller), Phi, and (TA77S) dzeta */
+void orc_ta77s_collide_single(double m_a, double m_b, double q_a, double q_b,
+                              double *va, double wa, double *vb, double wb,
+                              double dens_a, double dens_b, double debye, double dt,
+                              orc_rng_stream *rng);
+void orc_sk98_collide_single(double m_a, double m_b, double q_a, double q_b,
+                             double *va, double wa, double *vb, double wb,
+                             double dens_a, double dens_b, double debye, double dt,
+                             orc_rng_stream *rng);
+/* Philox-4x32-10 (Salmon et al., SC11) as the device library computes it:
+ * counter (ctr, draw, 0x5eed), key = seed */
+void orc_philox4x32(uint64_t ctr, uint32_t draw, uint64_t seed, uint32_t out[4]);
+void orc_philox4x32_raw(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* One collision pass of `scheme` on domain d with the counter-based draws of
+ * the device library: each cell's particles are shuffled by sorting on a
+ * Philox key of (particle id, step) (ties by id) -- the reference's
+ * time-seeded random_shuffle has no reproducible permutation -- and pair q of
+ * global cell c takes the four uniforms orc_coll_uniforms(seed, c, q, step). */
+void orc_smb_collide(orc_smb *s, int d, int scheme, int se, int si, uint64_t seed, uint64_t step);
+void orc_coll_uniforms(uint64_t seed, uint64_t cell, uint32_t q, uint64_t step, double u[4]);
+uint32_t orc_coll_key(uint64_t seed, uint64_t id, uint64_t step);
+/* enable the collision phase of orc_smb_step (scheme -1: off) */
+void orc_smb_set_collisions(orc_smb *s, int scheme, int se, int si, uint64_t seed);
 double orc_coulomb_logarithm(double m_a, double m_b, double debye, double v_rel);
 
 #ifdef __cplusplus
