@@ -74,8 +74,8 @@ def run_gpu(w, steps, warmup, prec):
 def cpu_baseline(w, threads):
     """The oracle (a C restatement of the reference CPU path, AoS particles,
     OpenMP over domains) on a slab of the same plasma: all 512 radial cells x
-    512 axial cells, 32 domains, same ppc, 1 warm-up + 12 timed steps (about
-    10-15 s on 16 cores)."""
+    512 axial cells, 32 domains, same ppc, 1 warm-up + 20 timed steps (about
+    13 s on 16 cores)."""
     import numpy as np
 
     from oracle import oracle as O
@@ -99,7 +99,7 @@ def cpu_baseline(w, threads):
     O.set_threads(threads)
     smb.step(1)
     t0 = time.perf_counter()
-    nsteps = 12
+    nsteps = 20
     smb.step(nsteps)
     dt_wall = time.perf_counter() - t0
     total = 2 * n

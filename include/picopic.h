@@ -40,6 +40,9 @@ enum { PCP_CCS_VB = 0, PCP_CCS_ZIGZAG = 1 };
  * MaxwellSolverYee::field_h_at_et, Current::current, epsilon, sigma */
 enum { PCP_GRID_E = 0, PCP_GRID_H = 1, PCP_GRID_H_AT_ET = 2, PCP_GRID_J = 3,
        PCP_GRID_EPS = 4, PCP_GRID_SIGMA = 5 };
+/* binary collision schemes: configure.ac --with-coulomb-collisions
+ * (CollisionsP12 is the reference default) */
+enum { PCP_COLL_P12 = 0, PCP_COLL_TA77S = 1, PCP_COLL_SK98 = 2 };
 /* precision of the particle/field storage, fixed at creation.  FP32_SPLIT
  * stores each position as a packed cell index plus FP32 in-cell offsets and
  * keeps fields, velocities and J in FP32; such contexts run pcp_step and the
@@ -142,6 +145,22 @@ int pcp_deposit(pcp_ctx *ctx);
 /* n whole steps: solve_maxvell, advance_particles, solve_current
  * (src/PiCoPiC.cpp:299-345) with push + move + deposit fused */
 int pcp_step(pcp_ctx *ctx, int n_steps);
+
+/* Collisions::operator() (include/collisions.hpp:49; P12
+ * src/collisions/collisionsP12.cpp:571-579, TA77S
+ * src/collisions/collisionsTA77S.cpp:568-576, SK98
+ * src/collisions/collisionsSK98.cpp:441-450) for every domain: one pass
+ * between electron species sp_el and ion species sp_ion on the current
+ * velocities.  random_sort is an ordering of each cell's particles by a
+ * Philox key of (particle id, pass) -- the reference shuffles with a
+ * time-seeded std::random_shuffle -- and pair q of global cell c takes the
+ * uniforms Philox(seed; c << 24 | q, 2 pass + 1).  At most 256 macro-particles
+ * of a species per cell (PCP_E_STATE beyond). */
+int pcp_collide(pcp_ctx *ctx, int scheme, int sp_el, int sp_ion, uint64_t seed, uint64_t pass);
+/* Enable (scheme >= 0) or disable (scheme < 0) the collision phase of
+ * pcp_step, run between the push and the move like Domain::collide in
+ * SMB::advance_particles (src/SMB.cpp:1287-1289); passes count from 0. */
+int pcp_set_collisions(pcp_ctx *ctx, int scheme, int sp_el, int sp_ion, uint64_t seed);
 
 /* Counting sort of species sp by (domain, cell), dropping removed particles
  * (no reference counterpart: the reference keeps per-domain vectors).  The

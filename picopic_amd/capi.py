@@ -26,7 +26,9 @@ EXPORTS = [
     "pcp_grid_points", "pcp_yee_e", "pcp_yee_h", "pcp_stitch", "pcp_push", "pcp_move", "pcp_migrate",
     "pcp_deposit", "pcp_step", "pcp_get_field_e", "pcp_get_field_h", "pcp_stream", "pcp_set_timing",
     "pcp_kernel_ms", "pcp_sort", "pcp_set_sort_interval", "pcp_copy_bandwidth", "pcp_field_energy", "pcp_kinetic_energy",
+    "pcp_collide", "pcp_set_collisions",
 ]
+COLL = {"p12": 0, "ta77s": 1, "sk98": 2}
 
 
 class SpeciesDesc(C.Structure):
@@ -87,6 +89,8 @@ def load(path=LIB_PATH):
     L.pcp_copy_bandwidth.argtypes = [vp, C.c_long, C.c_int, C.POINTER(C.c_double)]
     L.pcp_field_energy.argtypes = [vp, C.POINTER(C.c_double)]
     L.pcp_kinetic_energy.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
+    L.pcp_collide.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_uint64]
+    L.pcp_set_collisions.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_uint64]
     _lib = L
     return L
 
@@ -222,6 +226,14 @@ class Context:
 
     def sort(self, sp):
         _check(_lib.pcp_sort(self.h, sp))
+
+    def collide(self, scheme, se, si, seed, step):
+        """One collision pass (P12 / TA77S / SK98) on the current velocities."""
+        _check(_lib.pcp_collide(self.h, COLL[scheme], se, si, seed, step))
+
+    def set_collisions(self, scheme, se=0, si=1, seed=0):
+        """Collision phase inside step() (scheme None: off)."""
+        _check(_lib.pcp_set_collisions(self.h, -1 if scheme is None else COLL[scheme], se, si, seed))
 
     def set_sort_interval(self, sp, every):
         _check(_lib.pcp_set_sort_interval(self.h, sp, every))

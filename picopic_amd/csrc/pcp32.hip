@@ -9,6 +9,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include "pcp32.hpp"
+#include "pcp_rng.hpp"
 #include "pcp_kernels.hpp"
 #include "pcp_physics.hpp"
 #include "pcp_wave.hpp"
@@ -433,24 +434,53 @@ __device__ __forceinline__ void append_migrant32(Migrants mg, StepState *st, int
 // ---------------------------------------------------------------------------
 // the fused FP32 step kernel
 // ---------------------------------------------------------------------------
-// Gather, push and move of live particle p from slot `from` (positions and
-// velocities).  The exact re-deposition calls it again on the pre-step slot
-// and gets bit-identical (n, v, rd, zd).
+// Gather, push and move of live particle p: positions from slot `from`,
+// velocities from slot `vfrom`; do_push = false moves with the stored
+// velocity (steps whose collision phase sits between push and move).  The
+// exact re-deposition calls it again on the pre-step slots and gets
+// bit-identical (n, v, rd, zd).
 __device__ __forceinline__ bool advance32(const Geom &g, const FieldsT<float> &f, const Species32 &s, long p,
-                                          uint32_t cell, int from, const PushConst &K, float dtr, float dtz, Split &o,
-                                          Split &n, float v[3], double &rd, double &zd, int &d_old, int *err) {
+                                          uint32_t cell, int from, int vfrom, bool do_push, const PushConst &K,
+                                          float dtr, float dtz, Split &o, Split &n, float v[3], double &rd,
+                                          double &zd, int &d_old, int *err) {
   o = unpack(cell, s.fr[from][p], s.fz[from][p], g.Nz);
   d_old = domain_of32(g, o.cr, o.cz);
   const Dom dm = make_dom(g, d_old);
-  GridView32 EV{f.E + (long)d_old * 3 * g.P, g.P, g.yr, dm.bot_r, dm.left_z};
-  GridView32 HV{f.H + (long)d_old * 3 * g.P, g.P, g.yr, dm.bot_r, dm.left_z};
-  float e[3], h[3];
-  gather32_e(EV, o.cr, o.cz, o.fr, o.fz, e);
-  gather32_h(HV, o.cr, o.cz, o.fr, o.fz, h);
-  v[0] = s.vr[from][p]; v[1] = s.vphi[from][p]; v[2] = s.vz[from][p];
-  push32(g.pusher, K, e, h, v, err);
+  v[0] = s.vr[vfrom][p]; v[1] = s.vphi[vfrom][p]; v[2] = s.vz[vfrom][p];
+  if (do_push) {
+    GridView32 EV{f.E + (long)d_old * 3 * g.P, g.P, g.yr, dm.bot_r, dm.left_z};
+    GridView32 HV{f.H + (long)d_old * 3 * g.P, g.P, g.yr, dm.bot_r, dm.left_z};
+    float e[3], h[3];
+    gather32_e(EV, o.cr, o.cz, o.fr, o.fz, e);
+    gather32_h(HV, o.cr, o.cz, o.fr, o.fz, h);
+    push32(g.pusher, K, e, h, v, err);
+  }
   n = o;
   return move32(g, dm, s.beam, dtr, dtz, n, v, rd, zd);
+}
+
+// gather + push only (steps with a collision phase): velocities of slot cur
+// pushed into slot cur ^ 1
+__global__ void __launch_bounds__(256) k32_push(Geom g, FieldsT<float> f, Species32 s, int cur, StepState *st,
+                                                PushConst K) {
+  int err = 0;
+  const int nxt = cur ^ 1;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < s.n; p += (long)gridDim.x * blockDim.x) {
+    uint32_t cell = s.cell[cur][p];
+    if (cell == DEAD_CELL) continue;
+    Split o = unpack(cell, s.fr[cur][p], s.fz[cur][p], g.Nz);
+    const int d = domain_of32(g, o.cr, o.cz);
+    const Dom dm = make_dom(g, d);
+    GridView32 EV{f.E + (long)d * 3 * g.P, g.P, g.yr, dm.bot_r, dm.left_z};
+    GridView32 HV{f.H + (long)d * 3 * g.P, g.P, g.yr, dm.bot_r, dm.left_z};
+    float e[3], h[3];
+    gather32_e(EV, o.cr, o.cz, o.fr, o.fz, e);
+    gather32_h(HV, o.cr, o.cz, o.fr, o.fz, h);
+    float v[3] = {s.vr[cur][p], s.vphi[cur][p], s.vz[cur][p]};
+    push32(g.pusher, K, e, h, v, &err);
+    s.vr[nxt][p] = v[0]; s.vphi[nxt][p] = v[1]; s.vz[nxt][p] = v[2];
+  }
+  if (err) atomicOr(&st->err, err);
 }
 
 // domain whose grid receives the current of a particle ending at (rd, zd)
@@ -460,7 +490,10 @@ __device__ __forceinline__ int deposit_domain32(const Geom &g, double rd, double
   return domain_of32(g, cr, cz);
 }
 
-template <int CCS>
+// PUSH: velocities slot cur -> pushed, moved -> slot cur ^ 1.
+// !PUSH: pushed (and collided) velocities of slot cur ^ 1 -> moved -> slot
+// cur; the caller then exchanges the two velocity slots.
+template <int CCS, bool PUSH>
 __global__ void __launch_bounds__(256) k32_step_fused(Geom g, FieldsT<float> f, Species32 s, int sp, int cur,
                                                       StepState *st, Migrants mg, PushConst K) {
   int err = 0;
@@ -479,8 +512,10 @@ __global__ void __launch_bounds__(256) k32_step_fused(Geom g, FieldsT<float> f, 
     if (p < s.n) {
       uint32_t cell = s.cell[cur][p];
       int d_old = 0;
-      if (cell != DEAD_CELL && advance32(g, f, s, p, cell, cur, K, dtr, dtz, o, n, v, rd, zd, d_old, &err)) {
-        s.vr[nxt][p] = v[0]; s.vphi[nxt][p] = v[1]; s.vz[nxt][p] = v[2];
+      const int vin = PUSH ? cur : nxt, vout = PUSH ? nxt : cur;
+      if (cell != DEAD_CELL &&
+          advance32(g, f, s, p, cell, cur, vin, PUSH, K, dtr, dtz, o, n, v, rd, zd, d_old, &err)) {
+        s.vr[vout][p] = v[0]; s.vphi[vout][p] = v[1]; s.vz[vout][p] = v[2];
         s.cell[nxt][p] = (uint32_t)n.cr * (uint32_t)g.Nz + (uint32_t)n.cz;
         s.fr[nxt][p] = n.fr;
         s.fz[nxt][p] = n.fz;
@@ -547,7 +582,7 @@ __global__ void __launch_bounds__(256) k32_step_fused(Geom g, FieldsT<float> f, 
 // recover the deposit end point, then mark the reset nodes and re-deposit in
 // the reference's particle order, with the FP64 terms of the fused kernel.
 __global__ void __launch_bounds__(256) k32_mark(Geom g, FieldsT<float> f, Species32 s, int sp, int cur, StepState *st,
-                                                uint64_t *marker, PushConst K, int only_if_exact) {
+                                                uint64_t *marker, PushConst K, int do_push, int only_if_exact) {
   if (only_if_exact && !st->need_exact) return;
   const uint64_t stamp = st->step & 0xFFFFFF;
   const float dtr = (float)(g.dt / g.dr), dtz = (float)(g.dt / g.dz);
@@ -559,7 +594,8 @@ __global__ void __launch_bounds__(256) k32_mark(Geom g, FieldsT<float> f, Specie
     float v[3];
     double r, z;
     int d_old;
-    if (!advance32(g, f, s, p, co, cur ^ 1, K, dtr, dtz, o, n, v, r, z, d_old, &dummy)) continue;
+    if (!advance32(g, f, s, p, co, cur ^ 1, cur ^ 1, do_push != 0, K, dtr, dtz, o, n, v, r, z, d_old, &dummy))
+      continue;
     const double ro = ((double)o.cr + (double)o.fr) * g.dr, zo = ((double)o.cz + (double)o.fz) * g.dz;
     if (!vb_is_strict_reset(g.dr, g.dz, r, z, ro, zo)) continue;
     int d = deposit_domain32(g, r, z);
@@ -573,7 +609,7 @@ __global__ void __launch_bounds__(256) k32_mark(Geom g, FieldsT<float> f, Specie
 
 __global__ void __launch_bounds__(256) k32_deposit_exact(Geom g, FieldsT<float> f, Species32 s, int sp, int cur,
                                                          StepState *st, const uint64_t *marker, PushConst K,
-                                                         int only_if_exact) {
+                                                         int do_push, int only_if_exact) {
   if (only_if_exact && !st->need_exact) return;
   int err = 0, dummy = 0;
   const uint64_t stamp = st->step & 0xFFFFFF;
@@ -585,7 +621,8 @@ __global__ void __launch_bounds__(256) k32_deposit_exact(Geom g, FieldsT<float> 
     float v[3];
     double r, z;
     int d_old;
-    if (!advance32(g, f, s, p, co, cur ^ 1, K, dtr, dtz, o, n, v, r, z, d_old, &dummy)) continue;
+    if (!advance32(g, f, s, p, co, cur ^ 1, cur ^ 1, do_push != 0, K, dtr, dtz, o, n, v, r, z, d_old, &dummy))
+      continue;
     const double ro = ((double)o.cr + (double)o.fr) * g.dr, zo = ((double)o.cz + (double)o.fz) * g.dz;
     int d = deposit_domain32(g, r, z);
     Dom dm = make_dom(g, d);
@@ -674,35 +711,6 @@ __global__ void k32_count_live(const uint32_t *cell, long n, unsigned long long 
 // ---------------------------------------------------------------------------
 // synthetic inputs in split form (same sampling as pcp_init.hip)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void philox_round32(uint32_t c[4], uint32_t k[2]) {
-  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
-  uint64_t p0 = (uint64_t)M0 * c[0];
-  uint64_t p1 = (uint64_t)M1 * c[2];
-  uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k[0];
-  uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k[1];
-  c[0] = n0; c[1] = (uint32_t)p1; c[2] = n2; c[3] = (uint32_t)p0;
-}
-
-__device__ __forceinline__ void uniform2_32(uint64_t ctr, uint32_t draw, uint64_t seed, double &u0, double &u1) {
-  uint32_t c[4] = {(uint32_t)ctr, (uint32_t)(ctr >> 32), draw, 0x5eedu};
-  uint32_t k[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
-  for (int r = 0; r < 10; ++r) {
-    philox_round32(c, k);
-    k[0] += 0x9E3779B9u;
-    k[1] += 0xBB67AE85u;
-  }
-  const double s = 1.0 / 9007199254740992.0;
-  u0 = (double)(((uint64_t)(c[0] >> 5) << 26) | (c[1] >> 6)) * s;
-  u1 = (double)(((uint64_t)(c[2] >> 5) << 26) | (c[3] >> 6)) * s;
-}
-
-__device__ __forceinline__ void normal2_32(uint64_t ctr, uint32_t draw, uint64_t seed, double &n0, double &n1) {
-  double u0, u1;
-  uniform2_32(ctr, draw, seed, u0, u1);
-  double rad = sqrt(-2.0 * log(1.0 - u0));
-  n0 = rad * cos(2.0 * PI * u1);
-  n1 = rad * sin(2.0 * PI * u1);
-}
 
 __device__ __forceinline__ void store_split(Species32 &s, long q, const Geom &g, double r, double z) {
   double ur = r / g.dr, uz = z / g.dz;
@@ -733,9 +741,9 @@ __global__ void k32_init_plasma(Geom g, Species32 s, long base, long n_new, cons
     Dom dm = make_dom(g, d);
     long gi = dm.bot_r + li, gk = dm.left_z + lk;
     double u0, u1, n0, n1, n2, n3;
-    uniform2_32((uint64_t)(base + p), 0, seed, u0, u1);
-    normal2_32((uint64_t)(base + p), 1, seed, n0, n1);
-    normal2_32((uint64_t)(base + p), 2, seed, n2, n3);
+    uniform2((uint64_t)(base + p), 0, seed, u0, u1);
+    normal2((uint64_t)(base + p), 1, seed, n0, n1);
+    normal2((uint64_t)(base + p), 2, seed, n2, n3);
     double r0 = gi * g.dr, r1 = (gi + 1) * g.dr;
     double r = sqrt(r0 * r0 + u0 * (r1 * r1 - r0 * r0));
     double z = (gk + u1) * g.dz;
@@ -758,8 +766,8 @@ __global__ void k32_init_bunch(Geom g, Species32 s, long base, long n_new, doubl
   double R = g.gsize_r, L = g.gsize_z;
   for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n_new; p += (long)gridDim.x * blockDim.x) {
     double n0, n1, n2, n3;
-    normal2_32((uint64_t)(base + p), 7, seed, n0, n1);
-    normal2_32((uint64_t)(base + p), 8, seed, n2, n3);
+    normal2((uint64_t)(base + p), 7, seed, n0, n1);
+    normal2((uint64_t)(base + p), 8, seed, n2, n3);
     double x = n0 * sigma_r, y = n1 * sigma_r;
     double r = sqrt(x * x + y * y);
     double z = z0 + n2 * sigma_z;
@@ -942,30 +950,52 @@ static PushConst push_const(const Species32 &s, double dt) {
   return K;
 }
 
-void launch32_step_fused(pcp_ctx *c, int sp) {
+void launch32_push(pcp_ctx *c, int sp) {
+  Species32 &s = c->s32->sp[sp];
+  if (!s.n) return;
+  hipLaunchKernelGGL(k32_push, dim3(pgrid32(c, s.n, 256)), dim3(256), 0, c->stream, c->g, c->s32->f, s, c->cur, c->st,
+                     push_const(s, c->g.dt));
+}
+
+void launch32_step_fused(pcp_ctx *c, int sp, bool push) {
   Species32 &s = c->s32->sp[sp];
   if (!s.n) return;
   PushConst K = push_const(s, c->g.dt);
-  if (c->g.ccs == PCP_CCS_ZIGZAG)
-    hipLaunchKernelGGL(k32_step_fused<PCP_CCS_ZIGZAG>, dim3(pgrid32(c, s.n, 256)), dim3(256), 0, c->stream, c->g,
-                       c->s32->f, s, sp, c->cur, c->st, c->mig, K);
-  else
-    hipLaunchKernelGGL(k32_step_fused<PCP_CCS_VB>, dim3(pgrid32(c, s.n, 256)), dim3(256), 0, c->stream, c->g,
-                       c->s32->f, s, sp, c->cur, c->st, c->mig, K);
+  const dim3 grid(pgrid32(c, s.n, 256)), block(256);
+  if (c->g.ccs == PCP_CCS_ZIGZAG) {
+    if (push)
+      hipLaunchKernelGGL((k32_step_fused<PCP_CCS_ZIGZAG, true>), grid, block, 0, c->stream, c->g, c->s32->f, s, sp,
+                         c->cur, c->st, c->mig, K);
+    else
+      hipLaunchKernelGGL((k32_step_fused<PCP_CCS_ZIGZAG, false>), grid, block, 0, c->stream, c->g, c->s32->f, s, sp,
+                         c->cur, c->st, c->mig, K);
+  } else {
+    if (push)
+      hipLaunchKernelGGL((k32_step_fused<PCP_CCS_VB, true>), grid, block, 0, c->stream, c->g, c->s32->f, s, sp, c->cur,
+                         c->st, c->mig, K);
+    else
+      hipLaunchKernelGGL((k32_step_fused<PCP_CCS_VB, false>), grid, block, 0, c->stream, c->g, c->s32->f, s, sp,
+                         c->cur, c->st, c->mig, K);
+  }
+  if (!push) {
+    std::swap(s.vr[0], s.vr[1]);
+    std::swap(s.vphi[0], s.vphi[1]);
+    std::swap(s.vz[0], s.vz[1]);
+  }
 }
 
-void launch32_mark(pcp_ctx *c, int sp, int only_if_exact) {
+void launch32_mark(pcp_ctx *c, int sp, int do_push, int only_if_exact) {
   Species32 &s = c->s32->sp[sp];
   if (!s.n || c->g.ccs != PCP_CCS_VB) return;
   hipLaunchKernelGGL(k32_mark, dim3(pgrid32(c, s.n, 256)), dim3(256), 0, c->stream, c->g, c->s32->f, s, sp, c->cur,
-                     c->st, c->marker, push_const(s, c->g.dt), only_if_exact);
+                     c->st, c->marker, push_const(s, c->g.dt), do_push, only_if_exact);
 }
 
-void launch32_deposit_exact(pcp_ctx *c, int sp, int only_if_exact) {
+void launch32_deposit_exact(pcp_ctx *c, int sp, int do_push, int only_if_exact) {
   Species32 &s = c->s32->sp[sp];
   if (!s.n || c->g.ccs != PCP_CCS_VB) return;
   hipLaunchKernelGGL(k32_deposit_exact, dim3(pgrid32(c, s.n, 256)), dim3(256), 0, c->stream, c->g, c->s32->f, s, sp,
-                     c->cur, c->st, c->marker, push_const(s, c->g.dt), only_if_exact);
+                     c->cur, c->st, c->marker, push_const(s, c->g.dt), do_push, only_if_exact);
 }
 
 void launch32_zero_j_if_exact(pcp_ctx *c) {

@@ -50,9 +50,13 @@ void free32(pcp_ctx *c);
 void launch32_yee_e(pcp_ctx *c, int clear_j);
 void launch32_yee_h(pcp_ctx *c);
 void launch32_stitch(pcp_ctx *c, int which);
-void launch32_step_fused(pcp_ctx *c, int sp);
-void launch32_mark(pcp_ctx *c, int sp, int only_if_exact);
-void launch32_deposit_exact(pcp_ctx *c, int sp, int only_if_exact);
+// push = false: the step's velocities were pushed (launch32_push) and
+// collided beforehand; the launcher exchanges the velocity slots after the
+// move so that slot cur ^ 1 keeps the pre-move velocities for the exact path
+void launch32_push(pcp_ctx *c, int sp);
+void launch32_step_fused(pcp_ctx *c, int sp, bool push = true);
+void launch32_mark(pcp_ctx *c, int sp, int do_push, int only_if_exact);
+void launch32_deposit_exact(pcp_ctx *c, int sp, int do_push, int only_if_exact);
 void launch32_zero_j_if_exact(pcp_ctx *c);
 void launch32_migrant_keys(pcp_ctx *c);
 int launch32_sort(pcp_ctx *c, int sp);

@@ -14,6 +14,7 @@
 #include "pcp_kernels.hpp"
 #include "pcp_sort.hpp"
 #include "pcp32.hpp"
+#include "pcp_collide.hpp"
 
 using namespace pcp;
 
@@ -88,10 +89,12 @@ int check_state(pcp_ctx *c) {
   StepState h;
   HIP_TRY(hipMemcpyAsync(&h, c->st, sizeof h, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  int fatal = h.err & (ERR_NAN_POSITION | ERR_BETA_GT_1 | ERR_NAN_FIELD | ERR_MIGRANT_OVERFLOW);
+  int fatal = h.err & (ERR_NAN_POSITION | ERR_BETA_GT_1 | ERR_NAN_FIELD | ERR_MIGRANT_OVERFLOW | ERR_COLL_OVERFLOW);
   if (fatal) {
-    char buf[160];
-    snprintf(buf, sizeof buf, "device reported error bits 0x%x (1 NaN position, 2 beta>1, 4 NaN field, 8 migrant overflow)",
+    char buf[256];
+    snprintf(buf, sizeof buf,
+             "device reported error bits 0x%x (1 NaN position, 2 beta>1, 4 NaN field, 8 migrant overflow, "
+             "32 more than 256 macro-particles of a species in a collision cell)",
              h.err);
     int zero = 0;
     hipMemcpy(&c->st->err, &zero, sizeof zero, hipMemcpyHostToDevice);
@@ -296,6 +299,7 @@ int pcp_create(const pcp_config *cfg, pcp_ctx **out) {
 
 int pcp_destroy(pcp_ctx *c) {
   if (!c) return 0;
+  coll_free(c);
   hipFree(c->f.E); hipFree(c->f.H); hipFree(c->f.Hat); hipFree(c->f.J);
   hipFree(c->f.eps); hipFree(c->f.sigma); hipFree(c->st); hipFree(c->marker);
   for (int s = 0; s < MAX_SPECIES; ++s) {
@@ -716,6 +720,36 @@ int pcp_deposit(pcp_ctx *c) {
   return pcp_sync(c);
 }
 
+static int check_coll_args(pcp_ctx *c, int scheme, int se, int si) {
+  if (!c) return fail(PCP_E_ARG, "null context");
+  if (scheme < PCP_COLL_P12 || scheme > PCP_COLL_SK98) return fail(PCP_E_ARG, "unknown collision scheme");
+  if (se < 0 || si < 0 || se >= c->cfg.n_species || si >= c->cfg.n_species || se == si)
+    return fail(PCP_E_ARG, "collisions need two distinct species (electrons, ions)");
+  return 0;
+}
+
+int pcp_collide(pcp_ctx *c, int scheme, int se, int si, uint64_t seed, uint64_t pass) {
+  if (int rc = check_coll_args(c, scheme, se, si)) return rc;
+  if (launch_collide(c, scheme, se, si, seed, pass, c->cur) < 0) return fail(PCP_E_HIP, "collision pass failed");
+  HIP_TRY(hipGetLastError());
+  return pcp_sync(c);
+}
+
+int pcp_set_collisions(pcp_ctx *c, int scheme, int se, int si, uint64_t seed) {
+  if (!c) return fail(PCP_E_ARG, "null context");
+  if (scheme < 0) {
+    c->coll_scheme = -1;
+    return 0;
+  }
+  if (int rc = check_coll_args(c, scheme, se, si)) return rc;
+  c->coll_scheme = scheme;
+  c->coll_se = se;
+  c->coll_si = si;
+  c->coll_seed = seed;
+  c->coll_calls = 0;
+  return 0;
+}
+
 int pcp_step(pcp_ctx *c, int n_steps) {
   if (!c || n_steps < 0) return fail(PCP_E_ARG, "bad argument");
   // optional per-phase device timing: events on the context stream
@@ -748,9 +782,19 @@ int pcp_step(pcp_ctx *c, int n_steps) {
           (f32 ? launch32_sort(c, s) : launch_sort(c, s)) < 0)
         return fail(PCP_E_HIP, "sort failed");
     mark(t, 1);
+    const bool coll = c->coll_scheme >= 0;
+    if (coll) {
+      // SMB::advance_particles: push, collide (before the move), move
+      for (int s = 0; s < c->cfg.n_species; ++s) {
+        if (f32) launch32_push(c, s);
+        else launch_push(c, s);
+      }
+      if (launch_collide(c, c->coll_scheme, c->coll_se, c->coll_si, c->coll_seed, c->coll_calls++, c->cur ^ 1) < 0)
+        return fail(PCP_E_HIP, "collision pass failed");
+    }
     for (int s = 0; s < c->cfg.n_species; ++s) {
-      if (f32) launch32_step_fused(c, s);
-      else launch_step_fused(c, s);
+      if (f32) launch32_step_fused(c, s, !coll);
+      else launch_step_fused(c, s, !coll);
     }
     c->cur ^= 1;
     mark(t, 2);
@@ -760,8 +804,8 @@ int pcp_step(pcp_ctx *c, int n_steps) {
     if (c->g.ccs == PCP_CCS_VB) {
       if (f32) {
         launch32_zero_j_if_exact(c);
-        for (int s = 0; s < c->cfg.n_species; ++s) launch32_mark(c, s, 1);
-        for (int s = 0; s < c->cfg.n_species; ++s) launch32_deposit_exact(c, s, 1);
+        for (int s = 0; s < c->cfg.n_species; ++s) launch32_mark(c, s, !coll, 1);
+        for (int s = 0; s < c->cfg.n_species; ++s) launch32_deposit_exact(c, s, !coll, 1);
       } else {
         launch_zero_j_if_exact(c);
         for (int s = 0; s < c->cfg.n_species; ++s) launch_mark(c, s, 1);

@@ -1,0 +1,590 @@
+// Binary Coulomb collisions on the device: the Collisions base pass
+// (src/collisions.cpp:87-236: sort_to_cells, random_sort,
+// collect_weighted_params_tot_grid) and the pairing loop shared by
+// CollisionsP12 / TA77S / SK98 (collisionsP12.cpp:330-530), plus TA77S
+// correct_velocities (collisionsTA77S.cpp:419-566).
+//
+// Pipeline per pass (two species: electrons `se`, ions `si`):
+//   k_coll_key      per particle: its (domain, cell) bucket and a slot in it
+//   scan            bucket offsets (hipCUB)
+//   k_coll_scatter  bucket lists of particle indices
+//   k_coll_prep     one wavefront per cell: shuffle each bucket by sorting on
+//                   a Philox key of (particle id, pass), per-cell sums in the
+//                   shuffled order, density / temperature / Debye length, and
+//                   the first cell of each r-row whose values are not normal
+//                   (the reference's `break` ends the rest of that row)
+//   k_coll_pairs    one wavefront per cell: the reference's pair schedule
+//                   with lanes on independent pair chains, velocities staged
+//                   in LDS in FP64; pair q of global cell c draws the uniforms
+//                   Philox(seed; c << 24 | q, 2 pass + 1)
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <climits>
+
+#include "pcp32.hpp"
+#include "pcp_collide.hpp"
+#include "pcp_collide_physics.hpp"
+#include "pcp_rng.hpp"
+#include "pcp_wave.hpp"
+
+namespace pcp {
+
+namespace {
+
+constexpr int COLL_BLOCK = 64;   // one wavefront per cell
+
+// particle access for FP64 and FP32 split storage
+struct View64 {
+  SpeciesDev s;
+  int cur;
+  __device__ __forceinline__ bool cell_of(const Geom &g, long p, int &cr, int &cz) const {
+    double r = s.r[cur][p];
+    if (!(r >= 0)) return false;
+    cr = cell_number(r, g.dr);
+    cz = cell_number(s.z[cur][p], g.dz);
+    return true;
+  }
+  __device__ __forceinline__ void load(long p, double v[3]) const {
+    v[0] = s.vr[p]; v[1] = s.vphi[p]; v[2] = s.vz[p];
+  }
+  __device__ __forceinline__ void store(long p, const double v[3]) const {
+    s.vr[p] = v[0]; s.vphi[p] = v[1]; s.vz[p] = v[2];
+  }
+  __device__ __forceinline__ double w(long p) const { return s.w[p]; }
+  __device__ __forceinline__ uint64_t id(long p) const { return s.id[p]; }
+  __device__ __forceinline__ long n() const { return s.n; }
+};
+
+struct View32 {
+  Species32 s;
+  int cur;    // position slot
+  int vslot;  // velocity slot
+  __device__ __forceinline__ bool cell_of(const Geom &g, long p, int &cr, int &cz) const {
+    uint32_t c = s.cell[cur][p];
+    if (c == DEAD_CELL) return false;
+    cr = (int)(c / (uint32_t)g.Nz);
+    cz = (int)(c - (uint32_t)cr * (uint32_t)g.Nz);
+    return true;
+  }
+  __device__ __forceinline__ void load(long p, double v[3]) const {
+    v[0] = s.vr[vslot][p]; v[1] = s.vphi[vslot][p]; v[2] = s.vz[vslot][p];
+  }
+  __device__ __forceinline__ void store(long p, const double v[3]) const {
+    s.vr[vslot][p] = (float)v[0]; s.vphi[vslot][p] = (float)v[1]; s.vz[vslot][p] = (float)v[2];
+  }
+  __device__ __forceinline__ double w(long p) const { return (double)s.w[p]; }
+  __device__ __forceinline__ uint64_t id(long p) const { return s.id[p]; }
+  __device__ __forceinline__ long n() const { return s.n; }
+};
+
+__device__ __forceinline__ uint32_t local_cell_key(const Geom &g, int cr, int cz) {
+  int d = (cr / g.nr) * g.Dz + cz / g.nz;
+  long lr = cr - (long)(d / g.Dz) * g.nr, lz = cz - (long)(d % g.Dz) * g.nz;
+  return (uint32_t)((long)d * g.nr * g.nz + lr * g.nz + lz);
+}
+
+template <class V>
+__global__ void __launch_bounds__(256) k_coll_key(Geom g, V v, uint32_t *key, uint32_t *slot, uint32_t *cnt,
+                                                  uint32_t sentinel) {
+  const long n = v.n();
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long b0 = (long)blockIdx.x * blockDim.x; b0 < n; b0 += stride) {
+    const long p = b0 + threadIdx.x;
+    uint32_t k = sentinel;
+    int cr, cz;
+    if (p < n && v.cell_of(g, p, cr, cz)) k = local_cell_key(g, cr, cz);
+    const bool a = p < n && k != sentinel;
+    uint32_t s = wave_reserve(cnt, a ? k : 0, a);
+    if (p < n) {
+      key[p] = k;
+      slot[p] = s;
+    }
+  }
+}
+
+__global__ void k_coll_scatter(long n, const uint32_t *key, const uint32_t *slot, const uint32_t *off,
+                               uint32_t *perm, uint32_t sentinel) {
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long)gridDim.x * blockDim.x) {
+    uint32_t k = key[p];
+    if (k != sentinel) perm[off[k] + slot[p]] = (uint32_t)p;
+  }
+}
+
+__global__ void k_fill_i32(int *p, long n, int v) {
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) p[t] = v;
+}
+
+__global__ void k_fill_u32(uint32_t *p, long n) {
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) p[t] = 0;
+}
+
+// cell c (domain-major key) -> domain, local (i, j), global (cr, cz)
+struct CellPos {
+  int d, i, j;
+  long cr, cz;
+};
+__device__ __forceinline__ CellPos cell_pos(const Geom &g, long c) {
+  CellPos q;
+  long per = (long)g.nr * g.nz;
+  q.d = (int)(c / per);
+  long lc = c - (long)q.d * per;
+  q.i = (int)(lc / g.nz);
+  q.j = (int)(lc - (long)q.i * g.nz);
+  q.cr = (long)(q.d / g.Dz) * g.nr + q.i;
+  q.cz = (long)(q.d % g.Dz) * g.nz + q.j;
+  return q;
+}
+
+// ascending bitonic sort of n (<= COLL_MAXC) keys with their indices, by
+// one wavefront
+__device__ void wave_sort(uint64_t *key, uint32_t *idx, int n) {
+  int P = 1;
+  while (P < n) P <<= 1;
+  for (int t = n + (int)threadIdx.x; t < P; t += COLL_BLOCK) {
+    key[t] = ~0ull;
+    idx[t] = 0xFFFFFFFFu;
+  }
+  __syncthreads();
+  for (int k = 2; k <= P; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = threadIdx.x; t < P; t += COLL_BLOCK) {
+        int l = t ^ j;
+        if (l > t) {
+          bool asc = (t & k) == 0;
+          uint64_t a = key[t], b = key[l];
+          // (key, index) order: padding (key ~0, index ~0) stays behind
+          // every real entry
+          bool gt = a > b || (a == b && idx[t] > idx[l]);
+          if (asc ? gt : !gt) {
+            key[t] = b; key[l] = a;
+            uint32_t x = idx[t]; idx[t] = idx[l]; idx[l] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+}
+
+struct CellSums {
+  double amount, energy, m[3];
+};
+
+// sequential sums in bucket order (collect_weighted_params_tot_grid)
+__device__ CellSums cell_sums(const double *vx, const double *vy, const double *vz, const double *w, int n,
+                              double mass) {
+  CellSums t{0, 0, {0, 0, 0}};
+  for (int k = 0; k < n; ++k) {
+    double vr = vx[k], vphi = vy[k], vzz = vz[k];
+    double v_sq = vr * vr + vphi * vphi + vzz * vzz;
+    double weighted_m = w[k] * mass;
+    t.m[0] += weighted_m * vr;
+    t.m[1] += weighted_m * vphi;
+    t.m[2] += weighted_m * vzz;
+    t.amount += w[k];
+    t.energy += weighted_m * v_sq / 2;
+  }
+  return t;
+}
+
+// per-cell values kept between the two kernels
+constexpr int TOT = 16;   // a_e E_e m_e[3] a_i E_i m_i[3] dens_e dens_i debye
+
+struct CellStage {
+  double vx[2][COLL_MAXC], vy[2][COLL_MAXC], vz[2][COLL_MAXC], w[2][COLL_MAXC];
+  uint32_t idx[2][COLL_MAXC];
+};
+
+template <class V>
+__device__ __forceinline__ void stage_species(const V &v, const uint32_t *perm, uint32_t o, int n, int s,
+                                              CellStage &S) {
+  for (int k = threadIdx.x; k < n; k += COLL_BLOCK) {
+    uint32_t p = perm[o + k];
+    double vv[3];
+    v.load(p, vv);
+    S.idx[s][k] = p;
+    S.vx[s][k] = vv[0]; S.vy[s][k] = vv[1]; S.vz[s][k] = vv[2];
+    S.w[s][k] = v.w(p);
+  }
+}
+
+// shuffle of one bucket: order by (Philox key, low id bits)
+template <class V>
+__device__ __forceinline__ void shuffle_bucket(const V &v, uint32_t *perm, uint32_t o, int n, uint64_t seed,
+                                               uint64_t step, uint64_t *skey, uint32_t *sidx) {
+  for (int k = threadIdx.x; k < n; k += COLL_BLOCK) {
+    uint32_t p = perm[o + k];
+    uint64_t id = v.id(p);
+    uint32_t ph[4];
+    philox4x32(id, (uint32_t)(2 * step), seed, ph);
+    skey[k] = ((uint64_t)ph[0] << 32) | (id & 0xFFFFFFFFull);
+    sidx[k] = p;
+  }
+  __syncthreads();
+  wave_sort(skey, sidx, n);
+  for (int k = threadIdx.x; k < n; k += COLL_BLOCK) perm[o + k] = sidx[k];
+  __syncthreads();
+}
+
+template <class V>
+__global__ void __launch_bounds__(COLL_BLOCK) k_coll_prep(Geom g, V ve, V vi, const uint32_t *off_e,
+                                                          const uint32_t *off_i, uint32_t *perm_e, uint32_t *perm_i,
+                                                          double m_e, double m_i, uint64_t seed, uint64_t step,
+                                                          double *tot, int *row_bad, StepState *st) {
+  __shared__ uint64_t skey[COLL_MAXC];
+  __shared__ CellStage S;
+  const long ncells = (long)g.Dr * g.Dz * g.nr * g.nz;
+  int err = 0;
+  for (long c = blockIdx.x; c < ncells; c += gridDim.x) {
+    int nn[2] = {(int)(off_e[c + 1] - off_e[c]), (int)(off_i[c + 1] - off_i[c])};
+    if (nn[0] > COLL_MAXC || nn[1] > COLL_MAXC) {
+      err |= ERR_COLL_OVERFLOW;
+      continue;
+    }
+    shuffle_bucket(ve, perm_e, off_e[c], nn[0], seed, step, skey, S.idx[0]);
+    shuffle_bucket(vi, perm_i, off_i[c], nn[1], seed, step, skey, S.idx[1]);
+    stage_species(ve, perm_e, off_e[c], nn[0], 0, S);
+    stage_species(vi, perm_i, off_i[c], nn[1], 1, S);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      CellSums te = cell_sums(S.vx[0], S.vy[0], S.vz[0], S.w[0], nn[0], m_e);
+      CellSums ti = cell_sums(S.vx[1], S.vy[1], S.vz[1], S.w[1], nn[1], m_i);
+      const CellPos q = cell_pos(g, c);
+      const double inv_e = 1. / EL_CHARGE;
+      double temperature_el = te.energy / te.amount * inv_e;
+      double temperature_ion = ti.energy / ti.amount * inv_e;
+      double cell_volume = PCP_CELL_VOLUME((double)q.cr, g.dr, g.dz);
+      double sum_el = 0, sum_ion = 0;
+      for (int k = 0; k < nn[0]; ++k) sum_el += S.w[0][k];
+      for (int k = 0; k < nn[1]; ++k) sum_ion += S.w[1][k];
+      double density_el = sum_el / cell_volume;
+      double density_ion = sum_ion / cell_volume;
+      double *t = tot + c * TOT;
+      t[0] = te.amount; t[1] = te.energy; t[2] = te.m[0]; t[3] = te.m[1]; t[4] = te.m[2];
+      t[5] = ti.amount; t[6] = ti.energy; t[7] = ti.m[0]; t[8] = ti.m[1]; t[9] = ti.m[2];
+      t[10] = density_el; t[11] = density_ion;
+      bool bad = !isnormal(temperature_ion) || !isnormal(temperature_el) || !isnormal(density_ion) ||
+                 !isnormal(density_el);
+      t[12] = bad ? 0.0 : debye_length(density_el, density_ion, temperature_el, temperature_ion);
+      if (bad) atomicMin(&row_bad[(long)q.d * g.nr + q.i], q.j);
+    }
+    __syncthreads();
+  }
+  if (err && threadIdx.x == 0) atomicOr(&st->err, err);
+}
+
+// the pair schedule of one cell (collisionsP12.cpp:356-528)
+template <int SCHEME>
+struct PairRunner {
+  CellStage &S;
+  double m[2], q[2];   // electrons, ions
+  double dens[2], debye, dt;
+  uint64_t seed, step, gcell;
+  int *err;
+  __device__ __forceinline__ void pair(int sa, int ka, int sb, int kb, uint32_t ord) {
+    PairDraws R;
+    uint64_t ctr = (gcell << 24) | (uint64_t)(ord & 0xFFFFFFu);
+    uint32_t draw = (uint32_t)(2 * step + 1);
+    uniform2(ctr, draw, seed, R.u[0], R.u[1]);
+    uniform2(ctr ^ (1ull << 63), draw, seed, R.u[2], R.u[3]);
+    R.pos = 0;
+    double va[3] = {S.vx[sa][ka], S.vy[sa][ka], S.vz[sa][ka]};
+    double vb[3] = {S.vx[sb][kb], S.vy[sb][kb], S.vz[sb][kb]};
+    collide_pair<SCHEME>(m[sa], m[sb], q[sa], q[sb], va, S.w[sa][ka], vb, S.w[sb][kb], dens[sa], dens[sb], debye, dt,
+                         R, err);
+    S.vx[sa][ka] = va[0]; S.vy[sa][ka] = va[1]; S.vz[sa][ka] = va[2];
+    S.vx[sb][kb] = vb[0]; S.vy[sb][kb] = vb[1]; S.vz[sb][kb] = vb[2];
+  }
+  // like-particle pairs of species s (cases 1a / 1b), ordinals from qb
+  __device__ void like(int s, int n, uint32_t qb) {
+    if (n % 2 == 0) {
+      for (int k = threadIdx.x; k < n / 2; k += COLL_BLOCK) pair(s, 2 * k, s, 2 * k + 1, qb + (uint32_t)k);
+    } else {
+      if (n >= 3 && threadIdx.x == 0) {
+        pair(s, 0, s, 1, qb);
+        pair(s, 1, s, 2, qb + 1);
+        pair(s, 2, s, 0, qb + 2);
+      }
+      if (n >= 5)
+        for (int k = threadIdx.x; k < (n - 3) / 2; k += COLL_BLOCK)
+          pair(s, 3 + 2 * k, s, 4 + 2 * k, qb + 3 + (uint32_t)k);
+    }
+  }
+};
+
+__device__ __forceinline__ uint32_t like_pairs(int n) {
+  if (n % 2 == 0) return (uint32_t)(n / 2);
+  return (uint32_t)((n >= 3 ? 3 : 0) + (n >= 5 ? (n - 3) / 2 : 0));
+}
+
+__device__ __forceinline__ int grp(unsigned f, float div) { return (int)floorf((float)f / div); }
+
+// TA77S correct_velocities for species s of the staged cell; t = the cell's
+// {amount, energy, moment[3]} from before the collisions
+__device__ void correct_species(CellStage &S, int s, int n, double mass, const double *t) {
+  __shared__ double prm[8];   // ok, alpha, V0[3], dV[3]
+  if (threadIdx.x == 0) {
+    double m_new[3] = {0, 0, 0}, E_new = 0;
+    for (int k = 0; k < n; ++k) {
+      double vr = S.vx[s][k], vphi = S.vy[s][k], vz = S.vz[s][k];
+      double v_sq = vr * vr + vphi * vphi + vz * vz;
+      double mass_m = mass * S.w[s][k];
+      m_new[0] += mass_m * vr;
+      m_new[1] += mass_m * vphi;
+      m_new[2] += mass_m * vz;
+      E_new += mass_m * v_sq / 2;
+    }
+    const double amount = t[0], energy = t[1];
+    double mass_tot = amount * mass;
+    double dV[3], V0[3];
+    for (int k = 0; k < 3; ++k) dV[k] = (m_new[k] - t[2 + k]) / mass_tot;
+    for (int k = 0; k < 3; ++k) V0[k] = t[2 + k] / mass_tot;
+    double delta_E = E_new - energy;
+    double V0_sq = V0[0] * V0[0] + V0[1] * V0[1] + V0[2] * V0[2];
+    double alpha = (energy - mass_tot * V0_sq / 2) /
+                   (energy + delta_E -
+                    mass_tot * (pow(V0[0] + dV[0], 2) + pow(V0[1] + dV[1], 2) + pow(V0[2] + dV[2], 2)) / 2);
+    bool ok = isnormal(alpha);
+    if (ok) {
+      if (alpha < 0) alpha = -alpha;
+      if (alpha == 0) alpha = 1;
+      alpha = sqrt(alpha);
+    }
+    prm[0] = ok ? 1.0 : 0.0;
+    prm[1] = alpha;
+    for (int k = 0; k < 3; ++k) { prm[2 + k] = V0[k]; prm[5 + k] = dV[k]; }
+  }
+  __syncthreads();
+  if (prm[0] != 0.0) {
+    const double alpha = prm[1];
+    for (int k = threadIdx.x; k < n; k += COLL_BLOCK) {
+      double vr = S.vx[s][k], vphi = S.vy[s][k], vz = S.vz[s][k];
+      S.vx[s][k] = prm[2] + alpha * (vr - prm[2] - prm[5]);
+      S.vy[s][k] = prm[3] + alpha * (vphi - prm[3] - prm[6]);
+      S.vz[s][k] = prm[4] + alpha * (vz - prm[4] - prm[7]);
+    }
+  }
+  __syncthreads();
+}
+
+template <int SCHEME, class V>
+__global__ void __launch_bounds__(COLL_BLOCK) k_coll_pairs(Geom g, V ve, V vi, const uint32_t *off_e,
+                                                           const uint32_t *off_i, const uint32_t *perm_e,
+                                                           const uint32_t *perm_i, double m_e, double m_i,
+                                                           double q_e, double q_i, uint64_t seed, uint64_t step,
+                                                           const double *tot, const int *row_bad, StepState *st) {
+  __shared__ CellStage S;
+  const long ncells = (long)g.Dr * g.Dz * g.nr * g.nz;
+  int err = 0;
+  for (long c = blockIdx.x; c < ncells; c += gridDim.x) {
+    int nn[2] = {(int)(off_e[c + 1] - off_e[c]), (int)(off_i[c + 1] - off_i[c])};
+    if (nn[0] > COLL_MAXC || nn[1] > COLL_MAXC) {
+      err |= ERR_COLL_OVERFLOW;
+      continue;
+    }
+    stage_species(ve, perm_e, off_e[c], nn[0], 0, S);
+    stage_species(vi, perm_i, off_i[c], nn[1], 1, S);
+    __syncthreads();
+    const CellPos q = cell_pos(g, c);
+    const double *t = tot + c * TOT;
+    const bool active = q.j < row_bad[(long)q.d * g.nr + q.i];
+    if (active) {
+      PairRunner<SCHEME> P{S, {m_e, m_i}, {q_e, q_i}, {t[10], t[11]}, t[12], g.dt, seed, step,
+                           (uint64_t)q.cr * (uint64_t)g.Nz + (uint64_t)q.cz, &err};
+      const int nel = nn[0], nion = nn[1];
+      // cases 1a/1b in reference order: ions even, electrons even, ions odd, electrons odd
+      const uint32_t n_ion = like_pairs(nion), n_el = like_pairs(nel);
+      // blocks in reference order: ion pairs if nion is even, electron pairs
+      // if nel is even, ion pairs if nion is odd, electron pairs if nel is odd
+      uint32_t q_ion = 0, q_el = n_ion;
+      if (nion % 2 != 0 && nel % 2 == 0) { q_el = 0; q_ion = n_el; }
+      P.like(1, nion, q_ion);
+      __syncthreads();
+      P.like(0, nel, q_el);
+      __syncthreads();
+      const uint32_t qc = n_ion + n_el;
+      // case 2a / 2b
+      if (nion == nel) {
+        for (int k = threadIdx.x; k < nel; k += COLL_BLOCK) P.pair(0, k, 1, k, qc + (uint32_t)k);
+      } else if (nion > nel && nel > 0) {
+        unsigned int c_i = (unsigned int)floorf((float)nion / (float)nel);
+        double c_r = (double)((float)nion / (float)nel - (float)c_i);
+        unsigned int ions_1st = (unsigned int)((c_i + 1) * c_r * nel);
+        unsigned int els_1st = (unsigned int)(c_r * nel);
+        unsigned int ions_2nd = (unsigned int)(c_i * (1 - c_r) * nel);
+        const float d1 = (float)(c_i + 1), d2 = (float)c_i;
+        // 1st group: electron e meets ions fgi with floor(fgi / (c_i + 1)) == e
+        for (int e = threadIdx.x; e < nel; e += COLL_BLOCK) {
+          unsigned lo = (unsigned)e * (c_i + 1);
+          lo = lo > 0 ? lo - 1 : 0;
+          for (unsigned fgi = lo; fgi < ions_1st && fgi <= (unsigned)(e + 1) * (c_i + 1); ++fgi)
+            if (grp(fgi, d1) == e && (int)fgi < nion) P.pair(0, e, 1, (int)fgi, qc + fgi);
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < nel; e += COLL_BLOCK) {
+          if (e < (int)els_1st) continue;
+          int le = e - (int)els_1st;
+          unsigned lo = (unsigned)le * c_i;
+          lo = lo > 0 ? lo - 1 : 0;
+          for (unsigned fgi = lo; fgi < ions_2nd && fgi <= (unsigned)(le + 1) * c_i; ++fgi)
+            if (grp(fgi, d2) == le && (int)(fgi + ions_1st) < nion)
+              P.pair(0, e, 1, (int)(fgi + ions_1st), qc + ions_1st + fgi);
+        }
+      } else if (nion < nel && nion > 0) {
+        double c_i = floorf((float)nel / (float)nion);
+        double c_r = (double)((float)nel / (float)nion) - c_i;
+        unsigned int els_1st = (unsigned int)((c_i + 1) * c_r * nion);
+        unsigned int ions_1st = (unsigned int)(c_r * nion);
+        unsigned int els_2nd = (unsigned int)(c_i * (1 - c_r) * nion);
+        const float d1 = (float)(c_i + 1), d2 = (float)c_i;
+        const unsigned ci1 = (unsigned)c_i + 1, ci0 = (unsigned)c_i;
+        for (int i = threadIdx.x; i < nion; i += COLL_BLOCK) {
+          unsigned lo = (unsigned)i * ci1;
+          lo = lo > 0 ? lo - 1 : 0;
+          for (unsigned fge = lo; fge < els_1st && fge <= (unsigned)(i + 1) * ci1; ++fge)
+            if (grp(fge, d1) == i && (int)fge < nel) P.pair(0, (int)fge, 1, i, qc + fge);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < nion; i += COLL_BLOCK) {
+          if (i < (int)ions_1st) continue;
+          int li = i - (int)ions_1st;
+          unsigned lo = (unsigned)li * ci0;
+          lo = lo > 0 ? lo - 1 : 0;
+          for (unsigned fge = lo; fge < els_2nd && fge <= (unsigned)(li + 1) * ci0; ++fge)
+            if (grp(fge, d2) == li && (int)(fge + els_1st) < nel)
+              P.pair(0, (int)(fge + els_1st), 1, i, qc + els_1st + fge);
+        }
+      }
+      __syncthreads();
+    }
+    if (SCHEME == COLL_TA77S) {
+      // correct_velocities runs over every cell, ions first
+      correct_species(S, 1, nn[1], m_i, t + 5);
+      correct_species(S, 0, nn[0], m_e, t + 0);
+    }
+    for (int k = threadIdx.x; k < nn[0]; k += COLL_BLOCK) {
+      double v3[3] = {S.vx[0][k], S.vy[0][k], S.vz[0][k]};
+      ve.store(S.idx[0][k], v3);
+    }
+    for (int k = threadIdx.x; k < nn[1]; k += COLL_BLOCK) {
+      double v3[3] = {S.vx[1][k], S.vy[1][k], S.vz[1][k]};
+      vi.store(S.idx[1][k], v3);
+    }
+    __syncthreads();
+  }
+  if (err && threadIdx.x == 0) atomicOr(&st->err, err);
+}
+
+}  // namespace
+
+struct CollBuffers {
+  uint32_t *cnt[2] = {nullptr, nullptr}, *off[2] = {nullptr, nullptr};
+  uint32_t *key[2] = {nullptr, nullptr}, *slot[2] = {nullptr, nullptr}, *perm[2] = {nullptr, nullptr};
+  long cap[2] = {0, 0};
+  double *tot = nullptr;
+  int *row_bad = nullptr;
+  void *scan_tmp = nullptr;
+  size_t scan_bytes = 0;
+  long ncells = 0;
+};
+
+void coll_free(pcp_ctx *c) {
+  CollBuffers *b = c->coll;
+  if (!b) return;
+  for (int s = 0; s < 2; ++s) {
+    (void)hipFree(b->cnt[s]); (void)hipFree(b->off[s]); (void)hipFree(b->key[s]);
+    (void)hipFree(b->slot[s]); (void)hipFree(b->perm[s]);
+  }
+  (void)hipFree(b->tot); (void)hipFree(b->row_bad); (void)hipFree(b->scan_tmp);
+  delete b;
+  c->coll = nullptr;
+}
+
+static int coll_alloc(pcp_ctx *c, long cap_e, long cap_i) {
+  CollBuffers *b = c->coll;
+  const long ncells = (long)c->D * c->g.nr * c->g.nz;
+  if (b && b->cap[0] >= cap_e && b->cap[1] >= cap_i) return 0;
+  coll_free(c);
+  b = new CollBuffers();
+  c->coll = b;
+  b->ncells = ncells;
+  const long caps[2] = {std::max(cap_e, 1L), std::max(cap_i, 1L)};
+  for (int s = 0; s < 2; ++s) {
+    b->cap[s] = caps[s];
+    if (hipMalloc(&b->cnt[s], (ncells + 1) * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&b->off[s], (ncells + 1) * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&b->key[s], caps[s] * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&b->slot[s], caps[s] * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&b->perm[s], caps[s] * sizeof(uint32_t)) != hipSuccess)
+      return -1;
+  }
+  if (hipMalloc(&b->tot, ncells * TOT * sizeof(double)) != hipSuccess ||
+      hipMalloc(&b->row_bad, (long)c->D * c->g.nr * sizeof(int)) != hipSuccess)
+    return -1;
+  if (hipcub::DeviceScan::ExclusiveSum(nullptr, b->scan_bytes, b->cnt[0], b->off[0], (int)(ncells + 1)) !=
+      hipSuccess)
+    return -1;
+  if (hipMalloc(&b->scan_tmp, std::max<size_t>(b->scan_bytes, 16)) != hipSuccess) return -1;
+  return 0;
+}
+
+template <class V>
+static int coll_run(pcp_ctx *c, int scheme, V ve, V vi, long n_e, long n_i, double m_e, double m_i, double q_e,
+                    double q_i, uint64_t seed, uint64_t step) {
+  CollBuffers *b = c->coll;
+  const Geom &g = c->g;
+  const long ncells = b->ncells;
+  const uint32_t sentinel = (uint32_t)ncells;
+  const long nn[2] = {n_e, n_i};
+  const V vv[2] = {ve, vi};
+  unsigned gz = (unsigned)std::min<long>((ncells + 256) / 256, (long)c->n_cu * 16);
+  for (int s = 0; s < 2; ++s) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(gz), dim3(256), 0, c->stream, b->cnt[s], ncells + 1);
+    if (nn[s] > 0) {
+      unsigned gp = (unsigned)std::min<long>((nn[s] + 255) / 256, (long)c->n_cu * 16);
+      hipLaunchKernelGGL(k_coll_key<V>, dim3(gp), dim3(256), 0, c->stream, g, vv[s], b->key[s], b->slot[s],
+                         b->cnt[s], sentinel);
+    }
+    size_t bytes = b->scan_bytes;
+    if (hipcub::DeviceScan::ExclusiveSum(b->scan_tmp, bytes, b->cnt[s], b->off[s], (int)(ncells + 1), c->stream) !=
+        hipSuccess)
+      return -1;
+    if (nn[s] > 0) {
+      unsigned gp = (unsigned)std::min<long>((nn[s] + 255) / 256, (long)c->n_cu * 16);
+      hipLaunchKernelGGL(k_coll_scatter, dim3(gp), dim3(256), 0, c->stream, nn[s], b->key[s], b->slot[s], b->off[s],
+                         b->perm[s], sentinel);
+    }
+  }
+  const long rows = (long)c->D * g.nr;
+  hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)std::max<long>(1, std::min<long>((rows + 255) / 256, 1024))),
+                     dim3(256), 0, c->stream, b->row_bad, rows, INT_MAX);
+  unsigned gc = (unsigned)std::min<long>(ncells, (long)c->n_cu * 32);
+  hipLaunchKernelGGL(k_coll_prep<V>, dim3(gc), dim3(COLL_BLOCK), 0, c->stream, g, ve, vi, b->off[0], b->off[1],
+                     b->perm[0], b->perm[1], m_e, m_i, seed, step, b->tot, b->row_bad, c->st);
+  if (scheme == COLL_TA77S)
+    hipLaunchKernelGGL((k_coll_pairs<COLL_TA77S, V>), dim3(gc), dim3(COLL_BLOCK), 0, c->stream, g, ve, vi, b->off[0],
+                       b->off[1], b->perm[0], b->perm[1], m_e, m_i, q_e, q_i, seed, step, b->tot, b->row_bad, c->st);
+  else if (scheme == COLL_SK98)
+    hipLaunchKernelGGL((k_coll_pairs<COLL_SK98, V>), dim3(gc), dim3(COLL_BLOCK), 0, c->stream, g, ve, vi, b->off[0],
+                       b->off[1], b->perm[0], b->perm[1], m_e, m_i, q_e, q_i, seed, step, b->tot, b->row_bad, c->st);
+  else
+    hipLaunchKernelGGL((k_coll_pairs<COLL_P12, V>), dim3(gc), dim3(COLL_BLOCK), 0, c->stream, g, ve, vi, b->off[0],
+                       b->off[1], b->perm[0], b->perm[1], m_e, m_i, q_e, q_i, seed, step, b->tot, b->row_bad, c->st);
+  return 0;
+}
+
+int launch_collide(pcp_ctx *c, int scheme, int se, int si, uint64_t seed, uint64_t step, int vslot) {
+  if (c->s32) {
+    Species32 &e = c->s32->sp[se], &i = c->s32->sp[si];
+    if (coll_alloc(c, e.cap, i.cap) < 0) return -1;
+    View32 ve{e, c->cur, vslot}, vi{i, c->cur, vslot};
+    return coll_run(c, scheme, ve, vi, e.n, i.n, e.m, i.m, e.q, i.q, seed, step);
+  }
+  SpeciesDev &e = c->sp[se], &i = c->sp[si];
+  if (coll_alloc(c, e.cap, i.cap) < 0) return -1;
+  View64 ve{e, c->cur}, vi{i, c->cur};
+  return coll_run(c, scheme, ve, vi, e.n, i.n, e.m, i.m, e.q, i.q, seed, step);
+}
+
+}  // namespace pcp

@@ -1,55 +1,15 @@
 // Seeded synthetic inputs generated on the device.
 //
-// Random numbers come from Philox-4x32-10 (Salmon, Moraes, Dror, Shaw,
-// "Parallel random numbers: as easy as 1, 2, 3", SC11), keyed by the seed and
-// counted by (particle index, draw), so every particle's draws are
-// independent of the launch shape.
+// Random numbers come from Philox-4x32-10 (pcp_rng.hpp) counted by
+// (particle index, draw), so every particle's draws are independent of the
+// launch shape.
 #include <hip/hip_runtime.h>
 
 #include "pcp_init.hpp"
 #include "pcp_physics.hpp"
+#include "pcp_rng.hpp"
 
 namespace pcp {
-
-__device__ __forceinline__ void philox_round(uint32_t c[4], uint32_t k[2]) {
-  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
-  uint64_t p0 = (uint64_t)M0 * c[0];
-  uint64_t p1 = (uint64_t)M1 * c[2];
-  uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
-  uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
-  uint32_t n0 = hi1 ^ c[1] ^ k[0];
-  uint32_t n2 = hi0 ^ c[3] ^ k[1];
-  c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
-}
-
-__device__ __forceinline__ void philox4x32(uint64_t ctr, uint32_t draw, uint64_t seed, uint32_t out[4]) {
-  uint32_t c[4] = {(uint32_t)ctr, (uint32_t)(ctr >> 32), draw, 0x5eedu};
-  uint32_t k[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
-  for (int r = 0; r < 10; ++r) {
-    philox_round(c, k);
-    k[0] += 0x9E3779B9u;
-    k[1] += 0xBB67AE85u;
-  }
-  out[0] = c[0]; out[1] = c[1]; out[2] = c[2]; out[3] = c[3];
-}
-
-// two doubles in [0, 1) from one Philox block
-__device__ __forceinline__ void uniform2(uint64_t ctr, uint32_t draw, uint64_t seed, double &u0, double &u1) {
-  uint32_t o[4];
-  philox4x32(ctr, draw, seed, o);
-  const double s = 1.0 / 9007199254740992.0;   // 2^-53
-  u0 = (double)(((uint64_t)(o[0] >> 5) << 26) | (o[1] >> 6)) * s;
-  u1 = (double)(((uint64_t)(o[2] >> 5) << 26) | (o[3] >> 6)) * s;
-}
-
-// Box-Muller on (0,1] x [0,1)
-__device__ __forceinline__ void normal2(uint64_t ctr, uint32_t draw, uint64_t seed, double &n0, double &n1) {
-  double u0, u1;
-  uniform2(ctr, draw, seed, u0, u1);
-  double rad = sqrt(-2.0 * log(1.0 - u0));
-  n0 = rad * cos(2.0 * PI * u1);
-  n1 = rad * sin(2.0 * PI * u1);
-}
 
 // One thread per particle; cell_of[p] found by binary search over the
 // per-cell exclusive offsets (cells in domain-major order).

@@ -41,6 +41,7 @@ enum : int {
   ERR_NAN_FIELD = 4,      // maxwellSolverYee.cpp:253-255
   ERR_MIGRANT_OVERFLOW = 8,
   ERR_MOVED_TOO_FAR = 16, // currentVB.cpp:199-204 (LOG_S(ERROR), not fatal)
+  ERR_COLL_OVERFLOW = 32, // a cell holds more macro-particles than a collision pass stages
 };
 
 // Geometry shared by every domain of the SMB (domains are equal).
@@ -127,7 +128,8 @@ struct Migrants {
   int ngroups;
 };
 
-struct State32;   // FP32 split-precision state (pcp32.hpp)
+struct State32;      // FP32 split-precision state (pcp32.hpp)
+struct CollBuffers;  // collision pass buffers (pcp_collide.hip)
 
 }  // namespace pcp
 
@@ -150,6 +152,10 @@ struct pcp_ctx {
   int sort_every[pcp::MAX_SPECIES]; // 0: never; k: before every k-th step
   long steps_done;
   pcp::State32 *s32 = nullptr;    // non-null in FP32 split mode
+  pcp::CollBuffers *coll = nullptr;
+  int coll_scheme = -1;            // collision phase of pcp_step (-1: off)
+  int coll_se = 0, coll_si = 1;
+  uint64_t coll_seed = 0, coll_calls = 0;
   int n_cu;
   int timing;
   std::vector<std::pair<std::string, double>> ktimes;

@@ -380,10 +380,11 @@ __global__ void k_migrate(Geom g, SpeciesDev s, int sp, int cur, StepState *st, 
 // fused push + move + deposit (the hot kernel).  Every thread of a block runs
 // the same number of loop trips so that the ZigZag deposit, which has no
 // data-dependent branches, executes converged and aggregates its atomics per
-// wavefront.
-template <int CCS>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CCS == PCP_CCS_ZIGZAG ? 3 : 2))) k_step_fused(Geom g, Fields f, SpeciesDev s, int sp, int cur, StepState *st,
-                                                    Migrants mg) {
+// wavefront.  PUSH = false skips the push (steps with a collision phase push
+// and collide first).
+template <int CCS, bool PUSH>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CCS == PCP_CCS_ZIGZAG ? 3 : 2)))
+k_step_fused(Geom g, Fields f, SpeciesDev s, int sp, int cur, StepState *st, Migrants mg) {
   int err = 0;
   const int nxt = cur ^ 1;
   const long stride = (long)gridDim.x * blockDim.x;
@@ -402,7 +403,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CCS ==
         s.r[nxt][p] = -1; s.z[nxt][p] = z;
       } else {
         double v[3] = {s.vr[p], s.vphi[p], s.vz[p]};
-        push_one(g, f, s.q, s.m, r, z, v, &err);
+        if (PUSH) push_one(g, f, s.q, s.m, r, z, v, &err);
         ro = r; zo = z;
         double vr = v[0], vz = v[2];
         vphi = v[1];
@@ -653,15 +654,25 @@ void launch_migrate_mark(pcp_ctx *c, int sp) {
   hipLaunchKernelGGL(k_migrate, dim3(pgrid(c, s.n, 256)), dim3(256), 0, c->stream, c->g, s, sp, c->cur, c->st,
                      c->mig);
 }
-void launch_step_fused(pcp_ctx *c, int sp) {
+void launch_step_fused(pcp_ctx *c, int sp, bool push) {
   SpeciesDev &s = c->sp[sp];
   if (!s.n) return;
-  if (c->g.ccs == PCP_CCS_ZIGZAG)
-    hipLaunchKernelGGL(k_step_fused<PCP_CCS_ZIGZAG>, dim3(pgrid(c, s.n, 256)), dim3(256), 0, c->stream, c->g, c->f, s,
-                       sp, c->cur, c->st, c->mig);
-  else
-    hipLaunchKernelGGL(k_step_fused<PCP_CCS_VB>, dim3(pgrid(c, s.n, 256)), dim3(256), 0, c->stream, c->g, c->f, s, sp,
-                       c->cur, c->st, c->mig);
+  const dim3 grid(pgrid(c, s.n, 256)), block(256);
+  if (c->g.ccs == PCP_CCS_ZIGZAG) {
+    if (push)
+      hipLaunchKernelGGL((k_step_fused<PCP_CCS_ZIGZAG, true>), grid, block, 0, c->stream, c->g, c->f, s, sp, c->cur,
+                         c->st, c->mig);
+    else
+      hipLaunchKernelGGL((k_step_fused<PCP_CCS_ZIGZAG, false>), grid, block, 0, c->stream, c->g, c->f, s, sp, c->cur,
+                         c->st, c->mig);
+  } else {
+    if (push)
+      hipLaunchKernelGGL((k_step_fused<PCP_CCS_VB, true>), grid, block, 0, c->stream, c->g, c->f, s, sp, c->cur, c->st,
+                         c->mig);
+    else
+      hipLaunchKernelGGL((k_step_fused<PCP_CCS_VB, false>), grid, block, 0, c->stream, c->g, c->f, s, sp, c->cur,
+                         c->st, c->mig);
+  }
 }
 void launch_mark(pcp_ctx *c, int sp, int only_if_exact) {
   SpeciesDev &s = c->sp[sp];

@@ -26,7 +26,7 @@ void launch_stamp(pcp_ctx *c);
 void launch_push(pcp_ctx *c, int sp);
 void launch_move(pcp_ctx *c, int sp);
 void launch_migrate_mark(pcp_ctx *c, int sp);   // removal + migrant list (phase API)
-void launch_step_fused(pcp_ctx *c, int sp);
+void launch_step_fused(pcp_ctx *c, int sp, bool push = true);
 void launch_mark(pcp_ctx *c, int sp, int only_if_exact);
 void launch_deposit_exact(pcp_ctx *c, int sp, int only_if_exact);
 void launch_zero_j_if_exact(pcp_ctx *c);

@@ -151,7 +151,8 @@ State read_state(DeviceBlock &b, const pcp_config &c) {
   }
   const int which[3] = {PCP_GRID_E, PCP_GRID_H, PCP_GRID_J};
   for (int k = 0; k < 3; ++k) {
-    s.grids[k].resize((size_t)pcp_grid_points(b.ctx()) * 3);
+    // [domain][component][Nr_d + 4][Nz_d + 4]
+    s.grids[k].resize((size_t)pcp_grid_points(b.ctx()) * 3 * (size_t)b.domains());
     picopic_hip::check(pcp_download_grid(b.ctx(), which[k], s.grids[k].data()), "pcp_download_grid");
   }
   return s;

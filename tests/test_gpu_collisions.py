@@ -1,0 +1,171 @@
+"""Device collisions (P12 / TA77S / SK98) against the oracle's keyed pass.
+
+Both sides shuffle each cell by the same Philox key and give pair q of
+global cell c the same four uniforms, so one pass is compared particle by
+particle.  FP64: 1e-12 relative to the largest speed (libm and the device
+math library may differ in the last ulp of pow/log/sin and the final
+Lorentz factor uses 1/sqrt for pow(x, -0.5)).  FP32 split contexts collide
+in FP64 from FP32 velocities and store FP32: 1e-6.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from picopic_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+M_E = 9.1e-31
+Q_E = 1.6e-19
+
+
+def plasma(rng, grid, size, counts_e, counts_i, temp_e=5.0, temp_i=1.0, fp32=False, weights="equal"):
+    """Per-cell counts given as arrays [nr, nz] for electrons and ions."""
+    nr, nz = grid
+    dr, dz = size[0] / nr, size[1] / nz
+    out = []
+    for sp, (cnt, m, T) in enumerate(((counts_e, M_E, temp_e), (counts_i, 1836 * M_E, temp_i))):
+        ci, cj = np.meshgrid(np.arange(nr), np.arange(nz), indexing="ij")
+        ci = np.repeat(ci.ravel(), cnt.ravel())
+        cj = np.repeat(cj.ravel(), cnt.ravel())
+        n = len(ci)
+        fr = rng.uniform(0.05, 0.95, n)
+        fz = rng.uniform(0.05, 0.95, n)
+        if fp32:
+            fr = fr.astype(np.float32).astype(np.float64)
+            fz = fz.astype(np.float32).astype(np.float64)
+        r = (ci + fr) * dr
+        z = (cj + fz) * dz
+        vth = np.sqrt(T * Q_E / m)
+        v = rng.normal(0, vth, (3, n))
+        if weights == "equal":
+            w = np.full(n, 1e8)
+        else:
+            w = rng.uniform(0.5, 1.5, n) * 1e8
+        if fp32:
+            v = v.astype(np.float32).astype(np.float64)
+            w = w.astype(np.float32).astype(np.float64)
+        ids = np.arange(n, dtype=np.uint64) + np.uint64(sp * 10 ** 6)
+        order = rng.permutation(n)
+        out.append(tuple(a[order] for a in (r, z, v[0], v[1], v[2], w, ids)))
+    return out
+
+
+def counts(rng, grid, lo=1, hi=12, hole=None):
+    ce = rng.integers(lo, hi, grid)
+    ci = rng.integers(lo, hi, grid)
+    ci[0, 0] = ce[0, 0]          # nion == nel
+    ci[0, 1] = 2 * ce[0, 1] + 1  # nion > nel, non-integer ratio
+    ce[1, 0] = 3 * ci[1, 0]      # nel = 3 nion
+    if hole is not None:
+        ce[hole] = 0             # an empty electron bucket ends the row
+    return ce, ci
+
+
+def build(grid, domains, data, precision, dt=1e-13):
+    size = (grid[0] * 1e-4, grid[1] * 1e-4)
+    species = [(-1, 1, False), (1, 1836, False)]
+    orc = O.SMB(size, grid, domains, dt, species)
+    cap = max(len(d[0]) for d in data) + 16
+    gpu = capi.Context(size, grid, domains, dt, [s + (cap,) for s in species], precision=precision)
+    for sp, d in enumerate(data):
+        orc.add_particles(sp, *d)
+        gpu.upload(sp, *d)
+    return orc, gpu
+
+
+def by_id(p):
+    o = np.argsort(p["id"])
+    return {k: v[o] for k, v in p.items()}
+
+
+def compare(orc, gpu, tol):
+    worst = 0.0
+    for sp in range(2):
+        a, b = by_id(orc.particles(sp)), by_id(gpu.particles(sp))
+        assert np.array_equal(a["id"], b["id"])
+        scale = max(np.abs(a[k]).max() for k in ("vr", "vphi", "vz"))
+        for k in ("vr", "vphi", "vz"):
+            e = np.abs(a[k] - b[k]).max() / scale
+            worst = max(worst, e)
+    assert worst <= tol, worst
+    return worst
+
+
+@pytest.mark.parametrize("scheme", ["p12", "ta77s", "sk98"])
+@pytest.mark.parametrize("domains", [(1, 1), (2, 2)])
+def test_collision_pass_fp64(scheme, domains):
+    rng = np.random.default_rng(31)
+    grid = (6, 8)
+    size = (grid[0] * 1e-4, grid[1] * 1e-4)
+    ce, ci = counts(rng, grid, hole=(2, 5))
+    data = plasma(rng, grid, size, ce, ci, weights="equal" if scheme != "ta77s" else "mixed")
+    orc, gpu = build(grid, domains, data, "fp64")
+    print("collide", scheme, domains, flush=True)
+    orc.collide(scheme, 0, 1, seed=77, step=3)
+    gpu.collide(scheme, 0, 1, 77, 3)
+    err = compare(orc, gpu, 1e-12)
+    print(scheme, domains, err)
+    # and something happened
+    before = by_id({k: v for k, v in zip(("r", "z", "vr", "vphi", "vz", "w", "id"), data[0])})
+    after = by_id(gpu.particles(0))
+    assert np.abs(after["vr"] - before["vr"]).max() > 0
+
+
+@pytest.mark.parametrize("scheme", ["p12", "ta77s"])
+def test_collision_pass_fp32(scheme):
+    rng = np.random.default_rng(32)
+    grid = (6, 8)
+    size = (grid[0] * 1e-4, grid[1] * 1e-4)
+    ce, ci = counts(rng, grid)
+    data = plasma(rng, grid, size, ce, ci, fp32=True)
+    orc, gpu = build(grid, (2, 2), data, "fp32")
+    print("collide fp32", scheme, flush=True)
+    orc.collide(scheme, 0, 1, seed=5, step=0)
+    gpu.collide(scheme, 0, 1, 5, 0)
+    print(scheme, compare(orc, gpu, 1e-6))
+
+
+def test_collision_cell_overflow_is_reported():
+    rng = np.random.default_rng(33)
+    grid = (2, 2)
+    size = (grid[0] * 1e-4, grid[1] * 1e-4)
+    ce = np.array([[300, 4], [4, 4]])
+    ci = np.array([[4, 4], [4, 4]])
+    data = plasma(rng, grid, size, ce, ci)
+    _, gpu = build(grid, (1, 1), data, "fp64")
+    print("collide overflow", flush=True)
+    with pytest.raises(capi.PcpError, match="collision cell"):
+        gpu.collide("p12", 0, 1, 1, 0)
+
+
+@pytest.mark.parametrize("precision,tol,tol_fields", [("fp64", 1e-11, 1e-10), ("fp32", 2e-5, 1e-4)])
+def test_steps_with_collision_phase(precision, tol, tol_fields):
+    """pcp_step with the collision phase between push and move, against the
+    oracle's step with its collision phase (3 steps, fields on)."""
+    rng = np.random.default_rng(34)
+    grid = (8, 16)
+    size = (grid[0] * 1e-4, grid[1] * 1e-4)
+    ce, ci = counts(rng, grid, lo=3, hi=10)
+    data = plasma(rng, grid, size, ce, ci, temp_e=20.0, temp_i=2.0, fp32=precision == "fp32")
+    dt = 0.5 * size[1] / grid[1] / 3e8
+    orc, gpu = build(grid, (2, 2), data, precision, dt=dt)
+    orc.set_collisions("p12", 0, 1, seed=9)
+    gpu.set_collisions("p12", 0, 1, 9)
+    print("steps with collisions", precision, flush=True)
+    orc.step(3)
+    gpu.step(3)
+    worst = 0.0
+    for sp in range(2):
+        a, b = by_id(orc.particles(sp)), by_id(gpu.particles(sp))
+        assert np.array_equal(a["id"], b["id"])
+        for k in ("r", "z", "vr", "vphi", "vz"):
+            scale = np.abs(a[k]).max()
+            worst = max(worst, np.abs(a[k] - b[k]).max() / scale)
+    print(precision, "particles", worst)
+    assert worst <= tol
+    for nm in ("E", "J"):
+        A, B = orc.get_grid(nm)[:, :, 2:-2, 2:-2], gpu.get_grid(nm)[:, :, 2:-2, 2:-2]
+        e = np.abs(A - B).max() / np.abs(A).max()
+        print(nm, e)
+        assert e <= tol_fields
