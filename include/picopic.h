@@ -42,7 +42,10 @@ enum { PCP_GRID_E = 0, PCP_GRID_H = 1, PCP_GRID_H_AT_ET = 2, PCP_GRID_J = 3,
        PCP_GRID_EPS = 4, PCP_GRID_SIGMA = 5 };
 /* binary collision schemes: configure.ac --with-coulomb-collisions
  * (CollisionsP12 is the reference default) */
-enum { PCP_COLL_P12 = 0, PCP_COLL_TA77S = 1, PCP_COLL_SK98 = 2 };
+enum { PCP_COLL_P12 = 0, PCP_COLL_TA77S = 1, PCP_COLL_SK98 = 2,
+       /* no reference counterpart: the fitted per-cell network (below) on
+        * the cells it covers, P12 on the others */
+       PCP_COLL_SURROGATE = 3 };
 /* precision of the particle/field storage, fixed at creation.  FP32_SPLIT
  * stores each position as a packed cell index plus FP32 in-cell offsets and
  * keeps fields, velocities and J in FP32; such contexts run pcp_step and the
@@ -162,6 +165,42 @@ int pcp_collide(pcp_ctx *ctx, int scheme, int sp_el, int sp_ion, uint64_t seed, 
  * SMB::advance_particles (src/SMB.cpp:1287-1289); passes count from 0. */
 int pcp_set_collisions(pcp_ctx *ctx, int scheme, int sp_el, int sp_ion, uint64_t seed);
 
+/* ---- collision surrogate: no counterpart in the reference ----
+ * A per-cell network (6 inputs: log10 of n_e, n_i, Te_par, Te_perp, Ti and
+ * dt; two tanh layers of 64; 4 outputs: relative changes of Te_par,
+ * Te_perp, Ti_par, Ti_perp per pass) evaluated and trained with BF16 MFMA
+ * kernels.  Its output rescales each species' thermal velocities about the
+ * species' mean velocity (z: par, r and phi: perp); a common factor then
+ * restores the cell's kinetic energy, so every species' momentum and the
+ * cell's energy are conserved.  Cells with fewer than 8 macro-particles of a
+ * species, degenerate moments or features beyond 4 standard deviations of
+ * the training set take the P12 pass instead (PCP_COLL_SURROGATE).
+ *
+ * Raw network access (rows of 8 floats in, first 6 used; rows of 4 out):
+ * the parameter block is W1[64][32] b1[64] W2[64][64] b2[64] W3[16][64]
+ * b3[16] (padding zero). */
+int pcp_mlp_nparams(void);
+int pcp_mlp_init(pcp_ctx *ctx, uint64_t seed);
+int pcp_mlp_get_params(pcp_ctx *ctx, float *host);
+int pcp_mlp_set_params(pcp_ctx *ctx, const float *host);
+int pcp_mlp_forward(pcp_ctx *ctx, const float *x, long n, float *y);
+/* gradient of sum (y - t)^2 / n over the batch; *loss = that sum */
+int pcp_mlp_grad(pcp_ctx *ctx, const float *x, const float *t, long n, float *grad, float *loss);
+/* one Adam step (beta 0.9 / 0.999) with the last gradient */
+int pcp_mlp_adam(pcp_ctx *ctx, float lr);
+/* one P12 pass (as pcp_collide) whose collided cells append a training row
+ * (raw features, relative temperature changes) to the context's training
+ * set; *rows = rows held */
+int pcp_surrogate_collect(pcp_ctx *ctx, int sp_el, int sp_ion, uint64_t seed, uint64_t pass, long *rows);
+/* normalise the training set and train: `epochs` shuffled passes of
+ * mini-batches with Adam; *loss = mean batch loss of the last pass */
+int pcp_surrogate_fit(pcp_ctx *ctx, int epochs, long batch, float lr, uint64_t seed, float *loss);
+/* normalisation of a fitted network: mean[6], std[6], target scale[4] */
+int pcp_surrogate_get_norm(pcp_ctx *ctx, float *out16);
+int pcp_surrogate_set_norm(pcp_ctx *ctx, const float *in16);
+/* cells of the last surrogate pass that took the P12 fallback */
+int pcp_surrogate_fallback_cells(pcp_ctx *ctx, long *cells);
+
 /* Counting sort of species sp by (domain, cell), dropping removed particles
  * (no reference counterpart: the reference keeps per-domain vectors).  The
  * reference order keys travel with the particles, so results do not depend
@@ -182,6 +221,11 @@ int pcp_get_field_h(pcp_ctx *ctx, int d, double r, double z, double *out);
  * kinetic energy of species sp: sum of w m c^2 (gamma - 1)    [J] */
 int pcp_field_energy(pcp_ctx *ctx, double *out);
 int pcp_kinetic_energy(pcp_ctx *ctx, int sp, double *out);
+/* velocity moments of species sp (the sums behind SpecieP::calc_temperature,
+ * src/specieP.cpp:725-839, taken over the whole block): out[7] = sum w,
+ * sum w v_r, sum w v_phi, sum w v_z, sum w v_z^2, sum w (v_r^2 + v_phi^2),
+ * number of live macro-particles */
+int pcp_species_moments(pcp_ctx *ctx, int sp, double *out);
 
 /* achievable HBM bandwidth of a plain device copy of `bytes` bytes (read +
  * write counted), averaged over iters launches [GB/s]; a calibration aid for

@@ -26,9 +26,12 @@ EXPORTS = [
     "pcp_grid_points", "pcp_yee_e", "pcp_yee_h", "pcp_stitch", "pcp_push", "pcp_move", "pcp_migrate",
     "pcp_deposit", "pcp_step", "pcp_get_field_e", "pcp_get_field_h", "pcp_stream", "pcp_set_timing",
     "pcp_kernel_ms", "pcp_sort", "pcp_set_sort_interval", "pcp_copy_bandwidth", "pcp_field_energy", "pcp_kinetic_energy",
-    "pcp_collide", "pcp_set_collisions",
+    "pcp_collide", "pcp_set_collisions", "pcp_mlp_nparams", "pcp_mlp_init", "pcp_mlp_get_params",
+    "pcp_mlp_set_params", "pcp_mlp_forward", "pcp_mlp_grad", "pcp_mlp_adam", "pcp_surrogate_collect",
+    "pcp_surrogate_fit", "pcp_surrogate_get_norm", "pcp_surrogate_set_norm", "pcp_surrogate_fallback_cells",
+    "pcp_species_moments",
 ]
-COLL = {"p12": 0, "ta77s": 1, "sk98": 2}
+COLL = {"p12": 0, "ta77s": 1, "sk98": 2, "surrogate": 3}
 
 
 class SpeciesDesc(C.Structure):
@@ -91,6 +94,20 @@ def load(path=LIB_PATH):
     L.pcp_kinetic_energy.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
     L.pcp_collide.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_uint64]
     L.pcp_set_collisions.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_uint64]
+    fp = C.POINTER(C.c_float)
+    L.pcp_mlp_nparams.argtypes = []
+    L.pcp_mlp_init.argtypes = [vp, C.c_uint64]
+    L.pcp_mlp_get_params.argtypes = [vp, fp]
+    L.pcp_mlp_set_params.argtypes = [vp, fp]
+    L.pcp_mlp_forward.argtypes = [vp, fp, C.c_long, fp]
+    L.pcp_mlp_grad.argtypes = [vp, fp, fp, C.c_long, fp, fp]
+    L.pcp_mlp_adam.argtypes = [vp, C.c_float]
+    L.pcp_surrogate_collect.argtypes = [vp, C.c_int, C.c_int, C.c_uint64, C.c_uint64, C.POINTER(C.c_long)]
+    L.pcp_surrogate_fit.argtypes = [vp, C.c_int, C.c_long, C.c_float, C.c_uint64, fp]
+    L.pcp_surrogate_get_norm.argtypes = [vp, fp]
+    L.pcp_surrogate_set_norm.argtypes = [vp, fp]
+    L.pcp_surrogate_fallback_cells.argtypes = [vp, C.POINTER(C.c_long)]
+    L.pcp_species_moments.argtypes = [vp, C.c_int, dp]
     _lib = L
     return L
 
@@ -102,6 +119,17 @@ def _check(rc):
 
 def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _rows8(x):
+    x = np.asarray(x, dtype=np.float32)
+    out = np.zeros((len(x), 8), dtype=np.float32)
+    out[:, :x.shape[1]] = x
+    return out
 
 
 def _u64p(a):
@@ -230,6 +258,75 @@ class Context:
     def collide(self, scheme, se, si, seed, step):
         """One collision pass (P12 / TA77S / SK98) on the current velocities."""
         _check(_lib.pcp_collide(self.h, COLL[scheme], se, si, seed, step))
+
+    def temperatures(self, sp, mass):
+        """(T_par, T_perp) [eV] of species sp over the whole block (z is par)."""
+        m = np.zeros(7)
+        _check(_lib.pcp_species_moments(self.h, sp, _dp(m)))
+        sw = m[0]
+        V = m[1:4] / sw
+        tpar = mass * (m[4] / sw - V[2] ** 2) / 1.6e-19
+        tperp = mass * (m[5] / sw - V[0] ** 2 - V[1] ** 2) / (2 * 1.6e-19)
+        return tpar, tperp
+
+    def moments(self, sp):
+        m = np.zeros(7)
+        _check(_lib.pcp_species_moments(self.h, sp, _dp(m)))
+        return m
+
+    # ---- collision surrogate network ----
+    def mlp_init(self, seed=0):
+        _check(_lib.pcp_mlp_init(self.h, seed))
+
+    def mlp_params(self):
+        out = np.zeros(_lib.pcp_mlp_nparams(), dtype=np.float32)
+        _check(_lib.pcp_mlp_get_params(self.h, _fp(out)))
+        return out
+
+    def mlp_set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(_lib.pcp_mlp_set_params(self.h, _fp(p)))
+
+    def mlp_forward(self, x):
+        x = _rows8(x)
+        y = np.zeros((len(x), 4), dtype=np.float32)
+        _check(_lib.pcp_mlp_forward(self.h, _fp(x), len(x), _fp(y)))
+        return y
+
+    def mlp_grad(self, x, t):
+        x = _rows8(x)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(len(x), 4)
+        g = np.zeros(_lib.pcp_mlp_nparams(), dtype=np.float32)
+        loss = C.c_float()
+        _check(_lib.pcp_mlp_grad(self.h, _fp(x), _fp(t), len(x), _fp(g), C.byref(loss)))
+        return g, loss.value
+
+    def mlp_adam(self, lr):
+        _check(_lib.pcp_mlp_adam(self.h, lr))
+
+    def surrogate_collect(self, se, si, seed, step):
+        rows = C.c_long()
+        _check(_lib.pcp_surrogate_collect(self.h, se, si, seed, step, C.byref(rows)))
+        return rows.value
+
+    def surrogate_fit(self, epochs, batch, lr, seed=0):
+        loss = C.c_float()
+        _check(_lib.pcp_surrogate_fit(self.h, epochs, batch, lr, seed, C.byref(loss)))
+        return loss.value
+
+    def surrogate_norm(self):
+        out = np.zeros(16, dtype=np.float32)
+        _check(_lib.pcp_surrogate_get_norm(self.h, _fp(out)))
+        return out
+
+    def surrogate_set_norm(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float32)
+        _check(_lib.pcp_surrogate_set_norm(self.h, _fp(v)))
+
+    def surrogate_fallback_cells(self):
+        n = C.c_long()
+        _check(_lib.pcp_surrogate_fallback_cells(self.h, C.byref(n)))
+        return n.value
 
     def set_collisions(self, scheme, se=0, si=1, seed=0):
         """Collision phase inside step() (scheme None: off)."""

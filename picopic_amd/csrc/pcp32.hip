@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <climits>
+
 #include "pcp32.hpp"
 #include "pcp_rng.hpp"
 #include "pcp_kernels.hpp"
@@ -318,6 +320,30 @@ struct WaveSink32 {
   }
 };
 
+// Per-block LDS tile of the current (the nodes of the block's bounding box in
+// one domain), accumulated in FP64 with LDS atomics and added to J once per
+// node and block.  A contribution outside the box goes straight to J.
+constexpr int TILE_NODES = 640;   // nodes per component
+
+struct TileSink32 {
+  double *t;
+  int i0, k0, th, tw;
+  float *Jd;   // J of the tile's domain
+  int P, yr, nr, nz;
+  int *err;
+  __device__ __forceinline__ void add_if(int c, int i, int k, double v, bool a) {
+    if (!a) return;
+    const int li = i - i0, lk = k - k0;
+    if (li >= 0 && li < th && lk >= 0 && lk < tw) {
+      atomicAdd(&t[(c * th + li) * tw + lk], v);
+    } else if (i >= -2 && i < nr + 2 && k >= -2 && k < nz + 2) {
+      unsafeAtomicAdd(&Jd[c * P + (i + 2) * yr + (k + 2)], (float)v);
+    } else {
+      *err |= ERR_MOVED_TOO_FAR;
+    }
+  }
+};
+
 struct AtomicSink32 {
   float *J;
   long P;
@@ -365,7 +391,8 @@ struct MarkSink32 {
 
 // ZigZag (currentZigZag.cpp:22-136) in FP32, positions relative to the old
 // cell: uo = f_old, un = (i_n - i_o) + f_new (likewise in z)
-__device__ __forceinline__ void zigzag32(WaveSink32 &S, bool live, const Geom &g, const Dom &dn, const Split &o,
+template <class Sink>
+__device__ __forceinline__ void zigzag32(Sink &S, bool live, const Geom &g, const Dom &dn, const Split &o,
                                          const Split &n, float vphi, float qw, float inv_dt) {
   const float dr = (float)g.dr, dz = (float)g.dz;
   const int di = n.cr - o.cr, dk = n.cz - o.cz;
@@ -490,12 +517,58 @@ __device__ __forceinline__ int deposit_domain32(const Geom &g, double rd, double
   return domain_of32(g, cr, cz);
 }
 
+// the deposit of one lane's particle (ZigZag in FP32 relative to the old
+// cell; Villasenor-Buneman terms in FP64 from the old split position and
+// the unrounded end point) into a sink; every lane of the block calls it
+template <int CCS, class Sink>
+__device__ __forceinline__ void deposit32(Sink &S, bool live, const Geom &g, const Dom &dn, const Split &o,
+                                          const Split &n, float vphi, float qw, float inv_dt, double rd, double zd,
+                                          double p_charge, const FieldsT<float> &f, int d_dep, StepState *st,
+                                          int *err) {
+  if (CCS == PCP_CCS_ZIGZAG) {
+    zigzag32(S, live, g, dn, o, n, vphi, qw, inv_dt);
+  } else {
+    const double r = rd, z = zd;
+    const double ro = ((double)o.cr + (double)o.fr) * g.dr, zo = ((double)o.cz + (double)o.fz) * g.dz;
+    AzTerms A = vb_azimuthal_terms(g.nr, g.nz, g.dr, g.dz, dn.bot_r, dn.left_z, r, z, ro, zo, (double)vphi, p_charge);
+    S.add_if(1, A.i, A.k, (float)A.w[0], live && A.in[0]);
+    S.add_if(1, A.i + 1, A.k, (float)A.w[1], live && A.in[1]);
+    S.add_if(1, A.i, A.k + 1, (float)A.w[2], live && A.in[2]);
+    S.add_if(1, A.i + 1, A.k + 1, (float)A.w[3], live && A.in[3]);
+    SegRecorder R;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      R.rn[q] = 1.5 * g.dr; R.zn[q] = 1.25 * g.dz; R.ro[q] = 1.25 * g.dr; R.zo[q] = 1.5 * g.dz;
+      R.i[q] = 1; R.k[q] = 1;
+    }
+    if (live) vb_rz_dispatch(R, g.dr, g.dz, r, z, ro, zo, err);
+    if (live && R.is_strict) {
+      AtomicSink32 A1{f.J + (long)d_dep * 3 * g.P, g.P, g.yr, g.nr, g.nz, err, &st->need_exact};
+      vb_strict(A1, g.dr, g.dz, g.dt, dn.bot_r, dn.left_z, r, z, ro, zo, p_charge);
+    }
+#pragma unroll
+    for (int sg = 0; sg < 3; ++sg) {
+      const bool a = live && !R.is_strict && sg < R.n;
+      if (__ballot(a) == 0) break;
+      VBTerms T = vb_simple_terms(g.dr, g.dz, g.dt, dn.bot_r, dn.left_z, R.rn[sg], R.zn[sg], R.ro[sg], R.zo[sg],
+                                  R.i[sg], R.k[sg], p_charge);
+      const bool va = a && T.valid;
+      S.add_if(2, T.i, T.k, (float)T.jz0, va);
+      S.add_if(2, T.i + 1, T.k, (float)T.jz1, va);
+      S.add_if(0, T.i, T.k, (float)T.jr0, va);
+      S.add_if(0, T.i, T.k + 1, (float)T.jr1, va);
+    }
+  }
+}
+
 // PUSH: velocities slot cur -> pushed, moved -> slot cur ^ 1.
 // !PUSH: pushed (and collided) velocities of slot cur ^ 1 -> moved -> slot
 // cur; the caller then exchanges the two velocity slots.
 template <int CCS, bool PUSH>
 __global__ void __launch_bounds__(256) k32_step_fused(Geom g, FieldsT<float> f, Species32 s, int sp, int cur,
                                                       StepState *st, Migrants mg, PushConst K) {
+  __shared__ double tile[3 * TILE_NODES];
+  __shared__ int box[6];
   int err = 0;
   const int nxt = cur ^ 1;
   const float dtr = (float)(g.dt / g.dr), dtz = (float)(g.dt / g.dz);
@@ -529,48 +602,56 @@ __global__ void __launch_bounds__(256) k32_step_fused(Geom g, FieldsT<float> f, 
     }
     if (!live) { o = n = Split{1, 1, 0.5f, 0.5f}; d_new = 0; v[1] = 0.f; w = 0.f; rd = 1.5 * g.dr; zd = 1.5 * g.dz; }
     const float qw = (float)s.q * w;
+    // deposit domain and the lane's node box (local indices, one cell of
+    // margin): ZigZag deposits around the old and new split cells, VB around
+    // the cells of its FP64 end points
+    const int d_dep = CCS == PCP_CCS_ZIGZAG ? d_new : (live ? deposit_domain32(g, rd, zd) : 0);
+    const Dom dn = make_dom(g, d_dep);
+    int ci0, ci1, ck0, ck1;
     if (CCS == PCP_CCS_ZIGZAG) {
-      const Dom dn = make_dom(g, d_new);
-      WaveSink32 S{f.J, (int)((long)d_new * 3 * g.P), (int)g.P, g.yr, g.nr, g.nz, &err};
-      zigzag32(S, live, g, dn, o, n, v[1], qw, inv_dt);
+      ci0 = o.cr; ci1 = n.cr; ck0 = o.cz; ck1 = n.cz;
     } else {
-      // Villasenor-Buneman terms in FP64 from the old split position and the
-      // unrounded end point
-      const int d_dep = live ? deposit_domain32(g, rd, zd) : 0;
-      const Dom dn = make_dom(g, d_dep);
+      ci0 = cell_number(((double)o.cr + (double)o.fr) * g.dr, g.dr); ci1 = cell_number(rd, g.dr);
+      ck0 = cell_number(((double)o.cz + (double)o.fz) * g.dz, g.dz); ck1 = cell_number(zd, g.dz);
+    }
+    const int bi0 = min(ci0, ci1) - (int)dn.bot_r - 1, bi1 = max(ci0, ci1) - (int)dn.bot_r + 2;
+    const int bk0 = min(ck0, ck1) - (int)dn.left_z - 1, bk1 = max(ck0, ck1) - (int)dn.left_z + 2;
+    if (threadIdx.x == 0) {
+      box[0] = INT_MAX; box[1] = INT_MIN; box[2] = INT_MAX; box[3] = INT_MIN; box[4] = INT_MAX; box[5] = INT_MIN;
+    }
+    __syncthreads();
+    if (live) {
+      atomicMin(&box[0], bi0); atomicMax(&box[1], bi1);
+      atomicMin(&box[2], bk0); atomicMax(&box[3], bk1);
+      atomicMin(&box[4], d_dep); atomicMax(&box[5], d_dep);
+    }
+    __syncthreads();
+    const int ti0 = box[0], ti1 = box[1], tk0 = box[2], tk1 = box[3], td0 = box[4], td1 = box[5];
+    __syncthreads();
+    if (ti0 > ti1) continue;   // no live particle in this block trip
+    const int th = ti1 - ti0 + 1, tw = tk1 - tk0 + 1;
+    if (td0 == td1 && th * tw <= TILE_NODES) {
+      const int nodes = th * tw;
+      for (int t = threadIdx.x; t < 3 * nodes; t += blockDim.x) tile[t] = 0.0;
+      __syncthreads();
+      TileSink32 S{tile, ti0, tk0, th, tw, f.J + (long)td0 * 3 * g.P, (int)g.P, g.yr, g.nr, g.nz, &err};
+      deposit32<CCS>(S, live, g, dn, o, n, v[1], qw, inv_dt, rd, zd, s.q * (double)w, f, d_dep, st, &err);
+      __syncthreads();
+      float *Jd = f.J + (long)td0 * 3 * g.P;
+      for (int t = threadIdx.x; t < 3 * nodes; t += blockDim.x) {
+        const double val = tile[t];
+        if (val == 0.0) continue;
+        const int c = t / nodes, rem = t - c * nodes;
+        const int i = ti0 + rem / tw, k = tk0 + rem % tw;
+        if (i >= -2 && i < g.nr + 2 && k >= -2 && k < g.nz + 2)
+          unsafeAtomicAdd(&Jd[c * g.P + (long)(i + 2) * g.yr + (k + 2)], (float)val);
+        else
+          err |= ERR_MOVED_TOO_FAR;
+      }
+      __syncthreads();
+    } else {
       WaveSink32 S{f.J, (int)((long)d_dep * 3 * g.P), (int)g.P, g.yr, g.nr, g.nz, &err};
-      const double r = rd, z = zd;
-      const double ro = ((double)o.cr + (double)o.fr) * g.dr, zo = ((double)o.cz + (double)o.fz) * g.dz;
-      const double p_charge = s.q * (double)w;
-      AzTerms A = vb_azimuthal_terms(g.nr, g.nz, g.dr, g.dz, dn.bot_r, dn.left_z, r, z, ro, zo, (double)v[1],
-                                     p_charge);
-      S.add_if(1, A.i, A.k, (float)A.w[0], live && A.in[0]);
-      S.add_if(1, A.i + 1, A.k, (float)A.w[1], live && A.in[1]);
-      S.add_if(1, A.i, A.k + 1, (float)A.w[2], live && A.in[2]);
-      S.add_if(1, A.i + 1, A.k + 1, (float)A.w[3], live && A.in[3]);
-      SegRecorder R;
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        R.rn[q] = 1.5 * g.dr; R.zn[q] = 1.25 * g.dz; R.ro[q] = 1.25 * g.dr; R.zo[q] = 1.5 * g.dz;
-        R.i[q] = 1; R.k[q] = 1;
-      }
-      if (live) vb_rz_dispatch(R, g.dr, g.dz, r, z, ro, zo, &err);
-      if (live && R.is_strict) {
-        AtomicSink32 A1{f.J + (long)d_dep * 3 * g.P, g.P, g.yr, g.nr, g.nz, &err, &st->need_exact};
-        vb_strict(A1, g.dr, g.dz, g.dt, dn.bot_r, dn.left_z, r, z, ro, zo, p_charge);
-      }
-#pragma unroll
-      for (int sg = 0; sg < 3; ++sg) {
-        const bool a = live && !R.is_strict && sg < R.n;
-        if (__ballot(a) == 0) break;
-        VBTerms T = vb_simple_terms(g.dr, g.dz, g.dt, dn.bot_r, dn.left_z, R.rn[sg], R.zn[sg], R.ro[sg], R.zo[sg],
-                                    R.i[sg], R.k[sg], p_charge);
-        const bool va = a && T.valid;
-        S.add_if(2, T.i, T.k, (float)T.jz0, va);
-        S.add_if(2, T.i + 1, T.k, (float)T.jz1, va);
-        S.add_if(0, T.i, T.k, (float)T.jr0, va);
-        S.add_if(0, T.i, T.k + 1, (float)T.jr1, va);
-      }
+      deposit32<CCS>(S, live, g, dn, o, n, v[1], qw, inv_dt, rd, zd, s.q * (double)w, f, d_dep, st, &err);
     }
   }
   if (err) atomicOr(&st->err, err);
@@ -1071,6 +1152,60 @@ void launch_field_energy(pcp_ctx *c, double *dev_out) {
     hipLaunchKernelGGL(k_field_energy<float>, dim3(grid), dim3(256), 0, c->stream, c->g, c->s32->f, dev_out);
   else
     hipLaunchKernelGGL(k_field_energy<double>, dim3(grid), dim3(256), 0, c->stream, c->g, c->f, dev_out);
+}
+
+// velocity moments of a species: sum w, sum w v (3), sum w v_z^2,
+// sum w (v_r^2 + v_phi^2), live count
+template <class S, class F>
+__global__ void k_species_moments(S s, long n, F live_at, double *out) {
+  double a[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long)gridDim.x * blockDim.x) {
+    double w, v[3];
+    if (!live_at(s, p, w, v)) continue;
+    a[0] += w;
+    a[1] += w * v[0];
+    a[2] += w * v[1];
+    a[3] += w * v[2];
+    a[4] += w * v[2] * v[2];
+    a[5] += w * (v[0] * v[0] + v[1] * v[1]);
+    a[6] += 1.0;
+  }
+  for (int k = 0; k < 7; ++k) {
+    double v = a[k];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&out[k], v);
+  }
+}
+
+struct Live64 {
+  int cur;
+  __device__ bool operator()(const SpeciesDev &s, long p, double &w, double v[3]) const {
+    if (!(s.r[cur][p] >= 0)) return false;
+    w = s.w[p]; v[0] = s.vr[p]; v[1] = s.vphi[p]; v[2] = s.vz[p];
+    return true;
+  }
+};
+struct Live32 {
+  int cur;
+  __device__ bool operator()(const Species32 &s, long p, double &w, double v[3]) const {
+    if (s.cell[cur][p] == DEAD_CELL) return false;
+    w = s.w[p]; v[0] = s.vr[cur][p]; v[1] = s.vphi[cur][p]; v[2] = s.vz[cur][p];
+    return true;
+  }
+};
+
+void launch_species_moments(pcp_ctx *c, int sp, double *dev_out) {
+  if (c->s32) {
+    Species32 &s = c->s32->sp[sp];
+    if (!s.n) return;
+    hipLaunchKernelGGL((k_species_moments<Species32, Live32>), dim3(pgrid32(c, s.n, 256)), dim3(256), 0, c->stream,
+                       s, s.n, Live32{c->cur}, dev_out);
+  } else {
+    SpeciesDev &s = c->sp[sp];
+    if (!s.n) return;
+    hipLaunchKernelGGL((k_species_moments<SpeciesDev, Live64>), dim3(pgrid32(c, s.n, 256)), dim3(256), 0, c->stream,
+                       s, s.n, Live64{c->cur}, dev_out);
+  }
 }
 
 void launch_kinetic_energy(pcp_ctx *c, int sp, double *dev_out) {

@@ -71,5 +71,8 @@ void launch32_gather_point(pcp_ctx *c, int d, double r, double z, int which, dou
 // sum of w m c^2 (gamma - 1) of a species
 void launch_field_energy(pcp_ctx *c, double *dev_out);
 void launch_kinetic_energy(pcp_ctx *c, int sp, double *dev_out);
+// out[7] += sum w, sum w v_r, sum w v_phi, sum w v_z, sum w v_z^2,
+// sum w (v_r^2 + v_phi^2), number of live particles
+void launch_species_moments(pcp_ctx *c, int sp, double *dev_out);
 
 }  // namespace pcp

@@ -15,6 +15,7 @@
 #include "pcp_sort.hpp"
 #include "pcp32.hpp"
 #include "pcp_collide.hpp"
+#include "pcp_mlp.hpp"
 
 using namespace pcp;
 
@@ -300,6 +301,7 @@ int pcp_create(const pcp_config *cfg, pcp_ctx **out) {
 int pcp_destroy(pcp_ctx *c) {
   if (!c) return 0;
   coll_free(c);
+  mlp_free(c);
   hipFree(c->f.E); hipFree(c->f.H); hipFree(c->f.Hat); hipFree(c->f.J);
   hipFree(c->f.eps); hipFree(c->f.sigma); hipFree(c->st); hipFree(c->marker);
   for (int s = 0; s < MAX_SPECIES; ++s) {
@@ -722,7 +724,9 @@ int pcp_deposit(pcp_ctx *c) {
 
 static int check_coll_args(pcp_ctx *c, int scheme, int se, int si) {
   if (!c) return fail(PCP_E_ARG, "null context");
-  if (scheme < PCP_COLL_P12 || scheme > PCP_COLL_SK98) return fail(PCP_E_ARG, "unknown collision scheme");
+  if (scheme < PCP_COLL_P12 || scheme > PCP_COLL_SURROGATE) return fail(PCP_E_ARG, "unknown collision scheme");
+  if (scheme == PCP_COLL_SURROGATE && (!c->mlp || !c->mlp->trained))
+    return fail(PCP_E_STATE, "the collision surrogate has no fitted network (pcp_surrogate_fit)");
   if (se < 0 || si < 0 || se >= c->cfg.n_species || si >= c->cfg.n_species || se == si)
     return fail(PCP_E_ARG, "collisions need two distinct species (electrons, ions)");
   return 0;
@@ -733,6 +737,134 @@ int pcp_collide(pcp_ctx *c, int scheme, int se, int si, uint64_t seed, uint64_t 
   if (launch_collide(c, scheme, se, si, seed, pass, c->cur) < 0) return fail(PCP_E_HIP, "collision pass failed");
   HIP_TRY(hipGetLastError());
   return pcp_sync(c);
+}
+
+static int need_mlp(pcp_ctx *c) {
+  if (!c) return fail(PCP_E_ARG, "null context");
+  if (mlp_alloc(c) < 0) return fail(PCP_E_NOMEM, "network buffers");
+  return 0;
+}
+
+int pcp_mlp_nparams(void) { return MLP_NPARAM; }
+
+int pcp_mlp_init(pcp_ctx *c, uint64_t seed) {
+  if (int rc = need_mlp(c)) return rc;
+  if (mlp_init(c, seed) < 0) return fail(PCP_E_HIP, "network initialisation");
+  c->mlp->trained = 0;
+  return 0;
+}
+
+int pcp_mlp_get_params(pcp_ctx *c, float *host) {
+  if (int rc = need_mlp(c)) return rc;
+  if (!host) return fail(PCP_E_ARG, "null buffer");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(host, c->mlp->params, MLP_NPARAM * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pcp_mlp_set_params(pcp_ctx *c, const float *host) {
+  if (int rc = need_mlp(c)) return rc;
+  if (!host) return fail(PCP_E_ARG, "null buffer");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(c->mlp->params, host, MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice));
+  c->mlp->inited = 1;
+  return 0;
+}
+
+static int mlp_batch_buffers(pcp_ctx *c, long n) {
+  MlpState *m = c->mlp;
+  if (m->bcap >= n) return 0;
+  (void)hipFree(m->xb); (void)hipFree(m->tb); (void)hipFree(m->yb);
+  m->xb = m->tb = m->yb = nullptr;
+  HIP_TRY(hipMalloc(&m->xb, n * MLP_XSTRIDE * sizeof(float)));
+  HIP_TRY(hipMalloc(&m->tb, n * MLP_YSTRIDE * sizeof(float)));
+  HIP_TRY(hipMalloc(&m->yb, n * MLP_YSTRIDE * sizeof(float)));
+  m->bcap = n;
+  return 0;
+}
+
+int pcp_mlp_forward(pcp_ctx *c, const float *X, long n, float *Y) {
+  if (int rc = need_mlp(c)) return rc;
+  if (n <= 0 || !X || !Y) return fail(PCP_E_ARG, "bad batch");
+  if (int rc = mlp_batch_buffers(c, n)) return rc;
+  MlpState *m = c->mlp;
+  HIP_TRY(hipMemcpy(m->xb, X, n * MLP_XSTRIDE * sizeof(float), hipMemcpyHostToDevice));
+  mlp_forward(c, m->xb, n, m->yb);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(Y, m->yb, n * MLP_YSTRIDE * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pcp_mlp_grad(pcp_ctx *c, const float *X, const float *T, long n, float *grad, float *loss) {
+  if (int rc = need_mlp(c)) return rc;
+  if (n <= 0 || !X || !T) return fail(PCP_E_ARG, "bad batch");
+  if (int rc = mlp_batch_buffers(c, n)) return rc;
+  MlpState *m = c->mlp;
+  HIP_TRY(hipMemcpy(m->xb, X, n * MLP_XSTRIDE * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(m->tb, T, n * MLP_YSTRIDE * sizeof(float), hipMemcpyHostToDevice));
+  mlp_grad(c, m->xb, m->tb, n, m->yb);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (grad) HIP_TRY(hipMemcpy(grad, m->grads, MLP_NPARAM * sizeof(float), hipMemcpyDeviceToHost));
+  if (loss) HIP_TRY(hipMemcpy(loss, m->yb, sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pcp_mlp_adam(pcp_ctx *c, float lr) {
+  if (int rc = need_mlp(c)) return rc;
+  mlp_adam(c, lr);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int pcp_surrogate_collect(pcp_ctx *c, int se, int si, uint64_t seed, uint64_t pass, long *rows) {
+  if (int rc = check_coll_args(c, PCP_COLL_P12, se, si)) return rc;
+  if (int rc = need_mlp(c)) return rc;
+  if (mlp_reserve(c, (long)c->D * c->g.nr * c->g.nz) < 0) return fail(PCP_E_NOMEM, "training set");
+  if (surrogate_collect(c, se, si, seed, pass) < 0) return fail(PCP_E_HIP, "collection pass failed");
+  HIP_TRY(hipGetLastError());
+  if (rows) *rows = c->mlp->n;
+  return pcp_sync(c);
+}
+
+int pcp_surrogate_fit(pcp_ctx *c, int epochs, long batch, float lr, uint64_t seed, float *loss) {
+  if (int rc = need_mlp(c)) return rc;
+  if (epochs < 1 || batch < 1) return fail(PCP_E_ARG, "epochs and batch must be positive");
+  int rc = mlp_fit(c, epochs, batch, lr, seed, loss);
+  if (rc == -2) return fail(PCP_E_STATE, "the training set is too small (pcp_surrogate_collect first)");
+  if (rc < 0) return fail(PCP_E_HIP, "training failed");
+  HIP_TRY(hipGetLastError());
+  return pcp_sync(c);
+}
+
+int pcp_surrogate_get_norm(pcp_ctx *c, float *out) {
+  if (int rc = need_mlp(c)) return rc;
+  MlpState *m = c->mlp;
+  for (int k = 0; k < MLP_IN; ++k) { out[k] = m->feat_mean[k]; out[MLP_IN + k] = m->feat_std[k]; }
+  for (int k = 0; k < MLP_OUT; ++k) out[2 * MLP_IN + k] = m->targ_scale[k];
+  return 0;
+}
+
+int pcp_surrogate_set_norm(pcp_ctx *c, const float *in) {
+  if (int rc = need_mlp(c)) return rc;
+  MlpState *m = c->mlp;
+  for (int k = 0; k < MLP_IN; ++k) {
+    if (!(in[MLP_IN + k] > 0)) return fail(PCP_E_ARG, "feature scales must be positive");
+    m->feat_mean[k] = in[k];
+    m->feat_std[k] = in[MLP_IN + k];
+  }
+  for (int k = 0; k < MLP_OUT; ++k) m->targ_scale[k] = in[2 * MLP_IN + k];
+  m->trained = 1;
+  return 0;
+}
+
+int pcp_surrogate_fallback_cells(pcp_ctx *c, long *cells) {
+  if (!c || !cells) return fail(PCP_E_ARG, "bad argument");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  *cells = surrogate_fallback_cells(c);
+  return 0;
 }
 
 int pcp_set_collisions(pcp_ctx *c, int scheme, int se, int si, uint64_t seed) {
@@ -908,6 +1040,18 @@ int pcp_kinetic_energy(pcp_ctx *c, int sp, double *out) {
   HIP_TRY(hipMemsetAsync(d, 0, sizeof(double), c->stream));
   launch_kinetic_energy(c, sp, d);
   HIP_TRY(hipMemcpyAsync(out, d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  (void)hipFree(d);
+  return 0;
+}
+
+int pcp_species_moments(pcp_ctx *c, int sp, double *out) {
+  if (!c || !out || sp < 0 || sp >= c->cfg.n_species) return fail(PCP_E_ARG, "bad argument");
+  double *d;
+  HIP_TRY(hipMalloc(&d, 7 * sizeof(double)));
+  HIP_TRY(hipMemsetAsync(d, 0, 7 * sizeof(double), c->stream));
+  launch_species_moments(c, sp, d);
+  HIP_TRY(hipMemcpyAsync(out, d, 7 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   (void)hipFree(d);
   return 0;

@@ -26,6 +26,7 @@
 #include "pcp32.hpp"
 #include "pcp_collide.hpp"
 #include "pcp_collide_physics.hpp"
+#include "pcp_mlp.hpp"
 #include "pcp_rng.hpp"
 #include "pcp_wave.hpp"
 
@@ -231,12 +232,13 @@ template <class V>
 __global__ void __launch_bounds__(COLL_BLOCK) k_coll_prep(Geom g, V ve, V vi, const uint32_t *off_e,
                                                           const uint32_t *off_i, uint32_t *perm_e, uint32_t *perm_i,
                                                           double m_e, double m_i, uint64_t seed, uint64_t step,
-                                                          double *tot, int *row_bad, StepState *st) {
+                                                          double *tot, int *row_bad, const int *mask, StepState *st) {
   __shared__ uint64_t skey[COLL_MAXC];
   __shared__ CellStage S;
   const long ncells = (long)g.Dr * g.Dz * g.nr * g.nz;
   int err = 0;
   for (long c = blockIdx.x; c < ncells; c += gridDim.x) {
+    if (mask && !mask[c]) continue;
     int nn[2] = {(int)(off_e[c + 1] - off_e[c]), (int)(off_i[c + 1] - off_i[c])};
     if (nn[0] > COLL_MAXC || nn[1] > COLL_MAXC) {
       err |= ERR_COLL_OVERFLOW;
@@ -254,7 +256,12 @@ __global__ void __launch_bounds__(COLL_BLOCK) k_coll_prep(Geom g, V ve, V vi, co
       const double inv_e = 1. / EL_CHARGE;
       double temperature_el = te.energy / te.amount * inv_e;
       double temperature_ion = ti.energy / ti.amount * inv_e;
-      double cell_volume = PCP_CELL_VOLUME((double)q.cr, g.dr, g.dz);
+      // Collisions::geometry_cell_volume (collisions.cpp:162-170) passes
+      // `i+shift` to CELL_VOLUME, which expands to 2.0 * i + shift + 1
+      // (quirk kept: the macro argument is not parenthesised)
+      const double shift = (double)((long)(q.d / g.Dz) * g.nr);
+      const int i = q.i;
+      double cell_volume = PCP_CELL_VOLUME(i+shift, g.dr, g.dz);
       double sum_el = 0, sum_ion = 0;
       for (int k = 0; k < nn[0]; ++k) sum_el += S.w[0][k];
       for (int k = 0; k < nn[1]; ++k) sum_ion += S.w[1][k];
@@ -373,11 +380,13 @@ __global__ void __launch_bounds__(COLL_BLOCK) k_coll_pairs(Geom g, V ve, V vi, c
                                                            const uint32_t *off_i, const uint32_t *perm_e,
                                                            const uint32_t *perm_i, double m_e, double m_i,
                                                            double q_e, double q_i, uint64_t seed, uint64_t step,
-                                                           const double *tot, const int *row_bad, StepState *st) {
+                                                           const double *tot, const int *row_bad, const int *mask,
+                                                           StepState *st) {
   __shared__ CellStage S;
   const long ncells = (long)g.Dr * g.Dz * g.nr * g.nz;
   int err = 0;
   for (long c = blockIdx.x; c < ncells; c += gridDim.x) {
+    if (mask && !mask[c]) continue;
     int nn[2] = {(int)(off_e[c + 1] - off_e[c]), (int)(off_i[c + 1] - off_i[c])};
     if (nn[0] > COLL_MAXC || nn[1] > COLL_MAXC) {
       err |= ERR_COLL_OVERFLOW;
@@ -476,6 +485,173 @@ __global__ void __launch_bounds__(COLL_BLOCK) k_coll_pairs(Geom g, V ve, V vi, c
   if (err && threadIdx.x == 0) atomicOr(&st->err, err);
 }
 
+// ---------------------------------------------------------------------------
+// collision surrogate (no reference counterpart)
+// ---------------------------------------------------------------------------
+// per-cell moments, per species: count, sum w, sum w v (3), sum w v_z^2,
+// sum w (v_r^2 + v_phi^2)
+constexpr int SST = 16;
+
+template <class V>
+__device__ __forceinline__ void species_moments(const V &v, const uint32_t *perm, uint32_t o, int n, double *out) {
+  double a[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int k = threadIdx.x; k < n; k += COLL_BLOCK) {
+    const uint32_t p = perm[o + k];
+    double vv[3];
+    v.load(p, vv);
+    const double w = v.w(p);
+    a[0] += 1.0;
+    a[1] += w;
+    a[2] += w * vv[0];
+    a[3] += w * vv[1];
+    a[4] += w * vv[2];
+    a[5] += w * vv[2] * vv[2];
+    a[6] += w * (vv[0] * vv[0] + vv[1] * vv[1]);
+  }
+#pragma unroll
+  for (int q = 0; q < 7; ++q) a[q] = wave_sum(a[q]);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 7; ++q) out[q] = a[q];
+}
+
+template <class V>
+__global__ void __launch_bounds__(COLL_BLOCK) k_surr_moments(Geom g, V ve, V vi, const uint32_t *off_e,
+                                                             const uint32_t *off_i, const uint32_t *perm_e,
+                                                             const uint32_t *perm_i, double *mom) {
+  const long ncells = (long)g.Dr * g.Dz * g.nr * g.nz;
+  for (long c = blockIdx.x; c < ncells; c += gridDim.x) {
+    species_moments(ve, perm_e, off_e[c], (int)(off_e[c + 1] - off_e[c]), mom + c * SST);
+    species_moments(vi, perm_i, off_i[c], (int)(off_i[c + 1] - off_i[c]), mom + c * SST + 8);
+  }
+}
+
+// temperatures [eV] of one species from its moments: T_par (z), T_perp
+// (per degree of freedom of r, phi), and the mean velocity
+__device__ __forceinline__ bool species_temps(const double *m, double mass, double &tpar, double &tperp, double V[3]) {
+  const double sw = m[1];
+  if (!(m[0] >= SURR_MIN_PARTICLES) || !(sw > 0)) return false;
+  V[0] = m[2] / sw; V[1] = m[3] / sw; V[2] = m[4] / sw;
+  tpar = mass * (m[5] / sw - V[2] * V[2]) / EL_CHARGE;
+  tperp = mass * (m[6] / sw - V[0] * V[0] - V[1] * V[1]) / (2.0 * EL_CHARGE);
+  return isnormal(tpar) && tpar > 0 && isnormal(tperp) && tperp > 0;
+}
+
+// features: log10 n_e, log10 n_i, log10 Te_par, log10 Te_perp, log10 Ti,
+// log10 dt; normalised by the fitted mean / std when `norm` is given.
+// ok[c] = 1 for a cell the network may handle.
+__global__ void k_surr_features(Geom g, const double *mom, double m_e, double m_i, SurrNorm norm, int normalise,
+                                float *X, int *ok) {
+  const long ncells = (long)g.Dr * g.Dz * g.nr * g.nz;
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += (long)gridDim.x * blockDim.x) {
+    const double *me = mom + c * SST, *mi = me + 8;
+    double tep, tet, tip, tit, Ve[3], Vi[3];
+    bool good = species_temps(me, m_e, tep, tet, Ve) && species_temps(mi, m_i, tip, tit, Vi);
+    const CellPos q = cell_pos(g, c);
+    const double vol = PCP_CELL_VOLUME((double)q.cr, g.dr, g.dz);
+    double f[MLP_IN] = {log10(me[1] / vol), log10(mi[1] / vol), log10(tep), log10(tet),
+                        log10((tip + 2.0 * tit) / 3.0), log10(g.dt)};
+    for (int k = 0; k < MLP_IN; ++k) {
+      float x = (float)f[k];
+      if (normalise) {
+        x = (x - norm.mean[k]) / norm.std[k];
+        if (!(fabsf(x) <= SURR_MAX_Z)) good = false;
+      }
+      X[c * MLP_XSTRIDE + k] = good ? x : 0.f;
+    }
+    X[c * MLP_XSTRIDE + 6] = 0.f;
+    X[c * MLP_XSTRIDE + 7] = 0.f;
+    ok[c] = good ? 1 : 0;
+  }
+}
+
+// training rows of the cells that collided: relative changes of Te_par,
+// Te_perp, Ti_par, Ti_perp over the pass
+__global__ void k_surr_targets(Geom g, const double *before, const double *after, double m_e, double m_i,
+                               const int *ok, const int *row_bad, const float *X, float *DX, float *DT,
+                               unsigned long long *count, long cap) {
+  const long ncells = (long)g.Dr * g.Dz * g.nr * g.nz;
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += (long)gridDim.x * blockDim.x) {
+    if (!ok[c]) continue;
+    const CellPos q = cell_pos(g, c);
+    if (q.j >= row_bad[(long)q.d * g.nr + q.i]) continue;
+    double b[4], a[4], V[3];
+    if (!species_temps(before + c * SST, m_e, b[0], b[1], V) || !species_temps(before + c * SST + 8, m_i, b[2], b[3], V) ||
+        !species_temps(after + c * SST, m_e, a[0], a[1], V) || !species_temps(after + c * SST + 8, m_i, a[2], a[3], V))
+      continue;
+    unsigned long long slot = atomicAdd(count, 1ull);
+    if ((long)slot >= cap) continue;
+    for (int k = 0; k < MLP_XSTRIDE; ++k) DX[slot * MLP_XSTRIDE + k] = X[c * MLP_XSTRIDE + k];
+    for (int k = 0; k < MLP_OUT; ++k) DT[slot * MLP_YSTRIDE + k] = (float)((a[k] - b[k]) / b[k]);
+  }
+}
+
+// The network's relative temperature changes applied per species as
+// rescalings of the thermal velocity about the species' mean velocity (z
+// by alpha_par, r and phi by alpha_perp), then one common factor on both
+// species' thermal velocities restores the cell's total kinetic energy:
+// each species' momentum and the cell's energy are conserved exactly (up to
+// rounding).  Cells the network may not handle are flagged for the binary
+// collision fallback.
+template <class V>
+__global__ void __launch_bounds__(COLL_BLOCK) k_surr_apply(Geom g, V ve, V vi, const uint32_t *off_e,
+                                                           const uint32_t *off_i, const uint32_t *perm_e,
+                                                           const uint32_t *perm_i, const double *mom, double m_e,
+                                                           double m_i, const float *Y, SurrNorm norm, const int *ok,
+                                                           int *fallback, unsigned long long *n_fallback) {
+  __shared__ CellStage S;
+  __shared__ double red[4];
+  const long ncells = (long)g.Dr * g.Dz * g.nr * g.nz;
+  for (long c = blockIdx.x; c < ncells; c += gridDim.x) {
+    const int nn[2] = {(int)(off_e[c + 1] - off_e[c]), (int)(off_i[c + 1] - off_i[c])};
+    const bool use = ok[c] && nn[0] <= COLL_MAXC && nn[1] <= COLL_MAXC;
+    if (threadIdx.x == 0) {
+      fallback[c] = use ? 0 : 1;
+      if (!use && (nn[0] > 0 || nn[1] > 0)) atomicAdd(n_fallback, 1ull);
+    }
+    if (!use) continue;
+    stage_species(ve, perm_e, off_e[c], nn[0], 0, S);
+    stage_species(vi, perm_i, off_i[c], nn[1], 1, S);
+    __syncthreads();
+    double Vs[2][3], apar[2], aperp[2];
+    const double mass[2] = {m_e, m_i};
+    for (int s = 0; s < 2; ++s) {
+      const double *m = mom + c * SST + 8 * s;
+      for (int k = 0; k < 3; ++k) Vs[s][k] = m[2 + k] / m[1];
+      const double ypar = (double)Y[c * MLP_YSTRIDE + 2 * s] * norm.tscale[2 * s];
+      const double yperp = (double)Y[c * MLP_YSTRIDE + 2 * s + 1] * norm.tscale[2 * s + 1];
+      apar[s] = sqrt(fmax(1.0 + ypar, SURR_MIN_FACTOR));
+      aperp[s] = sqrt(fmax(1.0 + yperp, SURR_MIN_FACTOR));
+    }
+    // thermal energy before and after the per-species rescaling
+    double e0 = 0, e1 = 0;
+    for (int s = 0; s < 2; ++s)
+      for (int k = threadIdx.x; k < nn[s]; k += COLL_BLOCK) {
+        const double dx = S.vx[s][k] - Vs[s][0], dy = S.vy[s][k] - Vs[s][1], dz = S.vz[s][k] - Vs[s][2];
+        const double wm = S.w[s][k] * mass[s];
+        e0 += wm * (dx * dx + dy * dy + dz * dz);
+        const double nx = aperp[s] * dx, ny = aperp[s] * dy, nz = apar[s] * dz;
+        e1 += wm * (nx * nx + ny * ny + nz * nz);
+      }
+    e0 = wave_sum(e0);
+    e1 = wave_sum(e1);
+    if (threadIdx.x == 0) red[0] = e1 > 0 ? sqrt(e0 / e1) : 1.0;
+    __syncthreads();
+    const double beta = red[0];
+    for (int s = 0; s < 2; ++s)
+      for (int k = threadIdx.x; k < nn[s]; k += COLL_BLOCK) {
+        double v3[3];
+        v3[0] = Vs[s][0] + beta * (aperp[s] * (S.vx[s][k] - Vs[s][0]));
+        v3[1] = Vs[s][1] + beta * (aperp[s] * (S.vy[s][k] - Vs[s][1]));
+        v3[2] = Vs[s][2] + beta * (apar[s] * (S.vz[s][k] - Vs[s][2]));
+        if (s == 0) ve.store(S.idx[0][k], v3);
+        else vi.store(S.idx[1][k], v3);
+      }
+    __syncthreads();
+  }
+}
+
+__global__ void k_fill_zero_ull(unsigned long long *p) { *p = 0ull; }
+
 }  // namespace
 
 struct CollBuffers {
@@ -487,6 +663,11 @@ struct CollBuffers {
   void *scan_tmp = nullptr;
   size_t scan_bytes = 0;
   long ncells = 0;
+  // surrogate
+  double *mom = nullptr, *mom2 = nullptr;
+  float *feat = nullptr, *pred = nullptr;
+  int *ok = nullptr, *fallback = nullptr;
+  unsigned long long *counter = nullptr;   // [0] fallback cells, [1] collected rows
 };
 
 void coll_free(pcp_ctx *c) {
@@ -497,6 +678,8 @@ void coll_free(pcp_ctx *c) {
     (void)hipFree(b->slot[s]); (void)hipFree(b->perm[s]);
   }
   (void)hipFree(b->tot); (void)hipFree(b->row_bad); (void)hipFree(b->scan_tmp);
+  (void)hipFree(b->mom); (void)hipFree(b->mom2); (void)hipFree(b->feat); (void)hipFree(b->pred);
+  (void)hipFree(b->ok); (void)hipFree(b->fallback); (void)hipFree(b->counter);
   delete b;
   c->coll = nullptr;
 }
@@ -520,8 +703,16 @@ static int coll_alloc(pcp_ctx *c, long cap_e, long cap_i) {
       return -1;
   }
   if (hipMalloc(&b->tot, ncells * TOT * sizeof(double)) != hipSuccess ||
-      hipMalloc(&b->row_bad, (long)c->D * c->g.nr * sizeof(int)) != hipSuccess)
+      hipMalloc(&b->row_bad, (long)c->D * c->g.nr * sizeof(int)) != hipSuccess ||
+      hipMalloc(&b->mom, ncells * SST * sizeof(double)) != hipSuccess ||
+      hipMalloc(&b->mom2, ncells * SST * sizeof(double)) != hipSuccess ||
+      hipMalloc(&b->feat, ncells * MLP_XSTRIDE * sizeof(float)) != hipSuccess ||
+      hipMalloc(&b->pred, ncells * MLP_YSTRIDE * sizeof(float)) != hipSuccess ||
+      hipMalloc(&b->ok, ncells * sizeof(int)) != hipSuccess ||
+      hipMalloc(&b->fallback, ncells * sizeof(int)) != hipSuccess ||
+      hipMalloc(&b->counter, 2 * sizeof(unsigned long long)) != hipSuccess)
     return -1;
+  if (hipMemset(b->counter, 0, 2 * sizeof(unsigned long long)) != hipSuccess) return -1;
   if (hipcub::DeviceScan::ExclusiveSum(nullptr, b->scan_bytes, b->cnt[0], b->off[0], (int)(ncells + 1)) !=
       hipSuccess)
     return -1;
@@ -529,22 +720,21 @@ static int coll_alloc(pcp_ctx *c, long cap_e, long cap_i) {
   return 0;
 }
 
+// bucket lists of both species (sort_to_cells)
 template <class V>
-static int coll_run(pcp_ctx *c, int scheme, V ve, V vi, long n_e, long n_i, double m_e, double m_i, double q_e,
-                    double q_i, uint64_t seed, uint64_t step) {
+static int coll_buckets(pcp_ctx *c, V ve, V vi, long n_e, long n_i) {
   CollBuffers *b = c->coll;
   const Geom &g = c->g;
   const long ncells = b->ncells;
   const uint32_t sentinel = (uint32_t)ncells;
   const long nn[2] = {n_e, n_i};
-  const V vv[2] = {ve, vi};
   unsigned gz = (unsigned)std::min<long>((ncells + 256) / 256, (long)c->n_cu * 16);
   for (int s = 0; s < 2; ++s) {
     hipLaunchKernelGGL(k_fill_u32, dim3(gz), dim3(256), 0, c->stream, b->cnt[s], ncells + 1);
     if (nn[s] > 0) {
       unsigned gp = (unsigned)std::min<long>((nn[s] + 255) / 256, (long)c->n_cu * 16);
-      hipLaunchKernelGGL(k_coll_key<V>, dim3(gp), dim3(256), 0, c->stream, g, vv[s], b->key[s], b->slot[s],
-                         b->cnt[s], sentinel);
+      hipLaunchKernelGGL(k_coll_key<V>, dim3(gp), dim3(256), 0, c->stream, g, s == 0 ? ve : vi, b->key[s],
+                         b->slot[s], b->cnt[s], sentinel);
     }
     size_t bytes = b->scan_bytes;
     if (hipcub::DeviceScan::ExclusiveSum(b->scan_tmp, bytes, b->cnt[s], b->off[s], (int)(ncells + 1), c->stream) !=
@@ -556,21 +746,105 @@ static int coll_run(pcp_ctx *c, int scheme, V ve, V vi, long n_e, long n_i, doub
                          b->perm[s], sentinel);
     }
   }
+  return 0;
+}
+
+// the binary pass (shuffle + statistics, then pairs) on the cells of `mask`
+// (all cells when null)
+template <class V>
+static void coll_binary(pcp_ctx *c, int scheme, V ve, V vi, double m_e, double m_i, double q_e, double q_i,
+                        uint64_t seed, uint64_t step, const int *mask) {
+  CollBuffers *b = c->coll;
+  const Geom &g = c->g;
+  const long ncells = b->ncells;
   const long rows = (long)c->D * g.nr;
   hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)std::max<long>(1, std::min<long>((rows + 255) / 256, 1024))),
                      dim3(256), 0, c->stream, b->row_bad, rows, INT_MAX);
   unsigned gc = (unsigned)std::min<long>(ncells, (long)c->n_cu * 32);
   hipLaunchKernelGGL(k_coll_prep<V>, dim3(gc), dim3(COLL_BLOCK), 0, c->stream, g, ve, vi, b->off[0], b->off[1],
-                     b->perm[0], b->perm[1], m_e, m_i, seed, step, b->tot, b->row_bad, c->st);
+                     b->perm[0], b->perm[1], m_e, m_i, seed, step, b->tot, b->row_bad, mask, c->st);
   if (scheme == COLL_TA77S)
     hipLaunchKernelGGL((k_coll_pairs<COLL_TA77S, V>), dim3(gc), dim3(COLL_BLOCK), 0, c->stream, g, ve, vi, b->off[0],
-                       b->off[1], b->perm[0], b->perm[1], m_e, m_i, q_e, q_i, seed, step, b->tot, b->row_bad, c->st);
+                       b->off[1], b->perm[0], b->perm[1], m_e, m_i, q_e, q_i, seed, step, b->tot, b->row_bad, mask,
+                       c->st);
   else if (scheme == COLL_SK98)
     hipLaunchKernelGGL((k_coll_pairs<COLL_SK98, V>), dim3(gc), dim3(COLL_BLOCK), 0, c->stream, g, ve, vi, b->off[0],
-                       b->off[1], b->perm[0], b->perm[1], m_e, m_i, q_e, q_i, seed, step, b->tot, b->row_bad, c->st);
+                       b->off[1], b->perm[0], b->perm[1], m_e, m_i, q_e, q_i, seed, step, b->tot, b->row_bad, mask,
+                       c->st);
   else
     hipLaunchKernelGGL((k_coll_pairs<COLL_P12, V>), dim3(gc), dim3(COLL_BLOCK), 0, c->stream, g, ve, vi, b->off[0],
-                       b->off[1], b->perm[0], b->perm[1], m_e, m_i, q_e, q_i, seed, step, b->tot, b->row_bad, c->st);
+                       b->off[1], b->perm[0], b->perm[1], m_e, m_i, q_e, q_i, seed, step, b->tot, b->row_bad, mask,
+                       c->st);
+}
+
+template <class V>
+static void surr_moments(pcp_ctx *c, V ve, V vi, double *mom) {
+  CollBuffers *b = c->coll;
+  unsigned gc = (unsigned)std::min<long>(b->ncells, (long)c->n_cu * 32);
+  hipLaunchKernelGGL(k_surr_moments<V>, dim3(gc), dim3(COLL_BLOCK), 0, c->stream, c->g, ve, vi, b->off[0], b->off[1],
+                     b->perm[0], b->perm[1], mom);
+}
+
+static SurrNorm surr_norm(const pcp_ctx *c) {
+  SurrNorm nm;
+  const MlpState *m = c->mlp;
+  for (int k = 0; k < MLP_IN; ++k) {
+    nm.mean[k] = m ? m->feat_mean[k] : 0.f;
+    nm.std[k] = m ? m->feat_std[k] : 1.f;
+  }
+  for (int k = 0; k < MLP_OUT; ++k) nm.tscale[k] = m ? m->targ_scale[k] : 1.f;
+  return nm;
+}
+
+template <class V>
+static int coll_run(pcp_ctx *c, int scheme, V ve, V vi, long n_e, long n_i, double m_e, double m_i, double q_e,
+                    double q_i, uint64_t seed, uint64_t step) {
+  CollBuffers *b = c->coll;
+  if (coll_buckets(c, ve, vi, n_e, n_i) < 0) return -1;
+  if (scheme != COLL_SURROGATE) {
+    coll_binary(c, scheme, ve, vi, m_e, m_i, q_e, q_i, seed, step, nullptr);
+    return 0;
+  }
+  // network on the cells it covers, P12 on the others
+  if (!c->mlp || !c->mlp->trained) return -2;
+  const long ncells = b->ncells;
+  unsigned gc = (unsigned)std::min<long>(ncells, (long)c->n_cu * 32);
+  unsigned gt = (unsigned)std::min<long>((ncells + 255) / 256, (long)c->n_cu * 16);
+  surr_moments(c, ve, vi, b->mom);
+  hipLaunchKernelGGL(k_surr_features, dim3(gt), dim3(256), 0, c->stream, c->g, b->mom, m_e, m_i, surr_norm(c), 1,
+                     b->feat, b->ok);
+  mlp_forward(c, b->feat, ncells, b->pred);
+  hipLaunchKernelGGL(k_fill_zero_ull, dim3(1), dim3(1), 0, c->stream, b->counter);
+  hipLaunchKernelGGL(k_surr_apply<V>, dim3(gc), dim3(COLL_BLOCK), 0, c->stream, c->g, ve, vi, b->off[0], b->off[1],
+                     b->perm[0], b->perm[1], b->mom, m_e, m_i, b->pred, surr_norm(c), b->ok, b->fallback,
+                     b->counter);
+  coll_binary(c, COLL_P12, ve, vi, m_e, m_i, q_e, q_i, seed, step, b->fallback);
+  return 0;
+}
+
+// one P12 pass that also appends (features, relative temperature changes)
+// rows of the cells it collided to the network's training set
+template <class V>
+static int coll_collect(pcp_ctx *c, V ve, V vi, long n_e, long n_i, double m_e, double m_i, double q_e, double q_i,
+                        uint64_t seed, uint64_t step) {
+  CollBuffers *b = c->coll;
+  MlpState *m = c->mlp;
+  if (coll_buckets(c, ve, vi, n_e, n_i) < 0) return -1;
+  const long ncells = b->ncells;
+  unsigned gt = (unsigned)std::min<long>((ncells + 255) / 256, (long)c->n_cu * 16);
+  surr_moments(c, ve, vi, b->mom);
+  hipLaunchKernelGGL(k_surr_features, dim3(gt), dim3(256), 0, c->stream, c->g, b->mom, m_e, m_i, surr_norm(c), 0,
+                     b->feat, b->ok);
+  coll_binary(c, COLL_P12, ve, vi, m_e, m_i, q_e, q_i, seed, step, nullptr);
+  surr_moments(c, ve, vi, b->mom2);
+  unsigned long long n0 = (unsigned long long)m->n;
+  if (hipMemcpyAsync(b->counter + 1, &n0, sizeof n0, hipMemcpyHostToDevice, c->stream) != hipSuccess) return -1;
+  hipLaunchKernelGGL(k_surr_targets, dim3(gt), dim3(256), 0, c->stream, c->g, b->mom, b->mom2, m_e, m_i, b->ok,
+                     b->row_bad, b->feat, m->X, m->T, b->counter + 1, m->cap);
+  unsigned long long n1 = 0;
+  if (hipMemcpyAsync(&n1, b->counter + 1, sizeof n1, hipMemcpyDeviceToHost, c->stream) != hipSuccess) return -1;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return -1;
+  m->n = std::min<long>((long)n1, m->cap);
   return 0;
 }
 
@@ -585,6 +859,26 @@ int launch_collide(pcp_ctx *c, int scheme, int se, int si, uint64_t seed, uint64
   if (coll_alloc(c, e.cap, i.cap) < 0) return -1;
   View64 ve{e, c->cur}, vi{i, c->cur};
   return coll_run(c, scheme, ve, vi, e.n, i.n, e.m, i.m, e.q, i.q, seed, step);
+}
+
+int surrogate_collect(pcp_ctx *c, int se, int si, uint64_t seed, uint64_t step) {
+  if (c->s32) {
+    Species32 &e = c->s32->sp[se], &i = c->s32->sp[si];
+    if (coll_alloc(c, e.cap, i.cap) < 0) return -1;
+    View32 ve{e, c->cur, c->cur}, vi{i, c->cur, c->cur};
+    return coll_collect(c, ve, vi, e.n, i.n, e.m, i.m, e.q, i.q, seed, step);
+  }
+  SpeciesDev &e = c->sp[se], &i = c->sp[si];
+  if (coll_alloc(c, e.cap, i.cap) < 0) return -1;
+  View64 ve{e, c->cur}, vi{i, c->cur};
+  return coll_collect(c, ve, vi, e.n, i.n, e.m, i.m, e.q, i.q, seed, step);
+}
+
+long surrogate_fallback_cells(pcp_ctx *c) {
+  if (!c->coll) return 0;
+  unsigned long long n = 0;
+  if (hipMemcpy(&n, c->coll->counter, sizeof n, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  return (long)n;
 }
 
 }  // namespace pcp

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "pcp_internal.hpp"
+#include "pcp_mlp.hpp"
 
 namespace pcp {
 
@@ -12,11 +13,27 @@ constexpr int COLL_MAXC = 256;
 
 struct CollBuffers;
 
+// the collision surrogate: network on the cells it covers, P12 on the rest
+constexpr int COLL_SURROGATE = 3;
+// a cell goes to the network only with at least this many macro-particles
+// of each species, normal moments, and every normalised feature within
+// SURR_MAX_Z standard deviations of the training set
+constexpr double SURR_MIN_PARTICLES = 8;
+constexpr float SURR_MAX_Z = 4.0f;
+// smallest squared rescaling factor applied to a thermal velocity spread
+constexpr double SURR_MIN_FACTOR = 0.25;
+
+
 // One collision pass (scheme P12 / TA77S / SK98) between electron species se
 // and ion species si, enqueued on the context stream.  `vslot` selects the
 // FP32 velocity slot (ignored for FP64 contexts).  Returns -1 on allocation
 // or launch failure.
 int launch_collide(pcp_ctx *c, int scheme, int se, int si, uint64_t seed, uint64_t step, int vslot);
 void coll_free(pcp_ctx *c);
+// one P12 pass whose collided cells append training rows (raw features,
+// relative temperature changes) to the network's training set
+int surrogate_collect(pcp_ctx *c, int se, int si, uint64_t seed, uint64_t step);
+// cells of the last surrogate pass that went to the P12 fallback
+long surrogate_fallback_cells(pcp_ctx *c);
 
 }  // namespace pcp

@@ -130,6 +130,7 @@ struct Migrants {
 
 struct State32;      // FP32 split-precision state (pcp32.hpp)
 struct CollBuffers;  // collision pass buffers (pcp_collide.hip)
+struct MlpState;     // collision surrogate network (pcp_mlp.hpp)
 
 }  // namespace pcp
 
@@ -153,6 +154,7 @@ struct pcp_ctx {
   long steps_done;
   pcp::State32 *s32 = nullptr;    // non-null in FP32 split mode
   pcp::CollBuffers *coll = nullptr;
+  pcp::MlpState *mlp = nullptr;
   int coll_scheme = -1;            // collision phase of pcp_step (-1: off)
   int coll_se = 0, coll_si = 1;
   uint64_t coll_seed = 0, coll_calls = 0;

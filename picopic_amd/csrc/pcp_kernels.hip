@@ -12,6 +12,8 @@
 //  k_mig_*              order keys of particles that changed domain
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include "pcp_kernels.hpp"
 #include "pcp_physics.hpp"
 #include "pcp_wave.hpp"
@@ -70,6 +72,33 @@ struct MarkSink {
 };
 
 // wave-aggregated sink (call sites must be converged across the wavefront)
+// Per-block LDS tile of the current (the nodes of the block's bounding box in
+// one domain), added to J once per node and block; a contribution outside
+// the box goes straight to J.
+constexpr int TILE_NODES64 = 640;   // nodes per component
+
+struct TileSink {
+  double *t;
+  int i0, k0, th, tw;
+  double *Jd;   // J of the tile's domain
+  int P, yr, nr, nz;
+  bool active;
+  int *err;
+  __device__ __forceinline__ void add_if(int c, int i, int k, double v, bool a) {
+    if (!a) return;
+    const int li = i - i0, lk = k - k0;
+    if (li >= 0 && li < th && lk >= 0 && lk < tw) {
+      atomicAdd(&t[(c * th + li) * tw + lk], v);
+    } else if (i >= -2 && i < nr + 2 && k >= -2 && k < nz + 2) {
+      unsafeAtomicAdd(&Jd[c * P + (i + 2) * yr + (k + 2)], v);
+    } else {
+      *err |= ERR_MOVED_TOO_FAR;
+    }
+  }
+  __device__ __forceinline__ void add(int c, int i, int k, double v) { add_if(c, i, k, v, active); }
+  __device__ __forceinline__ void strict_reset(int, int) {}
+};
+
 struct WaveSink {
   double *J;      // f.J, every domain
   int base;       // offset of component 0 of the destination domain
@@ -377,6 +406,51 @@ __global__ void k_migrate(Geom g, SpeciesDev s, int sp, int cur, StepState *st, 
   if (err) atomicOr(&st->err, err);
 }
 
+// the deposit of one lane's particle into a sink; every lane of the block
+// calls it (inactive lanes with live = false)
+template <int CCS, class Sink>
+__device__ __forceinline__ void deposit64(Sink &S, bool live, const Geom &g, const Dom &dn, double r, double z,
+                                          double ro, double zo, double vphi, double q, double w, const Fields &f,
+                                          int d_new, StepState *st, int *err) {
+  if (CCS == PCP_CCS_ZIGZAG) {
+    zigzag(S, g.dr, g.dz, g.dt, dn.bot_r, dn.left_z, r, z, ro, zo, vphi, q, w);
+  } else {
+    const double p_charge = q * w;
+    // azimuthal part (4 interior nodes)
+    AzTerms A = vb_azimuthal_terms(g.nr, g.nz, g.dr, g.dz, dn.bot_r, dn.left_z, r, z, ro, zo, vphi, p_charge);
+    S.add_if(1, A.i, A.k, A.w[0], live && A.in[0]);
+    S.add_if(1, A.i + 1, A.k, A.w[1], live && A.in[1]);
+    S.add_if(1, A.i, A.k + 1, A.w[2], live && A.in[2]);
+    S.add_if(1, A.i + 1, A.k + 1, A.w[3], live && A.in[3]);
+    // r-z part: branch selection per lane, arithmetic and atomics converged
+    SegRecorder R;
+#pragma unroll
+    for (int q2 = 0; q2 < 3; ++q2) {
+      R.rn[q2] = 1.5 * g.dr; R.zn[q2] = 1.25 * g.dz; R.ro[q2] = 1.25 * g.dr; R.zo[q2] = 1.5 * g.dz;
+      R.i[q2] = 1; R.k[q2] = 1;
+    }
+    if (live) vb_rz_dispatch(R, g.dr, g.dz, r, z, ro, zo, err);
+    if (live && R.is_strict) {
+      // strict motion is rare: plain atomics; a J_r reset flags the exact
+      // re-deposition of this step
+      AtomicSink A1{f.J + (long)d_new * 3 * g.P, g.P, g.yr, g.nr, g.nz, err, &st->need_exact};
+      vb_strict(A1, g.dr, g.dz, g.dt, dn.bot_r, dn.left_z, r, z, ro, zo, p_charge);
+    }
+#pragma unroll
+    for (int sg = 0; sg < 3; ++sg) {
+      const bool a = live && !R.is_strict && sg < R.n;
+      if (__ballot(a) == 0) break;
+      VBTerms T = vb_simple_terms(g.dr, g.dz, g.dt, dn.bot_r, dn.left_z, R.rn[sg], R.zn[sg], R.ro[sg], R.zo[sg],
+                                  R.i[sg], R.k[sg], p_charge);
+      const bool v = a && T.valid;
+      S.add_if(2, T.i, T.k, T.jz0, v);
+      S.add_if(2, T.i + 1, T.k, T.jz1, v);
+      S.add_if(0, T.i, T.k, T.jr0, v);
+      S.add_if(0, T.i, T.k + 1, T.jr1, v);
+    }
+  }
+}
+
 // fused push + move + deposit (the hot kernel).  Every thread of a block runs
 // the same number of loop trips so that the ZigZag deposit, which has no
 // data-dependent branches, executes converged and aggregates its atomics per
@@ -385,6 +459,8 @@ __global__ void k_migrate(Geom g, SpeciesDev s, int sp, int cur, StepState *st, 
 template <int CCS, bool PUSH>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CCS == PCP_CCS_ZIGZAG ? 3 : 2)))
 k_step_fused(Geom g, Fields f, SpeciesDev s, int sp, int cur, StepState *st, Migrants mg) {
+  __shared__ double tile[3 * TILE_NODES64];
+  __shared__ int box[6];
   int err = 0;
   const int nxt = cur ^ 1;
   const long stride = (long)gridDim.x * blockDim.x;
@@ -421,44 +497,48 @@ k_step_fused(Geom g, Fields f, SpeciesDev s, int sp, int cur, StepState *st, Mig
       }
     }
     if (!live) { r = ro = g.dr; z = zo = g.dz; d_new = 0; vphi = 0; w = 0; }
-    Dom dn = make_dom(g, d_new);
-    WaveSink S{f.J, (int)((long)d_new * 3 * g.P), (int)g.P, g.yr, g.nr, g.nz, live, &err};
-    if (CCS == PCP_CCS_ZIGZAG) {
-      zigzag(S, g.dr, g.dz, g.dt, dn.bot_r, dn.left_z, r, z, ro, zo, vphi, s.q, w);
+    const Dom dn = make_dom(g, d_new);
+    // the lane's node box (local indices of its domain, one cell of margin)
+    const int ci0 = cell_number(ro, g.dr), ci1 = cell_number(r, g.dr);
+    const int ck0 = cell_number(zo, g.dz), ck1 = cell_number(z, g.dz);
+    const int bi0 = min(ci0, ci1) - (int)dn.bot_r - 1, bi1 = max(ci0, ci1) - (int)dn.bot_r + 2;
+    const int bk0 = min(ck0, ck1) - (int)dn.left_z - 1, bk1 = max(ck0, ck1) - (int)dn.left_z + 2;
+    if (threadIdx.x == 0) {
+      box[0] = INT_MAX; box[1] = INT_MIN; box[2] = INT_MAX; box[3] = INT_MIN; box[4] = INT_MAX; box[5] = INT_MIN;
+    }
+    __syncthreads();
+    if (live) {
+      atomicMin(&box[0], bi0); atomicMax(&box[1], bi1);
+      atomicMin(&box[2], bk0); atomicMax(&box[3], bk1);
+      atomicMin(&box[4], d_new); atomicMax(&box[5], d_new);
+    }
+    __syncthreads();
+    const int ti0 = box[0], ti1 = box[1], tk0 = box[2], tk1 = box[3], td0 = box[4], td1 = box[5];
+    __syncthreads();
+    if (ti0 > ti1) continue;   // no live particle in this block trip
+    const int th = ti1 - ti0 + 1, tw = tk1 - tk0 + 1;
+    if (td0 == td1 && th * tw <= TILE_NODES64) {
+      const int nodes = th * tw;
+      for (int t = threadIdx.x; t < 3 * nodes; t += blockDim.x) tile[t] = 0.0;
+      __syncthreads();
+      TileSink S{tile, ti0, tk0, th, tw, f.J + (long)td0 * 3 * g.P, (int)g.P, g.yr, g.nr, g.nz, live, &err};
+      deposit64<CCS>(S, live, g, dn, r, z, ro, zo, vphi, s.q, w, f, d_new, st, &err);
+      __syncthreads();
+      double *Jd = f.J + (long)td0 * 3 * g.P;
+      for (int t = threadIdx.x; t < 3 * nodes; t += blockDim.x) {
+        const double val = tile[t];
+        if (val == 0.0) continue;
+        const int c = t / nodes, rem = t - c * nodes;
+        const int i = ti0 + rem / tw, k = tk0 + rem % tw;
+        if (i >= -2 && i < g.nr + 2 && k >= -2 && k < g.nz + 2)
+          unsafeAtomicAdd(&Jd[c * g.P + (long)(i + 2) * g.yr + (k + 2)], val);
+        else
+          err |= ERR_MOVED_TOO_FAR;
+      }
+      __syncthreads();
     } else {
-      const double p_charge = s.q * w;
-      // azimuthal part (4 interior nodes)
-      AzTerms A = vb_azimuthal_terms(g.nr, g.nz, g.dr, g.dz, dn.bot_r, dn.left_z, r, z, ro, zo, vphi, p_charge);
-      S.add_if(1, A.i, A.k, A.w[0], live && A.in[0]);
-      S.add_if(1, A.i + 1, A.k, A.w[1], live && A.in[1]);
-      S.add_if(1, A.i, A.k + 1, A.w[2], live && A.in[2]);
-      S.add_if(1, A.i + 1, A.k + 1, A.w[3], live && A.in[3]);
-      // r-z part: branch selection per lane, arithmetic and atomics converged
-      SegRecorder R;
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        R.rn[q] = 1.5 * g.dr; R.zn[q] = 1.25 * g.dz; R.ro[q] = 1.25 * g.dr; R.zo[q] = 1.5 * g.dz;
-        R.i[q] = 1; R.k[q] = 1;
-      }
-      if (live) vb_rz_dispatch(R, g.dr, g.dz, r, z, ro, zo, &err);
-      if (live && R.is_strict) {
-        // strict motion is rare: plain atomics; a J_r reset flags the
-        // exact re-deposition of this step
-        AtomicSink A1{f.J + (long)d_new * 3 * g.P, g.P, g.yr, g.nr, g.nz, &err, &st->need_exact};
-        vb_strict(A1, g.dr, g.dz, g.dt, dn.bot_r, dn.left_z, r, z, ro, zo, p_charge);
-      }
-#pragma unroll
-      for (int sg = 0; sg < 3; ++sg) {
-        const bool a = live && !R.is_strict && sg < R.n;
-        if (__ballot(a) == 0) break;
-        VBTerms T = vb_simple_terms(g.dr, g.dz, g.dt, dn.bot_r, dn.left_z, R.rn[sg], R.zn[sg], R.ro[sg], R.zo[sg],
-                                    R.i[sg], R.k[sg], p_charge);
-        const bool v = a && T.valid;
-        S.add_if(2, T.i, T.k, T.jz0, v);
-        S.add_if(2, T.i + 1, T.k, T.jz1, v);
-        S.add_if(0, T.i, T.k, T.jr0, v);
-        S.add_if(0, T.i, T.k + 1, T.jr1, v);
-      }
+      WaveSink S{f.J, (int)((long)d_new * 3 * g.P), (int)g.P, g.yr, g.nr, g.nz, live, &err};
+      deposit64<CCS>(S, live, g, dn, r, z, ro, zo, vphi, s.q, w, f, d_new, st, &err);
     }
   }
   if (err) atomicOr(&st->err, err);
