@@ -226,6 +226,14 @@ int pcp_kinetic_energy(pcp_ctx *ctx, int sp, double *out);
  * sum w v_r, sum w v_phi, sum w v_z, sum w v_z^2, sum w (v_r^2 + v_phi^2),
  * number of live macro-particles */
 int pcp_species_moments(pcp_ctx *ctx, int sp, double *out);
+/* Discrete continuity residual of the last step (no reference counterpart;
+ * it checks currentVB.cpp's charge conservation on the device): for every
+ * node (i dr, k dz), 0 <= i <= Nr-2, 1 <= k <= Nz-2, the change of the
+ * charge the particles' ring clouds ([r -+ dr/2] x [z -+ dz/2]) put in the
+ * node's control volume plus dt times the current leaving it through J_r
+ * (cylinder r = (i+1/2) dr) and J_z (annulus at z = (k+1/2) dz, pi dr^2/4 at
+ * the axis).  out[3] = max |residual|, max |charge change|, largest |q w|. */
+int pcp_continuity_residual(pcp_ctx *ctx, double *out);
 
 /* achievable HBM bandwidth of a plain device copy of `bytes` bytes (read +
  * write counted), averaged over iters launches [GB/s]; a calibration aid for

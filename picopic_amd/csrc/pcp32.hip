@@ -303,7 +303,8 @@ __device__ __forceinline__ bool move32(const Geom &g, const Dom &o, int beam, fl
   float vr = v[0], vp = v[1];
   v[0] = cs * vr - sn * vp;
   v[1] = sn * vr + cs * vp;
-  return p.cr >= 0 && p.cz >= 0 && p.cr < g.Nr && p.cz < g.Nz;
+  // a non-finite position leaves the area, as in the FP64 mover
+  return isfinite(p.fr) && isfinite(p.fz) && p.cr >= 0 && p.cz >= 0 && p.cr < g.Nr && p.cz < g.Nz;
 }
 
 // ---------------------------------------------------------------------------

@@ -16,6 +16,7 @@
 #include "pcp32.hpp"
 #include "pcp_collide.hpp"
 #include "pcp_mlp.hpp"
+#include "pcp_diag.hpp"
 
 using namespace pcp;
 
@@ -1053,6 +1054,16 @@ int pcp_species_moments(pcp_ctx *c, int sp, double *out) {
   launch_species_moments(c, sp, d);
   HIP_TRY(hipMemcpyAsync(out, d, 7 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  (void)hipFree(d);
+  return 0;
+}
+
+int pcp_continuity_residual(pcp_ctx *c, double *out) {
+  if (!c || !out) return fail(PCP_E_ARG, "bad argument");
+  double *d;
+  HIP_TRY(hipMalloc(&d, 3 * sizeof(double)));
+  if (launch_continuity_residual(c, d) < 0) return fail(PCP_E_HIP, "continuity residual");
+  HIP_TRY(hipMemcpy(out, d, 3 * sizeof(double), hipMemcpyDeviceToHost));
   (void)hipFree(d);
   return 0;
 }
