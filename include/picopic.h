@@ -201,6 +201,12 @@ int pcp_surrogate_set_norm(pcp_ctx *ctx, const float *in16);
 /* cells of the last surrogate pass that took the P12 fallback */
 int pcp_surrogate_fallback_cells(pcp_ctx *ctx, long *cells);
 
+/* Replay each step from a captured hipGraph (no reference counterpart).
+ * Steps with a collision phase, timed steps (pcp_set_timing) and the sorts
+ * run as ordinary launches; graphs are recaptured after anything that
+ * changes the particle buffers (upload, init, sort). */
+int pcp_set_graph(pcp_ctx *ctx, int on);
+
 /* Counting sort of species sp by (domain, cell), dropping removed particles
  * (no reference counterpart: the reference keeps per-domain vectors).  The
  * reference order keys travel with the particles, so results do not depend

@@ -29,7 +29,7 @@ EXPORTS = [
     "pcp_collide", "pcp_set_collisions", "pcp_mlp_nparams", "pcp_mlp_init", "pcp_mlp_get_params",
     "pcp_mlp_set_params", "pcp_mlp_forward", "pcp_mlp_grad", "pcp_mlp_adam", "pcp_surrogate_collect",
     "pcp_surrogate_fit", "pcp_surrogate_get_norm", "pcp_surrogate_set_norm", "pcp_surrogate_fallback_cells",
-    "pcp_species_moments", "pcp_continuity_residual",
+    "pcp_species_moments", "pcp_continuity_residual", "pcp_set_graph",
 ]
 COLL = {"p12": 0, "ta77s": 1, "sk98": 2, "surrogate": 3}
 
@@ -109,6 +109,7 @@ def load(path=LIB_PATH):
     L.pcp_surrogate_fallback_cells.argtypes = [vp, C.POINTER(C.c_long)]
     L.pcp_species_moments.argtypes = [vp, C.c_int, dp]
     L.pcp_continuity_residual.argtypes = [vp, dp]
+    L.pcp_set_graph.argtypes = [vp, C.c_int]
     _lib = L
     return L
 
@@ -269,6 +270,9 @@ class Context:
         tpar = mass * (m[4] / sw - V[2] ** 2) / 1.6e-19
         tperp = mass * (m[5] / sw - V[0] ** 2 - V[1] ** 2) / (2 * 1.6e-19)
         return tpar, tperp
+
+    def set_graph(self, on=True):
+        _check(_lib.pcp_set_graph(self.h, 1 if on else 0))
 
     def continuity_residual(self):
         """(max |residual|, max |charge change|, max |q w|) of the last step."""

@@ -129,6 +129,10 @@ int ensure_capacity(pcp_ctx *c, int sp, long add) {
 
 }  // namespace
 
+// drops the captured step graphs (their kernels hold buffer pointers and
+// particle counts)
+static void graph_invalidate(pcp_ctx *c);
+
 extern "C" {
 
 const char *pcp_last_error(void) { return g_last_error.c_str(); }
@@ -301,6 +305,7 @@ int pcp_create(const pcp_config *cfg, pcp_ctx **out) {
 
 int pcp_destroy(pcp_ctx *c) {
   if (!c) return 0;
+  graph_invalidate(c);
   coll_free(c);
   mlp_free(c);
   hipFree(c->f.E); hipFree(c->f.H); hipFree(c->f.Hat); hipFree(c->f.J);
@@ -337,6 +342,7 @@ long pcp_grid_points(pcp_ctx *c) { return c ? c->g.P : -1; }
 
 int pcp_upload_species(pcp_ctx *c, int sp, long n, const double *r, const double *z, const double *vr,
                        const double *vphi, const double *vz, const double *w, const uint64_t *id) {
+  if (c) graph_invalidate(c);
   if (!c || sp < 0 || sp >= c->cfg.n_species || n < 0) return fail(PCP_E_ARG, "bad argument");
   if (n == 0) return 0;
   if (!r || !z || !vr || !vphi || !vz || !w) return fail(PCP_E_ARG, "null particle array");
@@ -527,6 +533,7 @@ int pcp_download_species(pcp_ctx *c, int sp, double *r, double *z, double *vr, d
 
 int pcp_init_plasma(pcp_ctx *c, int sp, int n_per_cell, double density_left, double density_right,
                     double temperature_ev, uint64_t seed) {
+  if (c) graph_invalidate(c);
   if (!c || sp < 0 || sp >= c->cfg.n_species || n_per_cell <= 0 || !(density_left >= 0) || !(density_right >= 0))
     return fail(PCP_E_ARG, "bad argument");
   const Geom &g = c->g;
@@ -584,6 +591,7 @@ int pcp_init_plasma(pcp_ctx *c, int sp, int n_per_cell, double density_left, dou
 
 int pcp_init_bunch(pcp_ctx *c, int sp, long n, double n_peak, double sigma_r, double sigma_z, double z0, double gamma,
                    uint64_t seed) {
+  if (c) graph_invalidate(c);
   if (!c || sp < 0 || sp >= c->cfg.n_species || n <= 0 || !(gamma >= 1) || !(sigma_r > 0) || !(sigma_z > 0))
     return fail(PCP_E_ARG, "bad argument");
   int rc = ensure_capacity(c, sp, n);
@@ -883,6 +891,81 @@ int pcp_set_collisions(pcp_ctx *c, int scheme, int se, int si, uint64_t seed) {
   return 0;
 }
 
+}  // extern "C"
+
+// the kernels of one step after the sorts: solve_maxvell, advance_particles
+// (with the collision phase between push and move), solve_current
+// (src/PiCoPiC.cpp:299-345); flips c->cur
+template <class Mark>
+static int step_body(pcp_ctx *c, Mark mark) {
+  const bool f32 = c->s32 != nullptr;
+  launch_step_begin(c);
+  if (f32) {
+    launch32_yee_e(c, 1);
+    launch32_stitch(c, PCP_GRID_E);
+    launch32_yee_h(c);
+    launch32_stitch(c, PCP_GRID_H);
+  } else {
+    launch_yee_e(c, 1);
+    launch_stitch(c, PCP_GRID_E);
+    launch_yee_h(c);
+    launch_stitch(c, PCP_GRID_H);
+  }
+  mark(1);
+  const bool coll = c->coll_scheme >= 0;
+  if (coll) {
+    // SMB::advance_particles: push, collide (before the move), move
+    for (int s = 0; s < c->cfg.n_species; ++s) {
+      if (f32) launch32_push(c, s);
+      else launch_push(c, s);
+    }
+    if (launch_collide(c, c->coll_scheme, c->coll_se, c->coll_si, c->coll_seed, c->coll_calls++, c->cur ^ 1) < 0)
+      return fail(PCP_E_HIP, "collision pass failed");
+  }
+  for (int s = 0; s < c->cfg.n_species; ++s) {
+    if (f32) launch32_step_fused(c, s, !coll);
+    else launch_step_fused(c, s, !coll);
+  }
+  c->cur ^= 1;
+  mark(2);
+  if (f32) launch32_migrant_keys(c);
+  else launch_migrant_keys(c);
+  // exact re-deposition when a strict axial motion reset J_r nodes
+  if (c->g.ccs == PCP_CCS_VB) {
+    if (f32) {
+      launch32_zero_j_if_exact(c);
+      for (int s = 0; s < c->cfg.n_species; ++s) launch32_mark(c, s, !coll, 1);
+      for (int s = 0; s < c->cfg.n_species; ++s) launch32_deposit_exact(c, s, !coll, 1);
+    } else {
+      launch_zero_j_if_exact(c);
+      for (int s = 0; s < c->cfg.n_species; ++s) launch_mark(c, s, 1);
+      for (int s = 0; s < c->cfg.n_species; ++s) launch_deposit_exact(c, s, 1);
+    }
+  }
+  mark(3);
+  if (f32) launch32_stitch(c, PCP_GRID_J);
+  else launch_stitch(c, PCP_GRID_J);
+  mark(4);
+  return 0;
+}
+
+static void graph_invalidate(pcp_ctx *c) {
+  for (auto &e : c->gexec)
+    if (e) {
+      (void)hipGraphExecDestroy(e);
+      e = nullptr;
+    }
+}
+
+extern "C" {
+
+int pcp_set_graph(pcp_ctx *c, int on) {
+  if (!c) return fail(PCP_E_ARG, "null context");
+  c->graph_on = on != 0;
+  if (!c->graph_on) graph_invalidate(c);
+  return 0;
+}
+
 int pcp_step(pcp_ctx *c, int n_steps) {
   if (!c || n_steps < 0) return fail(PCP_E_ARG, "bad argument");
   // optional per-phase device timing: events on the context stream
@@ -892,63 +975,40 @@ int pcp_step(pcp_ctx *c, int n_steps) {
     ev.resize((size_t)n_steps * 5);
     for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
   }
-  auto mark = [&](int t, int k) {
-    if (c->timing) (void)hipEventRecord(ev[(size_t)t * 5 + k], c->stream);
-  };
+  const bool f32 = c->s32 != nullptr;
   for (int t = 0; t < n_steps; ++t) {
-    mark(t, 0);
-    const bool f32 = c->s32 != nullptr;
-    launch_step_begin(c);
-    if (f32) {
-      launch32_yee_e(c, 1);
-      launch32_stitch(c, PCP_GRID_E);
-      launch32_yee_h(c);
-      launch32_stitch(c, PCP_GRID_H);
-    } else {
-      launch_yee_e(c, 1);
-      launch_stitch(c, PCP_GRID_E);
-      launch_yee_h(c);
-      launch_stitch(c, PCP_GRID_H);
-    }
+    auto mark = [&](int k) {
+      if (c->timing) (void)hipEventRecord(ev[(size_t)t * 5 + k], c->stream);
+    };
+    mark(0);
+    // the sorts change the particle buffers, so they run outside any graph
     for (int s = 0; s < c->cfg.n_species; ++s)
-      if (c->sort_every[s] > 0 && c->steps_done % c->sort_every[s] == 0 &&
-          (f32 ? launch32_sort(c, s) : launch_sort(c, s)) < 0)
-        return fail(PCP_E_HIP, "sort failed");
-    mark(t, 1);
+      if (c->sort_every[s] > 0 && c->steps_done % c->sort_every[s] == 0) {
+        if ((f32 ? launch32_sort(c, s) : launch_sort(c, s)) < 0) return fail(PCP_E_HIP, "sort failed");
+        graph_invalidate(c);
+      }
     const bool coll = c->coll_scheme >= 0;
-    if (coll) {
-      // SMB::advance_particles: push, collide (before the move), move
-      for (int s = 0; s < c->cfg.n_species; ++s) {
-        if (f32) launch32_push(c, s);
-        else launch_push(c, s);
+    if (c->graph_on && !c->timing && !coll) {
+      // one graph per parity of `cur` (the kernels' buffer arguments differ)
+      const int par = c->cur;
+      if (!c->gexec[par]) {
+        hipGraph_t graph;
+        HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+        int rc = step_body(c, [](int) {});
+        hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        c->cur = par;   // nothing ran during the capture
+        if (rc) return rc;
+        if (e != hipSuccess) return fail(PCP_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        HIP_TRY(hipGraphInstantiate(&c->gexec[par], graph, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(graph);
       }
-      if (launch_collide(c, c->coll_scheme, c->coll_se, c->coll_si, c->coll_seed, c->coll_calls++, c->cur ^ 1) < 0)
-        return fail(PCP_E_HIP, "collision pass failed");
+      HIP_TRY(hipGraphLaunch(c->gexec[par], c->stream));
+      c->cur ^= 1;
+    } else {
+      if (int rc = step_body(c, mark)) return rc;
+      // an FP32 step with collisions exchanges velocity buffers
+      if (coll && f32) graph_invalidate(c);
     }
-    for (int s = 0; s < c->cfg.n_species; ++s) {
-      if (f32) launch32_step_fused(c, s, !coll);
-      else launch_step_fused(c, s, !coll);
-    }
-    c->cur ^= 1;
-    mark(t, 2);
-    if (f32) launch32_migrant_keys(c);
-    else launch_migrant_keys(c);
-    // exact re-deposition when a strict axial motion reset J_r nodes
-    if (c->g.ccs == PCP_CCS_VB) {
-      if (f32) {
-        launch32_zero_j_if_exact(c);
-        for (int s = 0; s < c->cfg.n_species; ++s) launch32_mark(c, s, !coll, 1);
-        for (int s = 0; s < c->cfg.n_species; ++s) launch32_deposit_exact(c, s, !coll, 1);
-      } else {
-        launch_zero_j_if_exact(c);
-        for (int s = 0; s < c->cfg.n_species; ++s) launch_mark(c, s, 1);
-        for (int s = 0; s < c->cfg.n_species; ++s) launch_deposit_exact(c, s, 1);
-      }
-    }
-    mark(t, 3);
-    if (f32) launch32_stitch(c, PCP_GRID_J);
-    else launch_stitch(c, PCP_GRID_J);
-    mark(t, 4);
     ++c->steps_done;
   }
   HIP_TRY(hipGetLastError());
@@ -974,6 +1034,7 @@ int pcp_step(pcp_ctx *c, int n_steps) {
 }
 
 int pcp_sort(pcp_ctx *c, int sp) {
+  if (c) graph_invalidate(c);
   if (!c || sp < 0 || sp >= c->cfg.n_species) return fail(PCP_E_ARG, "bad argument");
   if ((c->s32 ? launch32_sort(c, sp) : launch_sort(c, sp)) < 0) return fail(PCP_E_HIP, "sort failed");
   HIP_TRY(hipGetLastError());

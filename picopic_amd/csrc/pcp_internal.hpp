@@ -155,6 +155,8 @@ struct pcp_ctx {
   pcp::State32 *s32 = nullptr;    // non-null in FP32 split mode
   pcp::CollBuffers *coll = nullptr;
   pcp::MlpState *mlp = nullptr;
+  int graph_on = 0;                 // replay captured step graphs
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};   // per parity of `cur`
   int coll_scheme = -1;            // collision phase of pcp_step (-1: off)
   int coll_se = 0, coll_si = 1;
   uint64_t coll_seed = 0, coll_calls = 0;
