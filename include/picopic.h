@@ -207,13 +207,35 @@ int pcp_surrogate_fallback_cells(pcp_ctx *ctx, long *cells);
  * changes the particle buffers (upload, init, sort). */
 int pcp_set_graph(pcp_ctx *ctx, int on);
 
-/* Counting sort of species sp by (domain, cell), dropping removed particles
+/* Counting sort of species sp by (domain, sort bin, cell), dropping removed particles
  * (no reference counterpart: the reference keeps per-domain vectors).  The
  * reference order keys travel with the particles, so results do not depend
  * on the storage order.  pcp_step sorts species sp before every `every`-th
  * step (0: never; defaults: beams 1, plasma 50). */
 int pcp_sort(pcp_ctx *ctx, int sp);
 int pcp_set_sort_interval(pcp_ctx *ctx, int sp, int every);
+
+/* Launch tuning (no reference counterpart: the reference's OpenMP loops
+ * have no launch shape).  The sort bin orders the cells of a domain in
+ * bin_r x bin_z tiles (each must divide the domain's cell count in r and
+ * z; default 1 x nz, plain row-major), so that the particles of one
+ * workgroup trip cover a compact patch of field and current nodes.  The
+ * launch shape is the workgroup size of the FP32 particle kernel (128, 256
+ * or 512) and its resident workgroups per CU (0: as many as fit).
+ * pcp_autotune times the candidates on the live state (the simulation
+ * advances by the steps it takes: about 40 x steps + 24), keeps the fastest
+ * bin and shape, and sets the plasma species' sort interval from the
+ * measured sort cost and the growth of the step time after a sort. */
+typedef struct pcp_tuning {
+  int bin_r, bin_z;
+  int block, blocks_per_cu;
+  int sort_every;        /* largest sort interval of the non-beam species */
+  double ms_per_step;    /* best trial step time (pcp_autotune) */
+} pcp_tuning;
+int pcp_set_sort_bin(pcp_ctx *ctx, int bin_r, int bin_z);
+int pcp_set_launch(pcp_ctx *ctx, int block, int blocks_per_cu);
+int pcp_get_tuning(pcp_ctx *ctx, pcp_tuning *out);
+int pcp_autotune(pcp_ctx *ctx, int steps, pcp_tuning *out);
 
 /* field gather at one point of domain d (include/maxwellSolver.hpp:101-316),
  * evaluated on the device; out[3] = (r, phi, z) components */

@@ -29,7 +29,8 @@ EXPORTS = [
     "pcp_collide", "pcp_set_collisions", "pcp_mlp_nparams", "pcp_mlp_init", "pcp_mlp_get_params",
     "pcp_mlp_set_params", "pcp_mlp_forward", "pcp_mlp_grad", "pcp_mlp_adam", "pcp_surrogate_collect",
     "pcp_surrogate_fit", "pcp_surrogate_get_norm", "pcp_surrogate_set_norm", "pcp_surrogate_fallback_cells",
-    "pcp_species_moments", "pcp_continuity_residual", "pcp_set_graph",
+    "pcp_species_moments", "pcp_continuity_residual", "pcp_set_graph", "pcp_set_sort_bin", "pcp_set_launch",
+    "pcp_get_tuning", "pcp_autotune",
 ]
 COLL = {"p12": 0, "ta77s": 1, "sk98": 2, "surrogate": 3}
 
@@ -45,6 +46,14 @@ class Config(C.Structure):
                 ("pml_outer", C.c_double), ("pml_sigma1", C.c_double), ("pml_sigma2", C.c_double),
                 ("precision", C.c_int), ("n_species", C.c_int), ("species", SpeciesDesc * 8),
                 ("max_migrants", C.c_long)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("bin_r", C.c_int), ("bin_z", C.c_int), ("block", C.c_int), ("blocks_per_cu", C.c_int),
+                ("sort_every", C.c_int), ("ms_per_step", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class PcpError(RuntimeError):
@@ -110,6 +119,10 @@ def load(path=LIB_PATH):
     L.pcp_species_moments.argtypes = [vp, C.c_int, dp]
     L.pcp_continuity_residual.argtypes = [vp, dp]
     L.pcp_set_graph.argtypes = [vp, C.c_int]
+    L.pcp_set_sort_bin.argtypes = [vp, C.c_int, C.c_int]
+    L.pcp_set_launch.argtypes = [vp, C.c_int, C.c_int]
+    L.pcp_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    L.pcp_autotune.argtypes = [vp, C.c_int, C.POINTER(Tuning)]
     _lib = L
     return L
 
@@ -345,6 +358,24 @@ class Context:
 
     def set_sort_interval(self, sp, every):
         _check(_lib.pcp_set_sort_interval(self.h, sp, every))
+
+    def set_sort_bin(self, bin_r, bin_z):
+        _check(_lib.pcp_set_sort_bin(self.h, bin_r, bin_z))
+
+    def set_launch(self, block, blocks_per_cu=0):
+        _check(_lib.pcp_set_launch(self.h, block, blocks_per_cu))
+
+    def tuning(self):
+        t = Tuning()
+        _check(_lib.pcp_get_tuning(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def autotune(self, steps=4):
+        """Time the sort-bin / launch-shape candidates on the live state (the
+        simulation advances) and keep the fastest; returns the choice."""
+        t = Tuning()
+        _check(_lib.pcp_autotune(self.h, steps, C.byref(t)))
+        return t.as_dict()
 
     def copy_bandwidth(self, nbytes=1 << 30, iters=20):
         g = C.c_double()

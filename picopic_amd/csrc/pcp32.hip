@@ -147,6 +147,15 @@ struct PushConst {
   float inv_c2;  // 1 / c^2
 };
 
+// the species of one batched step launch; end[k] = trips of species 0..k
+struct Batch32 {
+  Species32 s[MAX_SPECIES];
+  PushConst K[MAX_SPECIES];
+  long end[MAX_SPECIES];
+  int sp[MAX_SPECIES];
+  int nsp;
+};
+
 __device__ __forceinline__ float len2f(const float v[3]) { return v[0] * v[0] + v[1] * v[1] + v[2] * v[2]; }
 
 __device__ __forceinline__ void push32(int pusher, const PushConst &K, float e[3], float b[3], float v[3], int *err) {
@@ -565,95 +574,117 @@ __device__ __forceinline__ void deposit32(Sink &S, bool live, const Geom &g, con
 // PUSH: velocities slot cur -> pushed, moved -> slot cur ^ 1.
 // !PUSH: pushed (and collided) velocities of slot cur ^ 1 -> moved -> slot
 // cur; the caller then exchanges the two velocity slots.
+// One block trip of the fused step: the blockDim.x particles from b0 on.
 template <int CCS, bool PUSH>
-__global__ void __launch_bounds__(256) k32_step_fused(Geom g, FieldsT<float> f, Species32 s, int sp, int cur,
-                                                      StepState *st, Migrants mg, PushConst K) {
-  __shared__ double tile[3 * TILE_NODES];
-  __shared__ int box[6];
-  int err = 0;
+__device__ __forceinline__ void step_trip32(const Geom &g, const FieldsT<float> &f, const Species32 &s, int sp,
+                                            const PushConst &K, long b0, int cur, StepState *st, const Migrants &mg,
+                                            double *tile, int *box, int &err) {
   const int nxt = cur ^ 1;
   const float dtr = (float)(g.dt / g.dr), dtz = (float)(g.dt / g.dz);
   const float inv_dt = (float)(1.0 / g.dt);
-  const long stride = (long)gridDim.x * blockDim.x;
-  for (long b0 = (long)blockIdx.x * blockDim.x; b0 < s.n; b0 += stride) {
-    const long p = b0 + threadIdx.x;
-    bool live = false;
-    Split o{1, 1, 0.5f, 0.5f}, n{1, 1, 0.5f, 0.5f};
-    float v[3] = {0.f, 0.f, 0.f};
-    double rd = 1.5 * g.dr, zd = 1.5 * g.dz;
-    float w = 0.f;
-    int d_new = 0;
-    if (p < s.n) {
-      uint32_t cell = s.cell[cur][p];
-      int d_old = 0;
-      const int vin = PUSH ? cur : nxt, vout = PUSH ? nxt : cur;
-      if (cell != DEAD_CELL &&
-          advance32(g, f, s, p, cell, cur, vin, PUSH, K, dtr, dtz, o, n, v, rd, zd, d_old, &err)) {
-        s.vr[vout][p] = v[0]; s.vphi[vout][p] = v[1]; s.vz[vout][p] = v[2];
-        s.cell[nxt][p] = (uint32_t)n.cr * (uint32_t)g.Nz + (uint32_t)n.cz;
-        s.fr[nxt][p] = n.fr;
-        s.fz[nxt][p] = n.fz;
-        d_new = domain_of32(g, n.cr, n.cz);
-        if (d_new != d_old) append_migrant32(mg, st, sp, p, d_old, d_new, s.seq[p], g, &err);
-        w = s.w[p];
-        live = true;
-      } else {
-        s.cell[nxt][p] = DEAD_CELL;
-      }
-    }
-    if (!live) { o = n = Split{1, 1, 0.5f, 0.5f}; d_new = 0; v[1] = 0.f; w = 0.f; rd = 1.5 * g.dr; zd = 1.5 * g.dz; }
-    const float qw = (float)s.q * w;
-    // deposit domain and the lane's node box (local indices, one cell of
-    // margin): ZigZag deposits around the old and new split cells, VB around
-    // the cells of its FP64 end points
-    const int d_dep = CCS == PCP_CCS_ZIGZAG ? d_new : (live ? deposit_domain32(g, rd, zd) : 0);
-    const Dom dn = make_dom(g, d_dep);
-    int ci0, ci1, ck0, ck1;
-    if (CCS == PCP_CCS_ZIGZAG) {
-      ci0 = o.cr; ci1 = n.cr; ck0 = o.cz; ck1 = n.cz;
+  const long p = b0 + threadIdx.x;
+  bool live = false;
+  Split o{1, 1, 0.5f, 0.5f}, n{1, 1, 0.5f, 0.5f};
+  float v[3] = {0.f, 0.f, 0.f};
+  double rd = 1.5 * g.dr, zd = 1.5 * g.dz;
+  float w = 0.f;
+  int d_new = 0;
+  if (p < s.n) {
+    uint32_t cell = s.cell[cur][p];
+    int d_old = 0;
+    const int vin = PUSH ? cur : nxt, vout = PUSH ? nxt : cur;
+    if (cell != DEAD_CELL &&
+        advance32(g, f, s, p, cell, cur, vin, PUSH, K, dtr, dtz, o, n, v, rd, zd, d_old, &err)) {
+      s.vr[vout][p] = v[0]; s.vphi[vout][p] = v[1]; s.vz[vout][p] = v[2];
+      s.cell[nxt][p] = (uint32_t)n.cr * (uint32_t)g.Nz + (uint32_t)n.cz;
+      s.fr[nxt][p] = n.fr;
+      s.fz[nxt][p] = n.fz;
+      d_new = domain_of32(g, n.cr, n.cz);
+      if (d_new != d_old) append_migrant32(mg, st, sp, p, d_old, d_new, s.seq[p], g, &err);
+      w = s.w[p];
+      live = true;
     } else {
-      ci0 = cell_number(((double)o.cr + (double)o.fr) * g.dr, g.dr); ci1 = cell_number(rd, g.dr);
-      ck0 = cell_number(((double)o.cz + (double)o.fz) * g.dz, g.dz); ck1 = cell_number(zd, g.dz);
+      s.cell[nxt][p] = DEAD_CELL;
     }
-    const int bi0 = min(ci0, ci1) - (int)dn.bot_r - 1, bi1 = max(ci0, ci1) - (int)dn.bot_r + 2;
-    const int bk0 = min(ck0, ck1) - (int)dn.left_z - 1, bk1 = max(ck0, ck1) - (int)dn.left_z + 2;
-    if (threadIdx.x == 0) {
-      box[0] = INT_MAX; box[1] = INT_MIN; box[2] = INT_MAX; box[3] = INT_MIN; box[4] = INT_MAX; box[5] = INT_MIN;
+  }
+  if (!live) { o = n = Split{1, 1, 0.5f, 0.5f}; d_new = 0; v[1] = 0.f; w = 0.f; rd = 1.5 * g.dr; zd = 1.5 * g.dz; }
+  const float qw = (float)s.q * w;
+  // deposit domain and the lane's node box (local indices, one cell of
+  // margin): ZigZag deposits around the old and new split cells, VB around
+  // the cells of its FP64 end points
+  const int d_dep = CCS == PCP_CCS_ZIGZAG ? d_new : (live ? deposit_domain32(g, rd, zd) : 0);
+  const Dom dn = make_dom(g, d_dep);
+  int ci0, ci1, ck0, ck1;
+  if (CCS == PCP_CCS_ZIGZAG) {
+    ci0 = o.cr; ci1 = n.cr; ck0 = o.cz; ck1 = n.cz;
+  } else {
+    ci0 = cell_number(((double)o.cr + (double)o.fr) * g.dr, g.dr); ci1 = cell_number(rd, g.dr);
+    ck0 = cell_number(((double)o.cz + (double)o.fz) * g.dz, g.dz); ck1 = cell_number(zd, g.dz);
+  }
+  const int bi0 = min(ci0, ci1) - (int)dn.bot_r - 1, bi1 = max(ci0, ci1) - (int)dn.bot_r + 2;
+  const int bk0 = min(ck0, ck1) - (int)dn.left_z - 1, bk1 = max(ck0, ck1) - (int)dn.left_z + 2;
+  if (threadIdx.x == 0) {
+    box[0] = INT_MAX; box[1] = INT_MIN; box[2] = INT_MAX; box[3] = INT_MIN; box[4] = INT_MAX; box[5] = INT_MIN;
+  }
+  __syncthreads();
+  if (live) {
+    atomicMin(&box[0], bi0); atomicMax(&box[1], bi1);
+    atomicMin(&box[2], bk0); atomicMax(&box[3], bk1);
+    atomicMin(&box[4], d_dep); atomicMax(&box[5], d_dep);
+  }
+  __syncthreads();
+  const int ti0 = box[0], ti1 = box[1], tk0 = box[2], tk1 = box[3], td0 = box[4], td1 = box[5];
+  __syncthreads();
+  if (ti0 > ti1) return;   // no live particle in this block trip
+  const int th = ti1 - ti0 + 1, tw = tk1 - tk0 + 1;
+  if (td0 == td1 && th * tw <= TILE_NODES) {
+    const int nodes = th * tw;
+    for (int t = threadIdx.x; t < 3 * nodes; t += blockDim.x) tile[t] = 0.0;
+    __syncthreads();
+    TileSink32 S{tile, ti0, tk0, th, tw, f.J + (long)td0 * 3 * g.P, (int)g.P, g.yr, g.nr, g.nz, &err};
+    deposit32<CCS>(S, live, g, dn, o, n, v[1], qw, inv_dt, rd, zd, s.q * (double)w, f, d_dep, st, &err);
+    __syncthreads();
+    float *Jd = f.J + (long)td0 * 3 * g.P;
+    for (int t = threadIdx.x; t < 3 * nodes; t += blockDim.x) {
+      const double val = tile[t];
+      if (val == 0.0) continue;
+      const int c = t / nodes, rem = t - c * nodes;
+      const int i = ti0 + rem / tw, k = tk0 + rem % tw;
+      if (i >= -2 && i < g.nr + 2 && k >= -2 && k < g.nz + 2)
+        unsafeAtomicAdd(&Jd[c * g.P + (long)(i + 2) * g.yr + (k + 2)], (float)val);
+      else
+        err |= ERR_MOVED_TOO_FAR;
     }
     __syncthreads();
-    if (live) {
-      atomicMin(&box[0], bi0); atomicMax(&box[1], bi1);
-      atomicMin(&box[2], bk0); atomicMax(&box[3], bk1);
-      atomicMin(&box[4], d_dep); atomicMax(&box[5], d_dep);
-    }
+  } else {
+    WaveSink32 S{f.J, (int)((long)d_dep * 3 * g.P), (int)g.P, g.yr, g.nr, g.nz, &err};
+    deposit32<CCS>(S, live, g, dn, o, n, v[1], qw, inv_dt, rd, zd, s.q * (double)w, f, d_dep, st, &err);
+  }
+}
+
+// The fused step of every species in one persistent launch: a block takes
+// the next trip (BLOCK consecutive particles of one species) from a counter
+// in StepState until the trips of all species are handed out, so that trips
+// of uneven cost (wave-path deposits, dead slots, domains of different
+// density) balance over the resident blocks.
+template <int CCS, bool PUSH, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k32_step_batch(Geom g, FieldsT<float> f, Batch32 b, int cur, StepState *st,
+                                                        Migrants mg) {
+  __shared__ double tile[3 * TILE_NODES];
+  __shared__ int box[6];
+  __shared__ unsigned ticket;
+  int err = 0;
+  const long total = b.end[b.nsp - 1];
+  for (;;) {
+    if (threadIdx.x == 0) ticket = atomicAdd(&st->queue, 1u);
     __syncthreads();
-    const int ti0 = box[0], ti1 = box[1], tk0 = box[2], tk1 = box[3], td0 = box[4], td1 = box[5];
+    const long t = (long)(unsigned)__builtin_amdgcn_readfirstlane((int)ticket);
     __syncthreads();
-    if (ti0 > ti1) continue;   // no live particle in this block trip
-    const int th = ti1 - ti0 + 1, tw = tk1 - tk0 + 1;
-    if (td0 == td1 && th * tw <= TILE_NODES) {
-      const int nodes = th * tw;
-      for (int t = threadIdx.x; t < 3 * nodes; t += blockDim.x) tile[t] = 0.0;
-      __syncthreads();
-      TileSink32 S{tile, ti0, tk0, th, tw, f.J + (long)td0 * 3 * g.P, (int)g.P, g.yr, g.nr, g.nz, &err};
-      deposit32<CCS>(S, live, g, dn, o, n, v[1], qw, inv_dt, rd, zd, s.q * (double)w, f, d_dep, st, &err);
-      __syncthreads();
-      float *Jd = f.J + (long)td0 * 3 * g.P;
-      for (int t = threadIdx.x; t < 3 * nodes; t += blockDim.x) {
-        const double val = tile[t];
-        if (val == 0.0) continue;
-        const int c = t / nodes, rem = t - c * nodes;
-        const int i = ti0 + rem / tw, k = tk0 + rem % tw;
-        if (i >= -2 && i < g.nr + 2 && k >= -2 && k < g.nz + 2)
-          unsafeAtomicAdd(&Jd[c * g.P + (long)(i + 2) * g.yr + (k + 2)], (float)val);
-        else
-          err |= ERR_MOVED_TOO_FAR;
-      }
-      __syncthreads();
-    } else {
-      WaveSink32 S{f.J, (int)((long)d_dep * 3 * g.P), (int)g.P, g.yr, g.nr, g.nz, &err};
-      deposit32<CCS>(S, live, g, dn, o, n, v[1], qw, inv_dt, rd, zd, s.q * (double)w, f, d_dep, st, &err);
-    }
+    if (t >= total) break;
+    int k = 0;
+    while (k + 1 < b.nsp && t >= b.end[k]) ++k;
+    const long first = k ? b.end[k - 1] : 0;
+    step_trip32<CCS, PUSH>(g, f, b.s[k], b.sp[k], b.K[k], (t - first) * BLOCK, cur, st, mg, tile, box, err);
   }
   if (err) atomicOr(&st->err, err);
 }
@@ -729,7 +760,7 @@ __device__ __forceinline__ uint32_t sort_key32(const Geom &g, uint32_t cell) {
   int cr = (int)(cell / (uint32_t)g.Nz), cz = (int)(cell - (uint32_t)cr * (uint32_t)g.Nz);
   int d = domain_of32(g, cr, cz);
   long lr = cr - (long)(d / g.Dz) * g.nr, lz = cz - (long)(d % g.Dz) * g.nz;
-  return (uint32_t)((long)d * g.nr * g.nz + lr * g.nz + lz);
+  return sort_cell_key(g, d, lr, lz);
 }
 
 __global__ void __launch_bounds__(256) k32_sort_count(Geom g, Species32 s, int cur, uint32_t *key, uint32_t *count,
@@ -1039,31 +1070,51 @@ void launch32_push(pcp_ctx *c, int sp) {
                      push_const(s, c->g.dt));
 }
 
-void launch32_step_fused(pcp_ctx *c, int sp, bool push) {
-  Species32 &s = c->s32->sp[sp];
-  if (!s.n) return;
-  PushConst K = push_const(s, c->g.dt);
-  const dim3 grid(pgrid32(c, s.n, 256)), block(256);
-  if (c->g.ccs == PCP_CCS_ZIGZAG) {
-    if (push)
-      hipLaunchKernelGGL((k32_step_fused<PCP_CCS_ZIGZAG, true>), grid, block, 0, c->stream, c->g, c->s32->f, s, sp,
-                         c->cur, c->st, c->mig, K);
-    else
-      hipLaunchKernelGGL((k32_step_fused<PCP_CCS_ZIGZAG, false>), grid, block, 0, c->stream, c->g, c->s32->f, s, sp,
-                         c->cur, c->st, c->mig, K);
-  } else {
-    if (push)
-      hipLaunchKernelGGL((k32_step_fused<PCP_CCS_VB, true>), grid, block, 0, c->stream, c->g, c->s32->f, s, sp, c->cur,
-                         c->st, c->mig, K);
-    else
-      hipLaunchKernelGGL((k32_step_fused<PCP_CCS_VB, false>), grid, block, 0, c->stream, c->g, c->s32->f, s, sp,
-                         c->cur, c->st, c->mig, K);
+template <int CCS, bool PUSH, int BLOCK>
+static void launch_batch_t(pcp_ctx *c, const Batch32 &b) {
+  static int occ = 0;   // resident blocks per CU of this instance
+  if (!occ && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k32_step_batch<CCS, PUSH, BLOCK>, BLOCK, 0) !=
+                   hipSuccess || occ < 1))
+    occ = 1;
+  const int per_cu = c->tune.blocks_per_cu > 0 ? c->tune.blocks_per_cu : occ;
+  const long grid = std::max(1L, std::min(b.end[b.nsp - 1], (long)c->n_cu * per_cu));
+  hipLaunchKernelGGL((k32_step_batch<CCS, PUSH, BLOCK>), dim3((unsigned)grid), dim3(BLOCK), 0, c->stream, c->g,
+                     c->s32->f, b, c->cur, c->st, c->mig);
+}
+
+template <int CCS>
+static void launch_batch_ccs(pcp_ctx *c, const Batch32 &b, bool push, int block) {
+  if (!push) launch_batch_t<CCS, false, 256>(c, b);
+  else if (block == 128) launch_batch_t<CCS, true, 128>(c, b);
+  else if (block == 512) launch_batch_t<CCS, true, 512>(c, b);
+  else launch_batch_t<CCS, true, 256>(c, b);
+}
+
+void launch32_step_batch(pcp_ctx *c, bool push) {
+  Batch32 b{};
+  const int block = push ? c->tune.block : 256;
+  long acc = 0;
+  for (int sp = 0; sp < c->cfg.n_species; ++sp) {
+    const Species32 &s = c->s32->sp[sp];
+    if (!s.n) continue;
+    acc += (s.n + block - 1) / block;
+    b.s[b.nsp] = s;
+    b.K[b.nsp] = push_const(s, c->g.dt);
+    b.end[b.nsp] = acc;
+    b.sp[b.nsp] = sp;
+    ++b.nsp;
   }
-  if (!push) {
-    std::swap(s.vr[0], s.vr[1]);
-    std::swap(s.vphi[0], s.vphi[1]);
-    std::swap(s.vz[0], s.vz[1]);
+  if (b.nsp) {
+    if (c->g.ccs == PCP_CCS_ZIGZAG) launch_batch_ccs<PCP_CCS_ZIGZAG>(c, b, push, block);
+    else launch_batch_ccs<PCP_CCS_VB>(c, b, push, block);
   }
+  if (!push)
+    for (int sp = 0; sp < c->cfg.n_species; ++sp) {
+      Species32 &s = c->s32->sp[sp];
+      std::swap(s.vr[0], s.vr[1]);
+      std::swap(s.vphi[0], s.vphi[1]);
+      std::swap(s.vz[0], s.vz[1]);
+    }
 }
 
 void launch32_mark(pcp_ctx *c, int sp, int do_push, int only_if_exact) {

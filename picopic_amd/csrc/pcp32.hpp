@@ -54,7 +54,8 @@ void launch32_stitch(pcp_ctx *c, int which);
 // collided beforehand; the launcher exchanges the velocity slots after the
 // move so that slot cur ^ 1 keeps the pre-move velocities for the exact path
 void launch32_push(pcp_ctx *c, int sp);
-void launch32_step_fused(pcp_ctx *c, int sp, bool push = true);
+// the fused step of all species in one persistent work-queue launch
+void launch32_step_batch(pcp_ctx *c, bool push = true);
 void launch32_mark(pcp_ctx *c, int sp, int do_push, int only_if_exact);
 void launch32_deposit_exact(pcp_ctx *c, int sp, int do_push, int only_if_exact);
 void launch32_zero_j_if_exact(pcp_ctx *c);

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -180,6 +181,8 @@ int pcp_create(const pcp_config *cfg, pcp_ctx **out) {
   g.pml_on = cfg->pml_on;
   g.ccs = cfg->ccs;
   g.pusher = cfg->pusher;
+  g.bin_r = 1;
+  g.bin_z = g.nz;
   c->D = g.Dr * g.Dz;
   c->cur = 0;
 
@@ -922,10 +925,9 @@ static int step_body(pcp_ctx *c, Mark mark) {
     if (launch_collide(c, c->coll_scheme, c->coll_se, c->coll_si, c->coll_seed, c->coll_calls++, c->cur ^ 1) < 0)
       return fail(PCP_E_HIP, "collision pass failed");
   }
-  for (int s = 0; s < c->cfg.n_species; ++s) {
-    if (f32) launch32_step_fused(c, s, !coll);
-    else launch_step_fused(c, s, !coll);
-  }
+  if (f32) launch32_step_batch(c, !coll);
+  else
+    for (int s = 0; s < c->cfg.n_species; ++s) launch_step_fused(c, s, !coll);
   c->cur ^= 1;
   mark(2);
   if (f32) launch32_migrant_keys(c);
@@ -1044,6 +1046,134 @@ int pcp_sort(pcp_ctx *c, int sp) {
 int pcp_set_sort_interval(pcp_ctx *c, int sp, int every) {
   if (!c || sp < 0 || sp >= c->cfg.n_species || every < 0) return fail(PCP_E_ARG, "bad argument");
   c->sort_every[sp] = every;
+  return 0;
+}
+
+int pcp_set_sort_bin(pcp_ctx *c, int bin_r, int bin_z) {
+  if (!c) return fail(PCP_E_ARG, "null context");
+  if (bin_r < 1 || bin_z < 1 || c->g.nr % bin_r || c->g.nz % bin_z)
+    return fail(PCP_E_ARG, "sort bin must divide the domain's cells (nr, nz)");
+  graph_invalidate(c);
+  c->g.bin_r = bin_r;
+  c->g.bin_z = bin_z;
+  return 0;
+}
+
+int pcp_set_launch(pcp_ctx *c, int block, int blocks_per_cu) {
+  if (!c) return fail(PCP_E_ARG, "null context");
+  if ((block != 128 && block != 256 && block != 512) || blocks_per_cu < 0 || blocks_per_cu > 32)
+    return fail(PCP_E_ARG, "block must be 128, 256 or 512 and blocks_per_cu in [0, 32]");
+  graph_invalidate(c);
+  c->tune.block = block;
+  c->tune.blocks_per_cu = blocks_per_cu;
+  return 0;
+}
+
+int pcp_get_tuning(pcp_ctx *c, pcp_tuning *out) {
+  if (!c || !out) return fail(PCP_E_ARG, "bad argument");
+  out->bin_r = c->g.bin_r;
+  out->bin_z = c->g.bin_z;
+  out->block = c->tune.block;
+  out->blocks_per_cu = c->tune.blocks_per_cu;
+  out->sort_every = 0;
+  for (int s = 0; s < c->cfg.n_species; ++s)
+    if (!c->cfg.species[s].is_beam) out->sort_every = std::max(out->sort_every, c->sort_every[s]);
+  out->ms_per_step = 0;
+  return 0;
+}
+
+namespace {
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// sort every species, then time `steps` steps with no sort among them
+int tune_trial(pcp_ctx *c, int steps, double *ms) {
+  for (int s = 0; s < c->cfg.n_species; ++s)
+    if (int rc = pcp_sort(c, s)) return rc;
+  const double t0 = now_ms();
+  if (int rc = pcp_step(c, steps)) return rc;
+  *ms = (now_ms() - t0) / steps;
+  return 0;
+}
+
+}  // namespace
+
+// The tuner times candidates on the live state, so the simulation advances
+// by every step it takes (about 40 x steps + 24 in all).
+int pcp_autotune(pcp_ctx *c, int steps, pcp_tuning *out) {
+  if (!c || steps < 1 || steps > 1000) return fail(PCP_E_ARG, "bad argument");
+  int saved[MAX_SPECIES];
+  for (int s = 0; s < c->cfg.n_species; ++s) {
+    saved[s] = c->sort_every[s];
+    c->sort_every[s] = 0;
+  }
+  const int timing = c->timing;
+  c->timing = 0;
+  auto restore = [&](int rc) {
+    for (int s = 0; s < c->cfg.n_species; ++s) c->sort_every[s] = saved[s];
+    c->timing = timing;
+    return rc;
+  };
+  const Geom &g = c->g;
+  // 1. sort bin, at the default launch shape
+  std::vector<std::pair<int, int>> bins = {{1, g.nz}};
+  for (int br : {1, 2, 4, 8, 16})
+    for (int bz : {4, 8, 16, 32, 64})
+      if (br * bz >= 8 && br * bz <= 512 && g.nr % br == 0 && g.nz % bz == 0 && bz < g.nz) bins.push_back({br, bz});
+  double best = 1e300;
+  std::pair<int, int> best_bin = bins[0];
+  for (auto b : bins) {
+    double ms;
+    if (int rc = pcp_set_sort_bin(c, b.first, b.second)) return restore(rc);
+    if (int rc = tune_trial(c, steps, &ms)) return restore(rc);
+    if (ms < best) best = ms, best_bin = b;
+  }
+  if (int rc = pcp_set_sort_bin(c, best_bin.first, best_bin.second)) return restore(rc);
+  // 2. launch shape of the batched FP32 kernel
+  if (c->s32) {
+    int best_block = c->tune.block, best_per_cu = c->tune.blocks_per_cu;
+    for (int block : {128, 256, 512})
+      for (int per_cu : {0, 1, 2}) {
+        double ms;
+        if (int rc = pcp_set_launch(c, block, per_cu)) return restore(rc);
+        if (int rc = tune_trial(c, steps, &ms)) return restore(rc);
+        if (ms < best) best = ms, best_block = block, best_per_cu = per_cu;
+      }
+    if (int rc = pcp_set_launch(c, best_block, best_per_cu)) return restore(rc);
+  }
+  // 3. sort interval of the plasma species: the step time t(j) j steps
+  // after a sort grows as the particles leave cell order; with t(j) ~ a + b j
+  // and sort time t_s, an interval k costs t_s / k + a + b (k - 1) / 2 per
+  // step, least at k = sqrt(2 t_s / b)
+  double t_sort = 0;
+  {
+    if (int rc = pcp_sync(c)) return restore(rc);
+    const double t0 = now_ms();
+    for (int s = 0; s < c->cfg.n_species; ++s)
+      if (!c->cfg.species[s].is_beam)
+        if (int rc = pcp_sort(c, s)) return restore(rc);
+    t_sort = now_ms() - t0;
+  }
+  const int J = 24;
+  double sj = 0, st = 0, sjj = 0, sjt = 0;
+  for (int j = 0; j < J; ++j) {
+    const double t0 = now_ms();
+    if (int rc = pcp_step(c, 1)) return restore(rc);
+    const double t = now_ms() - t0;
+    sj += j, st += t, sjj += (double)j * j, sjt += j * t;
+  }
+  const double b = (J * sjt - sj * st) / (J * sjj - sj * sj);
+  int every = 200;
+  if (b > 0) every = (int)std::lround(std::sqrt(2 * t_sort / b));
+  every = std::max(1, std::min(200, every));
+  for (int s = 0; s < c->cfg.n_species; ++s) saved[s] = c->cfg.species[s].is_beam ? saved[s] : every;
+  restore(0);
+  if (out) {
+    pcp_get_tuning(c, out);
+    out->ms_per_step = best;
+  }
   return 0;
 }
 

@@ -58,6 +58,7 @@ struct Geom {
   long gca_r, gca_z;                 // domain Geometry::global_cell_amount
   int pml_on;
   int ccs, pusher;
+  int bin_r, bin_z;  // sort bin (cells; divide nr and nz)
 };
 
 // Per-domain values derived from the domain index (src/SMB.cpp:49-121).
@@ -78,6 +79,16 @@ __host__ __device__ inline Dom make_dom(const Geom &g, int d) {
   o.w2 = o.I == g.Dr - 1;
   o.w3 = o.J == g.Dz - 1;
   return o;
+}
+
+// Sort key of local cell (lr, lz) of domain d: the cells of a domain in
+// bins of bin_r x bin_z cells, bins row-major, cells row-major inside a bin
+// (bin_r = 1, bin_z = nz is plain row-major order).  A permutation of
+// [0, D nr nz), so that every cell keeps one contiguous run of particles.
+__host__ __device__ inline uint32_t sort_cell_key(const Geom &g, int d, long lr, long lz) {
+  const long nbz = g.nz / g.bin_z, bsz = (long)g.bin_r * g.bin_z;
+  const long bi = lr / g.bin_r, bk = lz / g.bin_z;
+  return (uint32_t)((long)d * g.nr * g.nz + (bi * nbz + bk) * bsz + (lr - bi * g.bin_r) * g.bin_z + (lz - bk * g.bin_z));
 }
 
 // halo-grid index of local (i, k) (interior-relative, may be -2..n+1)
@@ -109,6 +120,7 @@ struct StepState {
   int err;
   int need_exact;              // a strict-axis J_r reset happened in this step
   unsigned int n_mig;          // migrants appended in this step
+  unsigned int queue;          // next trip of the batched particle launch
 };
 
 // second set of per-particle arrays used as the sort's scatter target
@@ -155,6 +167,11 @@ struct pcp_ctx {
   pcp::State32 *s32 = nullptr;    // non-null in FP32 split mode
   pcp::CollBuffers *coll = nullptr;
   pcp::MlpState *mlp = nullptr;
+  // launch shape of the batched FP32 particle kernel (autotuned)
+  struct {
+    int block = 256;                // threads per block: 128, 256 or 512
+    int blocks_per_cu = 0;          // resident blocks per CU (0: occupancy)
+  } tune;
   int graph_on = 0;                 // replay captured step graphs
   hipGraphExec_t gexec[2] = {nullptr, nullptr};   // per parity of `cur`
   int coll_scheme = -1;            // collision phase of pcp_step (-1: off)

@@ -302,6 +302,7 @@ __global__ void k_step_begin(StepState *st) {
   st->step += 1;
   st->need_exact = 0;
   st->n_mig = 0;
+  st->queue = 0;
 }
 
 // ---------------------------------------------------------------------------

@@ -1,4 +1,4 @@
-// Counting sort of one species by (domain, cell), with compaction.
+// Counting sort of one species by (domain, sort bin, cell), with compaction.
 //
 // Cell order makes the 64 particles of a wavefront share their field nodes
 // (the gather becomes broadcast loads) and their current nodes (the
@@ -25,7 +25,7 @@ __global__ void __launch_bounds__(256) k_sort_count(Geom g, SpeciesDev s, int cu
         long cr = (long)floor(r / g.dr), cz = (long)floor(z / g.dz);
         int d = (int)((cr / g.nr) * g.Dz + cz / g.nz);
         long lr = cr - (long)(d / g.Dz) * g.nr, lz = cz - (long)(d % g.Dz) * g.nz;
-        k = (uint32_t)((long)d * g.nr * g.nz + lr * g.nz + lz);
+        k = sort_cell_key(g, d, lr, lz);
       }
       key[p] = k;
     }

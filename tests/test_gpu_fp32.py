@@ -182,3 +182,61 @@ def test_fp32_vb_sub_resolution_radial_step():
     e = Hh.relerr(interior(gpu.get_grid("J")), interior(orc.get_grid("J")))
     print("J", e)
     assert e <= 1e-5
+
+
+@pytest.mark.parametrize("shape", [(256, 0, 0), (128, 4, 8), (512, 2, 16)])
+def test_fp32_batched_uneven_domains(shape):
+    """The batched-domain shape at small scale: 8 x 8 non-square domains
+    (8 x 32 cells each), density ramping 0.1 -> 1 along z so that the
+    domains hold about 10x different particle counts, Vay pusher, VB, FP32,
+    with the persistent launch at several workgroup sizes and sort bins,
+    against the FP64 oracle after 3 steps."""
+    block, br, bz = shape
+    capi = _cap()
+    size, grid = Hh.geometry(64, 256)
+    domains = (8, 8)
+    dt = 0.5 * (size[1] / grid[1]) / 3e8
+    species = Hh.species_list(True)
+    n = 24000
+    orc = O.SMB(size, grid, domains, dt, species, pusher="vay", ccs="vb")
+    gpu = capi.Context(size, grid, domains, dt, [s + (2 * n,) for s in species], pusher="vay", ccs="vb",
+                       precision="fp32")
+    gpu.set_launch(block)
+    if br:
+        gpu.set_sort_bin(br, bz)
+    rng = np.random.default_rng(31)
+    dr, dz = size[0] / grid[0], size[1] / grid[1]
+    for sp, (q, m, b) in enumerate(species):
+        r, z, vr, vp, vz, w = Hh.make_particles(rng, size, grid, n, m, fast_fraction=0.2, special=False,
+                                                temp_ev=1.0 if m == 1 else 0.1)
+        # linear ramp 0.1 -> 1 along z by inverse transform sampling
+        u = rng.uniform(0, 1, n)
+        x = (-0.1 + np.sqrt(0.01 + 4 * 0.45 * 0.55 * u)) / 0.9
+        z = np.clip(x * size[1], 0.5 * dz, size[1] - 0.5 * dz)
+        ir, iz = np.floor(r / dr), np.floor(z / dz)
+        r = (ir + (r / dr - ir).astype(np.float32).astype(np.float64)) * dr
+        z = (iz + (z / dz - iz).astype(np.float32).astype(np.float64)) * dz
+        vr, vp, vz, w = (a.astype(np.float32).astype(np.float64) for a in (vr, vp, vz, w))
+        ids = np.arange(n, dtype=np.uint64) + np.uint64(sp * 10 ** 6)
+        orc.add_particles(sp, r, z, vr, vp, vz, w, ids)
+        gpu.upload(sp, r, z, vr, vp, vz, w, ids)
+        if sp == 0:
+            dz_dom = size[1] / domains[1]
+            counts = np.bincount((z // dz_dom).astype(int), minlength=domains[1])
+            print("particles per domain column", counts, "ratio", counts.max() / counts.min())
+            assert counts.max() / counts.min() > 5
+    for sp in range(len(species)):
+        gpu.sort(sp)
+    nsteps = 3
+    orc.step(nsteps)
+    gpu.step(nsteps)
+    for sp in range(len(species)):
+        a, b = by_id(orc.particles(sp)), by_id(gpu.particles(sp))
+        assert np.array_equal(a["id"], b["id"])
+        for k in ("r", "z", "vr", "vphi", "vz"):
+            e = Hh.relerr(b[k], a[k])
+            assert e <= 1e-5, (sp, k, e)
+    for nm in ("E", "H", "J"):
+        e = Hh.relerr(interior(gpu.get_grid(nm)), interior(orc.get_grid(nm)))
+        print(shape, nm, e)
+        assert e <= 1e-4, (nm, e)
