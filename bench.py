@@ -1,12 +1,17 @@
 """Benchmark of the per-step particle loop on one MI355X.
 
-Workload (default): the beam-plasma wakefield case -- electron-proton plasma
+Workloads: ``wakefield`` (default) is the beam-plasma wakefield case -- electron-proton plasma
 n = 1e17 m^-3 on a 512 (r) x 4096 (z) grid, dr = dz = 1 mm, 64 macro-particles
 per cell per species (268,435,456 plasma particles), Te = 1 eV, Ti = 0.1 eV,
 plus a Gaussian electron bunch (gamma = 5, sigma_z = 5 mm, sigma_r = 2 mm,
 peak density 0.1 n, 1,048,576 macro-particles), ZigZag deposition, Boris
 pusher, dt = 0.5 dz / c, conducting walls.  Inputs are generated on the
 device from a fixed seed and are resident in HBM before timing starts.
+``batched`` is 8 x 8 uneven domains (1024 x 4096 cells, domains of 128 x 512,
+density ramping 0.1 n -> n along z, 32 per cell at the peak, about 1.5e8
+particles, Vay, Villasenor-Buneman); ``uniform`` is the 128 x 512 plasma with
+16 per cell.  The FP32 leg autotunes the sort bin and launch shape before
+its warm-up (``--no-tune`` keeps the defaults).
 
 One step = Yee E + stitch, Yee H + stitch, push + move + deposit of every
 particle, migration keys, J stitch.
@@ -32,29 +37,38 @@ def workload(name):
                     name="beam-plasma wakefield 512x4096, 64 ppc/species + Gaussian bunch (gamma 5)",
                     grid=(512, 4096), domains=(1, 1), ppc=64, density=1e17, ccs="zigzag", pusher="boris-adaptive",
                     bunch=dict(n=1 << 20, n_peak=1e16, sigma_r=2e-3, sigma_z=5e-3, z0=0.1, gamma=5.0))
+    if name == "batched":
+        return dict(key="batched",
+                    name="batched uneven domains 1024x4096, 8x8 domains of 128x512, ramp 0.1n->n, 32 ppc at peak, Vay",
+                    grid=(1024, 4096), domains=(8, 8), ppc=32, density=1e17, density_left=1e16, ccs="vb",
+                    pusher="vay", bunch=None)
     if name == "uniform":
         return dict(key="uniform", name="uniform e-p plasma 128x512, 16 ppc/species, VB", grid=(128, 512),
                     domains=(1, 1), ppc=16, density=1e17, ccs="vb", pusher="boris-adaptive", bunch=None)
     raise SystemExit("unknown workload " + name)
 
 
-def run_gpu(w, steps, warmup, prec):
+def run_gpu(w, steps, warmup, prec, tune):
     from picopic_amd import capi
     dr = dz = 1e-3
     grid = w["grid"]
     size = (grid[0] * dr, grid[1] * dz)
     dt = 0.5 * dz / C_LIGHT
     n_plasma = w["ppc"] * grid[0] * grid[1]
+    n_left = w.get("density_left", w["density"])
     species = [(-1, 1, False, n_plasma + n_plasma // 50), (1, 1836, False, n_plasma + n_plasma // 50)]
     if w["bunch"]:
         species.append((-1, 1, True, w["bunch"]["n"]))
     ctx = capi.Context(size, grid, w["domains"], dt, species, pusher=w["pusher"], ccs=w["ccs"], precision=prec)
-    ctx.init_plasma(0, w["ppc"], w["density"], w["density"], 1.0, 11)
-    ctx.init_plasma(1, w["ppc"], w["density"], w["density"], 0.1, 12)
+    ctx.init_plasma(0, w["ppc"], n_left, w["density"], 1.0, 11)
+    ctx.init_plasma(1, w["ppc"], n_left, w["density"], 0.1, 12)
     if w["bunch"]:
         b = w["bunch"]
         ctx.init_bunch(2, b["n"], b["n_peak"], b["sigma_r"], b["sigma_z"], b["z0"], b["gamma"], 13)
     n_part = sum(ctx.count(s) for s in range(len(species)))
+    # the autotuner times sort bins and launch shapes on the live state (its
+    # steps come before the warm-up and are not timed)
+    tuning = ctx.autotune(steps=2) if tune else ctx.tuning()
     ctx.step(warmup)
     L = capi.load()
     L.pcp_set_timing(ctx.h, 1)
@@ -68,7 +82,7 @@ def run_gpu(w, steps, warmup, prec):
     copy_gbs = ctx.copy_bandwidth(1 << 30, 20)
     ctx.close()
     return dict(n_particles=n_part, n_particles_end=n_end, wall=wall, phases=phases, cells=grid[0] * grid[1],
-                copy_gbs=copy_gbs, n_species=len(species))
+                copy_gbs=copy_gbs, n_species=len(species), tuning=tuning)
 
 
 def cpu_baseline(w, threads):
@@ -128,7 +142,7 @@ def traffic_from_profile(w, prec, n_particles, n_launches_per_step):
     return per_particle * n_particles / n_launches_per_step
 
 
-KERNEL = {"fp32": "k32_step_fused", "fp64": "k_step_fused"}
+KERNEL = {"fp32": "k32_step_batch", "fp64": "k_step_fused"}
 
 
 def main():
@@ -141,6 +155,7 @@ def main():
                     help="'both' times the FP32 split path (the reported value) and the plain FP64 path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-tune", action="store_true", help="FP32: keep the default sort bin and launch shape")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box in this pool has a single MI355X "
@@ -149,23 +164,24 @@ def main():
     w = workload(args.workload)
     runs = {}
     for prec in (["fp32", "fp64"] if args.prec == "both" else [args.prec]):
-        runs[prec] = run_gpu(w, args.steps, args.warmup, prec)
+        runs[prec] = run_gpu(w, args.steps, args.warmup, prec, tune=(prec == "fp32" and not args.no_tune))
     main_prec = "fp32" if "fp32" in runs else "fp64"
     g = runs[main_prec]
     n = g["n_particles"]
     value = n * args.steps / g["wall"]
     ms_step = g["wall"] * 1e3 / args.steps
-    # the fused push+move+deposit kernel is launched once per species and
-    # step; the "particles" phase is exactly those launches, timed with HIP
-    # events on the context stream
+    # the fused push+move+deposit kernel: one batched launch per step in
+    # FP32, one launch per species and step in FP64; the "particles" phase is
+    # exactly those launches, timed with HIP events on the context stream
     t_part = g["phases"]["particles"]
-    launches = g["n_species"]
+    launches = 1 if main_prec == "fp32" else g["n_species"]
     bytes_pp = 88 if main_prec == "fp64" else 52
     alg_bytes = bytes_pp * n
     achieved = alg_bytes / (t_part * 1e-3) / 1e9 if t_part > 0 else None
     traffic = traffic_from_profile(w, main_prec, n, launches)
     config = {"workload": w["name"], "particles": n, "cells": g["cells"], "precision": main_prec,
-              "deposition": w["ccs"], "pusher": w["pusher"], "phases_ms": g["phases"]}
+              "deposition": w["ccs"], "pusher": w["pusher"], "domains": list(w["domains"]), "phases_ms": g["phases"],
+              "tuning": g["tuning"]}
     if "fp64" in runs and main_prec != "fp64":
         g64 = runs["fp64"]
         v64 = g64["n_particles"] * args.steps / g64["wall"]
