@@ -240,3 +240,38 @@ def test_fp32_batched_uneven_domains(shape):
         e = Hh.relerr(interior(gpu.get_grid(nm)), interior(orc.get_grid(nm)))
         print(shape, nm, e)
         assert e <= 1e-4, (nm, e)
+
+
+@pytest.mark.parametrize("ccs", ["zigzag", "vb"])
+def test_fp32_error_per_component(ccs):
+    """Error of the FP32 split path against the FP64 oracle after 20 steps
+    from zero fields, for each field component (max |difference| over the
+    interior nodes / max |oracle value|) and each velocity moment of each
+    species (weighted mean and mean square of v_r, v_phi, v_z, relative to
+    the oracle's value scale).  The printed table is the per-component
+    error statement of this mode; the bounds are FP32-level."""
+    orc, gpu, species = pair32(grid=(32, 64), ccs=ccs, domains=(2, 2), n=20000, fields=False, fast=0.1,
+                               special=False)
+    nsteps = 20
+    orc.step(nsteps)
+    gpu.step(nsteps)
+    names = {"E": ("E_r", "E_phi", "E_z"), "H": ("H_r", "H_phi", "H_z"), "J": ("J_r", "J_phi", "J_z")}
+    for g, comps in names.items():
+        A, B = interior(orc.get_grid(g)), interior(gpu.get_grid(g))
+        for c, nm in enumerate(comps):
+            a, b = A[:, c], B[:, c]
+            e = np.abs(b - a).max() / max(np.abs(a).max(), 1e-300)
+            print("error %s %s %.3e" % (ccs, nm, e))
+            assert e <= 1e-3, (nm, e)
+    for sp in range(len(species)):
+        a, b = by_id(orc.particles(sp)), by_id(gpu.particles(sp))
+        assert np.array_equal(a["id"], b["id"])
+        wa, wb = a["w"], b["w"]
+        for k in ("vr", "vphi", "vz"):
+            scale = np.sqrt((wa * a[k] ** 2).sum() / wa.sum())
+            m1a, m1b = (wa * a[k]).sum() / wa.sum(), (wb * b[k]).sum() / wb.sum()
+            m2a, m2b = (wa * a[k] ** 2).sum() / wa.sum(), (wb * b[k] ** 2).sum() / wb.sum()
+            e1 = abs(m1b - m1a) / scale
+            e2 = abs(m2b - m2a) / m2a
+            print("error %s species %d <%s> %.3e <%s^2> %.3e" % (ccs, sp, k, e1, k, e2))
+            assert e1 <= 1e-5 and e2 <= 1e-5, (sp, k, e1, e2)
