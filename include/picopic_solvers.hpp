@@ -8,6 +8,7 @@
  *   DeviceCurrent       Current::current_distribution  include/current.hpp:35-54
  *   DeviceMaxwellSolver MaxwellSolver::set_pml, calc_field_h, calc_field_e,
  *                       get_field_h, get_field_e       include/maxwellSolver.hpp:35-317
+ *   DeviceCollisions    Collisions::operator()         include/collisions.hpp:32-78
  *   DeviceBlock         the SMB phases that are not solver virtuals:
  *                       overlays (src/SMB.cpp:566-596, 784-823, 1010-1044),
  *                       the per-particle kinematics of advance_particles
@@ -36,6 +37,7 @@
 #define PICOPIC_SOLVERS_HPP
 
 #include <array>
+#include <cstdint>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -57,7 +59,9 @@ inline void check(int rc, const char *where) {
 enum class Mode { PHASES, WHOLE_STEP };
 
 // the phases of one reference time step, in order (src/PiCoPiC.cpp:299-345)
-enum Phase { FIELD_E = 0, STITCH_E, FIELD_H, STITCH_H, PUSH, KINEMATICS, MIGRATE, DEPOSIT, STITCH_J, N_PHASES };
+// (the collision pass sits between the push and the move, as in
+// SMB::advance_particles)
+enum Phase { FIELD_E = 0, STITCH_E, FIELD_H, STITCH_H, PUSH, COLLIDE, KINEMATICS, MIGRATE, DEPOSIT, STITCH_J, N_PHASES };
 
 class DeviceBlock {
  public:
@@ -102,6 +106,22 @@ class DeviceBlock {
 
   void sync() { check(pcp_sync(ctx_), "pcp_sync"); }
 
+  // the block's collision operator (every domain's Collisions object names
+  // the same one); in WHOLE_STEP mode it becomes the collision phase of
+  // pcp_step, in PHASES mode the COLLIDE phase calls pcp_collide
+  void set_collisions(int scheme, int electrons, int ions, uint64_t seed) {
+    std::lock_guard<std::mutex> lock(mu_);
+    if (coll_on_ && scheme == coll_[0] && electrons == coll_[1] && ions == coll_[2] && seed == coll_seed_) return;
+    if (mode_ == Mode::WHOLE_STEP)
+      check(pcp_set_collisions(ctx_, scheme, electrons, ions, seed), "pcp_set_collisions");
+    coll_on_ = true;
+    coll_[0] = scheme;
+    coll_[1] = electrons;
+    coll_[2] = ions;
+    coll_seed_ = seed;
+    coll_pass_ = 0;
+  }
+
  private:
   void once(Phase ph) {
     std::lock_guard<std::mutex> lock(mu_);
@@ -121,6 +141,10 @@ class DeviceBlock {
       case FIELD_H: check(pcp_yee_h(ctx_), "pcp_yee_h"); break;
       case STITCH_H: check(pcp_stitch(ctx_, PCP_GRID_H), "pcp_stitch H"); break;
       case PUSH: check(pcp_push(ctx_), "pcp_push"); break;
+      case COLLIDE:
+        if (!coll_on_) throw Error(PCP_E_STATE, "collision phase without a collision operator");
+        check(pcp_collide(ctx_, coll_[0], coll_[1], coll_[2], coll_seed_, coll_pass_++), "pcp_collide");
+        break;
       case KINEMATICS: check(pcp_move(ctx_), "pcp_move"); break;
       case MIGRATE: check(pcp_migrate(ctx_), "pcp_migrate"); break;
       case DEPOSIT: check(pcp_deposit(ctx_), "pcp_deposit"); break;
@@ -133,6 +157,9 @@ class DeviceBlock {
   }
 
   pcp_ctx *ctx_ = nullptr;
+  bool coll_on_ = false;
+  int coll_[3] = {0, 0, 0};
+  uint64_t coll_seed_ = 0, coll_pass_ = 0;
   int domains_ = 0;
   Mode mode_;
   long steps_ = 0;
@@ -183,6 +210,26 @@ class DeviceMaxwellSolver {
     std::array<double, 3> out{};
     check(pcp_get_field_h(b_.ctx(), d_, radius, longitude, out.data()), "pcp_get_field_h");
     return out;
+  }
+
+ private:
+  DeviceBlock &b_;
+  int d_;
+};
+
+// Collisions (include/collisions.hpp:32-78; CollisionsP12 / TA77S / SK98
+// operator(), src/collisions/*.cpp): one object per domain as the reference
+// builds them; the pass runs over all domains of the block when the last
+// domain arrives.  scheme: PCP_COLL_P12, PCP_COLL_TA77S, PCP_COLL_SK98 or
+// PCP_COLL_SURROGATE; electrons / ions: species indices.
+class DeviceCollisions {
+ public:
+  DeviceCollisions(DeviceBlock &b, int domain, int scheme, int electrons, int ions, uint64_t seed)
+      : b_(b), d_(domain) {
+    b_.set_collisions(scheme, electrons, ions, seed);
+  }
+  void operator()() {
+    if (b_.mode() == Mode::PHASES) b_.arrive(COLLIDE, d_);
   }
 
  private:

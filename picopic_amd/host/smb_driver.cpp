@@ -4,11 +4,13 @@
 // phase boundary (overlay, runaway collector).  The solvers are the
 // device-backed drop-ins of picopic_solvers.hpp.
 //
-// usage: smb_driver [steps]
+// usage: smb_driver [steps] [p12|ta77s|sk98]
 // Runs the same seeded two-species input twice on the device -- once with
 // every reference phase mapped to its entry point (PHASES) and once with one
 // fused step per time step (WHOLE_STEP) -- and prints one JSON line with the
-// largest relative differences between the two end states.
+// largest relative differences between the two end states.  With a
+// collision scheme, every domain holds a Collisions object and the pass
+// runs between the push and the move.
 #include <omp.h>
 
 #include <algorithm>
@@ -17,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "picopic_solvers.hpp"
@@ -31,12 +34,19 @@ struct Domain {
   picopic_hip::DevicePusher pusher;
   picopic_hip::DeviceCurrent current;
   picopic_hip::DeviceMaxwellSolver maxwell_solver;
+  std::unique_ptr<picopic_hip::DeviceCollisions> collisions;
   DeviceBlock &block;
   int index;
-  Domain(DeviceBlock &b, int d) : pusher(b, d), current(b, d), maxwell_solver(b, d), block(b), index(d) {}
+  Domain(DeviceBlock &b, int d, int scheme)
+      : pusher(b, d), current(b, d), maxwell_solver(b, d), block(b), index(d) {
+    if (scheme >= 0) collisions.reset(new picopic_hip::DeviceCollisions(b, d, scheme, 0, 1, 77));
+  }
   void weight_field_e() { maxwell_solver.calc_field_e(); }
   void weight_field_h() { maxwell_solver.calc_field_h(); }
   void push_particles() { pusher(); }
+  void collide() {
+    if (collisions) (*collisions)();
+  }
   void advance_kinematics() { block.kinematics(index); }
   void weight_current() { current.current_distribution(); }
 };
@@ -45,9 +55,9 @@ struct SMB {
   DeviceBlock block;
   int r_domains, z_domains;
   std::vector<std::unique_ptr<Domain>> domains;
-  SMB(const pcp_config &cfg, picopic_hip::Mode mode)
+  SMB(const pcp_config &cfg, picopic_hip::Mode mode, int scheme)
       : block(cfg, mode), r_domains((int)cfg.domains_r), z_domains((int)cfg.domains_z) {
-    for (int d = 0; d < r_domains * z_domains; ++d) domains.emplace_back(new Domain(block, d));
+    for (int d = 0; d < r_domains * z_domains; ++d) domains.emplace_back(new Domain(block, d, scheme));
   }
   Domain &at(int i, int j) { return *domains[(size_t)(i * z_domains + j)]; }
   void solve_maxvell() {
@@ -65,6 +75,7 @@ struct SMB {
     for (int i = 0; i < r_domains; ++i)
       for (int j = 0; j < z_domains; ++j) {
         at(i, j).push_particles();
+        at(i, j).collide();
         at(i, j).advance_kinematics();
       }
     block.particles_runaway_collector();
@@ -188,11 +199,20 @@ double col_err(const std::vector<double> &a, const std::vector<double> &b) {
 
 int main(int argc, char **argv) {
   const int steps = argc > 1 ? std::atoi(argv[1]) : 5;
+  int scheme = -1;
+  if (argc > 2) {
+    const std::string s = argv[2];
+    scheme = s == "p12" ? PCP_COLL_P12 : s == "ta77s" ? PCP_COLL_TA77S : s == "sk98" ? PCP_COLL_SK98 : -2;
+    if (scheme == -2) {
+      std::fprintf(stderr, "smb_driver: unknown collision scheme %s\n", argv[2]);
+      return 2;
+    }
+  }
   const pcp_config cfg = make_config();
   const long n = 8000;
   try {
-    SMB phases(cfg, picopic_hip::Mode::PHASES);
-    SMB fused(cfg, picopic_hip::Mode::WHOLE_STEP);
+    SMB phases(cfg, picopic_hip::Mode::PHASES, scheme);
+    SMB fused(cfg, picopic_hip::Mode::WHOLE_STEP, scheme);
     upload(phases.block, cfg, n);
     upload(fused.block, cfg, n);
     for (int t = 0; t < steps; ++t) {
@@ -206,9 +226,9 @@ int main(int argc, char **argv) {
     double part = 0;
     for (int sp = 0; sp < cfg.n_species; ++sp)
       for (int k = 0; k < 5; ++k) part = std::max(part, col_err(a.cols[sp][(size_t)k], b.cols[sp][(size_t)k]));
-    std::printf("{\"steps\": %d, \"threads\": %d, \"launched_phases\": %ld, \"launched_fused\": %ld, "
+    std::printf("{\"steps\": %d, \"collisions\": %d, \"threads\": %d, \"launched_phases\": %ld, \"launched_fused\": %ld, "
                 "\"particles\": %.3e, \"E\": %.3e, \"H\": %.3e, \"J\": %.3e}\n",
-                steps, omp_get_max_threads(), phases.block.steps_launched(), fused.block.steps_launched(), part,
+                steps, scheme, omp_get_max_threads(), phases.block.steps_launched(), fused.block.steps_launched(), part,
                 grid_err(a.grids[0], b.grids[0], cfg), grid_err(a.grids[1], b.grids[1], cfg),
                 grid_err(a.grids[2], b.grids[2], cfg));
   } catch (const picopic_hip::Error &e) {
