@@ -68,7 +68,16 @@ def run_gpu(w, steps, warmup, prec, tune):
     n_part = sum(ctx.count(s) for s in range(len(species)))
     # the autotuner times sort bins and launch shapes on the live state (its
     # steps come before the warm-up and are not timed)
-    tuning = ctx.autotune(steps=2) if tune else ctx.tuning()
+    if isinstance(tune, dict):
+        ctx.set_sort_bin(tune["bin_r"], tune["bin_z"])
+        if prec == "fp32":
+            ctx.set_launch(tune["block"], tune["blocks_per_cu"])
+        for s in range(len(species)):
+            if not species[s][2]:
+                ctx.set_sort_interval(s, tune["sort_every"])
+        tuning = ctx.tuning()
+    else:
+        tuning = ctx.autotune(steps=2) if tune else ctx.tuning()
     ctx.step(warmup)
     L = capi.load()
     L.pcp_set_timing(ctx.h, 1)
@@ -156,6 +165,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-tune", action="store_true", help="FP32: keep the default sort bin and launch shape")
+    ap.add_argument("--tuning", default=None,
+                    help="FP32: bin_r,bin_z,block,blocks_per_cu,sort_every instead of autotuning (e.g. to profile "
+                         "the configuration an earlier run chose)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box in this pool has a single MI355X "
@@ -164,7 +176,11 @@ def main():
     w = workload(args.workload)
     runs = {}
     for prec in (["fp32", "fp64"] if args.prec == "both" else [args.prec]):
-        runs[prec] = run_gpu(w, args.steps, args.warmup, prec, tune=(prec == "fp32" and not args.no_tune))
+        tune = prec == "fp32" and not args.no_tune
+        if prec == "fp32" and args.tuning:
+            v = [int(x) for x in args.tuning.split(",")]
+            tune = dict(zip(("bin_r", "bin_z", "block", "blocks_per_cu", "sort_every"), v))
+        runs[prec] = run_gpu(w, args.steps, args.warmup, prec, tune=tune)
     main_prec = "fp32" if "fp32" in runs else "fp64"
     g = runs[main_prec]
     n = g["n_particles"]

@@ -33,6 +33,13 @@ namespace pcp {
 // Axial weights are 1 - e_z and e_z for the node below and above.
 // Near the axis (i <= 1) the radial factors are evaluated in FP64.
 // ---------------------------------------------------------------------------
+// FP32-mode arithmetic uses the hardware reciprocal, reciprocal square
+// root and square root (v_rcp_f32, v_rsq_f32, v_sqrt_f32; each within one
+// ulp), where the correctly rounded forms cost 8-12 instructions each.
+__device__ __forceinline__ float rcp_f(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float rsqrt_f(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float sqrt_f(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 struct W2 {
   int i;
   float a1, a2;
@@ -49,8 +56,8 @@ __device__ __forceinline__ W2 rad_half(int cr, float fr) {
     w.a2 = (float)(ed * (2 * id + 2 + ed) / (2 * id + 4));
   } else {
     float fi = (float)w.i;
-    w.a1 = (1.f - e) * (2.f * fi + 1.f + e) / (2.f * fi + 2.f);
-    w.a2 = e * (2.f * fi + 2.f + e) / (2.f * fi + 4.f);
+    w.a1 = (1.f - e) * (2.f * fi + 1.f + e) * rcp_f(2.f * fi + 2.f);
+    w.a2 = e * (2.f * fi + 2.f + e) * rcp_f(2.f * fi + 4.f);
   }
   return w;
 }
@@ -65,8 +72,8 @@ __device__ __forceinline__ W2 rad_int(int cr, float fr) {
     w.a2 = (float)(ed * (2 * id + 1 + ed) / (2 * id + 3));
   } else {
     float fi = (float)w.i;
-    w.a1 = (1.f - e) * (2.f * fi + e) / (2.f * fi + 1.f);
-    w.a2 = e * (2.f * fi + 1.f + e) / (2.f * fi + 3.f);
+    w.a1 = (1.f - e) * (2.f * fi + e) * rcp_f(2.f * fi + 1.f);
+    w.a2 = e * (2.f * fi + 1.f + e) * rcp_f(2.f * fi + 3.f);
   }
   return w;
 }
@@ -165,32 +172,32 @@ __device__ __forceinline__ void push32(int pusher, const PushConst &K, float e[3
     if (rel) {
       float beta = len2f(v) * K.inv_c2;
       if (beta > 1.f) *err |= ERR_BETA_GT_1;
-      float gamma = 1.f / sqrtf(1.f - beta);
+      float gamma = rsqrt_f(1.f - beta);
       for (int i = 0; i < 3; ++i) v[i] *= gamma;
     }
     for (int i = 0; i < 3; ++i) v[i] += e[i];
     if (rel) {
-      float gi = 1.f / sqrtf(1.f + len2f(v) * K.inv_c2);
+      float gi = rsqrt_f(1.f + len2f(v) * K.inv_c2);
       for (int i = 0; i < 3; ++i) b[i] *= gi;
     }
-    float const2 = 2.f / (1.f + len2f(b));
+    float const2 = 2.f * rcp_f(1.f + len2f(b));
     float t0 = v[0], t1 = v[1], t2 = v[2];
     v[0] = t0 + const2 * ((t1 - t0 * b[2] + t2 * b[0]) * b[2] - (t2 + t0 * b[1] - t1 * b[0]) * b[1]);
     v[1] = t1 + const2 * (-(t0 + t1 * b[2] - t2 * b[1]) * b[2] + (t2 + t0 * b[1] - t1 * b[0]) * b[0]);
     v[2] = t2 + const2 * ((t0 + t1 * b[2] - t2 * b[1]) * b[1] - (t1 - t0 * b[2] + t2 * b[0]) * b[0]);
     for (int i = 0; i < 3; ++i) v[i] += e[i];
     if (rel) {
-      float gi = 1.f / sqrtf(1.f + len2f(v) * K.inv_c2);
+      float gi = rsqrt_f(1.f + len2f(v) * K.inv_c2);
       for (int i = 0; i < 3; ++i) v[i] *= gi;
     }
   } else if (pusher == PCP_PUSHER_VAY) {
     float beta = len2f(v) * K.inv_c2;
     if (beta > 1.f) *err |= ERR_BETA_GT_1;
-    float gamma = 1.f / sqrtf(1.f - beta);
+    float gamma = rsqrt_f(1.f - beta);
     for (int i = 0; i < 3; ++i) v[i] *= gamma;
     float up[3];
     for (int i = 0; i < 3; ++i) { up[i] = v[i] + e[i] * (2.f * K.c2m); b[i] *= K.c2m_mu; }
-    float gi = 1.f / sqrtf(1.f + len2f(v) * K.inv_c2);
+    float gi = rsqrt_f(1.f + len2f(v) * K.inv_c2);
     up[0] += gi * (v[1] * b[2] - v[2] * b[1]);
     up[1] += gi * (v[2] * b[0] - v[0] * b[2]);
     up[2] += gi * (v[0] * b[1] - v[1] * b[0]);
@@ -198,20 +205,20 @@ __device__ __forceinline__ void push32(int pusher, const PushConst &K, float e[3
     float B2 = len2f(b);
     float s = alpha - B2;
     float ub = up[0] * b[0] + up[1] * b[1] + up[2] * b[2];
-    alpha = 1.f / sqrtf(0.5f * (s + sqrtf(s * s + 4.f * (B2 + ub * ub))));
+    alpha = rsqrt_f(0.5f * (s + sqrt_f(s * s + 4.f * (B2 + ub * ub))));
     for (int i = 0; i < 3; ++i) b[i] *= alpha;
-    s = 1.f / (1.f + len2f(b));
+    s = rcp_f(1.f + len2f(b));
     alpha = up[0] * b[0] + up[1] * b[1] + up[2] * b[2];
     float psm[3];
     psm[0] = s * (up[0] + alpha * b[0] + b[2] * up[1] - b[1] * up[2]);
     psm[1] = s * (up[1] + alpha * b[1] + b[0] * up[2] - b[2] * up[0]);
     psm[2] = s * (up[2] + alpha * b[2] + b[1] * up[0] - b[0] * up[1]);
-    gi = 1.f / sqrtf(1.f + len2f(psm) * K.inv_c2);
+    gi = rsqrt_f(1.f + len2f(psm) * K.inv_c2);
     for (int i = 0; i < 3; ++i) v[i] = psm[i] * gi;
   } else {
     float beta = len2f(v) * K.inv_c2;
     if (beta > 1.f) *err |= ERR_BETA_GT_1;
-    float gamma = 1.f / sqrtf(1.f - beta);
+    float gamma = rsqrt_f(1.f - beta);
     float um[3], psm[3], b2[3], bc[3], up[3];
     for (int i = 0; i < 3; ++i) {
       v[i] *= gamma;
@@ -223,15 +230,15 @@ __device__ __forceinline__ void push32(int pusher, const PushConst &K, float e[3
     float B2 = len2f(b);
     float bu = b[0] * um[0] + b[1] * um[1] + b[2] * um[2];
     float gb = gfm2 - B2;
-    gamma = 1.f / sqrtf(0.5f * (gb + sqrtf(gb * gb + 4.f * (B2 + bu * bu))));
+    gamma = rsqrt_f(0.5f * (gb + sqrt_f(gb * gb + 4.f * (B2 + bu * bu))));
     for (int i = 0; i < 3; ++i) { b[i] *= gamma; b2[i] = b[i] * b[i]; }
     bc[0] = b[0] * b[1]; bc[1] = b[1] * b[2]; bc[2] = b[2] * b[0];
-    float inv_det_B = 1.f / (1.f + b2[0] + b2[1] + b2[0]);   // b2[0] twice (pusherHC.cpp:94)
+    float inv_det_B = rcp_f(1.f + b2[0] + b2[1] + b2[0]);   // b2[0] twice (pusherHC.cpp:94)
     up[0] = ((1.f + b2[0] - b2[1] - b2[2]) * um[0] + 2.f * (bc[0] + b[2]) * um[1] + 2.f * (bc[2] - b[2]) * um[2]) * inv_det_B;
     up[1] = (2.f * (bc[0] - b[2]) * um[0] + (1.f - b2[0] + b2[1] - b2[2]) * um[1] + 2.f * (bc[1] + b[0]) * um[2]) * inv_det_B;
     up[2] = (2.f * (bc[2] + b[1]) * um[0] + 2.f * (bc[1] - b[0]) * um[1] + (1.f - b2[0] - b2[1] + b2[2]) * um[2]) * inv_det_B;
     for (int i = 0; i < 3; ++i) psm[i] += up[i];
-    float gi = 1.f / sqrtf(1.f + len2f(psm) * K.inv_c2);
+    float gi = rsqrt_f(1.f + len2f(psm) * K.inv_c2);
     for (int i = 0; i < 3; ++i) v[i] = psm[i] * gi;
   }
 }
@@ -270,16 +277,16 @@ __device__ __forceinline__ bool move32(const Geom &g, const Dom &o, int beam, fl
   float sn = 0.f;
   double ur = (double)p.cr + (double)p.fr + (double)du;   // unrounded radius, cell units
   if (Y != 0.f) {
-    float R = sqrtf(X * X + Y * Y);
+    float R = sqrt_f(X * X + Y * Y);
     if (X > 0.f) {
-      float inc = Y * Y / (X + R);
+      float inc = Y * Y * rcp_f(X + R);
       xoff += inc;
       ur += (double)inc;
     } else {
       xoff = R - (float)p.cr;
       ur = (double)R;
     }
-    sn = Y / R;
+    sn = Y * rcp_f(R);
   } else if (X < 0.f) {
     xoff = -X - (float)p.cr;   // |X|: the cylinder radius of a particle that crossed the axis
     ur = -ur;
@@ -308,7 +315,7 @@ __device__ __forceinline__ bool move32(const Geom &g, const Dom &o, int beam, fl
   renorm(p.cr, p.fr);
   renorm(p.cz, p.fz);
   // back_velocity_to_rz
-  float cs = sqrtf(1.f - sn * sn);
+  float cs = sqrt_f(1.f - sn * sn);
   float vr = v[0], vp = v[1];
   v[0] = cs * vr - sn * vp;
   v[1] = sn * vr + cs * vp;
@@ -327,30 +334,6 @@ struct WaveSink32 {
     bool in = i >= -2 && i < nr + 2 && k >= -2 && k < nz + 2;
     if (a && !in) *err |= ERR_MOVED_TOO_FAR;
     wave_add(J, base + c * P + (i + 2) * yr + (k + 2), v, a && in);
-  }
-};
-
-// Per-block LDS tile of the current (the nodes of the block's bounding box in
-// one domain), accumulated in FP64 with LDS atomics and added to J once per
-// node and block.  A contribution outside the box goes straight to J.
-constexpr int TILE_NODES = 640;   // nodes per component
-
-struct TileSink32 {
-  double *t;
-  int i0, k0, th, tw;
-  float *Jd;   // J of the tile's domain
-  int P, yr, nr, nz;
-  int *err;
-  __device__ __forceinline__ void add_if(int c, int i, int k, double v, bool a) {
-    if (!a) return;
-    const int li = i - i0, lk = k - k0;
-    if (li >= 0 && li < th && lk >= 0 && lk < tw) {
-      atomicAdd(&t[(c * th + li) * tw + lk], v);
-    } else if (i >= -2 && i < nr + 2 && k >= -2 && k < nz + 2) {
-      unsafeAtomicAdd(&Jd[c * P + (i + 2) * yr + (k + 2)], (float)v);
-    } else {
-      *err |= ERR_MOVED_TOO_FAR;
-    }
   }
 };
 
@@ -412,11 +395,10 @@ __device__ __forceinline__ void zigzag32(Sink &S, bool live, const Geom &g, cons
   const float rr = fminf(fminf(0.f, (float)di) + 1.f, fmaxf(fmaxf(0.f, (float)di), 0.5f * (uo + un)));
   const float rz = fminf(fminf(0.f, (float)dk) + 1.f, fmaxf(fmaxf(0.f, (float)dk), 0.5f * (wo + wn)));
   const int i_middle = o.cr + (int)floorf(0.5f * (uo + un));
-  const float one_over_volume =
-      (float)(1.0 / (PI * g.dz * g.dr * g.dr * (2.0 * i_middle + 1)));
+  const float one_over_volume = (float)g.inv_vol0 * rcp_f((float)(2 * i_middle + 1));
   const float charge_over_dt = qw * inv_dt;
   const float F_phi = qw * vphi;
-  const float W_phi = vphi * (float)g.dt / (2.f * (float)PI * dr * ((float)(2 * o.cr) + uo + un));
+  const float W_phi = vphi * (float)g.dt * rcp_f(2.f * (float)PI * dr * ((float)(2 * o.cr) + uo + un));
   const float F_r1 = charge_over_dt * dr * (rr - uo);
   const float F_r2 = charge_over_dt * dr * (un - rr);
   const float F_z1 = charge_over_dt * dz * (rz - wo);
@@ -444,12 +426,15 @@ __device__ __forceinline__ void zigzag32(Sink &S, bool live, const Geom &g, cons
   S.add_if(2, in + 1, kn, A * F_z2 * W_r2 * mWphi + A * F_z2 * W_r2 * W_phi, live);
 }
 
-__device__ __forceinline__ int domain_of32(const Geom &g, int cr, int cz) { return (cr / g.nr) * g.Dz + (cz / g.nz); }
+// (cr, cz >= 0: cells of live particles)
+__device__ __forceinline__ int domain_of32(const Geom &g, int cr, int cz) {
+  return (int)g.div_nr.div((uint32_t)cr) * g.Dz + (int)g.div_nz.div((uint32_t)cz);
+}
 
-__device__ __forceinline__ Split unpack(uint32_t cell, float fr, float fz, long Nz) {
+__device__ __forceinline__ Split unpack(uint32_t cell, float fr, float fz, const Geom &g) {
   Split s;
-  s.cr = (int)(cell / (uint32_t)Nz);
-  s.cz = (int)(cell - (uint32_t)s.cr * (uint32_t)Nz);
+  s.cr = (int)g.div_Nz.div(cell);
+  s.cz = (int)(cell - (uint32_t)s.cr * (uint32_t)g.Nz);
   s.fr = fr;
   s.fz = fz;
   return s;
@@ -461,7 +446,7 @@ __device__ __forceinline__ void append_migrant32(Migrants mg, StepState *st, int
                                                  uint64_t seq, const Geom &g, int *err) {
   unsigned int slot = atomicAdd(&st->n_mig, 1u);
   if (slot >= (unsigned long)mg.cap) { *err |= ERR_MIGRANT_OVERFLOW; return; }
-  int I = d_old / g.Dz, J = d_old % g.Dz;
+  const int I = (int)g.div_Dz.div((uint32_t)d_old), J = d_old - I * g.Dz;
   uint64_t colour = (uint64_t)((I % 2) * 2 + (J % 2));
   mg.key[slot] = (colour << 47) | ((uint64_t)d_old << 35) | (seq & ((1ull << 35) - 1));
   mg.ref[slot] = ((uint64_t)sp << 40) | (uint64_t)p;
@@ -480,7 +465,7 @@ __device__ __forceinline__ bool advance32(const Geom &g, const FieldsT<float> &f
                                           uint32_t cell, int from, int vfrom, bool do_push, const PushConst &K,
                                           float dtr, float dtz, Split &o, Split &n, float v[3], double &rd,
                                           double &zd, int &d_old, int *err) {
-  o = unpack(cell, s.fr[from][p], s.fz[from][p], g.Nz);
+  o = unpack(cell, s.fr[from][p], s.fz[from][p], g);
   d_old = domain_of32(g, o.cr, o.cz);
   const Dom dm = make_dom(g, d_old);
   v[0] = s.vr[vfrom][p]; v[1] = s.vphi[vfrom][p]; v[2] = s.vz[vfrom][p];
@@ -505,7 +490,7 @@ __global__ void __launch_bounds__(256) k32_push(Geom g, FieldsT<float> f, Specie
   for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < s.n; p += (long)gridDim.x * blockDim.x) {
     uint32_t cell = s.cell[cur][p];
     if (cell == DEAD_CELL) continue;
-    Split o = unpack(cell, s.fr[cur][p], s.fz[cur][p], g.Nz);
+    Split o = unpack(cell, s.fr[cur][p], s.fz[cur][p], g);
     const int d = domain_of32(g, o.cr, o.cz);
     const Dom dm = make_dom(g, d);
     GridView32 EV{f.E + (long)d * 3 * g.P, g.P, g.yr, dm.bot_r, dm.left_z};
@@ -578,7 +563,7 @@ __device__ __forceinline__ void deposit32(Sink &S, bool live, const Geom &g, con
 template <int CCS, bool PUSH>
 __device__ __forceinline__ void step_trip32(const Geom &g, const FieldsT<float> &f, const Species32 &s, int sp,
                                             const PushConst &K, long b0, int cur, StepState *st, const Migrants &mg,
-                                            double *tile, int *box, int &err) {
+                                            int &err) {
   const int nxt = cur ^ 1;
   const float dtr = (float)(g.dt / g.dr), dtz = (float)(g.dt / g.dz);
   const float inv_dt = (float)(1.0 / g.dt);
@@ -609,57 +594,12 @@ __device__ __forceinline__ void step_trip32(const Geom &g, const FieldsT<float> 
   }
   if (!live) { o = n = Split{1, 1, 0.5f, 0.5f}; d_new = 0; v[1] = 0.f; w = 0.f; rd = 1.5 * g.dr; zd = 1.5 * g.dz; }
   const float qw = (float)s.q * w;
-  // deposit domain and the lane's node box (local indices, one cell of
-  // margin): ZigZag deposits around the old and new split cells, VB around
-  // the cells of its FP64 end points
+  // deposit domain: ZigZag deposits in the particle's new domain, VB in the
+  // domain of its FP64 end point
   const int d_dep = CCS == PCP_CCS_ZIGZAG ? d_new : (live ? deposit_domain32(g, rd, zd) : 0);
   const Dom dn = make_dom(g, d_dep);
-  int ci0, ci1, ck0, ck1;
-  if (CCS == PCP_CCS_ZIGZAG) {
-    ci0 = o.cr; ci1 = n.cr; ck0 = o.cz; ck1 = n.cz;
-  } else {
-    ci0 = cell_number(((double)o.cr + (double)o.fr) * g.dr, g.dr); ci1 = cell_number(rd, g.dr);
-    ck0 = cell_number(((double)o.cz + (double)o.fz) * g.dz, g.dz); ck1 = cell_number(zd, g.dz);
-  }
-  const int bi0 = min(ci0, ci1) - (int)dn.bot_r - 1, bi1 = max(ci0, ci1) - (int)dn.bot_r + 2;
-  const int bk0 = min(ck0, ck1) - (int)dn.left_z - 1, bk1 = max(ck0, ck1) - (int)dn.left_z + 2;
-  if (threadIdx.x == 0) {
-    box[0] = INT_MAX; box[1] = INT_MIN; box[2] = INT_MAX; box[3] = INT_MIN; box[4] = INT_MAX; box[5] = INT_MIN;
-  }
-  __syncthreads();
-  if (live) {
-    atomicMin(&box[0], bi0); atomicMax(&box[1], bi1);
-    atomicMin(&box[2], bk0); atomicMax(&box[3], bk1);
-    atomicMin(&box[4], d_dep); atomicMax(&box[5], d_dep);
-  }
-  __syncthreads();
-  const int ti0 = box[0], ti1 = box[1], tk0 = box[2], tk1 = box[3], td0 = box[4], td1 = box[5];
-  __syncthreads();
-  if (ti0 > ti1) return;   // no live particle in this block trip
-  const int th = ti1 - ti0 + 1, tw = tk1 - tk0 + 1;
-  if (td0 == td1 && th * tw <= TILE_NODES) {
-    const int nodes = th * tw;
-    for (int t = threadIdx.x; t < 3 * nodes; t += blockDim.x) tile[t] = 0.0;
-    __syncthreads();
-    TileSink32 S{tile, ti0, tk0, th, tw, f.J + (long)td0 * 3 * g.P, (int)g.P, g.yr, g.nr, g.nz, &err};
-    deposit32<CCS>(S, live, g, dn, o, n, v[1], qw, inv_dt, rd, zd, s.q * (double)w, f, d_dep, st, &err);
-    __syncthreads();
-    float *Jd = f.J + (long)td0 * 3 * g.P;
-    for (int t = threadIdx.x; t < 3 * nodes; t += blockDim.x) {
-      const double val = tile[t];
-      if (val == 0.0) continue;
-      const int c = t / nodes, rem = t - c * nodes;
-      const int i = ti0 + rem / tw, k = tk0 + rem % tw;
-      if (i >= -2 && i < g.nr + 2 && k >= -2 && k < g.nz + 2)
-        unsafeAtomicAdd(&Jd[c * g.P + (long)(i + 2) * g.yr + (k + 2)], (float)val);
-      else
-        err |= ERR_MOVED_TOO_FAR;
-    }
-    __syncthreads();
-  } else {
-    WaveSink32 S{f.J, (int)((long)d_dep * 3 * g.P), (int)g.P, g.yr, g.nr, g.nz, &err};
-    deposit32<CCS>(S, live, g, dn, o, n, v[1], qw, inv_dt, rd, zd, s.q * (double)w, f, d_dep, st, &err);
-  }
+  WaveSink32 S{f.J, (int)((long)d_dep * 3 * g.P), (int)g.P, g.yr, g.nr, g.nz, &err};
+  deposit32<CCS>(S, live, g, dn, o, n, v[1], qw, inv_dt, rd, zd, s.q * (double)w, f, d_dep, st, &err);
 }
 
 // The fused step of every species in one persistent launch: a block takes
@@ -670,8 +610,6 @@ __device__ __forceinline__ void step_trip32(const Geom &g, const FieldsT<float> 
 template <int CCS, bool PUSH, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k32_step_batch(Geom g, FieldsT<float> f, Batch32 b, int cur, StepState *st,
                                                         Migrants mg) {
-  __shared__ double tile[3 * TILE_NODES];
-  __shared__ int box[6];
   __shared__ unsigned ticket;
   int err = 0;
   const long total = b.end[b.nsp - 1];
@@ -684,7 +622,7 @@ __global__ void __launch_bounds__(BLOCK) k32_step_batch(Geom g, FieldsT<float> f
     int k = 0;
     while (k + 1 < b.nsp && t >= b.end[k]) ++k;
     const long first = k ? b.end[k - 1] : 0;
-    step_trip32<CCS, PUSH>(g, f, b.s[k], b.sp[k], b.K[k], (t - first) * BLOCK, cur, st, mg, tile, box, err);
+    step_trip32<CCS, PUSH>(g, f, b.s[k], b.sp[k], b.K[k], (t - first) * BLOCK, cur, st, mg, err);
   }
   if (err) atomicOr(&st->err, err);
 }

@@ -150,8 +150,8 @@ int pcp_create(const pcp_config *cfg, pcp_ctx **out) {
   if (cfg->ccs != PCP_CCS_VB && cfg->ccs != PCP_CCS_ZIGZAG) return fail(PCP_E_ARG, "unknown ccs");
   if (cfg->precision != PCP_PREC_FP64 && cfg->precision != PCP_PREC_FP32_SPLIT)
     return fail(PCP_E_ARG, "unknown precision");
-  if (cfg->precision == PCP_PREC_FP32_SPLIT && (long)cfg->grid_r * cfg->grid_z >= 0xFFFFFFFFL)
-    return fail(PCP_E_ARG, "FP32 split mode packs the cell index in 32 bits");
+  if ((long)cfg->grid_r * cfg->grid_z >= 0x7FFFFFFFL)
+    return fail(PCP_E_ARG, "at most 2^31 - 1 cells (32-bit cell indices)");
   if (cfg->domains_r * cfg->domains_z > 4096) return fail(PCP_E_ARG, "at most 4096 domains");
   if (!(cfg->dt > 0)) return fail(PCP_E_ARG, "dt must be positive");
 
@@ -183,6 +183,11 @@ int pcp_create(const pcp_config *cfg, pcp_ctx **out) {
   g.pusher = cfg->pusher;
   g.bin_r = 1;
   g.bin_z = g.nz;
+  g.div_nr = make_fastdiv((uint32_t)g.nr);
+  g.div_nz = make_fastdiv((uint32_t)g.nz);
+  g.div_Dz = make_fastdiv((uint32_t)g.Dz);
+  g.div_Nz = make_fastdiv((uint32_t)g.Nz);
+  g.inv_vol0 = 1.0 / (PI * g.dz * g.dr * g.dr);
   c->D = g.Dr * g.Dz;
   c->cur = 0;
 

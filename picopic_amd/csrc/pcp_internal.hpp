@@ -45,6 +45,26 @@ enum : int {
 };
 
 // Geometry shared by every domain of the SMB (domains are equal).
+// Division by an invariant divisor d >= 1 for dividends n < 2^31:
+// q = floor(n m / 2^p) with p = 31 + ceil(log2 d), m = ceil(2^p / d)
+// (m < 2^32).  Exact: n m / 2^p = n / d + n e / 2^p with 0 <= e < 1, and
+// n e / 2^p < 2^-ceil(log2 d) <= 1 / d never carries n / d past an integer.
+struct FastDiv {
+  uint32_t d, m;
+  int p;
+  __host__ __device__ __forceinline__ uint32_t div(uint32_t n) const { return (uint32_t)(((uint64_t)n * m) >> p); }
+};
+
+inline FastDiv make_fastdiv(uint32_t d) {
+  int l = 0;
+  while ((1ull << l) < d) ++l;
+  FastDiv f;
+  f.d = d;
+  f.p = 31 + l;
+  f.m = (uint32_t)(((1ull << f.p) + d - 1) / d);
+  return f;
+}
+
 struct Geom {
   int Dr, Dz;        // domains
   int nr, nz;        // cells per domain
@@ -59,6 +79,8 @@ struct Geom {
   int pml_on;
   int ccs, pusher;
   int bin_r, bin_z;  // sort bin (cells; divide nr and nz)
+  FastDiv div_nr, div_nz, div_Dz, div_Nz;
+  double inv_vol0;   // 1 / (pi dz dr^2): CELL_VOLUME(i) = (2 i + 1) / inv_vol0
 };
 
 // Per-domain values derived from the domain index (src/SMB.cpp:49-121).
@@ -70,8 +92,8 @@ struct Dom {
 
 __host__ __device__ inline Dom make_dom(const Geom &g, int d) {
   Dom o;
-  o.I = d / g.Dz;
-  o.J = d % g.Dz;
+  o.I = (int)g.div_Dz.div((uint32_t)d);
+  o.J = d - o.I * g.Dz;
   o.bot_r = (long)g.nr * o.I;
   o.left_z = (long)g.nz * o.J;
   o.w0 = o.I == 0;

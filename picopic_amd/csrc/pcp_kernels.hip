@@ -72,33 +72,6 @@ struct MarkSink {
 };
 
 // wave-aggregated sink (call sites must be converged across the wavefront)
-// Per-block LDS tile of the current (the nodes of the block's bounding box in
-// one domain), added to J once per node and block; a contribution outside
-// the box goes straight to J.
-constexpr int TILE_NODES64 = 640;   // nodes per component
-
-struct TileSink {
-  double *t;
-  int i0, k0, th, tw;
-  double *Jd;   // J of the tile's domain
-  int P, yr, nr, nz;
-  bool active;
-  int *err;
-  __device__ __forceinline__ void add_if(int c, int i, int k, double v, bool a) {
-    if (!a) return;
-    const int li = i - i0, lk = k - k0;
-    if (li >= 0 && li < th && lk >= 0 && lk < tw) {
-      atomicAdd(&t[(c * th + li) * tw + lk], v);
-    } else if (i >= -2 && i < nr + 2 && k >= -2 && k < nz + 2) {
-      unsafeAtomicAdd(&Jd[c * P + (i + 2) * yr + (k + 2)], v);
-    } else {
-      *err |= ERR_MOVED_TOO_FAR;
-    }
-  }
-  __device__ __forceinline__ void add(int c, int i, int k, double v) { add_if(c, i, k, v, active); }
-  __device__ __forceinline__ void strict_reset(int, int) {}
-};
-
 struct WaveSink {
   double *J;      // f.J, every domain
   int base;       // offset of component 0 of the destination domain
@@ -460,8 +433,6 @@ __device__ __forceinline__ void deposit64(Sink &S, bool live, const Geom &g, con
 template <int CCS, bool PUSH>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CCS == PCP_CCS_ZIGZAG ? 3 : 2)))
 k_step_fused(Geom g, Fields f, SpeciesDev s, int sp, int cur, StepState *st, Migrants mg) {
-  __shared__ double tile[3 * TILE_NODES64];
-  __shared__ int box[6];
   int err = 0;
   const int nxt = cur ^ 1;
   const long stride = (long)gridDim.x * blockDim.x;
@@ -499,48 +470,8 @@ k_step_fused(Geom g, Fields f, SpeciesDev s, int sp, int cur, StepState *st, Mig
     }
     if (!live) { r = ro = g.dr; z = zo = g.dz; d_new = 0; vphi = 0; w = 0; }
     const Dom dn = make_dom(g, d_new);
-    // the lane's node box (local indices of its domain, one cell of margin)
-    const int ci0 = cell_number(ro, g.dr), ci1 = cell_number(r, g.dr);
-    const int ck0 = cell_number(zo, g.dz), ck1 = cell_number(z, g.dz);
-    const int bi0 = min(ci0, ci1) - (int)dn.bot_r - 1, bi1 = max(ci0, ci1) - (int)dn.bot_r + 2;
-    const int bk0 = min(ck0, ck1) - (int)dn.left_z - 1, bk1 = max(ck0, ck1) - (int)dn.left_z + 2;
-    if (threadIdx.x == 0) {
-      box[0] = INT_MAX; box[1] = INT_MIN; box[2] = INT_MAX; box[3] = INT_MIN; box[4] = INT_MAX; box[5] = INT_MIN;
-    }
-    __syncthreads();
-    if (live) {
-      atomicMin(&box[0], bi0); atomicMax(&box[1], bi1);
-      atomicMin(&box[2], bk0); atomicMax(&box[3], bk1);
-      atomicMin(&box[4], d_new); atomicMax(&box[5], d_new);
-    }
-    __syncthreads();
-    const int ti0 = box[0], ti1 = box[1], tk0 = box[2], tk1 = box[3], td0 = box[4], td1 = box[5];
-    __syncthreads();
-    if (ti0 > ti1) continue;   // no live particle in this block trip
-    const int th = ti1 - ti0 + 1, tw = tk1 - tk0 + 1;
-    if (td0 == td1 && th * tw <= TILE_NODES64) {
-      const int nodes = th * tw;
-      for (int t = threadIdx.x; t < 3 * nodes; t += blockDim.x) tile[t] = 0.0;
-      __syncthreads();
-      TileSink S{tile, ti0, tk0, th, tw, f.J + (long)td0 * 3 * g.P, (int)g.P, g.yr, g.nr, g.nz, live, &err};
-      deposit64<CCS>(S, live, g, dn, r, z, ro, zo, vphi, s.q, w, f, d_new, st, &err);
-      __syncthreads();
-      double *Jd = f.J + (long)td0 * 3 * g.P;
-      for (int t = threadIdx.x; t < 3 * nodes; t += blockDim.x) {
-        const double val = tile[t];
-        if (val == 0.0) continue;
-        const int c = t / nodes, rem = t - c * nodes;
-        const int i = ti0 + rem / tw, k = tk0 + rem % tw;
-        if (i >= -2 && i < g.nr + 2 && k >= -2 && k < g.nz + 2)
-          unsafeAtomicAdd(&Jd[c * g.P + (long)(i + 2) * g.yr + (k + 2)], val);
-        else
-          err |= ERR_MOVED_TOO_FAR;
-      }
-      __syncthreads();
-    } else {
-      WaveSink S{f.J, (int)((long)d_new * 3 * g.P), (int)g.P, g.yr, g.nr, g.nz, live, &err};
-      deposit64<CCS>(S, live, g, dn, r, z, ro, zo, vphi, s.q, w, f, d_new, st, &err);
-    }
+    WaveSink S{f.J, (int)((long)d_new * 3 * g.P), (int)g.P, g.yr, g.nr, g.nz, live, &err};
+    deposit64<CCS>(S, live, g, dn, r, z, ro, zo, vphi, s.q, w, f, d_new, st, &err);
   }
   if (err) atomicOr(&st->err, err);
 }

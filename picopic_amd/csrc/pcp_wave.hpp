@@ -12,40 +12,47 @@
 
 namespace pcp {
 
+// DPP lane moves of a 64-bit value (every lane of the wavefront active)
+template <int CTRL>
+__device__ __forceinline__ double dpp64(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ double readlane64(double x, int lane) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over the 64 lanes in a fixed order, returned to every lane: pairs and
+// quads by quad permutes, the 16-lane rows by row rotations (4 and 8), then
+// the four row sums.  All lanes must be active.
 __device__ __forceinline__ double wave_sum(double x) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-  return x;
+  x += dpp64<0xB1>(x);    // quad_perm [1, 0, 3, 2]
+  x += dpp64<0x4E>(x);    // quad_perm [2, 3, 0, 1]
+  x += dpp64<0x124>(x);   // row_ror 4
+  x += dpp64<0x128>(x);   // row_ror 8
+  return (readlane64(x, 0) + readlane64(x, 16)) + (readlane64(x, 32) + readlane64(x, 48));
 }
 
-// FP32 wave sum carrying the rounding error of every pairwise addition
-// (Knuth's TwoSum), so the result is the correctly rounded sum of the 64
-// lanes up to a second-order term
-__device__ __forceinline__ float wave_sum(float x) {
-  float c = 0.f;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    float y = __shfl_xor(x, off, 64);
-    float yc = __shfl_xor(c, off, 64);
-    float s = x + y;
-    float bp = s - x;
-    float err = (x - (s - bp)) + (y - bp);
-    c = c + yc + err;
-    x = s;
-  }
-  return x + c;
-}
-
+// Scatter-add with wave aggregation: lanes with the same target index are
+// summed (in FP64, also for FP32 targets, so the aggregated FP32 current is
+// the correctly rounded sum of up to 64 contributions up to a 2^-53
+// relative term) and one lane per distinct index issues the atomic.
 template <class T>
 __device__ __forceinline__ void wave_add(T *base, int idx, T v, bool active) {
   unsigned long long pending = __ballot(active);
   const int lane = __lane_id();
   while (pending) {
     const int leader = __ffsll((long long)pending) - 1;
-    const int lk = __shfl(idx, leader, 64);
+    const int lk = __builtin_amdgcn_readlane(idx, leader);
     const bool m = active && idx == lk;
-    const T s = wave_sum(m ? v : (T)0);
-    if (lane == leader) unsafeAtomicAdd(base + lk, s);
+    const double s = wave_sum(m ? (double)v : 0.0);
+    if (lane == leader) unsafeAtomicAdd(base + lk, (T)s);
     pending &= ~__ballot(m);
     active = active && !m;
   }
