@@ -1,0 +1,46 @@
+"""A device context built from a JSON configuration in the reference's schema
+runs the same steps as one built by hand with the same parameters."""
+import copy
+import json
+import os
+
+import numpy as np
+import pytest
+
+from picopic_amd import capi, config
+
+pytestmark = pytest.mark.gpu
+
+PATH = os.path.join(os.path.dirname(__file__), "golden", "config_uniform.json")
+
+
+def small():
+    cfg = copy.deepcopy(config.read(PATH))
+    cfg["geometry"]["size"] = {"radius": 0.032, "longitude": 0.064}
+    cfg["geometry"]["grid"] = {"radius": 32, "longitude": 64}
+    cfg["macro_amount"] = 32 * 64 * 2 * 8
+    cfg["beams"] = []
+    return cfg
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_config_context_equals_manual(precision):
+    cfg = small()
+    cfg["device"]["precision"] = precision
+    ctx, sp = config.context(cfg, seed=5)
+    assert config.per_cell(cfg) == 8
+    dt = config.time(cfg)[2]
+    man = capi.Context((0.032, 0.064), (32, 64), (2, 4), dt,
+                       [(-1, 1, False, ctx.cfg.species[0].capacity), (1, 1836, False, ctx.cfg.species[1].capacity)],
+                       precision=precision)
+    man.init_plasma(0, 8, 1e17, 1e17, 1.0, 5)
+    man.init_plasma(1, 8, 1e17, 1e17, 0.1, 6)
+    for c in (ctx, man):
+        c.step(3)
+    for s in range(2):
+        assert ctx.count(s) == man.count(s) == 32 * 64 * 8
+        a, b = ctx.particles(s), man.particles(s)
+        oa, ob = np.argsort(a["id"]), np.argsort(b["id"])
+        for k in ("r", "z", "vr", "vz"):
+            scale = np.abs(a[k]).max()
+            assert np.abs(a[k][oa] - b[k][ob]).max() <= 1e-9 * scale
