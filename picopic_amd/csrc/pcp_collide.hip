@@ -536,6 +536,25 @@ __device__ __forceinline__ bool species_temps(const double *m, double mass, doub
   return isnormal(tpar) && tpar > 0 && isnormal(tperp) && tperp > 0;
 }
 
+// Collision-rate scale of each target (relative changes of Te_par,
+// Te_perp, Ti_par, Ti_perp over one pass): the network learns the relative
+// change divided by dt times the species' collision frequency up to a
+// constant -- electrons n_e / (Te_perp sqrt(Te_par)) (the isotropisation
+// rate of a bi-Maxwellian), ions n_i / Ti^1.5 -- so that its outputs stay
+// of order one across the density and temperature range.
+__device__ __forceinline__ bool surr_rate_scale(const double *me, const double *mi, double m_e, double m_i,
+                                                double vol, double dt, double sc[MLP_OUT]) {
+  double tep, tet, tip, tit, V[3];
+  if (!species_temps(me, m_e, tep, tet, V) || !species_temps(mi, m_i, tip, tit, V)) return false;
+  const double t = dt * 1e10;
+  const double se = (me[1] / vol) * 1e-20 * t / (tet * sqrt(tep));
+  const double ti = (tip + 2.0 * tit) / 3.0;
+  const double si = (mi[1] / vol) * 1e-20 * t / (ti * sqrt(ti));
+  sc[0] = sc[1] = se;
+  sc[2] = sc[3] = si;
+  return isnormal(se) && isnormal(si);
+}
+
 // features: log10 n_e, log10 n_i, log10 Te_par, log10 Te_perp, log10 Ti,
 // log10 dt; normalised by the fitted mean / std when `norm` is given.
 // ok[c] = 1 for a cell the network may handle.
@@ -565,7 +584,7 @@ __global__ void k_surr_features(Geom g, const double *mom, double m_e, double m_
 }
 
 // training rows of the cells that collided: relative changes of Te_par,
-// Te_perp, Ti_par, Ti_perp over the pass
+// Te_perp, Ti_par, Ti_perp over the pass, each over its rate scale
 __global__ void k_surr_targets(Geom g, const double *before, const double *after, double m_e, double m_i,
                                const int *ok, const int *row_bad, const float *X, float *DX, float *DT,
                                unsigned long long *count, long cap) {
@@ -574,14 +593,16 @@ __global__ void k_surr_targets(Geom g, const double *before, const double *after
     if (!ok[c]) continue;
     const CellPos q = cell_pos(g, c);
     if (q.j >= row_bad[(long)q.d * g.nr + q.i]) continue;
-    double b[4], a[4], V[3];
+    double b[4], a[4], V[3], sc[MLP_OUT];
     if (!species_temps(before + c * SST, m_e, b[0], b[1], V) || !species_temps(before + c * SST + 8, m_i, b[2], b[3], V) ||
         !species_temps(after + c * SST, m_e, a[0], a[1], V) || !species_temps(after + c * SST + 8, m_i, a[2], a[3], V))
       continue;
+    const double vol = PCP_CELL_VOLUME((double)q.cr, g.dr, g.dz);
+    if (!surr_rate_scale(before + c * SST, before + c * SST + 8, m_e, m_i, vol, g.dt, sc)) continue;
     unsigned long long slot = atomicAdd(count, 1ull);
     if ((long)slot >= cap) continue;
     for (int k = 0; k < MLP_XSTRIDE; ++k) DX[slot * MLP_XSTRIDE + k] = X[c * MLP_XSTRIDE + k];
-    for (int k = 0; k < MLP_OUT; ++k) DT[slot * MLP_YSTRIDE + k] = (float)((a[k] - b[k]) / b[k]);
+    for (int k = 0; k < MLP_OUT; ++k) DT[slot * MLP_YSTRIDE + k] = (float)((a[k] - b[k]) / b[k] / sc[k]);
   }
 }
 
@@ -612,13 +633,17 @@ __global__ void __launch_bounds__(COLL_BLOCK) k_surr_apply(Geom g, V ve, V vi, c
     stage_species(ve, perm_e, off_e[c], nn[0], 0, S);
     stage_species(vi, perm_i, off_i[c], nn[1], 1, S);
     __syncthreads();
-    double Vs[2][3], apar[2], aperp[2];
+    double Vs[2][3], apar[2], aperp[2], sc[MLP_OUT];
     const double mass[2] = {m_e, m_i};
+    const CellPos q = cell_pos(g, c);
+    // ok[c] guarantees normal moments, hence a normal rate scale
+    (void)surr_rate_scale(mom + c * SST, mom + c * SST + 8, m_e, m_i, PCP_CELL_VOLUME((double)q.cr, g.dr, g.dz), g.dt,
+                          sc);
     for (int s = 0; s < 2; ++s) {
       const double *m = mom + c * SST + 8 * s;
       for (int k = 0; k < 3; ++k) Vs[s][k] = m[2 + k] / m[1];
-      const double ypar = (double)Y[c * MLP_YSTRIDE + 2 * s] * norm.tscale[2 * s];
-      const double yperp = (double)Y[c * MLP_YSTRIDE + 2 * s + 1] * norm.tscale[2 * s + 1];
+      const double ypar = (double)Y[c * MLP_YSTRIDE + 2 * s] * norm.tscale[2 * s] * sc[2 * s];
+      const double yperp = (double)Y[c * MLP_YSTRIDE + 2 * s + 1] * norm.tscale[2 * s + 1] * sc[2 * s + 1];
       apar[s] = sqrt(fmax(1.0 + ypar, SURR_MIN_FACTOR));
       aperp[s] = sqrt(fmax(1.0 + yperp, SURR_MIN_FACTOR));
     }

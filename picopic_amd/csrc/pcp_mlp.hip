@@ -509,6 +509,8 @@ int mlp_fit(pcp_ctx *c, int epochs, long batch, float lr, uint64_t seed, float *
   std::mt19937_64 gen(seed);
   std::vector<float> hl(nb);
   for (int e = 0; e < epochs; ++e) {
+    // cosine annealing from lr to lr / 20 over the passes
+    const float lr_e = lr * (0.05f + 0.95f * 0.5f * (1.f + std::cos(3.14159265f * (float)e / (float)epochs)));
     for (long i = 0; i < n; ++i) perm[i] = i;
     std::shuffle(perm.begin(), perm.end(), gen);
     if (hipMemcpyAsync(m->idx, perm.data(), n * sizeof(long), hipMemcpyHostToDevice, c->stream) != hipSuccess)
@@ -516,7 +518,7 @@ int mlp_fit(pcp_ctx *c, int epochs, long batch, float lr, uint64_t seed, float *
     for (long b = 0; b < nb; ++b) {
       const long b0 = b * batch, cnt = std::min(batch, n - b0);
       mlp_grad_idx(c, m->Xn, m->Tn, m->idx + b0, cnt, ld + b);
-      mlp_adam(c, lr);
+      mlp_adam(c, lr_e);
     }
     // the host permutation buffer is reused by the next pass
     if (hipStreamSynchronize(c->stream) != hipSuccess) return -1;

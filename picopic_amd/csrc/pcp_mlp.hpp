@@ -71,8 +71,9 @@ int mlp_init(pcp_ctx *c, uint64_t seed);
 // room for `rows` more training rows
 int mlp_reserve(pcp_ctx *c, long rows);
 // normalise the training set (feature mean / std, target rms), then
-// `epochs` passes of shuffled mini-batches with Adam; *loss = mean batch
-// loss of the last pass (normalised units)
+// `epochs` passes of shuffled mini-batches with Adam, the step size
+// annealed from lr to lr / 20 along a cosine; *loss = mean batch loss of the
+// last pass (normalised units)
 int mlp_fit(pcp_ctx *c, int epochs, long batch, float lr, uint64_t seed, float *loss);
 
 }  // namespace pcp
