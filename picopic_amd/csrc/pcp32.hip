@@ -161,6 +161,8 @@ struct Batch32 {
   long end[MAX_SPECIES];
   int sp[MAX_SPECIES];
   int nsp;
+  uint64_t *marker;    // strict-motion reset markers [D][P]
+  uint32_t *cflag;     // cells near reset nodes [Nr][Nz]
 };
 
 __device__ __forceinline__ float len2f(const float v[3]) { return v[0] * v[0] + v[1] * v[1] + v[2] * v[2]; }
@@ -337,55 +339,12 @@ struct WaveSink32 {
   }
 };
 
-struct AtomicSink32 {
-  float *J;
-  long P;
-  int yr, nr, nz;
-  int *err;
-  int *need_exact;
-  __device__ __forceinline__ void add(int c, int i, int k, double v) {
-    if (i < -2 || i >= nr + 2 || k < -2 || k >= nz + 2) { *err |= ERR_MOVED_TOO_FAR; return; }
-    unsafeAtomicAdd(&J[c * P + (long)(i + 2) * yr + (k + 2)], (float)v);
-  }
-  __device__ __forceinline__ void strict_reset(int, int) {
-    if (need_exact) *need_exact = 1;
-  }
-};
 
-struct ExactSink32 {
-  float *J;
-  const uint64_t *marker;
-  long P;
-  int yr, nr, nz;
-  uint64_t stamp, key;
-  int *err;
-  __device__ __forceinline__ void add(int c, int i, int k, double v) {
-    if (i < -2 || i >= nr + 2 || k < -2 || k >= nz + 2) { *err |= ERR_MOVED_TOO_FAR; return; }
-    long o = (long)(i + 2) * yr + (k + 2);
-    if (c == 0) {
-      uint64_t m = marker[o];
-      if ((m >> 40) == stamp && (m & ((1ull << 40) - 1)) > key) return;
-    }
-    unsafeAtomicAdd(&J[c * P + o], (float)v);
-  }
-  __device__ __forceinline__ void strict_reset(int, int) {}
-};
 
-struct MarkSink32 {
-  uint64_t *marker;
-  int yr, nr, nz;
-  uint64_t val;
-  __device__ __forceinline__ void add(int, int, int, double) {}
-  __device__ __forceinline__ void strict_reset(int i, int k) {
-    if (i < -2 || i >= nr + 2 || k < -2 || k >= nz + 2) return;
-    atomicMax((unsigned long long *)&marker[(long)(i + 2) * yr + (k + 2)], (unsigned long long)val);
-  }
-};
 
 // ZigZag (currentZigZag.cpp:22-136) in FP32, positions relative to the old
 // cell: uo = f_old, un = (i_n - i_o) + f_new (likewise in z)
-template <class Sink>
-__device__ __forceinline__ void zigzag32(Sink &S, bool live, const Geom &g, const Dom &dn, const Split &o,
+__device__ __forceinline__ ZZTermsT<float> zigzag32(const Geom &g, const Dom &dn, const Split &o,
                                          const Split &n, float vphi, float qw, float inv_dt) {
   const float dr = (float)g.dr, dz = (float)g.dz;
   const int di = n.cr - o.cr, dk = n.cz - o.cz;
@@ -412,18 +371,21 @@ __device__ __forceinline__ void zigzag32(Sink &S, bool live, const Geom &g, cons
   const int io = o.cr - (int)dn.bot_r, ko = o.cz - (int)dn.left_z;
   const int in = n.cr - (int)dn.bot_r, kn = n.cz - (int)dn.left_z;
   const float A = one_over_volume;
-  S.add_if(0, io, ko, A * F_r1 * mWphi * mWz1 + A * F_r1 * W_phi * mWz1, live);
-  S.add_if(0, io, ko + 1, A * F_r1 * mWphi * W_z1 + A * F_r1 * W_phi * W_z1, live);
-  S.add_if(0, in, kn, A * F_r2 * mWphi * mWz2 + A * F_r2 * W_phi * mWz2, live);
-  S.add_if(0, in, kn + 1, A * F_r2 * mWphi * W_z2 + A * F_r2 * W_phi * W_z2, live);
-  S.add_if(1, in, kn, A * F_phi * mWr2 * mWz2, live);
-  S.add_if(1, in, kn + 1, A * F_phi * mWr2 * W_z1, live);   // uses W_z1 (currentZigZag.cpp:104)
-  S.add_if(1, in + 1, kn, A * F_phi * W_r2 * mWz2, live);
-  S.add_if(1, in + 1, kn + 1, A * F_phi * W_r2 * W_z2, live);
-  S.add_if(2, io, ko, A * F_z1 * mWr1 * mWphi + A * F_z1 * mWr1 * W_phi, live);
-  S.add_if(2, io + 1, ko, A * F_z1 * W_r1 * mWphi + A * F_z1 * W_r1 * W_phi, live);
-  S.add_if(2, in, kn, A * F_z2 * mWr2 * mWphi + A * F_z2 * mWr2 * W_phi, live);
-  S.add_if(2, in + 1, kn, A * F_z2 * W_r2 * mWphi + A * F_z2 * W_r2 * W_phi, live);
+  ZZTermsT<float> Z;
+  Z.io = io; Z.ko = ko; Z.in = in; Z.kn = kn;
+  Z.v[0] = A * F_r1 * mWphi * mWz1 + A * F_r1 * W_phi * mWz1;
+  Z.v[1] = A * F_r1 * mWphi * W_z1 + A * F_r1 * W_phi * W_z1;
+  Z.v[2] = A * F_r2 * mWphi * mWz2 + A * F_r2 * W_phi * mWz2;
+  Z.v[3] = A * F_r2 * mWphi * W_z2 + A * F_r2 * W_phi * W_z2;
+  Z.v[4] = A * F_phi * mWr2 * mWz2;
+  Z.v[5] = A * F_phi * mWr2 * W_z1;   // uses W_z1 (currentZigZag.cpp:104)
+  Z.v[6] = A * F_phi * W_r2 * mWz2;
+  Z.v[7] = A * F_phi * W_r2 * W_z2;
+  Z.v[8] = A * F_z1 * mWr1 * mWphi + A * F_z1 * mWr1 * W_phi;
+  Z.v[9] = A * F_z1 * W_r1 * mWphi + A * F_z1 * W_r1 * W_phi;
+  Z.v[10] = A * F_z2 * mWr2 * mWphi + A * F_z2 * mWr2 * W_phi;
+  Z.v[11] = A * F_z2 * W_r2 * mWphi + A * F_z2 * W_r2 * W_phi;
+  return Z;
 }
 
 // (cr, cz >= 0: cells of live particles)
@@ -519,17 +481,14 @@ template <int CCS, class Sink>
 __device__ __forceinline__ void deposit32(Sink &S, bool live, const Geom &g, const Dom &dn, const Split &o,
                                           const Split &n, float vphi, float qw, float inv_dt, double rd, double zd,
                                           double p_charge, const FieldsT<float> &f, int d_dep, StepState *st,
-                                          int *err) {
+                                          int *err, uint64_t *marker, uint32_t *cflag, const uint64_t *seq, long p,
+                                          int sp) {
   if (CCS == PCP_CCS_ZIGZAG) {
-    zigzag32(S, live, g, dn, o, n, vphi, qw, inv_dt);
+    zigzag_runs(f.J, (long)d_dep * 3 * g.P, g.P, g.yr, g.nr, g.nz, zigzag32(g, dn, o, n, vphi, qw, inv_dt), live, err);
   } else {
     const double r = rd, z = zd;
     const double ro = ((double)o.cr + (double)o.fr) * g.dr, zo = ((double)o.cz + (double)o.fz) * g.dz;
     AzTerms A = vb_azimuthal_terms(g.nr, g.nz, g.dr, g.dz, dn.bot_r, dn.left_z, r, z, ro, zo, (double)vphi, p_charge);
-    S.add_if(1, A.i, A.k, (float)A.w[0], live && A.in[0]);
-    S.add_if(1, A.i + 1, A.k, (float)A.w[1], live && A.in[1]);
-    S.add_if(1, A.i, A.k + 1, (float)A.w[2], live && A.in[2]);
-    S.add_if(1, A.i + 1, A.k + 1, (float)A.w[3], live && A.in[3]);
     SegRecorder R;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
@@ -538,21 +497,12 @@ __device__ __forceinline__ void deposit32(Sink &S, bool live, const Geom &g, con
     }
     if (live) vb_rz_dispatch(R, g.dr, g.dz, r, z, ro, zo, err);
     if (live && R.is_strict) {
-      AtomicSink32 A1{f.J + (long)d_dep * 3 * g.P, g.P, g.yr, g.nr, g.nz, err, &st->need_exact};
+      const ResetMark M{marker + (long)d_dep * g.P, cflag, ((st->step & 0xFFFFFF) << 40) | order_key32(sp, seq[p]),
+                        (uint32_t)st->step, dn.bot_r, dn.left_z, g.Nr, g.Nz, g.yr, g.nr, g.nz, &st->need_exact};
+      ResetSink<float> A1{f.J + (long)d_dep * 3 * g.P, g.P, g.yr, g.nr, g.nz, err, M};
       vb_strict(A1, g.dr, g.dz, g.dt, dn.bot_r, dn.left_z, r, z, ro, zo, p_charge);
     }
-#pragma unroll
-    for (int sg = 0; sg < 3; ++sg) {
-      const bool a = live && !R.is_strict && sg < R.n;
-      if (__ballot(a) == 0) break;
-      VBTerms T = vb_simple_terms(g.dr, g.dz, g.dt, dn.bot_r, dn.left_z, R.rn[sg], R.zn[sg], R.ro[sg], R.zo[sg],
-                                  R.i[sg], R.k[sg], p_charge);
-      const bool va = a && T.valid;
-      S.add_if(2, T.i, T.k, (float)T.jz0, va);
-      S.add_if(2, T.i + 1, T.k, (float)T.jz1, va);
-      S.add_if(0, T.i, T.k, (float)T.jr0, va);
-      S.add_if(0, T.i, T.k + 1, (float)T.jr1, va);
-    }
+    vb_runs(f.J, (long)d_dep * 3 * g.P, g.P, g.yr, g, dn, live, A, R, p_charge, err);
   }
 }
 
@@ -563,7 +513,7 @@ __device__ __forceinline__ void deposit32(Sink &S, bool live, const Geom &g, con
 template <int CCS, bool PUSH>
 __device__ __forceinline__ void step_trip32(const Geom &g, const FieldsT<float> &f, const Species32 &s, int sp,
                                             const PushConst &K, long b0, int cur, StepState *st, const Migrants &mg,
-                                            int &err) {
+                                            uint64_t *marker, uint32_t *cflag, int &err) {
   const int nxt = cur ^ 1;
   const float dtr = (float)(g.dt / g.dr), dtz = (float)(g.dt / g.dz);
   const float inv_dt = (float)(1.0 / g.dt);
@@ -599,7 +549,8 @@ __device__ __forceinline__ void step_trip32(const Geom &g, const FieldsT<float> 
   const int d_dep = CCS == PCP_CCS_ZIGZAG ? d_new : (live ? deposit_domain32(g, rd, zd) : 0);
   const Dom dn = make_dom(g, d_dep);
   WaveSink32 S{f.J, (int)((long)d_dep * 3 * g.P), (int)g.P, g.yr, g.nr, g.nz, &err};
-  deposit32<CCS>(S, live, g, dn, o, n, v[1], qw, inv_dt, rd, zd, s.q * (double)w, f, d_dep, st, &err);
+  deposit32<CCS>(S, live, g, dn, o, n, v[1], qw, inv_dt, rd, zd, s.q * (double)w, f, d_dep, st, &err, marker, cflag,
+                 s.seq, p, sp);
 }
 
 // The fused step of every species in one persistent launch: a block takes
@@ -622,73 +573,45 @@ __global__ void __launch_bounds__(BLOCK) k32_step_batch(Geom g, FieldsT<float> f
     int k = 0;
     while (k + 1 < b.nsp && t >= b.end[k]) ++k;
     const long first = k ? b.end[k - 1] : 0;
-    step_trip32<CCS, PUSH>(g, f, b.s[k], b.sp[k], b.K[k], (t - first) * BLOCK, cur, st, mg, err);
+    step_trip32<CCS, PUSH>(g, f, b.s[k], b.sp[k], b.K[k], (t - first) * BLOCK, cur, st, mg, b.marker, b.cflag, err);
   }
   if (err) atomicOr(&st->err, err);
 }
 
-// Exact path of the Villasenor-Buneman deposit (rare: a strict axial motion
-// reset J_r nodes this step).  Both kernels run after the step flipped `cur`:
-// they redo the particle's gather/push/move from the pre-step slot cur ^ 1 to
-// recover the deposit end point, then mark the reset nodes and re-deposit in
-// the reference's particle order, with the FP64 terms of the fused kernel.
-__global__ void __launch_bounds__(256) k32_mark(Geom g, FieldsT<float> f, Species32 s, int sp, int cur, StepState *st,
-                                                uint64_t *marker, PushConst K, int do_push, int only_if_exact) {
-  if (only_if_exact && !st->need_exact) return;
-  const uint64_t stamp = st->step & 0xFFFFFF;
-  const float dtr = (float)(g.dt / g.dr), dtz = (float)(g.dt / g.dz);
-  int dummy = 0;
-  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < s.n; p += (long)gridDim.x * blockDim.x) {
-    uint32_t c = s.cell[cur][p], co = s.cell[cur ^ 1][p];
-    if (c == DEAD_CELL || co == DEAD_CELL) continue;
-    Split o, n;
-    float v[3];
-    double r, z;
-    int d_old;
-    if (!advance32(g, f, s, p, co, cur ^ 1, cur ^ 1, do_push != 0, K, dtr, dtz, o, n, v, r, z, d_old, &dummy))
-      continue;
-    const double ro = ((double)o.cr + (double)o.fr) * g.dr, zo = ((double)o.cz + (double)o.fz) * g.dz;
-    if (!vb_is_strict_reset(g.dr, g.dz, r, z, ro, zo)) continue;
-    int d = deposit_domain32(g, r, z);
-    Dom dm = make_dom(g, d);
-    MarkSink32 S{marker + (long)d * g.P, g.yr, g.nr, g.nz, (stamp << 40) | order_key32(sp, s.seq[p])};
-    int i_n = cell_number(r, g.dr) - (int)dm.bot_r, k_n = cell_number(z, g.dz) - (int)dm.left_z;
-    S.strict_reset(i_n, k_n);
-    S.strict_reset(i_n, k_n + 1);
-  }
-}
-
-__global__ void __launch_bounds__(256) k32_deposit_exact(Geom g, FieldsT<float> f, Species32 s, int sp, int cur,
-                                                         StepState *st, const uint64_t *marker, PushConst K,
-                                                         int do_push, int only_if_exact) {
-  if (only_if_exact && !st->need_exact) return;
-  int err = 0, dummy = 0;
-  const uint64_t stamp = st->step & 0xFFFFFF;
-  const float dtr = (float)(g.dt / g.dr), dtz = (float)(g.dt / g.dz);
-  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < s.n; p += (long)gridDim.x * blockDim.x) {
-    uint32_t c = s.cell[cur][p], co = s.cell[cur ^ 1][p];
-    if (c == DEAD_CELL || co == DEAD_CELL) continue;
-    Split o, n;
-    float v[3];
-    double r, z;
-    int d_old;
-    if (!advance32(g, f, s, p, co, cur ^ 1, cur ^ 1, do_push != 0, K, dtr, dtz, o, n, v, r, z, d_old, &dummy))
-      continue;
-    const double ro = ((double)o.cr + (double)o.fr) * g.dr, zo = ((double)o.cz + (double)o.fz) * g.dz;
-    int d = deposit_domain32(g, r, z);
-    Dom dm = make_dom(g, d);
-    ExactSink32 S{f.J + (long)d * 3 * g.P, marker + (long)d * g.P, g.P, g.yr, g.nr, g.nz, stamp,
-                  order_key32(sp, s.seq[p]), &err};
-    const double p_charge = s.q * (double)s.w[p];
-    vb_azimuthal(S, g.nr, g.nz, g.dr, g.dz, dm.bot_r, dm.left_z, r, z, ro, zo, (double)v[1], p_charge);
-    vb_rz(S, g.dr, g.dz, g.dt, dm.bot_r, dm.left_z, r, z, ro, zo, p_charge, &err);
-  }
-  if (err) atomicOr(&st->err, err);
-}
-
-__global__ void k32_fill_if_exact(float *p, long n, const StepState *st) {
+// Fix-up pass of a step whose strict motions reset J_r nodes: particles
+// whose pre-step cell is flagged redo their gather/push/move from the
+// pre-step slots (bit-identical to the step) and take back their J_r
+// contributions at nodes that a later particle in the reference order reset.
+// All species in one launch, one thread per particle.
+__global__ void __launch_bounds__(256) k32_fix(Geom g, FieldsT<float> f, Batch32 b, int cur, StepState *st,
+                                               int do_push) {
   if (!st->need_exact) return;
-  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) p[t] = 0.f;
+  const uint64_t stamp = st->step & 0xFFFFFF;
+  const uint32_t fval = (uint32_t)st->step;
+  const float dtr = (float)(g.dt / g.dr), dtz = (float)(g.dt / g.dz);
+  int err = 0;
+  const long total = b.end[b.nsp - 1] * 256;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+    int k = 0;
+    while (k + 1 < b.nsp && t >= b.end[k] * 256) ++k;
+    const Species32 &s = b.s[k];
+    const long p = t - (k ? b.end[k - 1] * 256 : 0);
+    if (p >= s.n) continue;
+    const uint32_t c = s.cell[cur][p], co = s.cell[cur ^ 1][p];
+    if (c == DEAD_CELL || co == DEAD_CELL || b.cflag[co] != fval) continue;
+    Split o, n;
+    float v[3];
+    double r, z;
+    int d_old;
+    if (!advance32(g, f, s, p, co, cur ^ 1, cur ^ 1, do_push != 0, b.K[k], dtr, dtz, o, n, v, r, z, d_old, &err))
+      continue;
+    const double ro = ((double)o.cr + (double)o.fr) * g.dr, zo = ((double)o.cz + (double)o.fz) * g.dz;
+    const int d = deposit_domain32(g, r, z);
+    const Dom dm = make_dom(g, d);
+    FixSink<float> S{f.J + (long)d * 3 * g.P, b.marker + (long)d * g.P, g.yr, g.nr, g.nz, stamp,
+                     order_key32(b.sp[k], s.seq[p])};
+    vb_rz(S, g.dr, g.dz, g.dt, dm.bot_r, dm.left_z, r, z, ro, zo, s.q * (double)s.w[p], &err);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1042,6 +965,8 @@ void launch32_step_batch(pcp_ctx *c, bool push) {
     b.sp[b.nsp] = sp;
     ++b.nsp;
   }
+  b.marker = c->marker;
+  b.cflag = c->cell_flag;
   if (b.nsp) {
     if (c->g.ccs == PCP_CCS_ZIGZAG) launch_batch_ccs<PCP_CCS_ZIGZAG>(c, b, push, block);
     else launch_batch_ccs<PCP_CCS_VB>(c, b, push, block);
@@ -1055,23 +980,25 @@ void launch32_step_batch(pcp_ctx *c, bool push) {
     }
 }
 
-void launch32_mark(pcp_ctx *c, int sp, int do_push, int only_if_exact) {
-  Species32 &s = c->s32->sp[sp];
-  if (!s.n || c->g.ccs != PCP_CCS_VB) return;
-  hipLaunchKernelGGL(k32_mark, dim3(pgrid32(c, s.n, 256)), dim3(256), 0, c->stream, c->g, c->s32->f, s, sp, c->cur,
-                     c->st, c->marker, push_const(s, c->g.dt), do_push, only_if_exact);
-}
-
-void launch32_deposit_exact(pcp_ctx *c, int sp, int do_push, int only_if_exact) {
-  Species32 &s = c->s32->sp[sp];
-  if (!s.n || c->g.ccs != PCP_CCS_VB) return;
-  hipLaunchKernelGGL(k32_deposit_exact, dim3(pgrid32(c, s.n, 256)), dim3(256), 0, c->stream, c->g, c->s32->f, s, sp,
-                     c->cur, c->st, c->marker, push_const(s, c->g.dt), do_push, only_if_exact);
-}
-
-void launch32_zero_j_if_exact(pcp_ctx *c) {
-  long n = (long)c->D * 3 * c->g.P;
-  hipLaunchKernelGGL(k32_fill_if_exact, dim3(pgrid32(c, n, 256)), dim3(256), 0, c->stream, c->s32->f.J, n, c->st);
+void launch32_fix(pcp_ctx *c, bool push) {
+  if (c->g.ccs != PCP_CCS_VB) return;
+  Batch32 b{};
+  long acc = 0;
+  for (int sp = 0; sp < c->cfg.n_species; ++sp) {
+    const Species32 &s = c->s32->sp[sp];
+    if (!s.n) continue;
+    acc += (s.n + 255) / 256;
+    b.s[b.nsp] = s;
+    b.K[b.nsp] = push_const(s, c->g.dt);
+    b.end[b.nsp] = acc;
+    b.sp[b.nsp] = sp;
+    ++b.nsp;
+  }
+  if (!b.nsp) return;
+  b.marker = c->marker;
+  b.cflag = c->cell_flag;
+  const unsigned grid = (unsigned)std::max(1L, std::min(acc, (long)c->n_cu * 16));
+  hipLaunchKernelGGL(k32_fix, dim3(grid), dim3(256), 0, c->stream, c->g, c->s32->f, b, c->cur, c->st, push ? 1 : 0);
 }
 
 void launch32_migrant_keys(pcp_ctx *c) {

@@ -56,9 +56,10 @@ void launch32_stitch(pcp_ctx *c, int which);
 void launch32_push(pcp_ctx *c, int sp);
 // the fused step of all species in one persistent work-queue launch
 void launch32_step_batch(pcp_ctx *c, bool push = true);
-void launch32_mark(pcp_ctx *c, int sp, int do_push, int only_if_exact);
-void launch32_deposit_exact(pcp_ctx *c, int sp, int do_push, int only_if_exact);
-void launch32_zero_j_if_exact(pcp_ctx *c);
+// fix-up pass after a step whose strict motions reset J_r nodes (no-op
+// otherwise, decided on the device); push: the step pushed (no collision
+// phase)
+void launch32_fix(pcp_ctx *c, bool push);
 void launch32_migrant_keys(pcp_ctx *c);
 int launch32_sort(pcp_ctx *c, int sp);
 void launch32_count_live(pcp_ctx *c, int sp, unsigned long long *dev_out);

@@ -207,7 +207,8 @@ int pcp_create(const pcp_config *cfg, pcp_ctx **out) {
   int rc = 0;
   if ((rc = dalloc(&c->f.E, gridn)) || (rc = dalloc(&c->f.H, gridn)) || (rc = dalloc(&c->f.Hat, gridn)) ||
       (rc = dalloc(&c->f.J, gridn)) || (rc = dalloc(&c->f.eps, gp)) || (rc = dalloc(&c->f.sigma, gp)) ||
-      (rc = dalloc(&c->st, 1)) || (rc = dalloc(&c->marker, (size_t)c->D * g.P))) {
+      (rc = dalloc(&c->st, 1)) || (rc = dalloc(&c->marker, (size_t)c->D * g.P)) ||
+      (rc = dalloc(&c->cell_flag, (size_t)g.Nr * g.Nz))) {
     pcp_destroy(c);
     return rc;
   }
@@ -317,7 +318,7 @@ int pcp_destroy(pcp_ctx *c) {
   coll_free(c);
   mlp_free(c);
   hipFree(c->f.E); hipFree(c->f.H); hipFree(c->f.Hat); hipFree(c->f.J);
-  hipFree(c->f.eps); hipFree(c->f.sigma); hipFree(c->st); hipFree(c->marker);
+  hipFree(c->f.eps); hipFree(c->f.sigma); hipFree(c->st); hipFree(c->marker); hipFree(c->cell_flag);
   for (int s = 0; s < MAX_SPECIES; ++s) {
     SpeciesDev &sp = c->sp[s];
     hipFree(sp.r[0]); hipFree(sp.r[1]); hipFree(sp.z[0]); hipFree(sp.z[1]);
@@ -937,17 +938,13 @@ static int step_body(pcp_ctx *c, Mark mark) {
   mark(2);
   if (f32) launch32_migrant_keys(c);
   else launch_migrant_keys(c);
-  // exact re-deposition when a strict axial motion reset J_r nodes
+  // strict axial motions reset J_r nodes: take back the contributions of
+  // the particles that precede the resetting one (device-side no-op when no
+  // reset happened)
   if (c->g.ccs == PCP_CCS_VB) {
-    if (f32) {
-      launch32_zero_j_if_exact(c);
-      for (int s = 0; s < c->cfg.n_species; ++s) launch32_mark(c, s, !coll, 1);
-      for (int s = 0; s < c->cfg.n_species; ++s) launch32_deposit_exact(c, s, !coll, 1);
-    } else {
-      launch_zero_j_if_exact(c);
-      for (int s = 0; s < c->cfg.n_species; ++s) launch_mark(c, s, 1);
-      for (int s = 0; s < c->cfg.n_species; ++s) launch_deposit_exact(c, s, 1);
-    }
+    if (f32) launch32_fix(c, !coll);
+    else
+      for (int s = 0; s < c->cfg.n_species; ++s) launch_fix(c, s);
   }
   mark(3);
   if (f32) launch32_stitch(c, PCP_GRID_J);
@@ -1152,6 +1149,9 @@ int pcp_autotune(pcp_ctx *c, int steps, pcp_tuning *out) {
   // after a sort grows as the particles leave cell order; with t(j) ~ a + b j
   // and sort time t_s, an interval k costs t_s / k + a + b (k - 1) / 2 per
   // step, least at k = sqrt(2 t_s / b)
+  // beams keep their own interval while the plasma order ages
+  for (int s = 0; s < c->cfg.n_species; ++s)
+    if (c->cfg.species[s].is_beam) c->sort_every[s] = saved[s];
   double t_sort = 0;
   {
     if (int rc = pcp_sync(c)) return restore(rc);

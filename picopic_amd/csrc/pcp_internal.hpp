@@ -179,6 +179,7 @@ struct pcp_ctx {
   pcp::StepState *st;              // device
   pcp::Migrants mig;
   uint64_t *marker;                // [D][P] strict-motion J_r reset markers
+  uint32_t *cell_flag;             // [Nr][Nz] cells near reset nodes (step stamp)
   pcp::SortBuffers sortb[pcp::MAX_SPECIES];
   uint32_t *sort_key, *sort_count, *sort_offset, *sort_fill;
   void *sort_temp;

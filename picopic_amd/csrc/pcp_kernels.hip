@@ -266,10 +266,6 @@ __global__ void k_fill(double *p, long n, double v) {
   for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) p[t] = v;
 }
 
-__global__ void k_fill_if_exact(double *p, long n, const StepState *st) {
-  if (!st->need_exact) return;
-  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) p[t] = 0;
-}
 
 __global__ void k_step_begin(StepState *st) {
   st->step += 1;
@@ -385,17 +381,15 @@ __global__ void k_migrate(Geom g, SpeciesDev s, int sp, int cur, StepState *st, 
 template <int CCS, class Sink>
 __device__ __forceinline__ void deposit64(Sink &S, bool live, const Geom &g, const Dom &dn, double r, double z,
                                           double ro, double zo, double vphi, double q, double w, const Fields &f,
-                                          int d_new, StepState *st, int *err) {
+                                          int d_new, StepState *st, int *err, uint64_t *marker, uint32_t *cflag,
+                                          const uint64_t *seq, long p, int sp) {
   if (CCS == PCP_CCS_ZIGZAG) {
-    zigzag(S, g.dr, g.dz, g.dt, dn.bot_r, dn.left_z, r, z, ro, zo, vphi, q, w);
+    zigzag_runs(f.J, (long)d_new * 3 * g.P, g.P, g.yr, g.nr, g.nz,
+                zigzag_terms(g.dr, g.dz, g.dt, dn.bot_r, dn.left_z, r, z, ro, zo, vphi, q, w), live, err);
   } else {
     const double p_charge = q * w;
     // azimuthal part (4 interior nodes)
     AzTerms A = vb_azimuthal_terms(g.nr, g.nz, g.dr, g.dz, dn.bot_r, dn.left_z, r, z, ro, zo, vphi, p_charge);
-    S.add_if(1, A.i, A.k, A.w[0], live && A.in[0]);
-    S.add_if(1, A.i + 1, A.k, A.w[1], live && A.in[1]);
-    S.add_if(1, A.i, A.k + 1, A.w[2], live && A.in[2]);
-    S.add_if(1, A.i + 1, A.k + 1, A.w[3], live && A.in[3]);
     // r-z part: branch selection per lane, arithmetic and atomics converged
     SegRecorder R;
 #pragma unroll
@@ -405,23 +399,14 @@ __device__ __forceinline__ void deposit64(Sink &S, bool live, const Geom &g, con
     }
     if (live) vb_rz_dispatch(R, g.dr, g.dz, r, z, ro, zo, err);
     if (live && R.is_strict) {
-      // strict motion is rare: plain atomics; a J_r reset flags the exact
-      // re-deposition of this step
-      AtomicSink A1{f.J + (long)d_new * 3 * g.P, g.P, g.yr, g.nr, g.nz, err, &st->need_exact};
+      // strict motion is rare: plain atomics; a J_r reset marks the node for
+      // the fix-up pass of this step
+      const ResetMark M{marker + (long)d_new * g.P, cflag, ((st->step & 0xFFFFFF) << 40) | order_key(sp, seq[p]),
+                        (uint32_t)st->step, dn.bot_r, dn.left_z, g.Nr, g.Nz, g.yr, g.nr, g.nz, &st->need_exact};
+      ResetSink<double> A1{f.J + (long)d_new * 3 * g.P, g.P, g.yr, g.nr, g.nz, err, M};
       vb_strict(A1, g.dr, g.dz, g.dt, dn.bot_r, dn.left_z, r, z, ro, zo, p_charge);
     }
-#pragma unroll
-    for (int sg = 0; sg < 3; ++sg) {
-      const bool a = live && !R.is_strict && sg < R.n;
-      if (__ballot(a) == 0) break;
-      VBTerms T = vb_simple_terms(g.dr, g.dz, g.dt, dn.bot_r, dn.left_z, R.rn[sg], R.zn[sg], R.ro[sg], R.zo[sg],
-                                  R.i[sg], R.k[sg], p_charge);
-      const bool v = a && T.valid;
-      S.add_if(2, T.i, T.k, T.jz0, v);
-      S.add_if(2, T.i + 1, T.k, T.jz1, v);
-      S.add_if(0, T.i, T.k, T.jr0, v);
-      S.add_if(0, T.i, T.k + 1, T.jr1, v);
-    }
+    vb_runs(f.J, (long)d_new * 3 * g.P, g.P, g.yr, g, dn, live, A, R, p_charge, err);
   }
 }
 
@@ -432,7 +417,8 @@ __device__ __forceinline__ void deposit64(Sink &S, bool live, const Geom &g, con
 // and collide first).
 template <int CCS, bool PUSH>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CCS == PCP_CCS_ZIGZAG ? 3 : 2)))
-k_step_fused(Geom g, Fields f, SpeciesDev s, int sp, int cur, StepState *st, Migrants mg) {
+k_step_fused(Geom g, Fields f, SpeciesDev s, int sp, int cur, StepState *st, Migrants mg, uint64_t *marker,
+             uint32_t *cflag) {
   int err = 0;
   const int nxt = cur ^ 1;
   const long stride = (long)gridDim.x * blockDim.x;
@@ -471,7 +457,7 @@ k_step_fused(Geom g, Fields f, SpeciesDev s, int sp, int cur, StepState *st, Mig
     if (!live) { r = ro = g.dr; z = zo = g.dz; d_new = 0; vphi = 0; w = 0; }
     const Dom dn = make_dom(g, d_new);
     WaveSink S{f.J, (int)((long)d_new * 3 * g.P), (int)g.P, g.yr, g.nr, g.nz, live, &err};
-    deposit64<CCS>(S, live, g, dn, r, z, ro, zo, vphi, s.q, w, f, d_new, st, &err);
+    deposit64<CCS>(S, live, g, dn, r, z, ro, zo, vphi, s.q, w, f, d_new, st, &err, marker, cflag, s.seq, p, sp);
   }
   if (err) atomicOr(&st->err, err);
 }
@@ -518,6 +504,30 @@ __global__ void __launch_bounds__(256) k_deposit_exact(Geom g, Fields f, Species
     }
   }
   if (err) atomicOr(&st->err, err);
+}
+
+// fix-up pass of a step with strict-motion resets: particles whose old cell
+// is flagged recompute their r-z current and take back the J_r
+// contributions at nodes that a later particle reset
+__global__ void __launch_bounds__(256) k_fix(Geom g, Fields f, SpeciesDev s, int sp, int cur, StepState *st,
+                                             const uint64_t *marker, const uint32_t *cflag) {
+  if (!st->need_exact) return;
+  int err = 0;
+  const uint64_t stamp = st->step & 0xFFFFFF;
+  const uint32_t fval = (uint32_t)st->step;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < s.n; p += (long)gridDim.x * blockDim.x) {
+    const double r = s.r[cur][p], ro = s.r[cur ^ 1][p];
+    if (r < 0 || ro < 0) continue;
+    const double z = s.z[cur][p], zo = s.z[cur ^ 1][p];
+    const long ci = min(max((long)cell_number(ro, g.dr), 0L), g.Nr - 1);
+    const long ck = min(max((long)cell_number(zo, g.dz), 0L), g.Nz - 1);
+    if (cflag[ci * g.Nz + ck] != fval) continue;
+    const int d = domain_of(g, cell_number(r, g.dr), cell_number(z, g.dz));
+    const Dom dm = make_dom(g, d);
+    FixSink<double> S{f.J + (long)d * 3 * g.P, marker + (long)d * g.P, g.yr, g.nr, g.nz, stamp,
+                      order_key(sp, s.seq[p])};
+    vb_rz(S, g.dr, g.dz, g.dt, dm.bot_r, dm.left_z, r, z, ro, zo, s.q * s.w[p], &err);
+  }
 }
 
 __global__ void k_stamp(StepState *st) { st->step += 1; }
@@ -673,17 +683,17 @@ void launch_step_fused(pcp_ctx *c, int sp, bool push) {
   if (c->g.ccs == PCP_CCS_ZIGZAG) {
     if (push)
       hipLaunchKernelGGL((k_step_fused<PCP_CCS_ZIGZAG, true>), grid, block, 0, c->stream, c->g, c->f, s, sp, c->cur,
-                         c->st, c->mig);
+                         c->st, c->mig, c->marker, c->cell_flag);
     else
       hipLaunchKernelGGL((k_step_fused<PCP_CCS_ZIGZAG, false>), grid, block, 0, c->stream, c->g, c->f, s, sp, c->cur,
-                         c->st, c->mig);
+                         c->st, c->mig, c->marker, c->cell_flag);
   } else {
     if (push)
       hipLaunchKernelGGL((k_step_fused<PCP_CCS_VB, true>), grid, block, 0, c->stream, c->g, c->f, s, sp, c->cur, c->st,
-                         c->mig);
+                         c->mig, c->marker, c->cell_flag);
     else
       hipLaunchKernelGGL((k_step_fused<PCP_CCS_VB, false>), grid, block, 0, c->stream, c->g, c->f, s, sp, c->cur,
-                         c->st, c->mig);
+                         c->st, c->mig, c->marker, c->cell_flag);
   }
 }
 void launch_mark(pcp_ctx *c, int sp, int only_if_exact) {
@@ -698,9 +708,11 @@ void launch_deposit_exact(pcp_ctx *c, int sp, int only_if_exact) {
   hipLaunchKernelGGL(k_deposit_exact, dim3(pgrid(c, s.n, 256)), dim3(256), 0, c->stream, c->g, c->f, s, sp, c->cur,
                      c->st, c->marker, only_if_exact);
 }
-void launch_zero_j_if_exact(pcp_ctx *c) {
-  long n = (long)c->D * 3 * c->g.P;
-  hipLaunchKernelGGL(k_fill_if_exact, dim3(pgrid(c, n, 256)), dim3(256), 0, c->stream, c->f.J, n, c->st);
+void launch_fix(pcp_ctx *c, int sp) {
+  SpeciesDev &s = c->sp[sp];
+  if (!s.n || c->g.ccs != PCP_CCS_VB) return;
+  hipLaunchKernelGGL(k_fix, dim3(pgrid(c, s.n, 256)), dim3(256), 0, c->stream, c->g, c->f, s, sp, c->cur, c->st,
+                     c->marker, c->cell_flag);
 }
 void launch_migrant_keys_t(pcp_ctx *c, SeqTable t) {
   if (c->D == 1) return;

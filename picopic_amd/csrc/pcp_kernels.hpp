@@ -29,7 +29,9 @@ void launch_migrate_mark(pcp_ctx *c, int sp);   // removal + migrant list (phase
 void launch_step_fused(pcp_ctx *c, int sp, bool push = true);
 void launch_mark(pcp_ctx *c, int sp, int only_if_exact);
 void launch_deposit_exact(pcp_ctx *c, int sp, int only_if_exact);
-void launch_zero_j_if_exact(pcp_ctx *c);
+// fix-up pass after a step whose strict motions reset J_r nodes (no-op
+// otherwise, decided on the device)
+void launch_fix(pcp_ctx *c, int sp);
 void launch_migrant_keys(pcp_ctx *c);
 void launch_gather_point(pcp_ctx *c, int d, double r, double z, int which, double *dev_out);
 

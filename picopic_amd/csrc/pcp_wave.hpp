@@ -58,6 +58,72 @@ __device__ __forceinline__ void wave_add(T *base, int idx, T v, bool active) {
   }
 }
 
+// Runs of equal keys in lane order.  Cell-sorted particles put the lanes of
+// a wavefront into a few long runs (one per cell they share).  A segmented
+// inclusive scan -- row shifts by 1, 2, 4 and 8 lanes, then the broadcasts of
+// lane 15 of each row into the next row and of lane 31 into rows 2 and 3 --
+// leaves the total of each run in its last lane, which issues one atomic for
+// the run.  m[s] = ~0 where the lane adds its step-s source (no run head
+// between the lane and its source, and the step reaches the lane's row),
+// else 0; computed once per key and shared by every value deposited at nodes
+// fixed relative to that key.  All 64 lanes must be active.
+struct Runs {
+  int m[6];
+  bool tail;   // last lane of its run
+};
+
+// DPP move with lanes that have no source reading 0
+template <int CTRL>
+__device__ __forceinline__ int dpp_mov(int x) {
+  return __builtin_amdgcn_mov_dpp(x, CTRL, 0xF, 0xF, true);
+}
+
+__device__ __forceinline__ Runs runs_of(uint32_t key) {
+  const int lane = __lane_id();
+  const int row = lane >> 4;
+  const uint32_t prev = (uint32_t)dpp_mov<0x138>((int)key);   // wave_shr:1 -- key of lane - 1
+  const int h = (lane == 0 || prev != key) ? 1 : 0;            // first lane of a run
+  Runs R;
+  // masks as F - 1 (F is 0 or 1): ~0 where no head lies between
+  int F = h;   // a run head in (lane - d, lane] before the step of distance d
+  R.m[0] = F - 1;
+  F |= dpp_mov<0x111>(F);   // row_shr:1
+  R.m[1] = F - 1;
+  F |= dpp_mov<0x112>(F);   // row_shr:2
+  R.m[2] = F - 1;
+  F |= dpp_mov<0x114>(F);   // row_shr:4
+  R.m[3] = F - 1;
+  F |= dpp_mov<0x118>(F);   // row_shr:8 -- now: a head in this row up to the lane
+  R.m[4] = (F | (~row & 1)) - 1;                     // lane 15 of the previous row: rows 1, 3
+  const int Fb = dpp_mov<0x142>(F);
+  F |= Fb & row;                                     // heads of row 2 seen from row 3 (row & 1)
+  R.m[5] = (F | (row < 2 ? 1 : 0)) - 1;              // lane 31: rows 2, 3
+  // wave_shl:1 -- head flag of lane + 1; evaluated by every lane (a DPP
+  // move inside a short-circuit would read lane 63 as inactive)
+  const int hn = dpp_mov<0x130>(h);
+  R.tail = (lane == 63) | (hn != 0);
+  return R;
+}
+
+template <int CTRL>
+__device__ __forceinline__ double run_step(double x, int m) {
+  const long long b = __double_as_longlong(x);
+  const int lo = dpp_mov<CTRL>((int)b) & m;
+  const int hi = dpp_mov<CTRL>((int)(b >> 32)) & m;
+  return x + __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// partial sums of x along the runs; the last lane of a run holds its total
+__device__ __forceinline__ double run_sum(double x, const Runs &R) {
+  x = run_step<0x111>(x, R.m[0]);   // row_shr:1
+  x = run_step<0x112>(x, R.m[1]);   // row_shr:2
+  x = run_step<0x114>(x, R.m[2]);   // row_shr:4
+  x = run_step<0x118>(x, R.m[3]);   // row_shr:8
+  x = run_step<0x142>(x, R.m[4]);   // row_bcast:15
+  x = run_step<0x143>(x, R.m[5]);   // row_bcast:31
+  return x;
+}
+
 // one atomic per distinct key in the wavefront; each active lane receives a
 // distinct slot counted from the value the counter held
 __device__ __forceinline__ uint32_t wave_reserve(uint32_t *ctr, uint32_t key, bool active) {

@@ -37,10 +37,13 @@ def test_config_context_equals_manual(precision):
     man.init_plasma(1, 8, 1e17, 1e17, 0.1, 6)
     for c in (ctx, man):
         c.step(3)
+    # the deposits' atomics sum in a different order in the two contexts
+    tol = 1e-12 if precision == "fp64" else 1e-5
     for s in range(2):
         assert ctx.count(s) == man.count(s) == 32 * 64 * 8
         a, b = ctx.particles(s), man.particles(s)
         oa, ob = np.argsort(a["id"]), np.argsort(b["id"])
+        assert np.array_equal(a["id"][oa], b["id"][ob])
         for k in ("r", "z", "vr", "vz"):
             scale = np.abs(a[k]).max()
-            assert np.abs(a[k][oa] - b[k][ob]).max() <= 1e-9 * scale
+            assert np.abs(a[k][oa] - b[k][ob]).max() <= tol * scale
