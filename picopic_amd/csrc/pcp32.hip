@@ -154,7 +154,8 @@ struct PushConst {
   float inv_c2;  // 1 / c^2
 };
 
-// the species of one batched step launch; end[k] = trips of species 0..k
+// the species of one batched step launch; end[k] = block trips of species
+// 0..k
 struct Batch32 {
   Species32 s[MAX_SPECIES];
   PushConst K[MAX_SPECIES];
@@ -509,15 +510,15 @@ __device__ __forceinline__ void deposit32(Sink &S, bool live, const Geom &g, con
 // PUSH: velocities slot cur -> pushed, moved -> slot cur ^ 1.
 // !PUSH: pushed (and collided) velocities of slot cur ^ 1 -> moved -> slot
 // cur; the caller then exchanges the two velocity slots.
-// One block trip of the fused step: the blockDim.x particles from b0 on.
+// One block trip of the fused step: particle p of this thread (the threads
+// of a block take consecutive particles).
 template <int CCS, bool PUSH>
 __device__ __forceinline__ void step_trip32(const Geom &g, const FieldsT<float> &f, const Species32 &s, int sp,
-                                            const PushConst &K, long b0, int cur, StepState *st, const Migrants &mg,
+                                            const PushConst &K, long p, int cur, StepState *st, const Migrants &mg,
                                             uint64_t *marker, uint32_t *cflag, int &err) {
   const int nxt = cur ^ 1;
-  const float dtr = (float)(g.dt / g.dr), dtz = (float)(g.dt / g.dz);
-  const float inv_dt = (float)(1.0 / g.dt);
-  const long p = b0 + threadIdx.x;
+  const float dtr = g.dtr_f, dtz = g.dtz_f;
+  const float inv_dt = g.inv_dt_f;
   bool live = false;
   Split o{1, 1, 0.5f, 0.5f}, n{1, 1, 0.5f, 0.5f};
   float v[3] = {0.f, 0.f, 0.f};
@@ -556,15 +557,17 @@ __device__ __forceinline__ void step_trip32(const Geom &g, const FieldsT<float> 
 // The fused step of every species in one persistent launch: a block takes
 // the next trip (BLOCK consecutive particles of one species) from a counter
 // in StepState until the trips of all species are handed out, so that trips
-// of uneven cost (wave-path deposits, dead slots, domains of different
-// density) balance over the resident blocks.
+// of uneven cost (dead slots, domains of different density, strict motions,
+// walls) balance over the resident blocks.
 template <int CCS, bool PUSH, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k32_step_batch(Geom g, FieldsT<float> f, Batch32 b, int cur, StepState *st,
                                                         Migrants mg) {
   __shared__ unsigned ticket;
   int err = 0;
-  const long total = b.end[b.nsp - 1];
+  const long total = b.end[b.nsp - 1];   // trips of BLOCK particles
   for (;;) {
+    // one counter increment per block trip (a counter per wavefront trip
+    // serialises on the atomic: 4x slower on the wakefield case)
     if (threadIdx.x == 0) ticket = atomicAdd(&st->queue, 1u);
     __syncthreads();
     const long t = (long)(unsigned)__builtin_amdgcn_readfirstlane((int)ticket);
@@ -573,7 +576,8 @@ __global__ void __launch_bounds__(BLOCK) k32_step_batch(Geom g, FieldsT<float> f
     int k = 0;
     while (k + 1 < b.nsp && t >= b.end[k]) ++k;
     const long first = k ? b.end[k - 1] : 0;
-    step_trip32<CCS, PUSH>(g, f, b.s[k], b.sp[k], b.K[k], (t - first) * BLOCK, cur, st, mg, b.marker, b.cflag, err);
+    step_trip32<CCS, PUSH>(g, f, b.s[k], b.sp[k], b.K[k], (t - first) * BLOCK + threadIdx.x, cur, st, mg, b.marker,
+                           b.cflag, err);
   }
   if (err) atomicOr(&st->err, err);
 }
@@ -588,7 +592,7 @@ __global__ void __launch_bounds__(256) k32_fix(Geom g, FieldsT<float> f, Batch32
   if (!st->need_exact) return;
   const uint64_t stamp = st->step & 0xFFFFFF;
   const uint32_t fval = (uint32_t)st->step;
-  const float dtr = (float)(g.dt / g.dr), dtz = (float)(g.dt / g.dz);
+  const float dtr = g.dtr_f, dtz = g.dtz_f;
   int err = 0;
   const long total = b.end[b.nsp - 1] * 256;
   for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {

@@ -188,6 +188,9 @@ int pcp_create(const pcp_config *cfg, pcp_ctx **out) {
   g.div_Dz = make_fastdiv((uint32_t)g.Dz);
   g.div_Nz = make_fastdiv((uint32_t)g.Nz);
   g.inv_vol0 = 1.0 / (PI * g.dz * g.dr * g.dr);
+  g.dtr_f = (float)(g.dt / g.dr);
+  g.dtz_f = (float)(g.dt / g.dz);
+  g.inv_dt_f = (float)(1.0 / g.dt);
   c->D = g.Dr * g.Dz;
   c->cur = 0;
 

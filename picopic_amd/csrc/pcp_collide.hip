@@ -585,24 +585,46 @@ __global__ void k_surr_features(Geom g, const double *mom, double m_e, double m_
 
 // training rows of the cells that collided: relative changes of Te_par,
 // Te_perp, Ti_par, Ti_perp over the pass, each over its rate scale
+// a training row of cell c: valid when the cell collided (not after a row
+// break) and its moments are normal before and after the pass
+__device__ __forceinline__ bool surr_row(const Geom &g, long c, const double *before, const double *after, double m_e,
+                                         double m_i, const int *ok, const int *row_bad, double rel[MLP_OUT]) {
+  if (!ok[c]) return false;
+  const CellPos q = cell_pos(g, c);
+  if (q.j >= row_bad[(long)q.d * g.nr + q.i]) return false;
+  double b[4], a[4], V[3], sc[MLP_OUT];
+  if (!species_temps(before + c * SST, m_e, b[0], b[1], V) || !species_temps(before + c * SST + 8, m_i, b[2], b[3], V) ||
+      !species_temps(after + c * SST, m_e, a[0], a[1], V) || !species_temps(after + c * SST + 8, m_i, a[2], a[3], V))
+    return false;
+  const double vol = PCP_CELL_VOLUME((double)q.cr, g.dr, g.dz);
+  if (!surr_rate_scale(before + c * SST, before + c * SST + 8, m_e, m_i, vol, g.dt, sc)) return false;
+  for (int k = 0; k < MLP_OUT; ++k) rel[k] = (a[k] - b[k]) / b[k] / sc[k];
+  return true;
+}
+
+__global__ void k_surr_rowflag(Geom g, const double *before, const double *after, double m_e, double m_i,
+                               const int *ok, const int *row_bad, uint32_t *flag) {
+  const long ncells = (long)g.Dr * g.Dz * g.nr * g.nz;
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c <= ncells; c += (long)gridDim.x * blockDim.x) {
+    double rel[MLP_OUT];
+    flag[c] = c < ncells && surr_row(g, c, before, after, m_e, m_i, ok, row_bad, rel) ? 1u : 0u;
+  }
+}
+
+// training rows of the cells that collided, in cell order from row n0 on:
+// relative changes of Te_par, Te_perp, Ti_par, Ti_perp over the pass, each
+// over its rate scale
 __global__ void k_surr_targets(Geom g, const double *before, const double *after, double m_e, double m_i,
-                               const int *ok, const int *row_bad, const float *X, float *DX, float *DT,
-                               unsigned long long *count, long cap) {
+                               const int *ok, const int *row_bad, const float *X, const uint32_t *offset, long n0,
+                               float *DX, float *DT, long cap) {
   const long ncells = (long)g.Dr * g.Dz * g.nr * g.nz;
   for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += (long)gridDim.x * blockDim.x) {
-    if (!ok[c]) continue;
-    const CellPos q = cell_pos(g, c);
-    if (q.j >= row_bad[(long)q.d * g.nr + q.i]) continue;
-    double b[4], a[4], V[3], sc[MLP_OUT];
-    if (!species_temps(before + c * SST, m_e, b[0], b[1], V) || !species_temps(before + c * SST + 8, m_i, b[2], b[3], V) ||
-        !species_temps(after + c * SST, m_e, a[0], a[1], V) || !species_temps(after + c * SST + 8, m_i, a[2], a[3], V))
-      continue;
-    const double vol = PCP_CELL_VOLUME((double)q.cr, g.dr, g.dz);
-    if (!surr_rate_scale(before + c * SST, before + c * SST + 8, m_e, m_i, vol, g.dt, sc)) continue;
-    unsigned long long slot = atomicAdd(count, 1ull);
-    if ((long)slot >= cap) continue;
+    double rel[MLP_OUT];
+    if (!surr_row(g, c, before, after, m_e, m_i, ok, row_bad, rel)) continue;
+    const long slot = n0 + (long)offset[c];
+    if (slot >= cap) continue;
     for (int k = 0; k < MLP_XSTRIDE; ++k) DX[slot * MLP_XSTRIDE + k] = X[c * MLP_XSTRIDE + k];
-    for (int k = 0; k < MLP_OUT; ++k) DT[slot * MLP_YSTRIDE + k] = (float)((a[k] - b[k]) / b[k] / sc[k]);
+    for (int k = 0; k < MLP_OUT; ++k) DT[slot * MLP_YSTRIDE + k] = (float)rel[k];
   }
 }
 
@@ -692,7 +714,7 @@ struct CollBuffers {
   double *mom = nullptr, *mom2 = nullptr;
   float *feat = nullptr, *pred = nullptr;
   int *ok = nullptr, *fallback = nullptr;
-  unsigned long long *counter = nullptr;   // [0] fallback cells, [1] collected rows
+  unsigned long long *counter = nullptr;   // [0] fallback cells
 };
 
 void coll_free(pcp_ctx *c) {
@@ -862,14 +884,23 @@ static int coll_collect(pcp_ctx *c, V ve, V vi, long n_e, long n_i, double m_e, 
                      b->feat, b->ok);
   coll_binary(c, COLL_P12, ve, vi, m_e, m_i, q_e, q_i, seed, step, nullptr);
   surr_moments(c, ve, vi, b->mom2);
-  unsigned long long n0 = (unsigned long long)m->n;
-  if (hipMemcpyAsync(b->counter + 1, &n0, sizeof n0, hipMemcpyHostToDevice, c->stream) != hipSuccess) return -1;
+  // rows in cell order (the bucket arrays of species 0 are free again):
+  // flags, their exclusive sum, then the rows at their offsets
+  const long n0 = m->n;
+  unsigned gf = (unsigned)std::min<long>((ncells + 256) / 256, (long)c->n_cu * 16);
+  hipLaunchKernelGGL(k_surr_rowflag, dim3(gf), dim3(256), 0, c->stream, c->g, b->mom, b->mom2, m_e, m_i, b->ok,
+                     b->row_bad, b->cnt[0]);
+  size_t bytes = b->scan_bytes;
+  if (hipcub::DeviceScan::ExclusiveSum(b->scan_tmp, bytes, b->cnt[0], b->off[0], (int)(ncells + 1), c->stream) !=
+      hipSuccess)
+    return -1;
   hipLaunchKernelGGL(k_surr_targets, dim3(gt), dim3(256), 0, c->stream, c->g, b->mom, b->mom2, m_e, m_i, b->ok,
-                     b->row_bad, b->feat, m->X, m->T, b->counter + 1, m->cap);
-  unsigned long long n1 = 0;
-  if (hipMemcpyAsync(&n1, b->counter + 1, sizeof n1, hipMemcpyDeviceToHost, c->stream) != hipSuccess) return -1;
+                     b->row_bad, b->feat, b->off[0], n0, m->X, m->T, m->cap);
+  uint32_t rows = 0;
+  if (hipMemcpyAsync(&rows, b->off[0] + ncells, sizeof rows, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+    return -1;
   if (hipStreamSynchronize(c->stream) != hipSuccess) return -1;
-  m->n = std::min<long>((long)n1, m->cap);
+  m->n = std::min<long>(n0 + (long)rows, m->cap);
   return 0;
 }
 

@@ -81,6 +81,7 @@ struct Geom {
   int bin_r, bin_z;  // sort bin (cells; divide nr and nz)
   FastDiv div_nr, div_nz, div_Dz, div_Nz;
   double inv_vol0;   // 1 / (pi dz dr^2): CELL_VOLUME(i) = (2 i + 1) / inv_vol0
+  float dtr_f, dtz_f, inv_dt_f;   // dt / dr, dt / dz, 1 / dt rounded to FP32
 };
 
 // Per-domain values derived from the domain index (src/SMB.cpp:49-121).

@@ -499,7 +499,12 @@ int mlp_fit(pcp_ctx *c, int epochs, long batch, float lr, uint64_t seed, float *
     m->feat_mean[k] = nm.mean[k] = (float)mean;
     m->feat_std[k] = nm.std[k] = (float)std::sqrt(std::max(var, 1e-8));
   }
-  for (int k = 0; k < MLP_OUT; ++k) m->targ_scale[k] = nm.tscale[k] = (float)std::sqrt(std::max(h[2 * MLP_IN + k] / n, 1e-20));
+  // one scale for all targets (the largest rms): an output that is mostly
+  // cell-level noise (the ions' relative changes) then weighs little in the
+  // loss instead of competing with the electrons' for the hidden units
+  double tmax = 1e-20;
+  for (int k = 0; k < MLP_OUT; ++k) tmax = std::max(tmax, h[2 * MLP_IN + k] / n);
+  for (int k = 0; k < MLP_OUT; ++k) m->targ_scale[k] = nm.tscale[k] = (float)std::sqrt(tmax);
   hipLaunchKernelGGL(k_mlp_normalise, dim3(g), dim3(256), 0, c->stream, m->X, m->T, n, nm, m->Xn, m->Tn);
   // mini-batch Adam over shuffled passes
   const long nb = (n + batch - 1) / batch;

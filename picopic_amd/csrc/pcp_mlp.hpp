@@ -70,7 +70,8 @@ void mlp_grad_idx(pcp_ctx *c, const float *X, const float *T, const long *idx, l
 int mlp_init(pcp_ctx *c, uint64_t seed);
 // room for `rows` more training rows
 int mlp_reserve(pcp_ctx *c, long rows);
-// normalise the training set (feature mean / std, target rms), then
+// normalise the training set (feature mean / std, one target scale: the
+// largest rms of the four outputs), then
 // `epochs` passes of shuffled mini-batches with Adam, the step size
 // annealed from lr to lr / 20 along a cosine; *loss = mean batch loss of the
 // last pass (normalised units)

@@ -76,6 +76,28 @@ def test_vb_discrete_continuity(r_range):
     assert worst < 1e-13, worst
 
 
+def test_vb_path_through_a_cell_corner_deposits_nothing():
+    """The reference's ten-node case splits a path that crosses both cell
+    boundaries at the crossing of r = i dr, z1, and branches on z1 < k dz or
+    z1 > k dz (currentVB.cpp:298-309 and siblings); a path whose z1 equals
+    k dz exactly deposits no r-z current.  The same move shifted off the
+    corner deposits and conserves charge."""
+    nr = nz = 10
+    dr = dz = 1e-3
+    out = {}
+    # with these roundings the crossing of r = 4 dr lands exactly on z = 8 dz
+    for name, (r, z, vz) in {"corner": (4 * dr - 0.1 * dr, 8 * dz + 0.1 * dz, -1.0e8),
+                             "off": (4 * dr - 0.1 * dr, 8 * dz + 0.05 * dz, -0.8e8)}.items():
+        smb, dt = one_particle_smb(r, z, np.array([1.0e8, 0.0, vz]), nr, nz, dr, dz)
+        for ph in ("push", "move", "reset_j", "deposit"):
+            smb.phase(ph)
+        p = smb.particles(0)
+        assert p["r"][0] > 4 * dr and p["z"][0] < 8 * dz   # both boundaries crossed
+        out[name] = np.abs(smb.get_grid("J")[0][[0, 2]]).max()
+    assert out["corner"] == 0.0
+    assert out["off"] > 0.0
+
+
 def test_vb_azimuthal_current_matches_charge_times_velocity():
     """The azimuthal deposit weights the charge density of the mid-position
     ring with the cell volumes of CELL_VOLUME(i) and CELL_VOLUME(i+1) (= 2i+2,
