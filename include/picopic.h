@@ -200,6 +200,15 @@ int pcp_surrogate_get_norm(pcp_ctx *ctx, float *out16);
 int pcp_surrogate_set_norm(pcp_ctx *ctx, const float *in16);
 /* cells of the last surrogate pass that took the P12 fallback */
 int pcp_surrogate_fallback_cells(pcp_ctx *ctx, long *cells);
+/* A posteriori calibration of a fitted network (no reference counterpart):
+ * the relaxation rate of the electron anisotropy over `passes` passes is
+ * measured for P12 and for the surrogate on scratch contexts of this
+ * context's configuration holding copies of its two species (as much
+ * device memory again), and the electron target scales are multiplied by
+ * the factor (*factor) that makes the surrogate's rate P12's (two
+ * fixed-point iterations).  The context's own particles are not changed.
+ * The plasma must keep T_par > T_perp for the electrons over the passes. */
+int pcp_surrogate_calibrate(pcp_ctx *ctx, int sp_el, int sp_ion, uint64_t seed, int passes, double *factor);
 
 /* Replay each step from a captured hipGraph (no reference counterpart).
  * Steps with a collision phase, timed steps (pcp_set_timing) and the sorts

@@ -30,7 +30,7 @@ EXPORTS = [
     "pcp_mlp_set_params", "pcp_mlp_forward", "pcp_mlp_grad", "pcp_mlp_adam", "pcp_surrogate_collect",
     "pcp_surrogate_fit", "pcp_surrogate_get_norm", "pcp_surrogate_set_norm", "pcp_surrogate_fallback_cells",
     "pcp_species_moments", "pcp_continuity_residual", "pcp_set_graph", "pcp_set_sort_bin", "pcp_set_launch",
-    "pcp_get_tuning", "pcp_autotune",
+    "pcp_get_tuning", "pcp_autotune", "pcp_surrogate_calibrate",
 ]
 COLL = {"p12": 0, "ta77s": 1, "sk98": 2, "surrogate": 3}
 
@@ -116,6 +116,7 @@ def load(path=LIB_PATH):
     L.pcp_surrogate_get_norm.argtypes = [vp, fp]
     L.pcp_surrogate_set_norm.argtypes = [vp, fp]
     L.pcp_surrogate_fallback_cells.argtypes = [vp, C.POINTER(C.c_long)]
+    L.pcp_surrogate_calibrate.argtypes = [vp, C.c_int, C.c_int, C.c_uint64, C.c_int, C.POINTER(C.c_double)]
     L.pcp_species_moments.argtypes = [vp, C.c_int, dp]
     L.pcp_continuity_residual.argtypes = [vp, dp]
     L.pcp_set_graph.argtypes = [vp, C.c_int]
@@ -346,6 +347,14 @@ class Context:
     def surrogate_set_norm(self, v):
         v = np.ascontiguousarray(v, dtype=np.float32)
         _check(_lib.pcp_surrogate_set_norm(self.h, _fp(v)))
+
+    def surrogate_calibrate(self, se, si, seed, passes):
+        """Scale the fitted network's electron outputs so that the electron
+        anisotropy relaxes at P12's rate over `passes` passes on this
+        context's plasma (run on scratch copies); returns the factor."""
+        f = C.c_double()
+        _check(_lib.pcp_surrogate_calibrate(self.h, se, si, seed, passes, C.byref(f)))
+        return f.value
 
     def surrogate_fallback_cells(self):
         n = C.c_long()

@@ -165,6 +165,11 @@ int pcp_create(const pcp_config *cfg, pcp_ctx **out) {
   g.xr = g.nr + 4;
   g.yr = g.nz + 4;
   g.P = (long)g.xr * g.yr;
+  if (cfg->precision == PCP_PREC_FP32_SPLIT && 3L * (long)cfg->domains_r * cfg->domains_z * g.P * 4 > 0xFFFFFFFFL) {
+    delete c;
+    return fail(PCP_E_ARG, "FP32 mode: each field array (3 components of every domain, halos included) must stay "
+                           "below 4 GiB (32-bit byte offsets)");
+  }
   g.Nr = cfg->grid_r;
   g.Nz = cfg->grid_z;
   // domain geometry exactly as SMB::SMB builds it (src/SMB.cpp:96-103,
@@ -878,6 +883,96 @@ int pcp_surrogate_set_norm(pcp_ctx *c, const float *in) {
   }
   for (int k = 0; k < MLP_OUT; ++k) m->targ_scale[k] = in[2 * MLP_IN + k];
   m->trained = 1;
+  return 0;
+}
+
+// relaxation rate of the electrons' anisotropy <dvz^2> - <dvr^2 + dvphi^2> / 2
+// (proportional to T_par - T_perp) over `passes` collision passes: minus
+// the least-squares slope of its logarithm per pass
+static int anisotropy_rate(pcp_ctx *c, int scheme, int se, int si, uint64_t seed, int passes, double *rate) {
+  double sx = 0, sy = 0, sxx = 0, sxy = 0;
+  for (int k = 0; k <= passes; ++k) {
+    double m[7];
+    if (int rc = pcp_species_moments(c, se, m)) return rc;
+    const double sw = m[0], vr = m[1] / sw, vp = m[2] / sw, vz = m[3] / sw;
+    const double a = (m[4] / sw - vz * vz) - 0.5 * (m[5] / sw - vr * vr - vp * vp);
+    if (!(a > 0) || !std::isfinite(a))
+      return fail(PCP_E_STATE, "calibration needs an electron T_par above T_perp throughout the passes");
+    const double x = k, y = std::log(a);
+    sx += x; sy += y; sxx += x * x; sxy += x * y;
+    if (k < passes)
+      if (int rc = pcp_collide(c, scheme, se, si, seed, (uint64_t)k)) return rc;
+  }
+  const double n = passes + 1;
+  *rate = -(n * sxy - sx * sy) / (n * sxx - sx * sx);
+  return 0;
+}
+
+namespace {
+struct SpeciesCopy {
+  std::vector<double> r, z, vr, vp, vz, w;
+  std::vector<uint64_t> id;
+};
+}  // namespace
+
+// One run of `passes` passes of `scheme` on a scratch context of c's
+// configuration holding copies of species se and si; for the surrogate the
+// scratch network is c's with the electron target scales times `fac`.
+static int scratch_rate(pcp_ctx *c, const SpeciesCopy cp[2], int scheme, int se, int si, uint64_t seed, int passes,
+                        const std::vector<float> &params, const float norm[16], double fac, double *rate) {
+  pcp_ctx *t = nullptr;
+  if (int rc = pcp_create(&c->cfg, &t)) return rc;
+  int rc = 0;
+  const int sp[2] = {se, si};
+  for (int j = 0; j < 2 && !rc; ++j)
+    rc = pcp_upload_species(t, sp[j], (long)cp[j].r.size(), cp[j].r.data(), cp[j].z.data(), cp[j].vr.data(),
+                            cp[j].vp.data(), cp[j].vz.data(), cp[j].w.data(), cp[j].id.data());
+  if (!rc && scheme == PCP_COLL_SURROGATE) {
+    float nm[16];
+    std::copy(norm, norm + 16, nm);
+    nm[2 * MLP_IN + 0] = (float)(nm[2 * MLP_IN + 0] * fac);
+    nm[2 * MLP_IN + 1] = (float)(nm[2 * MLP_IN + 1] * fac);
+    rc = pcp_mlp_set_params(t, params.data());
+    if (!rc) rc = pcp_surrogate_set_norm(t, nm);
+  }
+  if (!rc) rc = anisotropy_rate(t, scheme, se, si, seed, passes, rate);
+  (void)pcp_destroy(t);
+  return rc;
+}
+
+int pcp_surrogate_calibrate(pcp_ctx *c, int se, int si, uint64_t seed, int passes, double *factor) {
+  if (int rc = check_coll_args(c, PCP_COLL_SURROGATE, se, si)) return rc;
+  if (passes < 2) return fail(PCP_E_ARG, "calibration needs at least 2 passes");
+  // copies of the two species as they are now
+  SpeciesCopy cp[2];
+  const int sp[2] = {se, si};
+  for (int j = 0; j < 2; ++j) {
+    long n = 0;
+    if (int rc = pcp_count(c, sp[j], &n)) return rc;
+    if (n == 0) return fail(PCP_E_STATE, "calibration needs particles of both species");
+    SpeciesCopy &q = cp[j];
+    q.r.resize(n); q.z.resize(n); q.vr.resize(n); q.vp.resize(n); q.vz.resize(n); q.w.resize(n); q.id.resize(n);
+    if (int rc = pcp_download_species(c, sp[j], q.r.data(), q.z.data(), q.vr.data(), q.vp.data(), q.vz.data(),
+                                      q.w.data(), q.id.data(), nullptr, nullptr, nullptr))
+      return rc;
+  }
+  std::vector<float> params(MLP_NPARAM);
+  float norm[16];
+  if (int rc = pcp_mlp_get_params(c, params.data())) return rc;
+  if (int rc = pcp_surrogate_get_norm(c, norm)) return rc;
+  double r_ref = 0, fac = 1.0;
+  if (int rc = scratch_rate(c, cp, PCP_COLL_P12, se, si, seed, passes, params, norm, 1.0, &r_ref)) return rc;
+  // the rate is close to proportional to the electron output scale: two
+  // fixed-point iterations
+  for (int it = 0; it < 2; ++it) {
+    double r = 0;
+    if (int rc = scratch_rate(c, cp, PCP_COLL_SURROGATE, se, si, seed, passes, params, norm, fac, &r)) return rc;
+    if (!(r > 0)) return fail(PCP_E_STATE, "the surrogate does not relax the anisotropy: no calibration possible");
+    fac *= r_ref / r;
+  }
+  c->mlp->targ_scale[0] = (float)(c->mlp->targ_scale[0] * fac);
+  c->mlp->targ_scale[1] = (float)(c->mlp->targ_scale[1] * fac);
+  if (factor) *factor = fac;
   return 0;
 }
 
