@@ -510,8 +510,9 @@ __device__ __forceinline__ void deposit32(Sink &S, bool live, const Geom &g, con
 // PUSH: velocities slot cur -> pushed, moved -> slot cur ^ 1.
 // !PUSH: pushed (and collided) velocities of slot cur ^ 1 -> moved -> slot
 // cur; the caller then exchanges the two velocity slots.
-// One block trip of the fused step: particle p of this thread (the threads
-// of a block take consecutive particles).
+// One wavefront trip of the fused step: particle p of this lane (the lanes
+// of a wavefront take consecutive particles; all 64 lanes take part, live
+// or not, since the deposit's run sums move data between lanes).
 template <int CCS, bool PUSH>
 __device__ __forceinline__ void step_trip32(const Geom &g, const FieldsT<float> &f, const Species32 &s, int sp,
                                             const PushConst &K, long p, int cur, StepState *st, const Migrants &mg,
@@ -554,30 +555,34 @@ __device__ __forceinline__ void step_trip32(const Geom &g, const FieldsT<float> 
                  s.seq, p, sp);
 }
 
-// The fused step of every species in one persistent launch: a block takes
-// the next trip (BLOCK consecutive particles of one species) from a counter
-// in StepState until the trips of all species are handed out, so that trips
-// of uneven cost (dead slots, domains of different density, strict motions,
-// walls) balance over the resident blocks.
+// The fused step of every species in one persistent launch.  A trip is one
+// wavefront of 64 consecutive particles of one species; each wavefront
+// takes `chunk` consecutive trips per increment of a counter in StepState
+// until the trips of all species are handed out, so that trips of uneven
+// cost (dead slots, domains of different density, strict motions, walls)
+// balance over the resident wavefronts.  The wavefronts never synchronise:
+// a block-wide ticket and its barrier made every wavefront wait for its
+// outstanding J atomics once per trip (1.6x slower on the wakefield case),
+// and a counter increment per trip serialises on the counter.
 template <int CCS, bool PUSH, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k32_step_batch(Geom g, FieldsT<float> f, Batch32 b, int cur, StepState *st,
-                                                        Migrants mg) {
-  __shared__ unsigned ticket;
+                                                        Migrants mg, int chunk) {
   int err = 0;
-  const long total = b.end[b.nsp - 1];   // trips of BLOCK particles
+  const long total = b.end[b.nsp - 1];   // trips of 64 particles
+  const int lane = threadIdx.x & 63;
   for (;;) {
-    // one counter increment per block trip (a counter per wavefront trip
-    // serialises on the atomic: 4x slower on the wakefield case)
-    if (threadIdx.x == 0) ticket = atomicAdd(&st->queue, 1u);
-    __syncthreads();
-    const long t = (long)(unsigned)__builtin_amdgcn_readfirstlane((int)ticket);
-    __syncthreads();
-    if (t >= total) break;
-    int k = 0;
-    while (k + 1 < b.nsp && t >= b.end[k]) ++k;
-    const long first = k ? b.end[k - 1] : 0;
-    step_trip32<CCS, PUSH>(g, f, b.s[k], b.sp[k], b.K[k], (t - first) * BLOCK + threadIdx.x, cur, st, mg, b.marker,
-                           b.cflag, err);
+    unsigned tk = 0;
+    if (lane == 0) tk = atomicAdd(&st->queue, 1u);
+    const long t0 = (long)(unsigned)__builtin_amdgcn_readfirstlane((int)tk) * chunk;
+    if (t0 >= total) break;
+    const long t1 = t0 + chunk < total ? t0 + chunk : total;
+    for (long t = t0; t < t1; ++t) {
+      int k = 0;
+      while (k + 1 < b.nsp && t >= b.end[k]) ++k;
+      const long first = k ? b.end[k - 1] : 0;
+      step_trip32<CCS, PUSH>(g, f, b.s[k], b.sp[k], b.K[k], (t - first) * 64 + lane, cur, st, mg, b.marker, b.cflag,
+                             err);
+    }
   }
   if (err) atomicOr(&st->err, err);
 }
@@ -935,6 +940,9 @@ void launch32_push(pcp_ctx *c, int sp) {
                      push_const(s, c->g.dt));
 }
 
+// consecutive wavefront trips per counter increment of the batched step
+constexpr int TRIPS_PER_TICKET = 16;
+
 template <int CCS, bool PUSH, int BLOCK>
 static void launch_batch_t(pcp_ctx *c, const Batch32 &b) {
   static int occ = 0;   // resident blocks per CU of this instance
@@ -942,9 +950,11 @@ static void launch_batch_t(pcp_ctx *c, const Batch32 &b) {
                    hipSuccess || occ < 1))
     occ = 1;
   const int per_cu = c->tune.blocks_per_cu > 0 ? c->tune.blocks_per_cu : occ;
-  const long grid = std::max(1L, std::min(b.end[b.nsp - 1], (long)c->n_cu * per_cu));
+  const int chunk = TRIPS_PER_TICKET;
+  const long waves = (b.end[b.nsp - 1] + chunk - 1) / chunk;
+  const long grid = std::max(1L, std::min((waves + BLOCK / 64 - 1) / (BLOCK / 64), (long)c->n_cu * per_cu));
   hipLaunchKernelGGL((k32_step_batch<CCS, PUSH, BLOCK>), dim3((unsigned)grid), dim3(BLOCK), 0, c->stream, c->g,
-                     c->s32->f, b, c->cur, c->st, c->mig);
+                     c->s32->f, b, c->cur, c->st, c->mig, chunk);
 }
 
 template <int CCS>
@@ -958,11 +968,11 @@ static void launch_batch_ccs(pcp_ctx *c, const Batch32 &b, bool push, int block)
 void launch32_step_batch(pcp_ctx *c, bool push) {
   Batch32 b{};
   const int block = push ? c->tune.block : 256;
-  long acc = 0;
+  long acc = 0;   // trips of 64 particles
   for (int sp = 0; sp < c->cfg.n_species; ++sp) {
     const Species32 &s = c->s32->sp[sp];
     if (!s.n) continue;
-    acc += (s.n + block - 1) / block;
+    acc += (s.n + 63) / 64;
     b.s[b.nsp] = s;
     b.K[b.nsp] = push_const(s, c->g.dt);
     b.end[b.nsp] = acc;
