@@ -515,18 +515,31 @@ __global__ void __launch_bounds__(256) k_fix(Geom g, Fields f, SpeciesDev s, int
   int err = 0;
   const uint64_t stamp = st->step & 0xFFFFFF;
   const uint32_t fval = (uint32_t)st->step;
-  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < s.n; p += (long)gridDim.x * blockDim.x) {
-    const double r = s.r[cur][p], ro = s.r[cur ^ 1][p];
-    if (r < 0 || ro < 0) continue;
-    const double z = s.z[cur][p], zo = s.z[cur ^ 1][p];
-    const long ci = min(max((long)cell_number(ro, g.dr), 0L), g.Nr - 1);
-    const long ck = min(max((long)cell_number(zo, g.dz), 0L), g.Nz - 1);
-    if (cflag[ci * g.Nz + ck] != fval) continue;
-    const int d = domain_of(g, cell_number(r, g.dr), cell_number(z, g.dz));
-    const Dom dm = make_dom(g, d);
-    FixSink<double> S{f.J + (long)d * 3 * g.P, marker + (long)d * g.P, g.yr, g.nr, g.nz, stamp,
-                      order_key(sp, s.seq[p])};
-    vb_rz(S, g.dr, g.dz, g.dt, dm.bot_r, dm.left_z, r, z, ro, zo, s.q * s.w[p], &err);
+  // a thread screens four consecutive particles by their pre-step cell
+  // (independent loads) and redoes the few in flagged cells
+  const long groups = (s.n + 3) / 4;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < groups; t += (long)gridDim.x * blockDim.x) {
+    bool hit[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const long p = t * 4 + j;
+      const double ro = p < s.n ? s.r[cur ^ 1][p] : -1.0, zo = p < s.n ? s.z[cur ^ 1][p] : 0.0;
+      const long ci = min(max((long)cell_number(ro, g.dr), 0L), g.Nr - 1);
+      const long ck = min(max((long)cell_number(zo, g.dz), 0L), g.Nz - 1);
+      hit[j] = ro >= 0 && cflag[ci * g.Nz + ck] == fval;
+    }
+    for (int j = 0; j < 4; ++j) {
+      if (!hit[j]) continue;
+      const long p = t * 4 + j;
+      const double r = s.r[cur][p], ro = s.r[cur ^ 1][p];
+      if (r < 0) continue;
+      const double z = s.z[cur][p], zo = s.z[cur ^ 1][p];
+      const int d = domain_of(g, cell_number(r, g.dr), cell_number(z, g.dz));
+      const Dom dm = make_dom(g, d);
+      FixSink<double> S{f.J + (long)d * 3 * g.P, marker + (long)d * g.P, g.yr, g.nr, g.nz, stamp,
+                        order_key(sp, s.seq[p])};
+      vb_rz(S, g.dr, g.dz, g.dt, dm.bot_r, dm.left_z, r, z, ro, zo, s.q * s.w[p], &err);
+    }
   }
 }
 
@@ -711,7 +724,7 @@ void launch_deposit_exact(pcp_ctx *c, int sp, int only_if_exact) {
 void launch_fix(pcp_ctx *c, int sp) {
   SpeciesDev &s = c->sp[sp];
   if (!s.n || c->g.ccs != PCP_CCS_VB) return;
-  hipLaunchKernelGGL(k_fix, dim3(pgrid(c, s.n, 256)), dim3(256), 0, c->stream, c->g, c->f, s, sp, c->cur, c->st,
+  hipLaunchKernelGGL(k_fix, dim3(pgrid(c, (s.n + 3) / 4, 256)), dim3(256), 0, c->stream, c->g, c->f, s, sp, c->cur, c->st,
                      c->marker, c->cell_flag);
 }
 void launch_migrant_keys_t(pcp_ctx *c, SeqTable t) {

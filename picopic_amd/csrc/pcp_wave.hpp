@@ -113,6 +113,24 @@ __device__ __forceinline__ double run_step(double x, int m) {
   return x + __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+template <int CTRL>
+__device__ __forceinline__ float run_step_f(float x, int m) {
+  return x + __int_as_float(dpp_mov<CTRL>(__float_as_int(x)) & m);
+}
+
+// run_sum in FP32 (the FP32 mode's currents): one DPP-and plus one add per
+// step; the scan's rounding error over a run of up to 64 terms stays below
+// 6 x 2^-24 of the run's magnitude
+__device__ __forceinline__ float run_sum_f(float x, const Runs &R) {
+  x = run_step_f<0x111>(x, R.m[0]);   // row_shr:1
+  x = run_step_f<0x112>(x, R.m[1]);   // row_shr:2
+  x = run_step_f<0x114>(x, R.m[2]);   // row_shr:4
+  x = run_step_f<0x118>(x, R.m[3]);   // row_shr:8
+  x = run_step_f<0x142>(x, R.m[4]);   // row_bcast:15
+  x = run_step_f<0x143>(x, R.m[5]);   // row_bcast:31
+  return x;
+}
+
 // partial sums of x along the runs; the last lane of a run holds its total
 __device__ __forceinline__ double run_sum(double x, const Runs &R) {
   x = run_step<0x111>(x, R.m[0]);   // row_shr:1
