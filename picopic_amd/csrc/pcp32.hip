@@ -101,6 +101,7 @@ struct GridView32 {
 
 // E (maxwellSolver.hpp:211-316) in cell units
 __device__ __forceinline__ void gather32_e(const GridView32 &E, int cr, int cz, float fr, float fz, float out[3]) {
+#pragma clang fp contract(fast)
   W2 ra = rad_half(cr, fr), ri = rad_int(cr, fr);
   Z2 zi = z_int(cz, fz), zh = z_half(cz, fz);
   float c0 = 0.f, c1 = 0.f, c2 = 0.f;
@@ -124,6 +125,7 @@ __device__ __forceinline__ void gather32_e(const GridView32 &E, int cr, int cz, 
 
 // H (maxwellSolver.hpp:101-209) in cell units
 __device__ __forceinline__ void gather32_h(const GridView32 &H, int cr, int cz, float fr, float fz, float out[3]) {
+#pragma clang fp contract(fast)
   W2 ra = rad_half(cr, fr), ri = rad_int(cr, fr);
   Z2 zi = z_int(cz, fz), zh = z_half(cz, fz);
   float c0 = 0.f, c1 = 0.f, c2 = 0.f;
@@ -358,7 +360,6 @@ __device__ __forceinline__ ZZTermsT<float> zigzag32(const Geom &g, const Dom &dn
   const float one_over_volume = (float)g.inv_vol0 * rcp_f((float)(2 * i_middle + 1));
   const float charge_over_dt = qw * inv_dt;
   const float F_phi = qw * vphi;
-  const float W_phi = vphi * (float)g.dt * rcp_f(2.f * (float)PI * dr * ((float)(2 * o.cr) + uo + un));
   const float F_r1 = charge_over_dt * dr * (rr - uo);
   const float F_r2 = charge_over_dt * dr * (un - rr);
   const float F_z1 = charge_over_dt * dz * (rz - wo);
@@ -368,24 +369,28 @@ __device__ __forceinline__ ZZTermsT<float> zigzag32(const Geom &g, const Dom &dn
   const float W_r2 = dr * (0.5f * (un + rr) - (float)di);
   const float W_z1 = dz * (0.5f * (wo + rz));
   const float W_z2 = dz * (0.5f * (wn + rz) - (float)dk);
-  const float mWr1 = 1.f - W_r1, mWr2 = 1.f - W_r2, mWz1 = 1.f - W_z1, mWz2 = 1.f - W_z2, mWphi = 1.f - W_phi;
+  const float mWr1 = 1.f - W_r1, mWr2 = 1.f - W_r2, mWz1 = 1.f - W_z1, mWz2 = 1.f - W_z2;
   const int io = o.cr - (int)dn.bot_r, ko = o.cz - (int)dn.left_z;
   const int in = n.cr - (int)dn.bot_r, kn = n.cz - (int)dn.left_z;
   const float A = one_over_volume;
   ZZTermsT<float> Z;
   Z.io = io; Z.ko = ko; Z.in = in; Z.kn = kn;
-  Z.v[0] = A * F_r1 * mWphi * mWz1 + A * F_r1 * W_phi * mWz1;
-  Z.v[1] = A * F_r1 * mWphi * W_z1 + A * F_r1 * W_phi * W_z1;
-  Z.v[2] = A * F_r2 * mWphi * mWz2 + A * F_r2 * W_phi * mWz2;
-  Z.v[3] = A * F_r2 * mWphi * W_z2 + A * F_r2 * W_phi * W_z2;
-  Z.v[4] = A * F_phi * mWr2 * mWz2;
-  Z.v[5] = A * F_phi * mWr2 * W_z1;   // uses W_z1 (currentZigZag.cpp:104)
-  Z.v[6] = A * F_phi * W_r2 * mWz2;
-  Z.v[7] = A * F_phi * W_r2 * W_z2;
-  Z.v[8] = A * F_z1 * mWr1 * mWphi + A * F_z1 * mWr1 * W_phi;
-  Z.v[9] = A * F_z1 * W_r1 * mWphi + A * F_z1 * W_r1 * W_phi;
-  Z.v[10] = A * F_z2 * mWr2 * mWphi + A * F_z2 * mWr2 * W_phi;
-  Z.v[11] = A * F_z2 * W_r2 * mWphi + A * F_z2 * W_r2 * W_phi;
+  // the reference's two azimuthal halves of each J_r and J_z term (weights
+  // 1 - W_phi and W_phi) land on the same r-z node; their sum is the term
+  // without the azimuthal weight
+  const float Ar1 = A * F_r1, Ar2 = A * F_r2, Az1 = A * F_z1, Az2 = A * F_z2, Aphi = A * F_phi;
+  Z.v[0] = Ar1 * mWz1;
+  Z.v[1] = Ar1 * W_z1;
+  Z.v[2] = Ar2 * mWz2;
+  Z.v[3] = Ar2 * W_z2;
+  Z.v[4] = Aphi * mWr2 * mWz2;
+  Z.v[5] = Aphi * mWr2 * W_z1;   // uses W_z1 (currentZigZag.cpp:104)
+  Z.v[6] = Aphi * W_r2 * mWz2;
+  Z.v[7] = Aphi * W_r2 * W_z2;
+  Z.v[8] = Az1 * mWr1;
+  Z.v[9] = Az1 * W_r1;
+  Z.v[10] = Az2 * mWr2;
+  Z.v[11] = Az2 * W_r2;
   return Z;
 }
 
