@@ -272,6 +272,9 @@ __device__ __forceinline__ void renorm(int &c, float &f) {
 // stored offset rounds displacements below 2^-24 of a cell away, and a
 // rounded-away radial step would turn an ordinary move into the reference's
 // strict axial motion (which resets J_r nodes).
+// EXACT_END = false (ZigZag, which deposits from the split positions): the
+// FP64 end point is only formed for a wall reflection, from the FP32 offset.
+template <bool EXACT_END = true>
 __device__ __forceinline__ bool move32(const Geom &g, const Dom &o, int beam, float dtr, float dtz, Split &p, float v[3],
                                        double &rd, double &zd) {
   const Split old = p;
@@ -298,14 +301,20 @@ __device__ __forceinline__ bool move32(const Geom &g, const Dom &o, int beam, fl
   }
   p.fr = xoff;
   p.fz = p.fz + dw;
-  rd = ur * g.dr;
-  zd = ((double)old.cz + (double)old.fz + (double)dw) * g.dz;
+  if (EXACT_END) {
+    rd = ur * g.dr;
+    zd = ((double)old.cz + (double)old.fz + (double)dw) * g.dz;
+  }
   // walls (local cell units): r < 1/2 or > nr - 1/2, z < 1/2 or > nz - 1/2
   const float lr = (float)(p.cr - (int)o.bot_r) + p.fr, lz = (float)(p.cz - (int)o.left_z) + p.fz;
   const bool hit = beam ? (o.w0 && lr < 0.5f)
                         : ((o.w0 && lr < 0.5f) || (o.w2 && lr > g.nr - 0.5f) || (o.w1 && lz < 0.5f) ||
                            (o.w3 && lz > g.nz - 0.5f));
   if (hit) {
+    if (!EXACT_END) {
+      rd = ((double)p.cr + (double)p.fr) * g.dr;
+      zd = ((double)p.cz + (double)p.fz) * g.dz;
+    }
     double por = ((double)old.cr + (double)old.fr) * g.dr, poz = ((double)old.cz + (double)old.fz) * g.dz;
     double vr = v[0], vz = v[2];
     if (beam) reflect_beam(g, o, rd, vr);
@@ -399,6 +408,22 @@ __device__ __forceinline__ int domain_of32(const Geom &g, int cr, int cz) {
   return (int)g.div_nr.div((uint32_t)cr) * g.Dz + (int)g.div_nz.div((uint32_t)cz);
 }
 
+// domain of a cell and its description; D1: the context has one domain
+// (the domain is 0, its walls are the area's)
+template <bool D1>
+__device__ __forceinline__ int domain_of32t(const Geom &g, int cr, int cz) {
+  return D1 ? 0 : domain_of32(g, cr, cz);
+}
+template <bool D1>
+__device__ __forceinline__ Dom make_dom_t(const Geom &g, int d) {
+  if (!D1) return make_dom(g, d);
+  Dom o;
+  o.I = o.J = 0;
+  o.bot_r = o.left_z = 0;
+  o.w0 = o.w1 = o.w2 = o.w3 = 1;
+  return o;
+}
+
 __device__ __forceinline__ Split unpack(uint32_t cell, float fr, float fz, const Geom &g) {
   Split s;
   s.cr = (int)g.div_Nz.div(cell);
@@ -429,13 +454,14 @@ __device__ __forceinline__ void append_migrant32(Migrants mg, StepState *st, int
 // velocity (steps whose collision phase sits between push and move).  The
 // exact re-deposition calls it again on the pre-step slots and gets
 // bit-identical (n, v, rd, zd).
+template <bool EXACT_END = true, bool D1 = false>
 __device__ __forceinline__ bool advance32(const Geom &g, const FieldsT<float> &f, const Species32 &s, long p,
                                           uint32_t cell, int from, int vfrom, bool do_push, const PushConst &K,
                                           float dtr, float dtz, Split &o, Split &n, float v[3], double &rd,
                                           double &zd, int &d_old, int *err) {
   o = unpack(cell, s.fr[from][p], s.fz[from][p], g);
-  d_old = domain_of32(g, o.cr, o.cz);
-  const Dom dm = make_dom(g, d_old);
+  d_old = domain_of32t<D1>(g, o.cr, o.cz);
+  const Dom dm = make_dom_t<D1>(g, d_old);
   v[0] = s.vr[vfrom][p]; v[1] = s.vphi[vfrom][p]; v[2] = s.vz[vfrom][p];
   if (do_push) {
     GridView32 EV{f.E + (long)d_old * 3 * g.P, g.P, g.yr, dm.bot_r, dm.left_z};
@@ -446,7 +472,7 @@ __device__ __forceinline__ bool advance32(const Geom &g, const FieldsT<float> &f
     push32(g.pusher, K, e, h, v, err);
   }
   n = o;
-  return move32(g, dm, s.beam, dtr, dtz, n, v, rd, zd);
+  return move32<EXACT_END>(g, dm, s.beam, dtr, dtz, n, v, rd, zd);
 }
 
 // gather + push only (steps with a collision phase): velocities of slot cur
@@ -518,7 +544,7 @@ __device__ __forceinline__ void deposit32(Sink &S, bool live, const Geom &g, con
 // One wavefront trip of the fused step: particle p of this lane (the lanes
 // of a wavefront take consecutive particles; all 64 lanes take part, live
 // or not, since the deposit's run sums move data between lanes).
-template <int CCS, bool PUSH>
+template <int CCS, bool PUSH, bool D1>
 __device__ __forceinline__ void step_trip32(const Geom &g, const FieldsT<float> &f, const Species32 &s, int sp,
                                             const PushConst &K, long p, int cur, StepState *st, const Migrants &mg,
                                             uint64_t *marker, uint32_t *cflag, int &err) {
@@ -536,13 +562,14 @@ __device__ __forceinline__ void step_trip32(const Geom &g, const FieldsT<float> 
     int d_old = 0;
     const int vin = PUSH ? cur : nxt, vout = PUSH ? nxt : cur;
     if (cell != DEAD_CELL &&
-        advance32(g, f, s, p, cell, cur, vin, PUSH, K, dtr, dtz, o, n, v, rd, zd, d_old, &err)) {
+        advance32<CCS != PCP_CCS_ZIGZAG, D1>(g, f, s, p, cell, cur, vin, PUSH, K, dtr, dtz, o, n, v, rd, zd, d_old,
+                                             &err)) {
       s.vr[vout][p] = v[0]; s.vphi[vout][p] = v[1]; s.vz[vout][p] = v[2];
       s.cell[nxt][p] = (uint32_t)n.cr * (uint32_t)g.Nz + (uint32_t)n.cz;
       s.fr[nxt][p] = n.fr;
       s.fz[nxt][p] = n.fz;
-      d_new = domain_of32(g, n.cr, n.cz);
-      if (d_new != d_old) append_migrant32(mg, st, sp, p, d_old, d_new, s.seq[p], g, &err);
+      d_new = domain_of32t<D1>(g, n.cr, n.cz);
+      if (!D1 && d_new != d_old) append_migrant32(mg, st, sp, p, d_old, d_new, s.seq[p], g, &err);
       w = s.w[p];
       live = true;
     } else {
@@ -553,8 +580,8 @@ __device__ __forceinline__ void step_trip32(const Geom &g, const FieldsT<float> 
   const float qw = (float)s.q * w;
   // deposit domain: ZigZag deposits in the particle's new domain, VB in the
   // domain of its FP64 end point
-  const int d_dep = CCS == PCP_CCS_ZIGZAG ? d_new : (live ? deposit_domain32(g, rd, zd) : 0);
-  const Dom dn = make_dom(g, d_dep);
+  const int d_dep = CCS == PCP_CCS_ZIGZAG || D1 ? d_new : (live ? deposit_domain32(g, rd, zd) : 0);
+  const Dom dn = make_dom_t<D1>(g, d_dep);
   WaveSink32 S{f.J, (int)((long)d_dep * 3 * g.P), (int)g.P, g.yr, g.nr, g.nz, &err};
   deposit32<CCS>(S, live, g, dn, o, n, v[1], qw, inv_dt, rd, zd, s.q * (double)w, f, d_dep, st, &err, marker, cflag,
                  s.seq, p, sp);
@@ -588,7 +615,7 @@ typedef __attribute__((address_space(4))) const BatchArgs32 KernargBatch32;
 // ZigZag kernel then fits 80 VGPRs without spilling: six wavefronts per
 // SIMD instead of five (4 % faster); the VB kernel fits 128 instead of 130:
 // four instead of three (9 % faster on the batched case).
-template <int CCS, bool PUSH, int BLOCK>
+template <int CCS, bool PUSH, int BLOCK, bool D1>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(CCS == PCP_CCS_ZIGZAG ? 6 : 4)))
 k32_step_batch(BatchArgs32 args) {
   (void)args;
@@ -611,7 +638,7 @@ k32_step_batch(BatchArgs32 args) {
       int k = 0;
       while (k + 1 < x.b.nsp && t >= x.b.end[k]) ++k;
       const long first = k ? x.b.end[k - 1] : 0;
-      step_trip32<CCS, PUSH>(x.g, x.f, x.b.s[k], x.b.sp[k], x.b.K[k], (t - first) * 64 + lane, x.cur, x.st, x.mg,
+      step_trip32<CCS, PUSH, D1>(x.g, x.f, x.b.s[k], x.b.sp[k], x.b.K[k], (t - first) * 64 + lane, x.cur, x.st, x.mg,
                              x.b.marker, x.b.cflag, err);
     }
   }
@@ -985,10 +1012,10 @@ void launch32_push(pcp_ctx *c, int sp) {
 // consecutive wavefront trips per counter increment of the batched step
 constexpr int TRIPS_PER_TICKET = 16;
 
-template <int CCS, bool PUSH, int BLOCK>
+template <int CCS, bool PUSH, int BLOCK, bool D1>
 static void launch_batch_t(pcp_ctx *c, const Batch32 &b) {
   static int occ = 0;   // resident blocks per CU of this instance
-  if (!occ && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k32_step_batch<CCS, PUSH, BLOCK>, BLOCK, 0) !=
+  if (!occ && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k32_step_batch<CCS, PUSH, BLOCK, D1>, BLOCK, 0) !=
                    hipSuccess || occ < 1))
     occ = 1;
   const int per_cu = c->tune.blocks_per_cu > 0 ? c->tune.blocks_per_cu : occ;
@@ -996,15 +1023,22 @@ static void launch_batch_t(pcp_ctx *c, const Batch32 &b) {
   const long waves = (b.end[b.nsp - 1] + chunk - 1) / chunk;
   const long grid = std::max(1L, std::min((waves + BLOCK / 64 - 1) / (BLOCK / 64), (long)c->n_cu * per_cu));
   BatchArgs32 args{c->g, c->s32->f, b, c->st, c->mig, c->cur, chunk};
-  hipLaunchKernelGGL((k32_step_batch<CCS, PUSH, BLOCK>), dim3((unsigned)grid), dim3(BLOCK), 0, c->stream, args);
+  hipLaunchKernelGGL((k32_step_batch<CCS, PUSH, BLOCK, D1>), dim3((unsigned)grid), dim3(BLOCK), 0, c->stream, args);
 }
 
 template <int CCS>
 static void launch_batch_ccs(pcp_ctx *c, const Batch32 &b, bool push, int block) {
-  if (!push) launch_batch_t<CCS, false, 256>(c, b);
-  else if (block == 128) launch_batch_t<CCS, true, 128>(c, b);
-  else if (block == 512) launch_batch_t<CCS, true, 512>(c, b);
-  else launch_batch_t<CCS, true, 256>(c, b);
+  if (c->D != 1) {
+    if (!push) launch_batch_t<CCS, false, 256, false>(c, b);
+    else if (block == 128) launch_batch_t<CCS, true, 128, false>(c, b);
+    else if (block == 512) launch_batch_t<CCS, true, 512, false>(c, b);
+    else launch_batch_t<CCS, true, 256, false>(c, b);
+    return;
+  }
+  if (!push) launch_batch_t<CCS, false, 256, true>(c, b);
+  else if (block == 128) launch_batch_t<CCS, true, 128, true>(c, b);
+  else if (block == 512) launch_batch_t<CCS, true, 512, true>(c, b);
+  else launch_batch_t<CCS, true, 256, true>(c, b);
 }
 
 void launch32_step_batch(pcp_ctx *c, bool push) {
