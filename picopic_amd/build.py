@@ -26,6 +26,12 @@ FLAGS = [
     "-Wno-unused-result", "-Wno-unused-value",
 ]
 
+# The FP32 split kernels carry FP64 only where FP32 is not enough (the VB
+# terms, the deposit end point); there one-ulp reciprocal-based division and
+# approximate library functions suffice (6 % faster on the VB workload), so
+# that file alone is built with them.  The FP64 kernels keep IEEE division.
+FILE_FLAGS = {"pcp32.hip": ["-freciprocal-math", "-fapprox-func"]}
+
 
 def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
@@ -37,7 +43,7 @@ def build(verbose=False):
     jobs = []
     for src in sources():
         obj = os.path.join(OUT_DIR, os.path.basename(src)[:-4] + ".o")
-        cmd = [HIPCC] + FLAGS + ["-c", src, "-o", obj]
+        cmd = [HIPCC] + FLAGS + FILE_FLAGS.get(os.path.basename(src), []) + ["-c", src, "-o", obj]
         jobs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
         objs.append(obj)
     failed = False
