@@ -42,6 +42,11 @@ def workload(name):
                     name="batched uneven domains 1024x4096, 8x8 domains of 128x512, ramp 0.1n->n, 32 ppc at peak, Vay",
                     grid=(1024, 4096), domains=(8, 8), ppc=32, density=1e17, density_left=1e16, ccs="vb",
                     pusher="vay", bunch=None)
+    if name == "full":
+        return dict(key="full",
+                    name="full form 1024x8192, 8x8 domains of 128x1024, ramp 0.1n->n, 32 ppc at peak + Gaussian bunch, HC",
+                    grid=(1024, 8192), domains=(8, 8), ppc=32, density=1e17, density_left=1e16, ccs="zigzag",
+                    pusher="hc", bunch=dict(n=1 << 20, n_peak=1e16, sigma_r=2e-3, sigma_z=5e-3, z0=0.1, gamma=5.0))
     if name == "uniform":
         return dict(key="uniform", name="uniform e-p plasma 128x512, 16 ppc/species, VB", grid=(128, 512),
                     domains=(1, 1), ppc=16, density=1e17, ccs="vb", pusher="boris-adaptive", bunch=None)
