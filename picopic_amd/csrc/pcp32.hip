@@ -452,8 +452,8 @@ __device__ __forceinline__ void append_migrant32(Migrants mg, StepState *st, int
 // Gather, push and move of live particle p: positions from slot `from`,
 // velocities from slot `vfrom`; do_push = false moves with the stored
 // velocity (steps whose collision phase sits between push and move).  The
-// exact re-deposition calls it again on the pre-step slots and gets
-// bit-identical (n, v, rd, zd).
+// fix-up pass calls it again on the pre-step slots and gets bit-identical
+// (n, v, rd, zd).
 template <bool EXACT_END = true, bool D1 = false>
 __device__ __forceinline__ bool advance32(const Geom &g, const FieldsT<float> &f, const Species32 &s, long p,
                                           uint32_t cell, int from, int vfrom, bool do_push, const PushConst &K,
@@ -666,8 +666,13 @@ __global__ void __launch_bounds__(256) k32_fix(Geom g, FieldsT<float> f, Batch32
     const Species32 &s = b.s[k];
     const long p0 = (t - (k ? b.end[k - 1] * 64 : 0)) * 4;
     uint32_t co[4];
+    if (p0 + 3 < s.n) {
+      // one 16-byte load of the four cells (4-byte aligned)
+      __builtin_memcpy(co, s.cell[cur ^ 1] + p0, sizeof co);
+    } else {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) co[j] = p0 + j < s.n ? s.cell[cur ^ 1][p0 + j] : DEAD_CELL;
+      for (int j = 0; j < 4; ++j) co[j] = p0 + j < s.n ? s.cell[cur ^ 1][p0 + j] : DEAD_CELL;
+    }
     bool hit[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) hit[j] = co[j] != DEAD_CELL && b.cflag[co[j]] == fval;
