@@ -701,10 +701,10 @@ __global__ void __launch_bounds__(256) k32_fix(Geom g, FieldsT<float> f, Batch32
 // sort by (domain, cell) and compaction
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t sort_key32(const Geom &g, uint32_t cell) {
-  int cr = (int)(cell / (uint32_t)g.Nz), cz = (int)(cell - (uint32_t)cr * (uint32_t)g.Nz);
-  int d = domain_of32(g, cr, cz);
-  long lr = cr - (long)(d / g.Dz) * g.nr, lz = cz - (long)(d % g.Dz) * g.nz;
-  return sort_cell_key(g, d, lr, lz);
+  const int cr = (int)g.div_Nz.div(cell), cz = (int)(cell - (uint32_t)cr * (uint32_t)g.Nz);
+  const int d = domain_of32(g, cr, cz);
+  const int I = (int)g.div_Dz.div((uint32_t)d), J = d - I * g.Dz;
+  return sort_cell_key(g, d, cr - (long)I * g.nr, cz - (long)J * g.nz);
 }
 
 __global__ void __launch_bounds__(256) k32_sort_count(Geom g, Species32 s, int cur, uint32_t *key, uint32_t *count,

@@ -188,6 +188,9 @@ int pcp_create(const pcp_config *cfg, pcp_ctx **out) {
   g.pusher = cfg->pusher;
   g.bin_r = 1;
   g.bin_z = g.nz;
+  g.nbz = 1;
+  g.div_bin_r = make_fastdiv(1);
+  g.div_bin_z = make_fastdiv((uint32_t)g.nz);
   g.div_nr = make_fastdiv((uint32_t)g.nr);
   g.div_nz = make_fastdiv((uint32_t)g.nz);
   g.div_Dz = make_fastdiv((uint32_t)g.Dz);
@@ -1156,6 +1159,9 @@ int pcp_set_sort_bin(pcp_ctx *c, int bin_r, int bin_z) {
   graph_invalidate(c);
   c->g.bin_r = bin_r;
   c->g.bin_z = bin_z;
+  c->g.nbz = c->g.nz / bin_z;
+  c->g.div_bin_r = make_fastdiv((uint32_t)bin_r);
+  c->g.div_bin_z = make_fastdiv((uint32_t)bin_z);
   return 0;
 }
 

@@ -79,7 +79,8 @@ struct Geom {
   int pml_on;
   int ccs, pusher;
   int bin_r, bin_z;  // sort bin (cells; divide nr and nz)
-  FastDiv div_nr, div_nz, div_Dz, div_Nz;
+  int nbz;           // sort bins along z per domain (nz / bin_z)
+  FastDiv div_nr, div_nz, div_Dz, div_Nz, div_bin_r, div_bin_z;
   double inv_vol0;   // 1 / (pi dz dr^2): CELL_VOLUME(i) = (2 i + 1) / inv_vol0
   float dtr_f, dtz_f, inv_dt_f;   // dt / dr, dt / dz, 1 / dt rounded to FP32
 };
@@ -109,9 +110,10 @@ __host__ __device__ inline Dom make_dom(const Geom &g, int d) {
 // (bin_r = 1, bin_z = nz is plain row-major order).  A permutation of
 // [0, D nr nz), so that every cell keeps one contiguous run of particles.
 __host__ __device__ inline uint32_t sort_cell_key(const Geom &g, int d, long lr, long lz) {
-  const long nbz = g.nz / g.bin_z, bsz = (long)g.bin_r * g.bin_z;
-  const long bi = lr / g.bin_r, bk = lz / g.bin_z;
-  return (uint32_t)((long)d * g.nr * g.nz + (bi * nbz + bk) * bsz + (lr - bi * g.bin_r) * g.bin_z + (lz - bk * g.bin_z));
+  const long bsz = (long)g.bin_r * g.bin_z;
+  const long bi = g.div_bin_r.div((uint32_t)lr), bk = g.div_bin_z.div((uint32_t)lz);
+  return (uint32_t)((long)d * g.nr * g.nz + (bi * g.nbz + bk) * bsz + (lr - bi * g.bin_r) * g.bin_z +
+                    (lz - bk * g.bin_z));
 }
 
 // halo-grid index of local (i, k) (interior-relative, may be -2..n+1)

@@ -23,8 +23,9 @@ __global__ void __launch_bounds__(256) k_sort_count(Geom g, SpeciesDev s, int cu
       double r = s.r[cur][p], z = s.z[cur][p];
       if (r >= 0) {
         long cr = (long)floor(r / g.dr), cz = (long)floor(z / g.dz);
-        int d = (int)((cr / g.nr) * g.Dz + cz / g.nz);
-        long lr = cr - (long)(d / g.Dz) * g.nr, lz = cz - (long)(d % g.Dz) * g.nz;
+        const int I = (int)g.div_nr.div((uint32_t)cr), J = (int)g.div_nz.div((uint32_t)cz);
+        const int d = I * g.Dz + J;
+        long lr = cr - (long)I * g.nr, lz = cz - (long)J * g.nz;
         k = sort_cell_key(g, d, lr, lz);
       }
       key[p] = k;
