@@ -175,8 +175,8 @@ def main():
                          "the configuration an earlier run chose)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: every box in this pool has a single MI355X "
-              "(the sharding plan is described in the documentation)", file=sys.stderr)
+        print("bench.py measures one GPU only: this build runs the particle loop on a single MI355X "
+              "(the multi-GPU sharding plan is described in the documentation)", file=sys.stderr)
         sys.exit(2)
     w = workload(args.workload)
     runs = {}
