@@ -1,7 +1,8 @@
 """ctypes binding of the CPU oracle (test infrastructure only).
 
 Only the test suite, the smoke check and the benchmark's CPU-baseline leg
-import this module; it is the checker, never the product path.
+import this module, to check the device results; it is never the product
+path.
 """
 import ctypes as C
 import os

@@ -10,8 +10,9 @@
  * follows; arithmetic is written in the same order as the reference so that
  * an IEEE build (-ffp-contract=off, no fast-math) reproduces it.
  *
- * This library is the checker: only tests, the smoke check and the CPU
- * baseline leg of the benchmark load it.  It is never the product path.
+ * This library is test infrastructure: only tests, the smoke check and the
+ * CPU baseline leg of the benchmark load it, to check the device results.
+ * It is never the product path.
  *
  * The parity status of each function (which known-answer vector, closed
  * form, invariant or brute-force check pins it, or "parity unpinned") is
