@@ -175,7 +175,7 @@ def main():
                          "the configuration an earlier run chose)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build runs the particle loop on a single MI355X "
+        print("bench.py measures one GPU only: picopic_amd runs the particle loop on a single MI355X "
               "(the multi-GPU sharding plan is described in the documentation)", file=sys.stderr)
         sys.exit(2)
     w = workload(args.workload)
