@@ -9,9 +9,12 @@ pusher, dt = 0.5 dz / c, conducting walls.  Inputs are generated on the
 device from a fixed seed and are resident in HBM before timing starts.
 ``batched`` is 8 x 8 uneven domains (1024 x 4096 cells, domains of 128 x 512,
 density ramping 0.1 n -> n along z, 32 per cell at the peak, about 1.5e8
-particles, Vay, Villasenor-Buneman); ``uniform`` is the 128 x 512 plasma with
-16 per cell.  The FP32 leg autotunes the sort bin and launch shape before
-its warm-up (``--no-tune`` keeps the defaults).
+particles, Vay, Villasenor-Buneman); ``full`` is the full form (1024 x 8192
+cells in 8 x 8 domains of 128 x 1024, the same ramp, 32 per cell at the peak,
+the bunch, Higuera-Cary, ZigZag, about 3e8 particles); ``uniform`` is the
+128 x 512 plasma with 16 per cell.  The FP32 leg autotunes the sort bin, the
+launch shape and the plasma sort interval before its warm-up (``--no-tune``
+keeps the defaults).
 
 One step = Yee E + stitch, Yee H + stitch, push + move + deposit of every
 particle, migration keys, J stitch.
