@@ -337,23 +337,6 @@ __device__ __forceinline__ bool move32(const Geom &g, const Dom &o, int beam, fl
   return isfinite(p.fr) && isfinite(p.fz) && p.cr >= 0 && p.cz >= 0 && p.cr < g.Nr && p.cz < g.Nz;
 }
 
-// ---------------------------------------------------------------------------
-// deposition sinks on FP32 J
-// ---------------------------------------------------------------------------
-struct WaveSink32 {
-  float *J;
-  int base, P, yr, nr, nz;
-  int *err;
-  __device__ __forceinline__ void add_if(int c, int i, int k, float v, bool a) {
-    bool in = i >= -2 && i < nr + 2 && k >= -2 && k < nz + 2;
-    if (a && !in) *err |= ERR_MOVED_TOO_FAR;
-    wave_add(J, base + c * P + (i + 2) * yr + (k + 2), v, a && in);
-  }
-};
-
-
-
-
 // ZigZag (currentZigZag.cpp:22-136) in FP32, positions relative to the old
 // cell: uo = f_old, un = (i_n - i_o) + f_new (likewise in z)
 __device__ __forceinline__ ZZTermsT<float> zigzag32(const Geom &g, const Dom &dn, const Split &o,
@@ -508,9 +491,9 @@ __device__ __forceinline__ int deposit_domain32(const Geom &g, double rd, double
 
 // the deposit of one lane's particle (ZigZag in FP32 relative to the old
 // cell; Villasenor-Buneman terms in FP64 from the old split position and
-// the unrounded end point) into a sink; every lane of the block calls it
-template <int CCS, class Sink>
-__device__ __forceinline__ void deposit32(Sink &S, bool live, const Geom &g, const Dom &dn, const Split &o,
+// the unrounded end point); every lane of the wavefront calls it
+template <int CCS>
+__device__ __forceinline__ void deposit32(bool live, const Geom &g, const Dom &dn, const Split &o,
                                           const Split &n, float vphi, float qw, float inv_dt, double rd, double zd,
                                           double p_charge, const FieldsT<float> &f, int d_dep, StepState *st,
                                           int *err, uint64_t *marker, uint32_t *cflag, const uint64_t *seq, long p,
@@ -582,8 +565,7 @@ __device__ __forceinline__ void step_trip32(const Geom &g, const FieldsT<float> 
   // domain of its FP64 end point
   const int d_dep = CCS == PCP_CCS_ZIGZAG || D1 ? d_new : (live ? deposit_domain32(g, rd, zd) : 0);
   const Dom dn = make_dom_t<D1>(g, d_dep);
-  WaveSink32 S{f.J, (int)((long)d_dep * 3 * g.P), (int)g.P, g.yr, g.nr, g.nz, &err};
-  deposit32<CCS>(S, live, g, dn, o, n, v[1], qw, inv_dt, rd, zd, s.q * (double)w, f, d_dep, st, &err, marker, cflag,
+  deposit32<CCS>(live, g, dn, o, n, v[1], qw, inv_dt, rd, zd, s.q * (double)w, f, d_dep, st, &err, marker, cflag,
                  s.seq, p, sp);
 }
 

@@ -71,22 +71,6 @@ struct MarkSink {
   }
 };
 
-// wave-aggregated sink (call sites must be converged across the wavefront)
-struct WaveSink {
-  double *J;      // f.J, every domain
-  int base;       // offset of component 0 of the destination domain
-  int P, yr, nr, nz;
-  bool active;
-  int *err;
-  __device__ __forceinline__ void add_if(int c, int i, int k, double v, bool a) {
-    bool in = i >= -2 && i < nr + 2 && k >= -2 && k < nz + 2;
-    if (a && !in) *err |= ERR_MOVED_TOO_FAR;
-    wave_add(J, base + c * P + (i + 2) * yr + (k + 2), v, a && in);
-  }
-  __device__ __forceinline__ void add(int c, int i, int k, double v) { add_if(c, i, k, v, active); }
-  __device__ __forceinline__ void strict_reset(int, int) {}
-};
-
 // domain owning global cell (cr, cz)
 __device__ __forceinline__ int domain_of(const Geom &g, int cr, int cz) { return (cr / g.nr) * g.Dz + (cz / g.nz); }
 
@@ -376,10 +360,10 @@ __global__ void k_migrate(Geom g, SpeciesDev s, int sp, int cur, StepState *st, 
   if (err) atomicOr(&st->err, err);
 }
 
-// the deposit of one lane's particle into a sink; every lane of the block
-// calls it (inactive lanes with live = false)
-template <int CCS, class Sink>
-__device__ __forceinline__ void deposit64(Sink &S, bool live, const Geom &g, const Dom &dn, double r, double z,
+// the deposit of one lane's particle; every lane of the wavefront calls it
+// (inactive lanes with live = false)
+template <int CCS>
+__device__ __forceinline__ void deposit64(bool live, const Geom &g, const Dom &dn, double r, double z,
                                           double ro, double zo, double vphi, double q, double w, const Fields &f,
                                           int d_new, StepState *st, int *err, uint64_t *marker, uint32_t *cflag,
                                           const uint64_t *seq, long p, int sp) {
@@ -410,13 +394,11 @@ __device__ __forceinline__ void deposit64(Sink &S, bool live, const Geom &g, con
   }
 }
 
-// fused push + move + deposit (the hot kernel).  Every thread of a block runs
-// the same number of loop trips so that the ZigZag deposit, which has no
-// data-dependent branches, executes converged and aggregates its atomics per
-// wavefront.  PUSH = false skips the push (steps with a collision phase push
-// and collide first).
-// One particle of the fused step (all lanes of the wavefront take part,
-// live or not, since the deposit's run sums move data between lanes).
+// One particle of the fused push + move + deposit step (all lanes of the
+// wavefront take part, live or not, since the deposit's run sums move data
+// between lanes; the grid-stride loops run the same trips in every lane).
+// PUSH = false skips the push (steps with a collision phase push and
+// collide first).
 template <int CCS, bool PUSH>
 __device__ __forceinline__ void fused_trip(const Geom &g, const Fields &f, const SpeciesDev &s, int sp, int cur,
                                            StepState *st, const Migrants &mg, uint64_t *marker, uint32_t *cflag,
@@ -454,8 +436,7 @@ __device__ __forceinline__ void fused_trip(const Geom &g, const Fields &f, const
   }
   if (!live) { r = ro = g.dr; z = zo = g.dz; d_new = 0; vphi = 0; w = 0; }
   const Dom dn = make_dom(g, d_new);
-  WaveSink S{f.J, (int)((long)d_new * 3 * g.P), (int)g.P, g.yr, g.nr, g.nz, live, err};
-  deposit64<CCS>(S, live, g, dn, r, z, ro, zo, vphi, s.q, w, f, d_new, st, err, marker, cflag, s.seq, p, sp);
+  deposit64<CCS>(live, g, dn, r, z, ro, zo, vphi, s.q, w, f, d_new, st, err, marker, cflag, s.seq, p, sp);
 }
 
 template <int CCS, bool PUSH>

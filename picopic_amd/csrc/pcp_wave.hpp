@@ -1,11 +1,10 @@
-// Wave-level aggregation of scatter-adds (64-lane CDNA wavefronts).
+// Wavefront-level reductions and run sums (64-lane CDNA wavefronts).
 //
 // Particles are stored in cell order, so the lanes of a wavefront mostly
-// deposit into the same few grid nodes.  wave_add() groups the lanes of the
-// (converged) wavefront by target index, sums each group across the wave and
-// issues one device-scope atomic per distinct node, instead of one per lane.
-// It must be called by all lanes of the wavefront (converged control flow);
-// `active` masks the lanes that carry a contribution.
+// deposit into the same few grid nodes.  The run sums below add each
+// contribution along the runs of lanes that share a node and let the last
+// lane of a run issue the one atomic; wave_sum is a fixed-order reduction
+// over the whole wavefront.  Both must be called by all 64 lanes.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -37,25 +36,6 @@ __device__ __forceinline__ double wave_sum(double x) {
   x += dpp64<0x124>(x);   // row_ror 4
   x += dpp64<0x128>(x);   // row_ror 8
   return (readlane64(x, 0) + readlane64(x, 16)) + (readlane64(x, 32) + readlane64(x, 48));
-}
-
-// Scatter-add with wave aggregation: lanes with the same target index are
-// summed (in FP64, also for FP32 targets, so the aggregated FP32 current is
-// the correctly rounded sum of up to 64 contributions up to a 2^-53
-// relative term) and one lane per distinct index issues the atomic.
-template <class T>
-__device__ __forceinline__ void wave_add(T *base, int idx, T v, bool active) {
-  unsigned long long pending = __ballot(active);
-  const int lane = __lane_id();
-  while (pending) {
-    const int leader = __ffsll((long long)pending) - 1;
-    const int lk = __builtin_amdgcn_readlane(idx, leader);
-    const bool m = active && idx == lk;
-    const double s = wave_sum(m ? (double)v : 0.0);
-    if (lane == leader) unsafeAtomicAdd(base + lk, (T)s);
-    pending &= ~__ballot(m);
-    active = active && !m;
-  }
 }
 
 // Runs of equal keys in lane order.  Cell-sorted particles put the lanes of
