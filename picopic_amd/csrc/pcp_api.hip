@@ -1204,10 +1204,32 @@ int tune_trial(pcp_ctx *c, int steps, double *ms) {
   return 0;
 }
 
+// Times each of n candidates (set(i) applies candidate i) twice, in forward
+// and then in reverse order, and picks the least sum: the simulation advances
+// with every trial and its step time drifts, which the two orders cancel to
+// first order (one forward pass favoured whichever candidates came first).
+extern "C++" {
+template <class Set>
+int tune_pick(pcp_ctx *c, int steps, int n, Set set, int *best_i, double *best_ms) {
+  std::vector<double> score((size_t)n, 0.0);
+  for (int pass = 0; pass < 2; ++pass)
+    for (int j = 0; j < n; ++j) {
+      const int i = pass ? n - 1 - j : j;
+      if (int rc = set(i)) return rc;
+      double ms;
+      if (int rc = tune_trial(c, steps, &ms)) return rc;
+      score[(size_t)i] += ms;
+    }
+  *best_i = (int)(std::min_element(score.begin(), score.end()) - score.begin());
+  *best_ms = score[(size_t)*best_i] / 2;
+  return 0;
+}
+}  // extern "C++"
+
 }  // namespace
 
 // The tuner times candidates on the live state, so the simulation advances
-// by every step it takes (about 40 x steps + 24 in all).
+// by every step it takes (about 80 x steps + 24 in all).
 int pcp_autotune(pcp_ctx *c, int steps, pcp_tuning *out) {
   if (!c || steps < 1 || steps > 1000) return fail(PCP_E_ARG, "bad argument");
   int saved[MAX_SPECIES];
@@ -1229,25 +1251,21 @@ int pcp_autotune(pcp_ctx *c, int steps, pcp_tuning *out) {
     for (int bz : {4, 8, 16, 32, 64})
       if (br * bz >= 8 && br * bz <= 512 && g.nr % br == 0 && g.nz % bz == 0 && bz < g.nz) bins.push_back({br, bz});
   double best = 1e300;
-  std::pair<int, int> best_bin = bins[0];
-  for (auto b : bins) {
-    double ms;
-    if (int rc = pcp_set_sort_bin(c, b.first, b.second)) return restore(rc);
-    if (int rc = tune_trial(c, steps, &ms)) return restore(rc);
-    if (ms < best) best = ms, best_bin = b;
-  }
-  if (int rc = pcp_set_sort_bin(c, best_bin.first, best_bin.second)) return restore(rc);
+  int bi = 0;
+  if (int rc = tune_pick(c, steps, (int)bins.size(),
+                         [&](int i) { return pcp_set_sort_bin(c, bins[(size_t)i].first, bins[(size_t)i].second); },
+                         &bi, &best))
+    return restore(rc);
+  if (int rc = pcp_set_sort_bin(c, bins[(size_t)bi].first, bins[(size_t)bi].second)) return restore(rc);
   // 2. launch shape of the batched FP32 kernel
   if (c->s32) {
-    int best_block = c->tune.block, best_per_cu = c->tune.blocks_per_cu;
-    for (int block : {128, 256, 512})
-      for (int per_cu : {0, 1, 2}) {
-        double ms;
-        if (int rc = pcp_set_launch(c, block, per_cu)) return restore(rc);
-        if (int rc = tune_trial(c, steps, &ms)) return restore(rc);
-        if (ms < best) best = ms, best_block = block, best_per_cu = per_cu;
-      }
-    if (int rc = pcp_set_launch(c, best_block, best_per_cu)) return restore(rc);
+    static const int shapes[9][2] = {{128, 0}, {128, 1}, {128, 2}, {256, 0}, {256, 1},
+                                     {256, 2}, {512, 0}, {512, 1}, {512, 2}};
+    int li = 0;
+    if (int rc = tune_pick(c, steps, 9, [&](int i) { return pcp_set_launch(c, shapes[i][0], shapes[i][1]); }, &li,
+                           &best))
+      return restore(rc);
+    if (int rc = pcp_set_launch(c, shapes[li][0], shapes[li][1])) return restore(rc);
   }
   // 3. sort interval of the plasma species: the step time t(j) j steps
   // after a sort grows as the particles leave cell order; with t(j) ~ a + b j
