@@ -165,7 +165,9 @@ KERNEL = {"fp32": "k32_step_batch", "fp64": "k_step_fused"}
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    # 50 timed steps span about two plasma sort intervals, so that the figure
+    # does not depend on where the window falls in the sort cycle
+    ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="wakefield")
     ap.add_argument("--prec", default="both", choices=["fp64", "fp32", "both"],
