@@ -126,6 +126,126 @@ def test_vb_azimuthal_current_matches_charge_times_velocity():
 
 
 # --------------------------------------------------------------------------
+# ZigZag: relay-point charge conservation in cell units
+# --------------------------------------------------------------------------
+def _bilinear(r, z, nr, nz):
+    """Bilinear (cloud-in-cell) node weights of a point given in cell units."""
+    out = np.zeros((nr + 4, nz + 4))
+    i, k = int(np.floor(r)), int(np.floor(z))
+    fr, fz = r - i, z - k
+    for di, wr in ((0, 1 - fr), (1, fr)):
+        for dk, wz in ((0, 1 - fz), (1, fz)):
+            out[i + di + 2, k + dk + 2] += wr * wz
+    return out
+
+
+def _zigzag_one(r, z, v, h, nr=10, nz=10):
+    """One particle (charge -e, weight 1) pushed in zero fields, moved and
+    deposited by ZigZag on an nr x nz grid of h x h cells; r, z in cells.
+    Returns the deposit in flux units A = J V(i_mid) dt / (q w) divided by
+    h (so in cells), the old and new positions in cells, v_phi and dt."""
+    smb, dt = one_particle_smb(r * h, z * h, v, nr, nz, h, h, ccs="zigzag")
+    for ph in ("push", "move", "reset_j", "deposit"):
+        smb.phase(ph)
+    p = smb.particles(0)
+    ro, zo, rn, zn = p["r_old"][0] / h, p["z_old"][0] / h, p["r"][0] / h, p["z"][0] / h
+    i_mid = int(np.floor((ro + rn) / 2))
+    vol = np.pi * h ** 3 * (2 * i_mid + 1)     # ring of width h around r = i_mid h
+    A = smb.get_grid("J")[0] * vol * dt / Q_E / h
+    return A, (ro, zo), (rn, zn), p["vphi"][0], dt
+
+
+def _move(rng, reach):
+    v = rng.normal(0, 1, 3)
+    return v * rng.uniform(0.05, reach) / np.linalg.norm(v)
+
+
+def test_zigzag_discrete_continuity_in_cell_units():
+    """Umeda et al. (2003): with the relay point and its two straight
+    segments, the bilinear node charge changes by minus the net flux,
+    S_new - S_old + (A_r[i] - A_r[i-1]) + (A_z[k] - A_z[k-1]) = 0, A_r stored
+    at the inner node of its face and A_z at the lower one.  With 1 m cells
+    the reference's weights in metres (src/current/currentZigZag.cpp:81-84)
+    are the cell-unit weights, and the azimuthal weight cancels from the r
+    and z sums, so this holds with v_phi != 0 for moves crossing 0, 1 or 2
+    cell boundaries.  A wrong relay point, flux split, weight or node index
+    leaves a residual."""
+    rng = np.random.default_rng(31)
+    nr = nz = 10
+    worst, crossings = 0.0, set()
+    for _ in range(300):
+        r, z = rng.uniform(2.5, 7.0), rng.uniform(2.5, 7.0)
+        A, (ro, zo), (rn, zn), _, _ = _zigzag_one(r, z, _move(rng, 2.5e8), 1.0)
+        crossings.add(int(np.floor(ro) != np.floor(rn)) + int(np.floor(zo) != np.floor(zn)))
+        div = A[0].copy()
+        div[1:, :] -= A[0][:-1, :]
+        div += A[2]
+        div[:, 1:] -= A[2][:, :-1]
+        res = _bilinear(rn, zn, nr, nz) - _bilinear(ro, zo, nr, nz) + div
+        worst = max(worst, np.abs(res).max())
+    assert crossings == {0, 1, 2}
+    assert worst < 1e-12, worst
+
+
+def test_zigzag_azimuthal_deposit_uses_the_second_segment():
+    """J_phi = q w v_phi / V(i_mid) on the four nodes around the new cell with
+    the bilinear weights of the second segment's midpoint (W_r2, W_z2, read
+    from the pinned J_z and J_r deposits of that segment), except that the
+    node (i_n, k_n + 1) takes (1 - W_r2) W_z1, the first segment's z weight
+    (src/current/currentZigZag.cpp:104)."""
+    rng = np.random.default_rng(37)
+    done = 0
+    for _ in range(300):
+        # starts near a cell corner, so that many moves cross both boundaries
+        r, z = rng.integers(3, 7) + rng.choice([0.1, 0.9]), rng.integers(3, 7) + rng.choice([0.1, 0.9])
+        r, z = r + rng.uniform(-0.08, 0.08), z + rng.uniform(-0.08, 0.08)
+        v = _move(rng, 2.5e8)
+        v[1] = rng.uniform(1e6, 1e8)
+        A, (ro, zo), (rn, zn), vphi, dt = _zigzag_one(r, z, v, 1.0)
+        i_o, k_o, i_n, k_n = (int(np.floor(x)) + 2 for x in (ro, zo, rn, zn))
+        if i_o == i_n or k_o == k_n:
+            continue    # the segments' deposits share nodes; the weights are read where they do not
+        w_z1 = A[0][i_o, k_o + 1] / (A[0][i_o, k_o] + A[0][i_o, k_o + 1])
+        w_z2 = A[0][i_n, k_n + 1] / (A[0][i_n, k_n] + A[0][i_n, k_n + 1])
+        w_r2 = A[2][i_n + 1, k_n] / (A[2][i_n, k_n] + A[2][i_n + 1, k_n])
+        f = vphi * dt        # J_phi = q w v_phi / V(i_mid) in the units of A
+        expect = np.zeros_like(A[1])
+        expect[i_n, k_n] = (1 - w_r2) * (1 - w_z2)
+        expect[i_n, k_n + 1] = (1 - w_r2) * w_z1
+        expect[i_n + 1, k_n] = w_r2 * (1 - w_z2)
+        expect[i_n + 1, k_n + 1] = w_r2 * w_z2
+        assert np.abs(A[1] - f * expect).max() <= 1e-12 * abs(f)
+        done += 1
+    assert done >= 10
+
+
+@pytest.mark.parametrize("h", [1e-3, 0.25])
+def test_zigzag_weights_are_in_metres(h):
+    """The reference's weights are positions in metres, not divided by the
+    cell size (src/current/currentZigZag.cpp:81-84): for a move inside one
+    cell (relay point = midpoint) on cells of h metres, the share of each
+    flux on the upper node is h times its share on 1 m cells, and the
+    flux totals are the same in cells."""
+    rng = np.random.default_rng(41)
+    done = 0
+    for _ in range(200):
+        r, z = rng.uniform(2.5, 7.0), rng.uniform(2.5, 7.0)
+        v = _move(rng, 1.0e8)
+        A1, (ro, zo), (rn, zn), _, _ = _zigzag_one(r, z, v, 1.0)
+        if np.floor(ro) != np.floor(rn) or np.floor(zo) != np.floor(zn):
+            continue
+        Ah, _, _, _, _ = _zigzag_one(r, z, v, h)
+        i, k = int(np.floor(ro)) + 2, int(np.floor(zo)) + 2
+        for c, lo, hi in ((0, (i, k), (i, k + 1)), (2, (i, k), (i + 1, k))):
+            tot1, toth = A1[c][lo] + A1[c][hi], Ah[c][lo] + Ah[c][hi]
+            # positions of a few cells carry round-off of ~1e-15 cells into the totals
+            assert abs(toth - tot1) <= 1e-12
+            assert abs(Ah[c][hi] / toth - h * A1[c][hi] / tot1) <= 1e-12 / abs(tot1)
+        done += 1
+    assert done > 50
+
+
+# --------------------------------------------------------------------------
 # gather: ring-volume weights
 # --------------------------------------------------------------------------
 def node_weights(smb, which, comp, r, z, i0, k0):
@@ -234,6 +354,109 @@ def test_pushers_closed_form_without_magnetic_field(pusher):
         u = v / np.sqrt(1 - v @ v / C ** 2) + Q_E * E * dt / M_E
         ref = u / np.sqrt(1 + u @ u / C ** 2)
         assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def _momentum(v):
+    """u = gamma v per unit mass, with the speed of light."""
+    v = np.asarray(v, float)
+    return v / np.sqrt(1 - v @ v / C ** 2)
+
+
+def _rotate(u, t):
+    """The magnetic rotation u -> u+ fixed by u+ - u = (u+ + u) x t, written
+    as (1 - t^2) u + 2 (u.t) t + 2 u x t over 1 + t^2."""
+    return ((1 - t @ t) * u + 2 * (u @ t) * t + 2 * np.cross(u, t)) / (1 + t @ t)
+
+
+# two input regimes: order-one units (q = m = dt = 1, speeds of metres per
+# second, |tau| of order one), where the factor of the magnetic term is of
+# order one, and plasma electrons, where it is about 1e-9 of u
+REGIMES = {
+    "unit": dict(q=1.0, m=1.0, dt=1.0, speed=(0.05, 3.0), e=1.0, h=1.5 / MU0),
+    "plasma": dict(q=Q_E, m=M_E, dt=1.6e-12, speed=(1e3, 2.5e8), e=1e5, h=1e4),
+}
+
+
+def _draw(rng, rg):
+    v = rng.normal(0, 1, 3)
+    v *= rng.uniform(*rg["speed"]) / np.linalg.norm(v)
+    return v, rng.normal(0, rg["e"], 3), rng.normal(0, rg["h"], 3)
+
+
+@pytest.mark.parametrize("regime", sorted(REGIMES))
+def test_vay_output_solves_the_implicit_update(regime):
+    """Vay (2008): u = u' + (u / g) x tau with u' = u0 + q E dt / m + v0 x tau,
+    tau = q dt mu0 H / (2 m) and g^2 = 1 + |u|^2 (the reference's solve for g
+    writes 1 where c^2 belongs, src/pusher/pusherVay.cpp:81; the last
+    u -> v uses c).  The closed-form output is checked against the implicit
+    equation it has to satisfy, so a wrong sign or factor in the solve for g,
+    in s = 1 / (1 + t^2), in (u'.t) t or in u' x t leaves a residual."""
+    rg = REGIMES[regime]
+    rng = np.random.default_rng(17)
+    worst = 0.0
+    for _ in range(300):
+        v0, E, H = _draw(rng, rg)
+        tau = rg["q"] * rg["dt"] * MU0 * H / (2 * rg["m"])
+        u_prime = _momentum(v0) + rg["q"] * E * rg["dt"] / rg["m"] + np.cross(v0, tau)
+        u = _momentum(O.push_velocity("vay", rg["q"], rg["m"], rg["dt"], E, H, v0))
+        g = np.sqrt(1 + u @ u)
+        res = u - np.cross(u / g, tau) - u_prime
+        worst = max(worst, np.abs(res).max() / np.abs(u_prime).max())
+    assert worst <= 1e-13, worst
+
+
+def _hc_gamma(um, tau):
+    """The HC factor g from its defining relation g^2 = 1 + |(u+ + um) / 2|^2,
+    u+ the rotation of um by tau / g, by bracketing (independent of the
+    quartic's closed form); the root lies in [1, sqrt(1 + |um|^2)]."""
+    from scipy.optimize import brentq
+
+    def f(g):
+        ubar = 0.5 * (_rotate(um, tau / g) + um)
+        return g * g - 1 - ubar @ ubar
+    return brentq(f, 1.0, np.sqrt(1 + um @ um) * (1 + 1e-12), xtol=1e-300, rtol=4 * np.finfo(float).eps,
+                  maxiter=500)
+
+
+@pytest.mark.parametrize("regime", sorted(REGIMES))
+def test_hc_equals_higuera_cary_with_the_reference_deviations(regime):
+    """Higuera-Cary (2017): um = u0 + eps, eps = q E dt / (2 m); u+ = the
+    rotation of um by t = tau / g with g^2 = 1 + |(u+ + um) / 2|^2 (1 where c^2
+    belongs, src/pusher/pusherHC.cpp:75); u = u+ + eps.  The reference's
+    rotation matrix deviates twice (src/pusher/pusherHC.cpp:94-101): its
+    denominator is 1 + tx^2 + ty^2 + tx^2 instead of 1 + t^2, and the um_z
+    coefficient of u+_x is 2 (tx tz - tz) instead of 2 (tx tz - ty).  The
+    expected value is the textbook rotation with those two changes applied."""
+    rg = REGIMES[regime]
+    rng = np.random.default_rng(23)
+    worst = 0.0
+    for _ in range(300):
+        v0, E, H = _draw(rng, rg)
+        eps = rg["q"] * E * rg["dt"] / (2 * rg["m"])
+        tau = rg["q"] * rg["dt"] * MU0 * H / (2 * rg["m"])
+        um = _momentum(v0) + eps
+        t = tau / _hc_gamma(um, tau)
+        numer = _rotate(um, t) * (1 + t @ t)
+        numer[0] += 2 * (t[1] - t[2]) * um[2]
+        u = numer / (1 + 2 * t[0] ** 2 + t[1] ** 2) + eps
+        ref = u / np.sqrt(1 + u @ u / C ** 2)
+        got = O.push_velocity("hc", rg["q"], rg["m"], rg["dt"], E, H, v0)
+        worst = max(worst, np.abs(got - ref).max() / np.abs(ref).max())
+    assert worst <= 1e-12, worst
+
+
+def test_hc_without_the_deviations_is_a_rotation():
+    """Where neither deviation of the reference's matrix acts (t along y,
+    um_z = 0), HC's magnetic step keeps |um|: |u - eps| = |um|."""
+    rg = REGIMES["unit"]
+    rng = np.random.default_rng(29)
+    for _ in range(200):
+        v0, E, H = _draw(rng, rg)
+        v0[2], E[2], H[0], H[2] = 0.0, 0.0, 0.0, 0.0
+        eps = rg["q"] * E * rg["dt"] / (2 * rg["m"])
+        um = _momentum(v0) + eps
+        u = _momentum(O.push_velocity("hc", rg["q"], rg["m"], rg["dt"], E, H, v0))
+        assert abs(np.linalg.norm(u - eps) - np.linalg.norm(um)) <= 1e-13 * np.linalg.norm(um)
 
 
 # --------------------------------------------------------------------------
