@@ -232,7 +232,7 @@ int pcp_set_sort_interval(pcp_ctx *ctx, int sp, int every);
  * launch shape is the workgroup size of the FP32 particle kernel (128, 256
  * or 512) and its resident workgroups per CU (0: as many as fit).
  * pcp_autotune times the candidates on the live state (the simulation
- * advances by the steps it takes: about 40 x steps + 24), keeps the fastest
+ * advances by the steps it takes: about 80 x steps + 24), keeps the fastest
  * bin and shape, and sets the plasma species' sort interval from the
  * measured sort cost and the growth of the step time after a sort. */
 typedef struct pcp_tuning {
