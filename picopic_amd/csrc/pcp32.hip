@@ -689,32 +689,29 @@ __device__ __forceinline__ uint32_t sort_key32(const Geom &g, uint32_t cell) {
   return sort_cell_key(g, d, cr - (long)I * g.nr, cz - (long)J * g.nz);
 }
 
-__global__ void __launch_bounds__(256) k32_sort_count(Geom g, Species32 s, int cur, uint32_t *key, uint32_t *count,
-                                                      uint32_t sentinel) {
-  const long stride = (long)gridDim.x * blockDim.x;
-  for (long b0 = (long)blockIdx.x * blockDim.x; b0 < s.n; b0 += stride) {
-    const long p = b0 + threadIdx.x;
-    uint32_t k = sentinel;
-    if (p < s.n) {
-      uint32_t c = s.cell[cur][p];
-      if (c != DEAD_CELL) k = sort_key32(g, c);
+__global__ void __launch_bounds__(SORT_BLOCK) k32_sort_count(Geom g, Species32 s, int cur, uint32_t *key,
+                                                             uint32_t *count, uint32_t sentinel) {
+  __shared__ SortTable t;
+  table_clear(t);
+  __syncthreads();
+  for (long c0 = (long)blockIdx.x * SORT_CHUNK; c0 < s.n; c0 += (long)gridDim.x * SORT_CHUNK)
+    table_count_chunk(t, c0, s.n, sentinel, count, [&](long p) {
+      const uint32_t c = s.cell[cur][p];
+      const uint32_t k = c != DEAD_CELL ? sort_key32(g, c) : sentinel;
       key[p] = k;
-    }
-    (void)wave_reserve(count, k, p < s.n && k != sentinel);
-  }
+      return k;
+    });
 }
 
-__global__ void __launch_bounds__(256) k32_sort_scatter(Species32 s, SortBuffers32 sb, int cur, const uint32_t *key,
-                                                        const uint32_t *offset, uint32_t *fill, uint32_t sentinel) {
+__global__ void __launch_bounds__(SORT_BLOCK) k32_sort_scatter(Species32 s, SortBuffers32 sb, int cur,
+                                                               const uint32_t *key, const uint32_t *offset,
+                                                               uint32_t *fill, uint32_t sentinel) {
+  __shared__ SortTable t;
   const int nxt = cur ^ 1;
-  const long stride = (long)gridDim.x * blockDim.x;
-  for (long b0 = (long)blockIdx.x * blockDim.x; b0 < s.n; b0 += stride) {
-    const long p = b0 + threadIdx.x;
-    uint32_t k = p < s.n ? key[p] : sentinel;
-    const bool a = p < s.n && k != sentinel;
-    uint32_t slot = wave_reserve(fill, a ? k : 0, a);
-    if (a) {
-      long q = (long)offset[k] + slot;
+  table_clear(t);
+  __syncthreads();
+  for (long c0 = (long)blockIdx.x * SORT_CHUNK; c0 < s.n; c0 += (long)gridDim.x * SORT_CHUNK)
+    table_scatter_chunk(t, c0, s.n, sentinel, key, offset, fill, [&](long p, long q) {
       s.cell[nxt][q] = s.cell[cur][p];
       s.fr[nxt][q] = s.fr[cur][p];
       s.fz[nxt][q] = s.fz[cur][p];
@@ -724,8 +721,7 @@ __global__ void __launch_bounds__(256) k32_sort_scatter(Species32 s, SortBuffers
       sb.w[q] = s.w[p];
       sb.seq[q] = s.seq[p];
       sb.id[q] = s.id[p];
-    }
-  }
+    });
 }
 
 __global__ void k32_sort_tail(Species32 s, const uint32_t *n_live_ptr) {
@@ -1093,16 +1089,17 @@ int launch32_sort(pcp_ctx *c, int sp) {
   const long ncells = (long)c->D * c->g.nr * c->g.nz;
   const uint32_t sentinel = (uint32_t)ncells;
   unsigned grid = pgrid32(c, s.n, 256);
+  const unsigned gsort = (unsigned)std::max(1L, std::min((s.n + SORT_CHUNK - 1) / SORT_CHUNK, (long)c->n_cu * 16));
   unsigned gz = (unsigned)std::min<long>((ncells + 256) / 256, (long)c->n_cu * 16);
   hipLaunchKernelGGL(k32_zero_u32, dim3(gz), dim3(256), 0, c->stream, c->sort_count, ncells + 1);
   hipLaunchKernelGGL(k32_zero_u32, dim3(gz), dim3(256), 0, c->stream, c->sort_fill, ncells + 1);
-  hipLaunchKernelGGL(k32_sort_count, dim3(grid), dim3(256), 0, c->stream, c->g, s, c->cur, c->sort_key, c->sort_count,
+  hipLaunchKernelGGL(k32_sort_count, dim3(gsort), dim3(SORT_BLOCK), 0, c->stream, c->g, s, c->cur, c->sort_key, c->sort_count,
                      sentinel);
   size_t bytes = c->sort_temp_bytes;
   if (hipcub::DeviceScan::ExclusiveSum(c->sort_temp, bytes, c->sort_count, c->sort_offset, (int)(ncells + 1),
                                        c->stream) != hipSuccess)
     return -1;
-  hipLaunchKernelGGL(k32_sort_scatter, dim3(grid), dim3(256), 0, c->stream, s, sb, c->cur, c->sort_key,
+  hipLaunchKernelGGL(k32_sort_scatter, dim3(gsort), dim3(SORT_BLOCK), 0, c->stream, s, sb, c->cur, c->sort_key,
                      c->sort_offset, c->sort_fill, sentinel);
   std::swap(s.cell[c->cur], s.cell[c->cur ^ 1]);
   std::swap(s.fr[c->cur], s.fr[c->cur ^ 1]);

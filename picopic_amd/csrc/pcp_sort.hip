@@ -2,7 +2,8 @@
 //
 // Cell order makes the 64 particles of a wavefront share their field nodes
 // (the gather becomes broadcast loads) and their current nodes (the
-// wave-aggregated atomics issue one add per node).  Removed particles
+// wave-aggregated atomics issue one add per node).  The counting and
+// scatter passes aggregate their counters per workgroup chunk in LDS.  Removed particles
 // (r < 0) are dropped; the tail [n_live, n) is marked removed so that the
 // host-side slot count stays valid without a synchronisation.
 #include <hip/hip_runtime.h>
@@ -13,38 +14,35 @@
 
 namespace pcp {
 
-__global__ void __launch_bounds__(256) k_sort_count(Geom g, SpeciesDev s, int cur, uint32_t *key, uint32_t *count,
-                                                    uint32_t sentinel) {
-  const long stride = (long)gridDim.x * blockDim.x;
-  for (long b0 = (long)blockIdx.x * blockDim.x; b0 < s.n; b0 += stride) {
-    const long p = b0 + threadIdx.x;
-    uint32_t k = sentinel;
-    if (p < s.n) {
-      double r = s.r[cur][p], z = s.z[cur][p];
+__global__ void __launch_bounds__(SORT_BLOCK) k_sort_count(Geom g, SpeciesDev s, int cur, uint32_t *key,
+                                                           uint32_t *count, uint32_t sentinel) {
+  __shared__ SortTable t;
+  table_clear(t);
+  __syncthreads();
+  for (long c0 = (long)blockIdx.x * SORT_CHUNK; c0 < s.n; c0 += (long)gridDim.x * SORT_CHUNK)
+    table_count_chunk(t, c0, s.n, sentinel, count, [&](long p) {
+      uint32_t k = sentinel;
+      const double r = s.r[cur][p], z = s.z[cur][p];
       if (r >= 0) {
-        long cr = (long)floor(r / g.dr), cz = (long)floor(z / g.dz);
+        const long cr = (long)floor(r / g.dr), cz = (long)floor(z / g.dz);
         const int I = (int)g.div_nr.div((uint32_t)cr), J = (int)g.div_nz.div((uint32_t)cz);
         const int d = I * g.Dz + J;
-        long lr = cr - (long)I * g.nr, lz = cz - (long)J * g.nz;
-        k = sort_cell_key(g, d, lr, lz);
+        k = sort_cell_key(g, d, cr - (long)I * g.nr, cz - (long)J * g.nz);
       }
       key[p] = k;
-    }
-    (void)wave_reserve(count, k, p < s.n && k != sentinel);
-  }
+      return k;
+    });
 }
 
-__global__ void __launch_bounds__(256) k_sort_scatter(SpeciesDev s, SortBuffers sb, int cur, const uint32_t *key,
-                                                      const uint32_t *offset, uint32_t *fill, uint32_t sentinel) {
+__global__ void __launch_bounds__(SORT_BLOCK) k_sort_scatter(SpeciesDev s, SortBuffers sb, int cur,
+                                                             const uint32_t *key, const uint32_t *offset,
+                                                             uint32_t *fill, uint32_t sentinel) {
+  __shared__ SortTable t;
   const int nxt = cur ^ 1;
-  const long stride = (long)gridDim.x * blockDim.x;
-  for (long b0 = (long)blockIdx.x * blockDim.x; b0 < s.n; b0 += stride) {
-    const long p = b0 + threadIdx.x;
-    uint32_t k = p < s.n ? key[p] : sentinel;
-    const bool a = p < s.n && k != sentinel;
-    uint32_t slot = wave_reserve(fill, a ? k : 0, a);
-    if (a) {
-      long q = (long)offset[k] + slot;
+  table_clear(t);
+  __syncthreads();
+  for (long c0 = (long)blockIdx.x * SORT_CHUNK; c0 < s.n; c0 += (long)gridDim.x * SORT_CHUNK)
+    table_scatter_chunk(t, c0, s.n, sentinel, key, offset, fill, [&](long p, long q) {
       s.r[nxt][q] = s.r[cur][p];
       s.z[nxt][q] = s.z[cur][p];
       sb.vr[q] = s.vr[p];
@@ -53,8 +51,7 @@ __global__ void __launch_bounds__(256) k_sort_scatter(SpeciesDev s, SortBuffers 
       sb.w[q] = s.w[p];
       sb.seq[q] = s.seq[p];
       sb.id[q] = s.id[p];
-    }
-  }
+    });
 }
 
 // mark [n_live, n) removed in both position buffers
@@ -83,16 +80,17 @@ int launch_sort(pcp_ctx *c, int sp) {
   const long ncells = (long)c->D * c->g.nr * c->g.nz;
   const uint32_t sentinel = (uint32_t)ncells;
   unsigned grid = (unsigned)std::min<long>((s.n + 255) / 256, (long)c->n_cu * 16);
+  const unsigned gsort = (unsigned)std::max(1L, std::min((s.n + SORT_CHUNK - 1) / SORT_CHUNK, (long)c->n_cu * 16));
   unsigned gz = (unsigned)std::min<long>((ncells + 256) / 256, (long)c->n_cu * 16);
   hipLaunchKernelGGL(k_zero_u32, dim3(gz), dim3(256), 0, c->stream, c->sort_count, ncells + 1);
   hipLaunchKernelGGL(k_zero_u32, dim3(gz), dim3(256), 0, c->stream, c->sort_fill, ncells + 1);
-  hipLaunchKernelGGL(k_sort_count, dim3(grid), dim3(256), 0, c->stream, c->g, s, c->cur, c->sort_key, c->sort_count,
+  hipLaunchKernelGGL(k_sort_count, dim3(gsort), dim3(SORT_BLOCK), 0, c->stream, c->g, s, c->cur, c->sort_key, c->sort_count,
                      sentinel);
   size_t bytes = c->sort_temp_bytes;
   if (hipcub::DeviceScan::ExclusiveSum(c->sort_temp, bytes, c->sort_count, c->sort_offset, (int)(ncells + 1),
                                        c->stream) != hipSuccess)
     return -1;
-  hipLaunchKernelGGL(k_sort_scatter, dim3(grid), dim3(256), 0, c->stream, s, sb, c->cur, c->sort_key, c->sort_offset,
+  hipLaunchKernelGGL(k_sort_scatter, dim3(gsort), dim3(SORT_BLOCK), 0, c->stream, s, sb, c->cur, c->sort_key, c->sort_offset,
                      c->sort_fill, sentinel);
   // swap the sorted buffers in (this species only; the shared `cur` index
   // keeps its meaning because the two position pointers are exchanged)

@@ -143,4 +143,137 @@ __device__ __forceinline__ uint32_t wave_reserve(uint32_t *ctr, uint32_t key, bo
   return slot;
 }
 
+// ---------------------------------------------------------------------------
+// Workgroup-level key table of the counting sort.  A workgroup of
+// SORT_BLOCK threads takes chunks of SORT_CHUNK consecutive particles; the
+// particles of one chunk are counted per key in an LDS table first, and the
+// table issues one global atomic per distinct key of the chunk.  Sorted or
+// nearly sorted particles span a few keys per chunk, so the global counters
+// of crowded cells (thousands of beam particles) see about SORT_CHUNK times
+// fewer atomics than with one per particle.  Keys that find no free entry
+// within SORT_PROBES probes are counted directly in global memory.
+// ---------------------------------------------------------------------------
+constexpr int SORT_BLOCK = 256;
+constexpr int SORT_ROUNDS = 4;
+constexpr long SORT_CHUNK = (long)SORT_BLOCK * SORT_ROUNDS;
+constexpr int SORT_TABLE = 256;          // a power of two
+constexpr int SORT_PROBES = 4;
+constexpr uint32_t SORT_EMPTY = 0xffffffffu;
+
+struct SortTable {
+  uint32_t key[SORT_TABLE];
+  uint32_t cnt[SORT_TABLE];
+};
+
+// every thread empties the entries it owns (no barrier)
+__device__ __forceinline__ void table_clear(SortTable &t) {
+  for (int h = (int)threadIdx.x; h < SORT_TABLE; h += SORT_BLOCK) {
+    t.key[h] = SORT_EMPTY;
+    t.cnt[h] = 0;
+  }
+}
+
+// Adds the active lanes' keys to the table, one LDS atomic per distinct key
+// of the wavefront.  Each active lane receives its entry (-1 when the table
+// had no room for its key) and its rank among the entry's particles.  Must be
+// called by all 64 lanes.
+__device__ __forceinline__ void table_add(SortTable &t, uint32_t key, bool active, int &entry, uint32_t &rank) {
+  entry = -1;
+  rank = 0;
+  unsigned long long pending = __ballot(active);
+  const int lane = __lane_id();
+  while (pending) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const uint32_t lk = __shfl(key, leader, 64);
+    const bool m = active && key == lk;
+    const unsigned long long mask = __ballot(m);
+    int e = -1;
+    uint32_t base = 0;
+    if (lane == leader) {
+      uint32_t h = lk & (uint32_t)(SORT_TABLE - 1);
+      for (int i = 0; i < SORT_PROBES; ++i) {
+        const uint32_t old = atomicCAS(&t.key[h], SORT_EMPTY, lk);
+        if (old == SORT_EMPTY || old == lk) {
+          e = (int)h;
+          base = atomicAdd(&t.cnt[h], (uint32_t)__popcll(mask));
+          break;
+        }
+        h = (h + 1) & (uint32_t)(SORT_TABLE - 1);
+      }
+    }
+    e = __shfl(e, leader, 64);
+    base = __shfl(base, leader, 64);
+    if (m) {
+      entry = e;
+      rank = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    }
+    pending &= ~mask;
+    active = active && !m;
+  }
+}
+
+// Counting pass over one chunk: key_of(p) gives particle p's key (the
+// sentinel when it is dropped) and is called once per particle below n.
+template <class KeyOf>
+__device__ __forceinline__ void table_count_chunk(SortTable &t, long c0, long n, uint32_t sentinel, uint32_t *count,
+                                                  KeyOf key_of) {
+#pragma unroll
+  for (int r = 0; r < SORT_ROUNDS; ++r) {
+    const long p = c0 + (long)r * SORT_BLOCK + threadIdx.x;
+    const uint32_t k = p < n ? key_of(p) : sentinel;
+    const bool a = k != sentinel;
+    int e;
+    uint32_t rank;
+    table_add(t, k, a, e, rank);
+    (void)wave_reserve(count, a && e < 0 ? k : 0, a && e < 0);
+  }
+  __syncthreads();
+  for (int h = (int)threadIdx.x; h < SORT_TABLE; h += SORT_BLOCK) {
+    if (t.key[h] != SORT_EMPTY) atomicAdd(count + t.key[h], t.cnt[h]);
+    t.key[h] = SORT_EMPTY;
+    t.cnt[h] = 0;
+  }
+  __syncthreads();
+}
+
+// Scatter pass over one chunk: every particle below n whose key is not the
+// sentinel receives its destination offset[key] + slot, the slots of a key
+// reserved from fill[key] one range per table entry; move(p, q) copies
+// particle p to q.
+template <class Move>
+__device__ __forceinline__ void table_scatter_chunk(SortTable &t, long c0, long n, uint32_t sentinel,
+                                                    const uint32_t *key, const uint32_t *offset, uint32_t *fill,
+                                                    Move move) {
+  uint32_t kk[SORT_ROUNDS], rr[SORT_ROUNDS];
+  int ee[SORT_ROUNDS];
+#pragma unroll
+  for (int r = 0; r < SORT_ROUNDS; ++r) {
+    const long p = c0 + (long)r * SORT_BLOCK + threadIdx.x;
+    const uint32_t k = p < n ? key[p] : sentinel;
+    const bool a = k != sentinel;
+    int e;
+    uint32_t rank;
+    table_add(t, k, a, e, rank);
+    const uint32_t direct = wave_reserve(fill, a && e < 0 ? k : 0, a && e < 0);
+    kk[r] = k;
+    ee[r] = a ? e : -2;
+    rr[r] = e >= 0 ? rank : direct;
+  }
+  __syncthreads();
+  // each entry's count becomes the first slot of its range
+  for (int h = (int)threadIdx.x; h < SORT_TABLE; h += SORT_BLOCK)
+    if (t.key[h] != SORT_EMPTY) t.cnt[h] = atomicAdd(fill + t.key[h], t.cnt[h]);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < SORT_ROUNDS; ++r) {
+    if (ee[r] == -2) continue;
+    const long p = c0 + (long)r * SORT_BLOCK + threadIdx.x;
+    const uint32_t slot = ee[r] >= 0 ? t.cnt[ee[r]] + rr[r] : rr[r];
+    move(p, (long)offset[kk[r]] + slot);
+  }
+  __syncthreads();
+  table_clear(t);
+  __syncthreads();
+}
+
 }  // namespace pcp
