@@ -162,7 +162,12 @@ int pcp_step(pcp_ctx *ctx, int n_steps);
 int pcp_collide(pcp_ctx *ctx, int scheme, int sp_el, int sp_ion, uint64_t seed, uint64_t pass);
 /* Enable (scheme >= 0) or disable (scheme < 0) the collision phase of
  * pcp_step, run between the push and the move like Domain::collide in
- * SMB::advance_particles (src/SMB.cpp:1287-1289); passes count from 0. */
+ * SMB::advance_particles (src/SMB.cpp:1287-1289); passes count from 0.
+ * PCP_COLL_SURROGATE in an FP32_SPLIT context runs fused into the step: the
+ * push kernel sums each cell's velocity moments, the network evaluates every
+ * cell, and the move kernel applies each cell's conserving rescale to the
+ * colliding species as it reads the velocities; cells the network does not
+ * cover take the P12 pass as in pcp_collide. */
 int pcp_set_collisions(pcp_ctx *ctx, int scheme, int sp_el, int sp_ion, uint64_t seed);
 
 /* ---- collision surrogate: no counterpart in the reference ----
@@ -271,6 +276,18 @@ int pcp_species_moments(pcp_ctx *ctx, int sp, double *out);
  * (cylinder r = (i+1/2) dr) and J_z (annulus at z = (k+1/2) dz, pi dr^2/4 at
  * the axis).  out[3] = max |residual|, max |charge change|, largest |q w|. */
 int pcp_continuity_residual(pcp_ctx *ctx, double *out);
+
+/* In-step FP64 energy log (no reference counterpart; the reference sums
+ * energies on the host from its output maps, src/specieP.cpp:665-839).
+ * While on, every pcp_step step records, on the device, one row of
+ * 2 + 8 doubles: the step number, the field energy (as pcp_field_energy,
+ * after the step's E and H updates) and the kinetic energy of each species
+ * (as pcp_kinetic_energy, after the move).  FP32 contexts sum the kinetic
+ * energies inside the fused particle kernel.  The log keeps the last 4096
+ * steps; pcp_energy_log copies the most recent min(max_rows, logged) rows,
+ * oldest first, into out (rows x 10 doubles) and sets *rows. */
+int pcp_set_energy_log(pcp_ctx *ctx, int on);
+int pcp_energy_log(pcp_ctx *ctx, long max_rows, double *out, long *rows);
 
 /* achievable HBM bandwidth of a plain device copy of `bytes` bytes (read +
  * write counted), averaged over iters launches [GB/s]; a calibration aid for

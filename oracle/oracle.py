@@ -57,6 +57,19 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+NATIVE_LIB_PATH = os.path.join(HERE, "_build", "libpicopic_oracle_native.so")
+
+
+def use_native_build():
+    """Load the -O3 -march=native build (made for this host on first use)
+    instead of the portable one; must be called before the first lib()."""
+    global LIB_PATH
+    if _lib is not None:
+        raise RuntimeError("the oracle library is already loaded")
+    subprocess.run(["make", "-s", "-C", HERE, "native"], check=True)
+    LIB_PATH = NATIVE_LIB_PATH
+
+
 def lib():
     global _lib
     if _lib is None:

@@ -124,6 +124,8 @@ def load(path=LIB_PATH):
     L.pcp_set_launch.argtypes = [vp, C.c_int, C.c_int]
     L.pcp_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
     L.pcp_autotune.argtypes = [vp, C.c_int, C.POINTER(Tuning)]
+    L.pcp_set_energy_log.argtypes = [vp, C.c_int]
+    L.pcp_energy_log.argtypes = [vp, C.c_long, dp, C.POINTER(C.c_long)]
     _lib = L
     return L
 
@@ -400,6 +402,19 @@ class Context:
         g = C.c_double()
         _check(_lib.pcp_kinetic_energy(self.h, sp, C.byref(g)))
         return g.value
+
+    def set_energy_log(self, on=True):
+        """Record the field and kinetic energies on the device in every step."""
+        _check(_lib.pcp_set_energy_log(self.h, 1 if on else 0))
+
+    def energy_log(self, max_rows=4096):
+        """Logged steps, oldest first: dict of arrays 'step', 'field' and
+        'kinetic' [rows, n_species] (J)."""
+        out = np.zeros((max_rows, 10))
+        rows = C.c_long()
+        _check(_lib.pcp_energy_log(self.h, max_rows, _dp(out), C.byref(rows)))
+        out = out[:rows.value]
+        return {"step": out[:, 0].astype(np.int64), "field": out[:, 1], "kinetic": out[:, 2:2 + self.n_species]}
 
     def sync(self):
         _check(_lib.pcp_sync(self.h))

@@ -166,7 +166,20 @@ struct Batch32 {
   int nsp;
   uint64_t *marker;    // strict-motion reset markers [D][P]
   uint32_t *cflag;     // cells near reset nodes [Nr][Nz]
+  // collision surrogate of the step (null: none): per global cell, for the
+  // electron (slot 0) and ion (slot 1) species, the mean velocity V and the
+  // factors (perp, par) of v -> V + a (v - V) (SURR_PAR_STRIDE floats per cell)
+  const float *cpar;
+  int cslot[MAX_SPECIES];   // -1: the species does not collide
 };
+
+__device__ __forceinline__ void apply_cell_rescale(const float *cpar, int cslot, uint32_t cell, float v[3]) {
+  const float *P = cpar + (size_t)cell * SURR_PAR_STRIDE + cslot * 8;
+  const float Vr = P[0], Vp = P[1], Vz = P[2], ap = P[3], az = P[4];
+  v[0] = Vr + ap * (v[0] - Vr);
+  v[1] = Vp + ap * (v[1] - Vp);
+  v[2] = Vz + az * (v[2] - Vz);
+}
 
 __device__ __forceinline__ float len2f(const float v[3]) { return v[0] * v[0] + v[1] * v[1] + v[2] * v[2]; }
 
@@ -437,15 +450,19 @@ __device__ __forceinline__ void append_migrant32(Migrants mg, StepState *st, int
 // velocity (steps whose collision phase sits between push and move).  The
 // fix-up pass calls it again on the pre-step slots and gets bit-identical
 // (n, v, rd, zd).
+// With !do_push and a surrogate collision phase (cpar, cslot >= 0), the
+// stored (pushed) velocity first takes its cell's rescaling.
 template <bool EXACT_END = true, bool D1 = false>
 __device__ __forceinline__ bool advance32(const Geom &g, const FieldsT<float> &f, const Species32 &s, long p,
                                           uint32_t cell, int from, int vfrom, bool do_push, const PushConst &K,
                                           float dtr, float dtz, Split &o, Split &n, float v[3], double &rd,
-                                          double &zd, int &d_old, int *err) {
+                                          double &zd, int &d_old, int *err, const float *cpar = nullptr,
+                                          int cslot = -1) {
   o = unpack(cell, s.fr[from][p], s.fz[from][p], g);
   d_old = domain_of32t<D1>(g, o.cr, o.cz);
   const Dom dm = make_dom_t<D1>(g, d_old);
   v[0] = s.vr[vfrom][p]; v[1] = s.vphi[vfrom][p]; v[2] = s.vz[vfrom][p];
+  if (!do_push && cpar && cslot >= 0) apply_cell_rescale(cpar, cslot, cell, v);
   if (do_push) {
     GridView32 EV{f.E + (long)d_old * 3 * g.P, g.P, g.yr, dm.bot_r, dm.left_z};
     GridView32 HV{f.H + (long)d_old * 3 * g.P, g.P, g.yr, dm.bot_r, dm.left_z};
@@ -478,6 +495,59 @@ __global__ void __launch_bounds__(256) k32_push(Geom g, FieldsT<float> f, Specie
     float v[3] = {s.vr[cur][p], s.vphi[cur][p], s.vz[cur][p]};
     push32(g.pusher, K, e, h, v, &err);
     s.vr[nxt][p] = v[0]; s.vphi[nxt][p] = v[1]; s.vz[nxt][p] = v[2];
+  }
+  if (err) atomicOr(&st->err, err);
+}
+
+// Gather + push of a step with the fused surrogate collision phase: the
+// pushed velocities go to slot cur ^ 1, and for a colliding species (slot >=
+// 0) their per-cell moments are added to mom[cell][slot][0..6]: count,
+// sum w, sum w v (r, phi, z), sum w v_z^2, sum w (v_r^2 + v_phi^2) -- the
+// sums behind the surrogate's temperatures.  A wavefront takes 64
+// consecutive (cell-ordered) particles; the lanes of a cell form runs whose
+// sums (DPP run sums in FP32) leave one FP64 atomic per run and moment.
+__global__ void __launch_bounds__(256) k32_push_mom(Geom g, FieldsT<float> f, Species32 s, int cur, StepState *st,
+                                                    PushConst K, double *mom, int slot) {
+  int err = 0;
+  const int nxt = cur ^ 1;
+  const int lane = threadIdx.x & 63;
+  const long waves = (s.n + 63) / 64;
+  for (long wv = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); wv < waves;
+       wv += (long)gridDim.x * (blockDim.x >> 6)) {
+    const long p = wv * 64 + lane;
+    uint32_t key = DEAD_CELL;
+    float v[3] = {0.f, 0.f, 0.f}, w = 0.f;
+    if (p < s.n) {
+      const uint32_t cell = s.cell[cur][p];
+      if (cell != DEAD_CELL) {
+        Split o = unpack(cell, s.fr[cur][p], s.fz[cur][p], g);
+        const int d = domain_of32(g, o.cr, o.cz);
+        const Dom dm = make_dom(g, d);
+        GridView32 EV{f.E + (long)d * 3 * g.P, g.P, g.yr, dm.bot_r, dm.left_z};
+        GridView32 HV{f.H + (long)d * 3 * g.P, g.P, g.yr, dm.bot_r, dm.left_z};
+        float e[3], h[3];
+        gather32_e(EV, o.cr, o.cz, o.fr, o.fz, e);
+        gather32_h(HV, o.cr, o.cz, o.fr, o.fz, h);
+        v[0] = s.vr[cur][p]; v[1] = s.vphi[cur][p]; v[2] = s.vz[cur][p];
+        push32(g.pusher, K, e, h, v, &err);
+        s.vr[nxt][p] = v[0]; s.vphi[nxt][p] = v[1]; s.vz[nxt][p] = v[2];
+        key = cell;
+        w = s.w[p];
+      }
+    }
+    if (slot >= 0) {
+      const Runs R = runs_of(key);
+      const bool live = key != DEAD_CELL;
+      float q[7] = {live ? 1.f : 0.f, w, w * v[0], w * v[1], w * v[2], w * v[2] * v[2],
+                    w * (v[0] * v[0] + v[1] * v[1])};
+#pragma unroll
+      for (int k = 0; k < 7; ++k) q[k] = run_sum_f(q[k], R);
+      if (R.tail && live) {
+        double *m = mom + (size_t)key * SURR_PAR_STRIDE + slot * 8;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) atomicAdd(m + k, (double)q[k]);
+      }
+    }
   }
   if (err) atomicOr(&st->err, err);
 }
@@ -527,10 +597,14 @@ __device__ __forceinline__ void deposit32(bool live, const Geom &g, const Dom &d
 // One wavefront trip of the fused step: particle p of this lane (the lanes
 // of a wavefront take consecutive particles; all 64 lanes take part, live
 // or not, since the deposit's run sums move data between lanes).
+// Returns the lane's kinetic energy w m v^2 / (s (1 + s)), s = sqrt(1 - v^2/c^2)
+// (= w m c^2 (gamma - 1) without cancellation) after the move when `ke` is
+// set, else 0.
 template <int CCS, bool PUSH, bool D1>
-__device__ __forceinline__ void step_trip32(const Geom &g, const FieldsT<float> &f, const Species32 &s, int sp,
-                                            const PushConst &K, long p, int cur, StepState *st, const Migrants &mg,
-                                            uint64_t *marker, uint32_t *cflag, int &err) {
+__device__ __forceinline__ float step_trip32(const Geom &g, const FieldsT<float> &f, const Species32 &s, int sp,
+                                             const PushConst &K, long p, int cur, StepState *st, const Migrants &mg,
+                                             uint64_t *marker, uint32_t *cflag, int &err, const float *cpar,
+                                             int cslot, bool ke) {
   const int nxt = cur ^ 1;
   const float dtr = g.dtr_f, dtz = g.dtz_f;
   const float inv_dt = g.inv_dt_f;
@@ -546,7 +620,7 @@ __device__ __forceinline__ void step_trip32(const Geom &g, const FieldsT<float> 
     const int vin = PUSH ? cur : nxt, vout = PUSH ? nxt : cur;
     if (cell != DEAD_CELL &&
         advance32<CCS != PCP_CCS_ZIGZAG, D1>(g, f, s, p, cell, cur, vin, PUSH, K, dtr, dtz, o, n, v, rd, zd, d_old,
-                                             &err)) {
+                                             &err, PUSH ? nullptr : cpar, cslot)) {
       s.vr[vout][p] = v[0]; s.vphi[vout][p] = v[1]; s.vz[vout][p] = v[2];
       s.cell[nxt][p] = (uint32_t)n.cr * (uint32_t)g.Nz + (uint32_t)n.cz;
       s.fr[nxt][p] = n.fr;
@@ -560,6 +634,12 @@ __device__ __forceinline__ void step_trip32(const Geom &g, const FieldsT<float> 
     }
   }
   if (!live) { o = n = Split{1, 1, 0.5f, 0.5f}; d_new = 0; v[1] = 0.f; w = 0.f; rd = 1.5 * g.dr; zd = 1.5 * g.dz; }
+  float kin = 0.f;
+  if (ke) {
+    const float b2 = len2f(v) * K.inv_c2;
+    const float sq = sqrt_f(1.f - b2);
+    kin = live ? w * (float)s.m * len2f(v) * rcp_f(sq * (1.f + sq)) : 0.f;
+  }
   const float qw = (float)s.q * w;
   // deposit domain: ZigZag deposits in the particle's new domain, VB in the
   // domain of its FP64 end point
@@ -567,6 +647,7 @@ __device__ __forceinline__ void step_trip32(const Geom &g, const FieldsT<float> 
   const Dom dn = make_dom_t<D1>(g, d_dep);
   deposit32<CCS>(live, g, dn, o, n, v[1], qw, inv_dt, rd, zd, s.q * (double)w, f, d_dep, st, &err, marker, cflag,
                  s.seq, p, sp);
+  return kin;
 }
 
 // The fused step of every species in one persistent launch.  A trip is one
@@ -584,6 +665,7 @@ struct BatchArgs32 {
   Batch32 b;
   StepState *st;
   Migrants mg;
+  double *ehist;   // energy log (null: off); kinetic energies summed per species
   int cur, chunk;
 };
 typedef __attribute__((address_space(4))) const BatchArgs32 KernargBatch32;
@@ -605,6 +687,15 @@ k32_step_batch(BatchArgs32 args) {
   KernargBatch32 *A = (KernargBatch32 *)__builtin_amdgcn_kernarg_segment_ptr();
   const long total = A->b.end[A->b.nsp - 1];   // trips of 64 particles
   const int lane = threadIdx.x & 63;
+  const bool ke = A->ehist != nullptr;
+  // kinetic energy of the trips since the last flush, all of species ke_sp
+  double ke_acc = 0.0;
+  int ke_sp = -1;
+  auto flush = [&]() {
+    const double tot = wave_sum(ke_acc);
+    if (lane == 0 && ke_sp >= 0) atomicAdd(ehist_row(A->ehist, A->st->step) + 2 + A->b.sp[ke_sp], tot);
+    ke_acc = 0.0;
+  };
   for (;;) {
     KernargBatch32 *a = A;
     asm volatile("" : "+s"(a));
@@ -620,10 +711,17 @@ k32_step_batch(BatchArgs32 args) {
       int k = 0;
       while (k + 1 < x.b.nsp && t >= x.b.end[k]) ++k;
       const long first = k ? x.b.end[k - 1] : 0;
-      step_trip32<CCS, PUSH, D1>(x.g, x.f, x.b.s[k], x.b.sp[k], x.b.K[k], (t - first) * 64 + lane, x.cur, x.st, x.mg,
-                             x.b.marker, x.b.cflag, err);
+      if (ke && k != ke_sp) {
+        flush();
+        ke_sp = k;
+      }
+      const float kin = step_trip32<CCS, PUSH, D1>(x.g, x.f, x.b.s[k], x.b.sp[k], x.b.K[k], (t - first) * 64 + lane,
+                                                   x.cur, x.st, x.mg, x.b.marker, x.b.cflag, err, x.b.cpar,
+                                                   x.b.cslot[k], ke);
+      if (ke) ke_acc += (double)kin;
     }
   }
+  if (ke) flush();
   if (err) atomicOr(&A->st->err, err);
 }
 
@@ -667,7 +765,7 @@ __global__ void __launch_bounds__(256) k32_fix(Geom g, FieldsT<float> f, Batch32
       double r, z;
       int d_old;
       if (!advance32(g, f, s, p, co[j], cur ^ 1, cur ^ 1, do_push != 0, b.K[k], dtr, dtz, o, n, v, r, z, d_old,
-                     &err))
+                     &err, do_push ? nullptr : b.cpar, b.cslot[k]))
         continue;
       const double ro = ((double)o.cr + (double)o.fr) * g.dr, zo = ((double)o.cz + (double)o.fz) * g.dz;
       const int d = deposit_domain32(g, r, z);
@@ -838,8 +936,14 @@ __global__ void k32_gather_point(Geom g, FieldsT<float> f, int d, double r, doub
 // ---------------------------------------------------------------------------
 // FP64 energy reductions (diagnostics)
 // ---------------------------------------------------------------------------
+// out: the sum's destination; with st, out is the energy log and the sum
+// goes to column `col` of the current step's row
+__device__ __forceinline__ double *log_dest(double *out, const StepState *st, int col) {
+  return st ? ehist_row(out, st->step) + col : out;
+}
+
 template <class T>
-__global__ void k_field_energy(Geom g, FieldsT<T> f, double *out) {
+__global__ void k_field_energy(Geom g, FieldsT<T> f, double *out, const StepState *st) {
   double acc = 0;
   long total = (long)g.Dr * g.Dz * g.nr * g.nz;
   for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
@@ -866,7 +970,7 @@ __global__ void k_field_energy(Geom g, FieldsT<T> f, double *out) {
     if ((int)threadIdx.x < s2) red[threadIdx.x] += red[threadIdx.x + s2];
     __syncthreads();
   }
-  if (threadIdx.x == 0) atomicAdd(out, red[0]);
+  if (threadIdx.x == 0) atomicAdd(log_dest(out, st, 1), red[0]);
 }
 
 template <class V>
@@ -878,7 +982,7 @@ __device__ __forceinline__ double kin(double m, double w, V vr, V vp, V vz) {
   return w * m * v2 / (s * (1.0 + s));
 }
 
-__global__ void k_kinetic_energy64(SpeciesDev s, int cur, double *out) {
+__global__ void k_kinetic_energy64(SpeciesDev s, int cur, double *out, const StepState *st, int col) {
   double acc = 0;
   for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < s.n; p += (long)gridDim.x * blockDim.x)
     if (s.r[cur][p] >= 0) acc += kin(s.m, s.w[p], s.vr[p], s.vphi[p], s.vz[p]);
@@ -889,7 +993,7 @@ __global__ void k_kinetic_energy64(SpeciesDev s, int cur, double *out) {
     if ((int)threadIdx.x < s2) red[threadIdx.x] += red[threadIdx.x + s2];
     __syncthreads();
   }
-  if (threadIdx.x == 0) atomicAdd(out, red[0]);
+  if (threadIdx.x == 0) atomicAdd(log_dest(out, st, col), red[0]);
 }
 
 __global__ void k_kinetic_energy32(Species32 s, int cur, double *out) {
@@ -992,6 +1096,15 @@ void launch32_push(pcp_ctx *c, int sp) {
                      push_const(s, c->g.dt));
 }
 
+void launch32_push_mom(pcp_ctx *c, int sp, double *mom, int slot) {
+  Species32 &s = c->s32->sp[sp];
+  if (!s.n) return;
+  const long waves = (s.n + 63) / 64;
+  const unsigned grid = (unsigned)std::max(1L, std::min((waves + 3) / 4, (long)c->n_cu * 32));
+  hipLaunchKernelGGL(k32_push_mom, dim3(grid), dim3(256), 0, c->stream, c->g, c->s32->f, s, c->cur, c->st,
+                     push_const(s, c->g.dt), mom, slot);
+}
+
 // consecutive wavefront trips per counter increment of the batched step
 constexpr int TRIPS_PER_TICKET = 16;
 
@@ -1005,7 +1118,7 @@ static void launch_batch_t(pcp_ctx *c, const Batch32 &b) {
   const int chunk = TRIPS_PER_TICKET;
   const long waves = (b.end[b.nsp - 1] + chunk - 1) / chunk;
   const long grid = std::max(1L, std::min((waves + BLOCK / 64 - 1) / (BLOCK / 64), (long)c->n_cu * per_cu));
-  BatchArgs32 args{c->g, c->s32->f, b, c->st, c->mig, c->cur, chunk};
+  BatchArgs32 args{c->g, c->s32->f, b, c->st, c->mig, c->elog ? c->ehist : nullptr, c->cur, chunk};
   hipLaunchKernelGGL((k32_step_batch<CCS, PUSH, BLOCK, D1>), dim3((unsigned)grid), dim3(BLOCK), 0, c->stream, args);
 }
 
@@ -1024,7 +1137,14 @@ static void launch_batch_ccs(pcp_ctx *c, const Batch32 &b, bool push, int block)
   else launch_batch_t<CCS, true, 256, true>(c, b);
 }
 
-void launch32_step_batch(pcp_ctx *c, bool push) {
+// the collision slots of the species (electrons 0, ions 1, others -1) when
+// the step's surrogate rescaling `cpar` is given
+static void batch_coll(pcp_ctx *c, Batch32 &b, const float *cpar, int se, int si) {
+  b.cpar = cpar;
+  for (int k = 0; k < b.nsp; ++k) b.cslot[k] = !cpar ? -1 : b.sp[k] == se ? 0 : b.sp[k] == si ? 1 : -1;
+}
+
+void launch32_step_batch(pcp_ctx *c, bool push, const float *cpar, int se, int si) {
   Batch32 b{};
   const int block = push ? c->tune.block : 256;
   long acc = 0;   // trips of 64 particles
@@ -1040,6 +1160,7 @@ void launch32_step_batch(pcp_ctx *c, bool push) {
   }
   b.marker = c->marker;
   b.cflag = c->cell_flag;
+  batch_coll(c, b, cpar, se, si);
   if (b.nsp) {
     if (c->g.ccs == PCP_CCS_ZIGZAG) launch_batch_ccs<PCP_CCS_ZIGZAG>(c, b, push, block);
     else launch_batch_ccs<PCP_CCS_VB>(c, b, push, block);
@@ -1053,7 +1174,7 @@ void launch32_step_batch(pcp_ctx *c, bool push) {
     }
 }
 
-void launch32_fix(pcp_ctx *c, bool push) {
+void launch32_fix(pcp_ctx *c, bool push, const float *cpar, int se, int si) {
   if (c->g.ccs != PCP_CCS_VB) return;
   Batch32 b{};
   long acc = 0;
@@ -1070,6 +1191,7 @@ void launch32_fix(pcp_ctx *c, bool push) {
   if (!b.nsp) return;
   b.marker = c->marker;
   b.cflag = c->cell_flag;
+  batch_coll(c, b, cpar, se, si);
   // 256-particle units, 64 threads per unit
   const unsigned grid = (unsigned)std::max(1L, std::min((acc + 3) / 4, (long)c->n_cu * 16));
   hipLaunchKernelGGL(k32_fix, dim3(grid), dim3(256), 0, c->stream, c->g, c->s32->f, b, c->cur, c->st, push ? 1 : 0);
@@ -1137,13 +1259,14 @@ void launch32_gather_point(pcp_ctx *c, int d, double r, double z, int which, dou
   hipLaunchKernelGGL(k32_gather_point, dim3(1), dim3(1), 0, c->stream, c->g, c->s32->f, d, r, z, which, dev_out);
 }
 
-void launch_field_energy(pcp_ctx *c, double *dev_out) {
+void launch_field_energy(pcp_ctx *c, double *dev_out, bool to_log) {
   long total = (long)c->D * c->g.nr * c->g.nz;
   unsigned grid = (unsigned)std::min<long>((total + 255) / 256, (long)c->n_cu * 8);
+  const StepState *st = to_log ? c->st : nullptr;
   if (c->s32)
-    hipLaunchKernelGGL(k_field_energy<float>, dim3(grid), dim3(256), 0, c->stream, c->g, c->s32->f, dev_out);
+    hipLaunchKernelGGL(k_field_energy<float>, dim3(grid), dim3(256), 0, c->stream, c->g, c->s32->f, dev_out, st);
   else
-    hipLaunchKernelGGL(k_field_energy<double>, dim3(grid), dim3(256), 0, c->stream, c->g, c->f, dev_out);
+    hipLaunchKernelGGL(k_field_energy<double>, dim3(grid), dim3(256), 0, c->stream, c->g, c->f, dev_out, st);
 }
 
 // velocity moments of a species: sum w, sum w v (3), sum w v_z^2,
@@ -1208,8 +1331,16 @@ void launch_kinetic_energy(pcp_ctx *c, int sp, double *dev_out) {
   } else {
     SpeciesDev &s = c->sp[sp];
     if (!s.n) return;
-    hipLaunchKernelGGL(k_kinetic_energy64, dim3(pgrid32(c, s.n, 256)), dim3(256), 0, c->stream, s, c->cur, dev_out);
+    hipLaunchKernelGGL(k_kinetic_energy64, dim3(pgrid32(c, s.n, 256)), dim3(256), 0, c->stream, s, c->cur, dev_out,
+                       nullptr, 0);
   }
+}
+
+void launch_kinetic_energy_log(pcp_ctx *c, int sp) {
+  SpeciesDev &s = c->sp[sp];
+  if (!s.n || c->s32) return;
+  hipLaunchKernelGGL(k_kinetic_energy64, dim3(pgrid32(c, s.n, 256)), dim3(256), 0, c->stream, s, c->cur, c->ehist,
+                     c->st, 2 + sp);
 }
 
 }  // namespace pcp

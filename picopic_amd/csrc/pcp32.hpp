@@ -19,6 +19,13 @@ namespace pcp {
 
 constexpr uint32_t DEAD_CELL = 0xFFFFFFFFu;
 
+// fused surrogate collision phase of an FP32 step: per global cell and
+// collision slot (0 electrons, 1 ions), 8 moments of the pushed velocities
+// (count, sum w, sum w v_r, sum w v_phi, sum w v_z, sum w v_z^2,
+// sum w (v_r^2 + v_phi^2), -) and 8 rescale parameters (V_r, V_phi, V_z,
+// a_perp, a_par, -, -, -) of v -> V + a (v - V)
+constexpr int SURR_PAR_STRIDE = 16;
+
 struct Species32 {
   uint32_t *cell[2];
   float *fr[2], *fz[2];
@@ -54,12 +61,18 @@ void launch32_stitch(pcp_ctx *c, int which);
 // collided beforehand; the launcher exchanges the velocity slots after the
 // move so that slot cur ^ 1 keeps the pre-move velocities for the exact path
 void launch32_push(pcp_ctx *c, int sp);
-// the fused step of all species in one persistent work-queue launch
-void launch32_step_batch(pcp_ctx *c, bool push = true);
+// push of a step with the fused surrogate collision phase; slot >= 0 adds
+// the pushed velocities' per-cell moments of the species to mom (layout of
+// SURR_PAR_STRIDE doubles per global cell, 8 per collision slot)
+void launch32_push_mom(pcp_ctx *c, int sp, double *mom, int slot);
+// the fused step of all species in one persistent work-queue launch; with
+// cpar (and !push) the electron (se) and ion (si) velocities take their
+// cell's surrogate rescaling before the move
+void launch32_step_batch(pcp_ctx *c, bool push = true, const float *cpar = nullptr, int se = -1, int si = -1);
 // fix-up pass after a step whose strict motions reset J_r nodes (no-op
 // otherwise, decided on the device); push: the step pushed (no collision
 // phase)
-void launch32_fix(pcp_ctx *c, bool push);
+void launch32_fix(pcp_ctx *c, bool push, const float *cpar = nullptr, int se = -1, int si = -1);
 void launch32_migrant_keys(pcp_ctx *c);
 int launch32_sort(pcp_ctx *c, int sp);
 void launch32_count_live(pcp_ctx *c, int sp, unsigned long long *dev_out);
@@ -71,8 +84,11 @@ void launch32_gather_point(pcp_ctx *c, int d, double r, double z, int which, dou
 // FP64 reductions: field energy sum over interior nodes of
 // eps0 E^2 / 2 + mu0 H^2 / 2 weighted by the cell volume; kinetic energy
 // sum of w m c^2 (gamma - 1) of a species
-void launch_field_energy(pcp_ctx *c, double *dev_out);
+void launch_field_energy(pcp_ctx *c, double *dev_out, bool to_log = false);
 void launch_kinetic_energy(pcp_ctx *c, int sp, double *dev_out);
+// FP64 contexts with the energy log on: the kinetic energy of species sp
+// into the current step's row (FP32 contexts sum it inside the step kernel)
+void launch_kinetic_energy_log(pcp_ctx *c, int sp);
 // out[7] += sum w, sum w v_r, sum w v_phi, sum w v_z, sum w v_z^2,
 // sum w (v_r^2 + v_phi^2), number of live particles
 void launch_species_moments(pcp_ctx *c, int sp, double *dev_out);

@@ -37,6 +37,40 @@ int fail(int code, const std::string &msg) {
       return fail(PCP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
   } while (0)
 
+// device scratch released on every return path
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t n) { return hipMalloc((void **)&p, (n ? n : 1) * sizeof(T)); }
+};
+
+// HIP events destroyed on every return path
+struct EventSet {
+  std::vector<hipEvent_t> ev;
+  EventSet() = default;
+  EventSet(const EventSet &) = delete;
+  EventSet &operator=(const EventSet &) = delete;
+  ~EventSet() {
+    for (auto e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+  hipError_t create(size_t n) {
+    ev.assign(n, nullptr);
+    for (auto &e : ev)
+      if (hipError_t r = hipEventCreate(&e)) {
+        e = nullptr;
+        return r;
+      }
+    return hipSuccess;
+  }
+};
+
 template <class T>
 int dalloc(T **p, size_t n) {
   if (n == 0) n = 1;
@@ -218,7 +252,8 @@ int pcp_create(const pcp_config *cfg, pcp_ctx **out) {
   int rc = 0;
   if ((rc = dalloc(&c->f.E, gridn)) || (rc = dalloc(&c->f.H, gridn)) || (rc = dalloc(&c->f.Hat, gridn)) ||
       (rc = dalloc(&c->f.J, gridn)) || (rc = dalloc(&c->f.eps, gp)) || (rc = dalloc(&c->f.sigma, gp)) ||
-      (rc = dalloc(&c->st, 1)) || (rc = dalloc(&c->marker, (size_t)c->D * g.P)) ||
+      (rc = dalloc(&c->st, 1)) || (rc = dalloc(&c->diag, DIAG_DOUBLES)) ||
+      (rc = dalloc(&c->marker, (size_t)c->D * g.P)) ||
       (rc = dalloc(&c->cell_flag, (size_t)g.Nr * g.Nz))) {
     pcp_destroy(c);
     return rc;
@@ -330,6 +365,7 @@ int pcp_destroy(pcp_ctx *c) {
   mlp_free(c);
   hipFree(c->f.E); hipFree(c->f.H); hipFree(c->f.Hat); hipFree(c->f.J);
   hipFree(c->f.eps); hipFree(c->f.sigma); hipFree(c->st); hipFree(c->marker); hipFree(c->cell_flag);
+  hipFree(c->diag); hipFree(c->ehist);
   for (int s = 0; s < MAX_SPECIES; ++s) {
     SpeciesDev &sp = c->sp[s];
     hipFree(sp.r[0]); hipFree(sp.r[1]); hipFree(sp.z[0]); hipFree(sp.z[1]);
@@ -434,14 +470,12 @@ int pcp_upload_species(pcp_ctx *c, int sp, long n, const double *r, const double
 int pcp_count(pcp_ctx *c, int sp, long *n) {
   if (!c || !n || sp < 0 || sp >= c->cfg.n_species) return fail(PCP_E_ARG, "bad argument");
   if (c->sp[sp].n == 0) { *n = 0; return 0; }
-  unsigned long long *d = nullptr, h = 0;
-  HIP_TRY(hipMalloc(&d, sizeof h));
+  unsigned long long *d = (unsigned long long *)c->diag, h = 0;
   HIP_TRY(hipMemsetAsync(d, 0, sizeof h, c->stream));
   if (c->s32) launch32_count_live(c, sp, d);
   else launch_count_live(c, sp, d);
   HIP_TRY(hipMemcpyAsync(&h, d, sizeof h, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  hipFree(d);
   *n = (long)h;
   return 0;
 }
@@ -591,17 +625,15 @@ int pcp_init_plasma(pcp_ctx *c, int sp, int n_per_cell, double density_left, dou
   double weight = n_peak * PI * g.gsize_r * g.gsize_r * g.gsize_z / ((double)n_per_cell * g.Nr * g.Nz);
   double m = c->sp[sp].m;
   double vth = std::sqrt(temperature_ev * EL_CHARGE / m);
-  long *d_off = nullptr, *d_doff = nullptr;
-  HIP_TRY(hipMalloc(&d_off, sizeof(long) * (ncell + 1)));
-  HIP_TRY(hipMalloc(&d_doff, sizeof(long) * c->D));
-  HIP_TRY(hipMemcpy(d_off, off.data(), sizeof(long) * (ncell + 1), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(d_doff, doff.data(), sizeof(long) * c->D, hipMemcpyHostToDevice));
-  if (c->s32) launch32_init_plasma(c, sp, c->s32->sp[sp].n, total, d_off, ncell, weight, vth, seed, d_doff);
-  else launch_init_plasma(c, sp, c->sp[sp].n, total, d_off, ncell, weight, vth, vth, vth, seed, d_doff);
+  DevBuf<long> d_off, d_doff;
+  HIP_TRY(d_off.alloc(ncell + 1));
+  HIP_TRY(d_doff.alloc(c->D));
+  HIP_TRY(hipMemcpy(d_off.p, off.data(), sizeof(long) * (ncell + 1), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d_doff.p, doff.data(), sizeof(long) * c->D, hipMemcpyHostToDevice));
+  if (c->s32) launch32_init_plasma(c, sp, c->s32->sp[sp].n, total, d_off.p, ncell, weight, vth, seed, d_doff.p);
+  else launch_init_plasma(c, sp, c->sp[sp].n, total, d_off.p, ncell, weight, vth, vth, vth, seed, d_doff.p);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(c->stream));
-  hipFree(d_off);
-  hipFree(d_doff);
   HIP_TRY(hipMemcpy(c->mig.gbase, gbase.data(), gbase.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
   // the kernel wrote both position buffers
   c->sp[sp].n += total;
@@ -993,6 +1025,8 @@ int pcp_set_collisions(pcp_ctx *c, int scheme, int se, int si, uint64_t seed) {
     return 0;
   }
   if (int rc = check_coll_args(c, scheme, se, si)) return rc;
+  if (scheme == COLL_SURROGATE && c->s32 && fsurr_prepare(c, se, si) < 0)
+    return fail(PCP_E_NOMEM, "surrogate collision buffers");
   c->coll_scheme = scheme;
   c->coll_se = se;
   c->coll_si = si;
@@ -1021,9 +1055,20 @@ static int step_body(pcp_ctx *c, Mark mark) {
     launch_yee_h(c);
     launch_stitch(c, PCP_GRID_H);
   }
+  if (c->elog) launch_field_energy(c, c->ehist, true);
   mark(1);
   const bool coll = c->coll_scheme >= 0;
-  if (coll) {
+  // FP32 with the surrogate: the fused phase (moments in the push, per-cell
+  // rescalings applied by the move kernel)
+  const bool fused = coll && f32 && c->coll_scheme == COLL_SURROGATE;
+  const float *cpar = fused ? fsurr_par(c) : nullptr;
+  if (fused) {
+    double *mom = fsurr_mom(c);
+    for (int s = 0; s < c->cfg.n_species; ++s)
+      launch32_push_mom(c, s, mom, s == c->coll_se ? 0 : s == c->coll_si ? 1 : -1);
+    if (fsurr_cells(c, c->coll_se, c->coll_si, c->coll_seed, c->coll_calls++) < 0)
+      return fail(PCP_E_HIP, "surrogate collision phase failed");
+  } else if (coll) {
     // SMB::advance_particles: push, collide (before the move), move
     for (int s = 0; s < c->cfg.n_species; ++s) {
       if (f32) launch32_push(c, s);
@@ -1032,25 +1077,29 @@ static int step_body(pcp_ctx *c, Mark mark) {
     if (launch_collide(c, c->coll_scheme, c->coll_se, c->coll_si, c->coll_seed, c->coll_calls++, c->cur ^ 1) < 0)
       return fail(PCP_E_HIP, "collision pass failed");
   }
-  if (f32) launch32_step_batch(c, !coll);
+  mark(2);
+  if (f32) launch32_step_batch(c, !coll, cpar, c->coll_se, c->coll_si);
   else
     for (int s = 0; s < c->cfg.n_species; ++s) launch_step_fused(c, s, !coll);
   c->cur ^= 1;
-  mark(2);
+  mark(3);
   if (f32) launch32_migrant_keys(c);
   else launch_migrant_keys(c);
   // strict axial motions reset J_r nodes: take back the contributions of
   // the particles that precede the resetting one (device-side no-op when no
   // reset happened)
   if (c->g.ccs == PCP_CCS_VB) {
-    if (f32) launch32_fix(c, !coll);
+    if (f32) launch32_fix(c, !coll, cpar, c->coll_se, c->coll_si);
     else
       for (int s = 0; s < c->cfg.n_species; ++s) launch_fix(c, s);
   }
-  mark(3);
+  mark(4);
   if (f32) launch32_stitch(c, PCP_GRID_J);
   else launch_stitch(c, PCP_GRID_J);
-  mark(4);
+  // FP64 kinetic energies of the log (FP32 sums them in the particle kernel)
+  if (c->elog && !f32)
+    for (int s = 0; s < c->cfg.n_species; ++s) launch_kinetic_energy_log(c, s);
+  mark(5);
   return 0;
 }
 
@@ -1074,16 +1123,16 @@ int pcp_set_graph(pcp_ctx *c, int on) {
 int pcp_step(pcp_ctx *c, int n_steps) {
   if (!c || n_steps < 0) return fail(PCP_E_ARG, "bad argument");
   // optional per-phase device timing: events on the context stream
-  static const char *names[4] = {"fields", "particles", "fixup", "stitch_j"};
-  std::vector<hipEvent_t> ev;
-  if (c->timing) {
-    ev.resize((size_t)n_steps * 5);
-    for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
-  }
+  // fields: Yee + stitches; collide: the push and collision phase of a
+  // step with collisions; particles: the fused particle kernel(s)
+  static const char *names[5] = {"fields", "collide", "particles", "fixup", "stitch_j"};
+  EventSet evs;
+  if (c->timing) HIP_TRY(evs.create((size_t)n_steps * 6));
+  std::vector<hipEvent_t> &ev = evs.ev;
   const bool f32 = c->s32 != nullptr;
   for (int t = 0; t < n_steps; ++t) {
     auto mark = [&](int k) {
-      if (c->timing) (void)hipEventRecord(ev[(size_t)t * 5 + k], c->stream);
+      if (c->timing) (void)hipEventRecord(ev[(size_t)t * 6 + k], c->stream);
     };
     mark(0);
     // the sorts change the particle buffers, so they run outside any graph
@@ -1119,21 +1168,20 @@ int pcp_step(pcp_ctx *c, int n_steps) {
   HIP_TRY(hipGetLastError());
   int rc = pcp_sync(c);
   if (c->timing && n_steps > 0) {
-    double acc[4] = {0, 0, 0, 0};
+    double acc[5] = {0, 0, 0, 0, 0};
     for (int t = 0; t < n_steps; ++t)
-      for (int k = 0; k < 4; ++k) {
+      for (int k = 0; k < 5; ++k) {
         float ms = 0;
-        (void)hipEventElapsedTime(&ms, ev[(size_t)t * 5 + k], ev[(size_t)t * 5 + k + 1]);
+        (void)hipEventElapsedTime(&ms, ev[(size_t)t * 6 + k], ev[(size_t)t * 6 + k + 1]);
         acc[k] += ms;
       }
     c->ktimes.clear();
     double tot = 0;
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < 5; ++k) {
       c->ktimes.emplace_back(names[k], acc[k] / n_steps);
       tot += acc[k];
     }
     c->ktimes.emplace_back("step", tot / n_steps);
-    for (auto &e : ev) (void)hipEventDestroy(e);
   }
   return rc;
 }
@@ -1312,13 +1360,11 @@ int pcp_get_field_e(pcp_ctx *c, int d, double r, double z, double *out) {
     if (!(r >= r0 && r < r0 + c->g.nr * c->g.dr && z >= z0 && z < z0 + c->g.nz * c->g.dz))
       return fail(PCP_E_ARG, "point outside domain");
   }
-  double *dev;
-  HIP_TRY(hipMalloc(&dev, 3 * sizeof(double)));
+  double *dev = c->diag;
   if (c->s32) launch32_gather_point(c, d, r, z, 0, dev);
   else launch_gather_point(c, d, r, z, 0, dev);
   HIP_TRY(hipMemcpyAsync(out, dev, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  hipFree(dev);
   return 0;
 }
 
@@ -1330,65 +1376,88 @@ int pcp_get_field_h(pcp_ctx *c, int d, double r, double z, double *out) {
     if (!(r >= r0 && r < r0 + c->g.nr * c->g.dr && z >= z0 && z < z0 + c->g.nz * c->g.dz))
       return fail(PCP_E_ARG, "point outside domain");
   }
-  double *dev;
-  HIP_TRY(hipMalloc(&dev, 3 * sizeof(double)));
+  double *dev = c->diag;
   if (c->s32) launch32_gather_point(c, d, r, z, 1, dev);
   else launch_gather_point(c, d, r, z, 1, dev);
   HIP_TRY(hipMemcpyAsync(out, dev, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  hipFree(dev);
   return 0;
 }
 
 int pcp_field_energy(pcp_ctx *c, double *out) {
   if (!c || !out) return fail(PCP_E_ARG, "bad argument");
-  double *d;
-  HIP_TRY(hipMalloc(&d, sizeof(double)));
+  double *d = c->diag;
   HIP_TRY(hipMemsetAsync(d, 0, sizeof(double), c->stream));
   launch_field_energy(c, d);
   HIP_TRY(hipMemcpyAsync(out, d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  (void)hipFree(d);
   return 0;
 }
 
 int pcp_kinetic_energy(pcp_ctx *c, int sp, double *out) {
   if (!c || !out || sp < 0 || sp >= c->cfg.n_species) return fail(PCP_E_ARG, "bad argument");
-  double *d;
-  HIP_TRY(hipMalloc(&d, sizeof(double)));
+  double *d = c->diag;
   HIP_TRY(hipMemsetAsync(d, 0, sizeof(double), c->stream));
   launch_kinetic_energy(c, sp, d);
   HIP_TRY(hipMemcpyAsync(out, d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  (void)hipFree(d);
   return 0;
 }
 
 int pcp_species_moments(pcp_ctx *c, int sp, double *out) {
   if (!c || !out || sp < 0 || sp >= c->cfg.n_species) return fail(PCP_E_ARG, "bad argument");
-  double *d;
-  HIP_TRY(hipMalloc(&d, 7 * sizeof(double)));
+  double *d = c->diag;
   HIP_TRY(hipMemsetAsync(d, 0, 7 * sizeof(double), c->stream));
   launch_species_moments(c, sp, d);
   HIP_TRY(hipMemcpyAsync(out, d, 7 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  (void)hipFree(d);
   return 0;
 }
 
 int pcp_continuity_residual(pcp_ctx *c, double *out) {
   if (!c || !out) return fail(PCP_E_ARG, "bad argument");
-  double *d;
-  HIP_TRY(hipMalloc(&d, 3 * sizeof(double)));
+  double *d = c->diag;
   if (launch_continuity_residual(c, d) < 0) return fail(PCP_E_HIP, "continuity residual");
   HIP_TRY(hipMemcpy(out, d, 3 * sizeof(double), hipMemcpyDeviceToHost));
-  (void)hipFree(d);
   return 0;
 }
 
 int pcp_copy_bandwidth(pcp_ctx *c, long bytes, int iters, double *gbps) {
   if (!c || !gbps || bytes < 16 || iters < 1) return fail(PCP_E_ARG, "bad argument");
   if (bench_copy(c, bytes, iters, gbps) != 0) return fail(PCP_E_NOMEM, "copy buffers");
+  return 0;
+}
+
+int pcp_set_energy_log(pcp_ctx *c, int on) {
+  if (!c) return fail(PCP_E_ARG, "null context");
+  if (on && !c->ehist) {
+    int rc = dalloc(&c->ehist, (size_t)EHIST * EHIST_COLS);
+    if (rc) return rc;
+  }
+  graph_invalidate(c);
+  c->elog = on != 0;
+  return 0;
+}
+
+int pcp_energy_log(pcp_ctx *c, long max_rows, double *out, long *rows) {
+  if (!c || !out || !rows || max_rows < 0) return fail(PCP_E_ARG, "bad argument");
+  *rows = 0;
+  if (!c->ehist) return 0;
+  std::vector<double> h((size_t)EHIST * EHIST_COLS);
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(h.data(), c->ehist, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+  // rows in step order, the most recent max_rows of them
+  std::vector<long> order;
+  for (long k = 0; k < EHIST; ++k)
+    if (h[(size_t)k * EHIST_COLS] > 0) order.push_back(k);
+  std::sort(order.begin(), order.end(),
+            [&](long a, long b) { return h[(size_t)a * EHIST_COLS] < h[(size_t)b * EHIST_COLS]; });
+  const long n = std::min<long>((long)order.size(), max_rows);
+  const long first = (long)order.size() - n;
+  for (long k = 0; k < n; ++k)
+    for (int j = 0; j < EHIST_COLS; ++j)
+      out[k * EHIST_COLS + j] = h[(size_t)order[first + k] * EHIST_COLS + j];
+  *rows = n;
   return 0;
 }
 

@@ -86,9 +86,15 @@ __device__ __forceinline__ uint32_t local_cell_key(const Geom &g, int cr, int cz
   return (uint32_t)((long)d * g.nr * g.nz + lr * g.nz + lz);
 }
 
+// gate (optional): a device counter; the pass does nothing while it is 0.
+// mask (optional): only particles of cells with mask[key] != 0 are bucketed.
+__device__ __forceinline__ bool gate_closed(const unsigned long long *gate) { return gate && *gate == 0ull; }
+
 template <class V>
 __global__ void __launch_bounds__(256) k_coll_key(Geom g, V v, uint32_t *key, uint32_t *slot, uint32_t *cnt,
-                                                  uint32_t sentinel) {
+                                                  uint32_t sentinel, const int *mask,
+                                                  const unsigned long long *gate) {
+  if (gate_closed(gate)) return;
   const long n = v.n();
   const long stride = (long)gridDim.x * blockDim.x;
   for (long b0 = (long)blockIdx.x * blockDim.x; b0 < n; b0 += stride) {
@@ -96,6 +102,7 @@ __global__ void __launch_bounds__(256) k_coll_key(Geom g, V v, uint32_t *key, ui
     uint32_t k = sentinel;
     int cr, cz;
     if (p < n && v.cell_of(g, p, cr, cz)) k = local_cell_key(g, cr, cz);
+    if (mask && k != sentinel && !mask[k]) k = sentinel;
     const bool a = p < n && k != sentinel;
     uint32_t s = wave_reserve(cnt, a ? k : 0, a);
     if (p < n) {
@@ -106,7 +113,8 @@ __global__ void __launch_bounds__(256) k_coll_key(Geom g, V v, uint32_t *key, ui
 }
 
 __global__ void k_coll_scatter(long n, const uint32_t *key, const uint32_t *slot, const uint32_t *off,
-                               uint32_t *perm, uint32_t sentinel) {
+                               uint32_t *perm, uint32_t sentinel, const unsigned long long *gate) {
+  if (gate_closed(gate)) return;
   for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long)gridDim.x * blockDim.x) {
     uint32_t k = key[p];
     if (k != sentinel) perm[off[k] + slot[p]] = (uint32_t)p;
@@ -232,9 +240,11 @@ template <class V>
 __global__ void __launch_bounds__(COLL_BLOCK) k_coll_prep(Geom g, V ve, V vi, const uint32_t *off_e,
                                                           const uint32_t *off_i, uint32_t *perm_e, uint32_t *perm_i,
                                                           double m_e, double m_i, uint64_t seed, uint64_t step,
-                                                          double *tot, int *row_bad, const int *mask, StepState *st) {
+                                                          double *tot, int *row_bad, const int *mask, StepState *st,
+                                                          const unsigned long long *gate) {
   __shared__ uint64_t skey[COLL_MAXC];
   __shared__ CellStage S;
+  if (gate_closed(gate)) return;
   const long ncells = (long)g.Dr * g.Dz * g.nr * g.nz;
   int err = 0;
   for (long c = blockIdx.x; c < ncells; c += gridDim.x) {
@@ -381,8 +391,9 @@ __global__ void __launch_bounds__(COLL_BLOCK) k_coll_pairs(Geom g, V ve, V vi, c
                                                            const uint32_t *perm_i, double m_e, double m_i,
                                                            double q_e, double q_i, uint64_t seed, uint64_t step,
                                                            const double *tot, const int *row_bad, const int *mask,
-                                                           StepState *st) {
+                                                           StepState *st, const unsigned long long *gate) {
   __shared__ CellStage S;
+  if (gate_closed(gate)) return;
   const long ncells = (long)g.Dr * g.Dz * g.nr * g.nz;
   int err = 0;
   for (long c = blockIdx.x; c < ncells; c += gridDim.x) {
@@ -699,6 +710,101 @@ __global__ void __launch_bounds__(COLL_BLOCK) k_surr_apply(Geom g, V ve, V vi, c
 
 __global__ void k_fill_zero_ull(unsigned long long *p) { *p = 0ull; }
 
+// ---------------------------------------------------------------------------
+// Fused surrogate phase of an FP32 step.  The push kernel (k32_push_mom)
+// leaves each global cell's moments of the pushed velocities in mom; the
+// network runs on the features of every cell; k_fsurr_params turns its
+// outputs into per-cell rescalings that the move kernel applies to each
+// particle on the fly (no buckets, no staging).  The rescaling is the one of
+// k_surr_apply, with the energy-restoring factor beta computed from the
+// moments: for species s with thermal sums P_perp = sum w (v_r^2 + v_phi^2)
+// - |sum w v_perp|^2 / sum w and P_par likewise,
+//   beta^2 = sum_s m_s (P_perp + P_par) / sum_s m_s (a_perp^2 P_perp + a_par^2 P_par).
+// Cells the network does not cover keep their velocities here and are
+// flagged (by local cell key) for the P12 fallback.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void global_cell(const Geom &g, long c, long &cr, long &cz) {
+  cr = (long)g.div_Nz.div((uint32_t)c);
+  cz = c - cr * g.Nz;
+}
+
+__global__ void k_fsurr_features(Geom g, const double *mom, double m_e, double m_i, SurrNorm norm, float *X,
+                                 int *ok) {
+  const long ncells = g.Nr * g.Nz;
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += (long)gridDim.x * blockDim.x) {
+    const double *me = mom + c * SURR_PAR_STRIDE, *mi = me + 8;
+    double tep, tet, tip, tit, Ve[3], Vi[3];
+    bool good = species_temps(me, m_e, tep, tet, Ve) && species_temps(mi, m_i, tip, tit, Vi);
+    long cr, cz;
+    global_cell(g, c, cr, cz);
+    const double vol = PCP_CELL_VOLUME((double)cr, g.dr, g.dz);
+    const double f[MLP_IN] = {log10(me[1] / vol), log10(mi[1] / vol), log10(tep), log10(tet),
+                              log10((tip + 2.0 * tit) / 3.0), log10(g.dt)};
+    for (int k = 0; k < MLP_IN; ++k) {
+      float x = ((float)f[k] - norm.mean[k]) / norm.std[k];
+      if (!(fabsf(x) <= SURR_MAX_Z)) good = false;
+      X[c * MLP_XSTRIDE + k] = good ? x : 0.f;
+    }
+    X[c * MLP_XSTRIDE + 6] = 0.f;
+    X[c * MLP_XSTRIDE + 7] = 0.f;
+    ok[c] = good ? 1 : 0;
+  }
+}
+
+__global__ void k_fsurr_params(Geom g, double *mom, double m_e, double m_i, const float *Y, SurrNorm norm,
+                               const int *ok, float *par, int *fallback, unsigned long long *n_fallback) {
+  const long ncells = g.Nr * g.Nz;
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += (long)gridDim.x * blockDim.x) {
+    double *m = mom + c * SURR_PAR_STRIDE;
+    float *P = par + c * SURR_PAR_STRIDE;
+    long cr, cz;
+    global_cell(g, c, cr, cz);
+    const uint32_t lk = local_cell_key(g, (int)cr, (int)cz);
+    if (ok[c]) {
+      double sc[MLP_OUT];
+      (void)surr_rate_scale(m, m + 8, m_e, m_i, PCP_CELL_VOLUME((double)cr, g.dr, g.dz), g.dt, sc);
+      const double mass[2] = {m_e, m_i};
+      double V[2][3], ap[2], az[2], pp[2], pz[2];
+      double e0 = 0, e1 = 0;
+      for (int s = 0; s < 2; ++s) {
+        const double *q = m + 8 * s;
+        for (int k = 0; k < 3; ++k) V[s][k] = q[2 + k] / q[1];
+        const double ypar = (double)Y[c * MLP_YSTRIDE + 2 * s] * norm.tscale[2 * s] * sc[2 * s];
+        const double yperp = (double)Y[c * MLP_YSTRIDE + 2 * s + 1] * norm.tscale[2 * s + 1] * sc[2 * s + 1];
+        az[s] = sqrt(fmax(1.0 + ypar, SURR_MIN_FACTOR));
+        ap[s] = sqrt(fmax(1.0 + yperp, SURR_MIN_FACTOR));
+        pp[s] = fmax(q[6] - (q[2] * q[2] + q[3] * q[3]) / q[1], 0.0);
+        pz[s] = fmax(q[5] - q[4] * q[4] / q[1], 0.0);
+        e0 += mass[s] * (pp[s] + pz[s]);
+        e1 += mass[s] * (ap[s] * ap[s] * pp[s] + az[s] * az[s] * pz[s]);
+      }
+      const double beta = e1 > 0 ? sqrt(e0 / e1) : 1.0;
+      for (int s = 0; s < 2; ++s) {
+        P[8 * s + 0] = (float)V[s][0];
+        P[8 * s + 1] = (float)V[s][1];
+        P[8 * s + 2] = (float)V[s][2];
+        P[8 * s + 3] = (float)(beta * ap[s]);
+        P[8 * s + 4] = (float)(beta * az[s]);
+      }
+      fallback[lk] = 0;
+    } else {
+      for (int s = 0; s < 2; ++s) {
+        P[8 * s + 0] = P[8 * s + 1] = P[8 * s + 2] = 0.f;
+        P[8 * s + 3] = P[8 * s + 4] = 1.f;
+      }
+      const bool any = m[0] > 0 || m[8] > 0;
+      fallback[lk] = any ? 1 : 0;
+      if (any) atomicAdd(n_fallback, 1ull);
+    }
+    // the next step's push accumulates into zeroed moments
+    for (int k = 0; k < SURR_PAR_STRIDE; ++k) m[k] = 0.0;
+  }
+}
+
+__global__ void k_fill_zero_f64(double *p, long n) {
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) p[t] = 0.0;
+}
+
 }  // namespace
 
 struct CollBuffers {
@@ -715,6 +821,11 @@ struct CollBuffers {
   float *feat = nullptr, *pred = nullptr;
   int *ok = nullptr, *fallback = nullptr;
   unsigned long long *counter = nullptr;   // [0] fallback cells
+  // fused surrogate phase (per global cell)
+  double *fmom = nullptr;
+  float *fX = nullptr, *fY = nullptr, *fpar = nullptr;
+  int *fok = nullptr;
+  long fcells = 0;
 };
 
 void coll_free(pcp_ctx *c) {
@@ -727,6 +838,7 @@ void coll_free(pcp_ctx *c) {
   (void)hipFree(b->tot); (void)hipFree(b->row_bad); (void)hipFree(b->scan_tmp);
   (void)hipFree(b->mom); (void)hipFree(b->mom2); (void)hipFree(b->feat); (void)hipFree(b->pred);
   (void)hipFree(b->ok); (void)hipFree(b->fallback); (void)hipFree(b->counter);
+  (void)hipFree(b->fmom); (void)hipFree(b->fX); (void)hipFree(b->fY); (void)hipFree(b->fpar); (void)hipFree(b->fok);
   delete b;
   c->coll = nullptr;
 }
@@ -769,7 +881,8 @@ static int coll_alloc(pcp_ctx *c, long cap_e, long cap_i) {
 
 // bucket lists of both species (sort_to_cells)
 template <class V>
-static int coll_buckets(pcp_ctx *c, V ve, V vi, long n_e, long n_i) {
+static int coll_buckets(pcp_ctx *c, V ve, V vi, long n_e, long n_i, const int *mask = nullptr,
+                        const unsigned long long *gate = nullptr) {
   CollBuffers *b = c->coll;
   const Geom &g = c->g;
   const long ncells = b->ncells;
@@ -781,7 +894,7 @@ static int coll_buckets(pcp_ctx *c, V ve, V vi, long n_e, long n_i) {
     if (nn[s] > 0) {
       unsigned gp = (unsigned)std::min<long>((nn[s] + 255) / 256, (long)c->n_cu * 16);
       hipLaunchKernelGGL(k_coll_key<V>, dim3(gp), dim3(256), 0, c->stream, g, s == 0 ? ve : vi, b->key[s],
-                         b->slot[s], b->cnt[s], sentinel);
+                         b->slot[s], b->cnt[s], sentinel, mask, gate);
     }
     size_t bytes = b->scan_bytes;
     if (hipcub::DeviceScan::ExclusiveSum(b->scan_tmp, bytes, b->cnt[s], b->off[s], (int)(ncells + 1), c->stream) !=
@@ -790,7 +903,7 @@ static int coll_buckets(pcp_ctx *c, V ve, V vi, long n_e, long n_i) {
     if (nn[s] > 0) {
       unsigned gp = (unsigned)std::min<long>((nn[s] + 255) / 256, (long)c->n_cu * 16);
       hipLaunchKernelGGL(k_coll_scatter, dim3(gp), dim3(256), 0, c->stream, nn[s], b->key[s], b->slot[s], b->off[s],
-                         b->perm[s], sentinel);
+                         b->perm[s], sentinel, gate);
     }
   }
   return 0;
@@ -800,7 +913,7 @@ static int coll_buckets(pcp_ctx *c, V ve, V vi, long n_e, long n_i) {
 // (all cells when null)
 template <class V>
 static void coll_binary(pcp_ctx *c, int scheme, V ve, V vi, double m_e, double m_i, double q_e, double q_i,
-                        uint64_t seed, uint64_t step, const int *mask) {
+                        uint64_t seed, uint64_t step, const int *mask, const unsigned long long *gate = nullptr) {
   CollBuffers *b = c->coll;
   const Geom &g = c->g;
   const long ncells = b->ncells;
@@ -809,19 +922,19 @@ static void coll_binary(pcp_ctx *c, int scheme, V ve, V vi, double m_e, double m
                      dim3(256), 0, c->stream, b->row_bad, rows, INT_MAX);
   unsigned gc = (unsigned)std::min<long>(ncells, (long)c->n_cu * 32);
   hipLaunchKernelGGL(k_coll_prep<V>, dim3(gc), dim3(COLL_BLOCK), 0, c->stream, g, ve, vi, b->off[0], b->off[1],
-                     b->perm[0], b->perm[1], m_e, m_i, seed, step, b->tot, b->row_bad, mask, c->st);
+                     b->perm[0], b->perm[1], m_e, m_i, seed, step, b->tot, b->row_bad, mask, c->st, gate);
   if (scheme == COLL_TA77S)
     hipLaunchKernelGGL((k_coll_pairs<COLL_TA77S, V>), dim3(gc), dim3(COLL_BLOCK), 0, c->stream, g, ve, vi, b->off[0],
                        b->off[1], b->perm[0], b->perm[1], m_e, m_i, q_e, q_i, seed, step, b->tot, b->row_bad, mask,
-                       c->st);
+                       c->st, gate);
   else if (scheme == COLL_SK98)
     hipLaunchKernelGGL((k_coll_pairs<COLL_SK98, V>), dim3(gc), dim3(COLL_BLOCK), 0, c->stream, g, ve, vi, b->off[0],
                        b->off[1], b->perm[0], b->perm[1], m_e, m_i, q_e, q_i, seed, step, b->tot, b->row_bad, mask,
-                       c->st);
+                       c->st, gate);
   else
     hipLaunchKernelGGL((k_coll_pairs<COLL_P12, V>), dim3(gc), dim3(COLL_BLOCK), 0, c->stream, g, ve, vi, b->off[0],
                        b->off[1], b->perm[0], b->perm[1], m_e, m_i, q_e, q_i, seed, step, b->tot, b->row_bad, mask,
-                       c->st);
+                       c->st, gate);
 }
 
 template <class V>
@@ -915,6 +1028,47 @@ int launch_collide(pcp_ctx *c, int scheme, int se, int si, uint64_t seed, uint64
   if (coll_alloc(c, e.cap, i.cap) < 0) return -1;
   View64 ve{e, c->cur}, vi{i, c->cur};
   return coll_run(c, scheme, ve, vi, e.n, i.n, e.m, i.m, e.q, i.q, seed, step);
+}
+
+int fsurr_prepare(pcp_ctx *c, int se, int si) {
+  if (!c->s32) return -1;
+  if (coll_alloc(c, c->s32->sp[se].cap, c->s32->sp[si].cap) < 0) return -1;
+  CollBuffers *b = c->coll;
+  const long nc = c->g.Nr * c->g.Nz;
+  if (b->fcells == nc) return 0;
+  if (hipMalloc(&b->fmom, nc * SURR_PAR_STRIDE * sizeof(double)) != hipSuccess ||
+      hipMalloc(&b->fX, nc * MLP_XSTRIDE * sizeof(float)) != hipSuccess ||
+      hipMalloc(&b->fY, nc * MLP_YSTRIDE * sizeof(float)) != hipSuccess ||
+      hipMalloc(&b->fpar, nc * SURR_PAR_STRIDE * sizeof(float)) != hipSuccess ||
+      hipMalloc(&b->fok, nc * sizeof(int)) != hipSuccess)
+    return -1;
+  unsigned gz = (unsigned)std::min<long>((nc * SURR_PAR_STRIDE + 255) / 256, (long)c->n_cu * 16);
+  hipLaunchKernelGGL(k_fill_zero_f64, dim3(gz), dim3(256), 0, c->stream, b->fmom, nc * SURR_PAR_STRIDE);
+  b->fcells = nc;
+  return 0;
+}
+
+double *fsurr_mom(pcp_ctx *c) { return c->coll ? c->coll->fmom : nullptr; }
+const float *fsurr_par(pcp_ctx *c) { return c->coll ? c->coll->fpar : nullptr; }
+
+int fsurr_cells(pcp_ctx *c, int se, int si, uint64_t seed, uint64_t step) {
+  CollBuffers *b = c->coll;
+  if (!b || !b->fmom || !c->mlp || !c->mlp->trained) return -2;
+  Species32 &e = c->s32->sp[se], &i = c->s32->sp[si];
+  const long nc = b->fcells;
+  const unsigned gt = (unsigned)std::min<long>((nc + 255) / 256, (long)c->n_cu * 16);
+  hipLaunchKernelGGL(k_fsurr_features, dim3(gt), dim3(256), 0, c->stream, c->g, b->fmom, e.m, i.m, surr_norm(c),
+                     b->fX, b->fok);
+  mlp_forward(c, b->fX, nc, b->fY);
+  hipLaunchKernelGGL(k_fill_zero_ull, dim3(1), dim3(1), 0, c->stream, b->counter);
+  hipLaunchKernelGGL(k_fsurr_params, dim3(gt), dim3(256), 0, c->stream, c->g, b->fmom, e.m, i.m, b->fY,
+                     surr_norm(c), b->fok, b->fpar, b->fallback, b->counter);
+  // P12 on the cells the network does not cover, on the pushed velocities
+  // (every kernel returns at once while no cell is flagged)
+  View32 ve{e, c->cur, c->cur ^ 1}, vi{i, c->cur, c->cur ^ 1};
+  if (coll_buckets(c, ve, vi, e.n, i.n, b->fallback, b->counter) < 0) return -1;
+  coll_binary(c, COLL_P12, ve, vi, e.m, i.m, e.q, i.q, seed, step, b->fallback, b->counter);
+  return 0;
 }
 
 int surrogate_collect(pcp_ctx *c, int se, int si, uint64_t seed, uint64_t step) {

@@ -36,4 +36,14 @@ int surrogate_collect(pcp_ctx *c, int se, int si, uint64_t seed, uint64_t step);
 // cells of the last surrogate pass that went to the P12 fallback
 long surrogate_fallback_cells(pcp_ctx *c);
 
+// Fused surrogate collision phase of FP32 steps: buffers (moments zeroed),
+// the moment array the push fills, the per-cell rescalings the move applies,
+// and the per-cell work between them (features, network, rescalings, P12
+// on the cells the network does not cover).  Return -1 on allocation or
+// launch failure, -2 without a trained network.
+int fsurr_prepare(pcp_ctx *c, int se, int si);
+double *fsurr_mom(pcp_ctx *c);
+const float *fsurr_par(pcp_ctx *c);
+int fsurr_cells(pcp_ctx *c, int se, int si, uint64_t seed, uint64_t step);
+
 }  // namespace pcp

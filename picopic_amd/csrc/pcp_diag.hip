@@ -132,8 +132,14 @@ __global__ void k_zero_d(double *p, long n) {
 int launch_continuity_residual(pcp_ctx *c, double *dev_out) {
   const Geom &g = c->g;
   const long ldq = g.Nz + 4, nq = (g.Nr + 4) * ldq;
-  double *Q = nullptr;
-  if (hipMalloc(&Q, nq * sizeof(double)) != hipSuccess) return -1;
+  struct Scratch {
+    double *p = nullptr;
+    ~Scratch() {
+      if (p) (void)hipFree(p);
+    }
+  } q;
+  if (hipMalloc(&q.p, nq * sizeof(double)) != hipSuccess) return -1;
+  double *Q = q.p;
   const unsigned gz = (unsigned)std::min<long>((nq + 255) / 256, (long)c->n_cu * 16);
   hipLaunchKernelGGL(k_zero_d, dim3(gz), dim3(256), 0, c->stream, Q, nq);
   hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, c->stream, dev_out, 3L);
@@ -159,7 +165,6 @@ int launch_continuity_residual(pcp_ctx *c, double *dev_out) {
   else
     hipLaunchKernelGGL(k_cont_residual<double>, dim3(gr), dim3(256), 0, c->stream, g, c->f.J, Q, ldq, dev_out);
   if (hipStreamSynchronize(c->stream) != hipSuccess) return -1;
-  (void)hipFree(Q);
   return 0;
 }
 

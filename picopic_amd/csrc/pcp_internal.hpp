@@ -165,6 +165,17 @@ struct Migrants {
   int ngroups;
 };
 
+// In-step energy log (pcp_set_energy_log): a ring of EHIST rows of
+// EHIST_COLS doubles -- step number, field energy, kinetic energy of each
+// species -- filled on the device during every logged step.
+constexpr int EHIST = 4096;
+constexpr int DIAG_DOUBLES = 16;   // per-context scratch of the diagnostic reductions
+constexpr int EHIST_COLS = 2 + MAX_SPECIES;
+
+__host__ __device__ inline double *ehist_row(double *h, unsigned long long step) {
+  return h + (long)(step % EHIST) * EHIST_COLS;
+}
+
 struct State32;      // FP32 split-precision state (pcp32.hpp)
 struct CollBuffers;  // collision pass buffers (pcp_collide.hip)
 struct MlpState;     // collision surrogate network (pcp_mlp.hpp)
@@ -205,5 +216,8 @@ struct pcp_ctx {
   uint64_t coll_seed = 0, coll_calls = 0;
   int n_cu;
   int timing;
+  double *diag = nullptr;           // scratch of the diagnostic reductions (DIAG_DOUBLES)
+  double *ehist = nullptr;          // energy log [EHIST][EHIST_COLS]
+  int elog = 0;                     // log energies in every step
   std::vector<std::pair<std::string, double>> ktimes;
 };

@@ -251,11 +251,20 @@ __global__ void k_fill(double *p, long n, double v) {
 }
 
 
-__global__ void k_step_begin(StepState *st) {
-  st->step += 1;
-  st->need_exact = 0;
-  st->n_mig = 0;
-  st->queue = 0;
+// ehist (optional): the energy log, whose row of the new step is cleared
+// and stamped with the step number
+__global__ void k_step_begin(StepState *st, double *ehist) {
+  if (threadIdx.x == 0) {
+    st->step += 1;
+    st->need_exact = 0;
+    st->n_mig = 0;
+    st->queue = 0;
+  }
+  __syncthreads();
+  if (ehist && threadIdx.x < EHIST_COLS) {
+    double *row = ehist_row(ehist, st->step);
+    row[threadIdx.x] = threadIdx.x == 0 ? (double)st->step : 0.0;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -692,7 +701,9 @@ void launch_stitch(pcp_ctx *c, int which) { launch_stitch_t<double>(c, c->f, whi
 void launch_fill(pcp_ctx *c, double *p, long n, double v) {
   hipLaunchKernelGGL(k_fill, dim3(pgrid(c, n, 256)), dim3(256), 0, c->stream, p, n, v);
 }
-void launch_step_begin(pcp_ctx *c) { hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, c->stream, c->st); }
+void launch_step_begin(pcp_ctx *c) {
+  hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, c->stream, c->st, c->elog ? c->ehist : nullptr);
+}
 void launch_stamp(pcp_ctx *c) { hipLaunchKernelGGL(k_stamp, dim3(1), dim3(1), 0, c->stream, c->st); }
 void launch_push(pcp_ctx *c, int sp) {
   SpeciesDev &s = c->sp[sp];
