@@ -117,6 +117,20 @@ int pcp_init_plasma(pcp_ctx *ctx, int sp, int n_per_cell, double density_left,
  * with Lorentz factor gamma, peak density n_peak, n macro-particles. */
 int pcp_init_bunch(pcp_ctx *ctx, int sp, long n, double n_peak, double sigma_r, double sigma_z,
                    double z0, double gamma, uint64_t seed);
+/* The reference's own initial plasma: SMB::SMB + Domain::distribute for one
+ * plasma species (src/SMB.cpp:123-137, src/specieP.cpp:79-189 "flat"
+ * placement, 280-331 set_weight, 346-384 thermal velocities,
+ * src/math/maxwellJuettner.cpp:182-255).  Every domain places
+ * macro_amount / Dr / Dz particles at radical-inverse positions (bases 13 and
+ * 11, as math::random::random_reverse), uniform in r with weights
+ * proportional to r, and gives them Maxwell-Juettner thermal velocities of
+ * temperature_ev (the reference's units); draws are Philox with `seed` in
+ * place of the reference's time-seeded mt19937.  Particles a domain places
+ * past an inner domain edge join the owning domain as the first runaway
+ * collector would move them.  Replaces SpecieP::fullyfill_spatial_distribution
+ * + velocity_distribution (include/specieP.hpp:225-229). */
+int pcp_init_reference(pcp_ctx *ctx, int sp, long macro_amount, double density_left,
+                       double density_right, double temperature_ev, uint64_t seed);
 
 /* Whole halo grids, domain-major [domain][component][Nr_d+4][Nz_d+4]
  * (one component for EPS and SIGMA), as Grid<double> stores them

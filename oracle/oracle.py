@@ -122,6 +122,19 @@ def lib():
         L.orc_debye_length.restype = C.c_double
         L.orc_coulomb_logarithm.argtypes = [C.c_double] * 4
         L.orc_coulomb_logarithm.restype = C.c_double
+        L.orc_smb_init_reference.argtypes = [C.c_void_p, C.c_int, C.c_long, C.c_double, C.c_double, C.c_double,
+                                             C.c_uint64, C.c_uint64]
+        L.orc_smb_init_reference.restype = C.c_int
+        L.orc_random_reverse.argtypes = [C.c_double, C.c_int]
+        L.orc_random_reverse.restype = C.c_double
+        L.orc_gamma32_quantile.argtypes = [C.c_double]
+        L.orc_gamma32_quantile.restype = C.c_double
+        L.orc_mj_g.argtypes = [C.c_double]
+        L.orc_mj_g.restype = C.c_double
+        L.orc_mj_g_inverse.argtypes = [C.c_double, C.c_double]
+        L.orc_mj_g_inverse.restype = C.c_double
+        L.orc_mj_energy.argtypes = [C.c_double, C.c_uint64, C.c_uint64]
+        L.orc_mj_energy.restype = C.c_double
         _lib = L
     return _lib
 
@@ -249,6 +262,14 @@ class SMB:
         """Collision phase inside step() (scheme None: off)."""
         lib().orc_smb_set_collisions(self.h, -1 if scheme is None else COLL[scheme], se, si, seed)
 
+    def init_reference(self, sp, macro_amount, density_left, density_right, temperature, seed, id_base=0):
+        """The reference's flat placement, weights and thermal velocities for
+        plasma species sp (macro_amount over all domains)."""
+        rc = lib().orc_smb_init_reference(self.h, sp, int(macro_amount), density_left, density_right, temperature,
+                                          seed, id_base)
+        if rc:
+            raise RuntimeError("orc_smb_init_reference failed (%d)" % rc)
+
     def get_field_e(self, d, r, z):
         out = np.zeros(3)
         lib().orc_smb_get_field_e(self.h, d, r, z, _dp(out))
@@ -314,3 +335,23 @@ def coll_uniforms(seed, cell, q, step):
     u = np.zeros(4)
     lib().orc_coll_uniforms(seed, cell, q, step, _dp(u))
     return u
+
+
+def random_reverse(vel, power):
+    return lib().orc_random_reverse(vel, power)
+
+
+def gamma32_quantile(u):
+    return lib().orc_gamma32_quantile(u)
+
+
+def mj_g(x):
+    return lib().orc_mj_g(x)
+
+
+def mj_g_inverse(t, x_lo):
+    return lib().orc_mj_g_inverse(t, x_lo)
+
+
+def mj_energy(temperature, seed, ctr):
+    return lib().orc_mj_energy(temperature, seed, ctr)

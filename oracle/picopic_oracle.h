@@ -189,6 +189,19 @@ uint32_t orc_coll_key(uint64_t seed, uint64_t id, uint64_t step);
 void orc_smb_set_collisions(orc_smb *s, int scheme, int se, int si, uint64_t seed);
 double orc_coulomb_logarithm(double m_a, double m_b, double debye, double v_rel);
 
+/* ---- initial distribution (src/specieP.cpp:79-189, 280-331, 346-384;
+ * src/math/maxwellJuettner.cpp:182-255): flat placement by radical inverses,
+ * r-proportional weights, Maxwell-Juettner thermal velocities with Philox
+ * draws (counter domain << 40 | index, key seed).  Returns -2 when an energy
+ * draw did not accept. */
+int orc_smb_init_reference(orc_smb *s, int sp, long macro_amount, double density_left, double density_right,
+                           double temperature, uint64_t seed, uint64_t id_base);
+double orc_random_reverse(double vel, int power);
+double orc_gamma32_quantile(double U);
+double orc_mj_g(double x);
+double orc_mj_g_inverse(double t, double x_lo);
+double orc_mj_energy(double temperature, uint64_t seed, uint64_t ctr);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,7 +30,7 @@ EXPORTS = [
     "pcp_mlp_set_params", "pcp_mlp_forward", "pcp_mlp_grad", "pcp_mlp_adam", "pcp_surrogate_collect",
     "pcp_surrogate_fit", "pcp_surrogate_get_norm", "pcp_surrogate_set_norm", "pcp_surrogate_fallback_cells",
     "pcp_species_moments", "pcp_continuity_residual", "pcp_set_graph", "pcp_set_sort_bin", "pcp_set_launch",
-    "pcp_get_tuning", "pcp_autotune", "pcp_surrogate_calibrate",
+    "pcp_get_tuning", "pcp_autotune", "pcp_surrogate_calibrate", "pcp_init_reference",
 ]
 COLL = {"p12": 0, "ta77s": 1, "sk98": 2, "surrogate": 3}
 
@@ -81,6 +81,7 @@ def load(path=LIB_PATH):
     L.pcp_download_species.argtypes = [vp, C.c_int] + [dp] * 6 + [u64p, u64p, dp, dp]
     L.pcp_init_plasma.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64]
     L.pcp_init_bunch.argtypes = [vp, C.c_int, C.c_long] + [C.c_double] * 5 + [C.c_uint64]
+    L.pcp_init_reference.argtypes = [vp, C.c_int, C.c_long, C.c_double, C.c_double, C.c_double, C.c_uint64]
     L.pcp_download_grid.argtypes = [vp, C.c_int, dp]
     L.pcp_upload_grid.argtypes = [vp, C.c_int, dp]
     L.pcp_grid_points.argtypes = [vp]
@@ -225,6 +226,11 @@ class Context:
 
     def init_plasma(self, sp, ppc, density_left, density_right, temperature_ev, seed):
         _check(_lib.pcp_init_plasma(self.h, sp, ppc, density_left, density_right, temperature_ev, seed))
+
+    def init_reference(self, sp, macro_amount, density_left, density_right, temperature, seed):
+        """The reference's flat placement, r-proportional weights and
+        Maxwell-Juettner thermal velocities (macro_amount over all domains)."""
+        _check(_lib.pcp_init_reference(self.h, sp, int(macro_amount), density_left, density_right, temperature, seed))
 
     def init_bunch(self, sp, n, n_peak, sigma_r, sigma_z, z0, gamma, seed):
         _check(_lib.pcp_init_bunch(self.h, sp, n, n_peak, sigma_r, sigma_z, z0, gamma, seed))

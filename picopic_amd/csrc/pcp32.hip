@@ -431,16 +431,6 @@ __device__ __forceinline__ Split unpack(uint32_t cell, float fr, float fz, const
 
 __device__ __forceinline__ uint64_t order_key32(int sp, uint64_t seq) { return ((uint64_t)sp << 36) | seq; }
 
-__device__ __forceinline__ void append_migrant32(Migrants mg, StepState *st, int sp, long p, int d_old, int d_new,
-                                                 uint64_t seq, const Geom &g, int *err) {
-  unsigned int slot = atomicAdd(&st->n_mig, 1u);
-  if (slot >= (unsigned long)mg.cap) { *err |= ERR_MIGRANT_OVERFLOW; return; }
-  const int I = (int)g.div_Dz.div((uint32_t)d_old), J = d_old - I * g.Dz;
-  uint64_t colour = (uint64_t)((I % 2) * 2 + (J % 2));
-  mg.key[slot] = (colour << 47) | ((uint64_t)d_old << 35) | (seq & ((1ull << 35) - 1));
-  mg.ref[slot] = ((uint64_t)sp << 40) | (uint64_t)p;
-  mg.group[slot] = (uint32_t)(d_new * MAX_SPECIES + sp);
-}
 
 // ---------------------------------------------------------------------------
 // the fused FP32 step kernel
@@ -626,7 +616,7 @@ __device__ __forceinline__ float step_trip32(const Geom &g, const FieldsT<float>
       s.fr[nxt][p] = n.fr;
       s.fz[nxt][p] = n.fz;
       d_new = domain_of32t<D1>(g, n.cr, n.cz);
-      if (!D1 && d_new != d_old) append_migrant32(mg, st, sp, p, d_old, d_new, s.seq[p], g, &err);
+      if (!D1 && d_new != d_old) append_migrant(mg, st, sp, p, d_old, d_new, s.seq[p], g, &err);
       w = s.w[p];
       live = true;
     } else {
@@ -845,18 +835,6 @@ __global__ void k32_count_live(const uint32_t *cell, long n, unsigned long long 
 // synthetic inputs in split form (same sampling as pcp_init.hip)
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ void store_split(Species32 &s, long q, const Geom &g, double r, double z) {
-  double ur = r / g.dr, uz = z / g.dz;
-  double fcr = floor(ur), fcz = floor(uz);
-  float fr = (float)(ur - fcr), fz = (float)(uz - fcz);
-  int cr = (int)fcr, cz = (int)fcz;
-  if (fr >= 1.f) { fr = 0.f; cr += 1; }
-  if (fz >= 1.f) { fz = 0.f; cz += 1; }
-  uint32_t cell = (uint32_t)cr * (uint32_t)g.Nz + (uint32_t)cz;
-  s.cell[0][q] = cell; s.cell[1][q] = cell;
-  s.fr[0][q] = fr; s.fr[1][q] = fr;
-  s.fz[0][q] = fz; s.fz[1][q] = fz;
-}
 
 __global__ void k32_init_plasma(Geom g, Species32 s, long base, long n_new, const long *cell_off, long n_cells,
                                 double weight, double vth, uint64_t seed, const long *dom_off) {

@@ -45,6 +45,20 @@ struct SortBuffers32 {
   uint64_t *seq, *id;
 };
 
+// position (r, z) in split form, written to both position slots of q
+__device__ __forceinline__ void store_split(Species32 &s, long q, const Geom &g, double r, double z) {
+  double ur = r / g.dr, uz = z / g.dz;
+  double fcr = floor(ur), fcz = floor(uz);
+  float fr = (float)(ur - fcr), fz = (float)(uz - fcz);
+  int cr = (int)fcr, cz = (int)fcz;
+  if (fr >= 1.f) { fr = 0.f; cr += 1; }
+  if (fz >= 1.f) { fz = 0.f; cz += 1; }
+  uint32_t cell = (uint32_t)cr * (uint32_t)g.Nz + (uint32_t)cz;
+  s.cell[0][q] = cell; s.cell[1][q] = cell;
+  s.fr[0][q] = fr; s.fr[1][q] = fr;
+  s.fz[0][q] = fz; s.fz[1][q] = fz;
+}
+
 // state of an FP32 context (owned by pcp_ctx)
 struct State32 {
   FieldsT<float> f;

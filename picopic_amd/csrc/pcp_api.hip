@@ -666,6 +666,98 @@ int pcp_init_bunch(pcp_ctx *c, int sp, long n, double n_peak, double sigma_r, do
   return 0;
 }
 
+// grows the migrant list to hold at least `need` entries (its contents are
+// per-pass scratch)
+static int grow_migrants(pcp_ctx *c, long need) {
+  Migrants &m = c->mig;
+  if (need <= m.cap) return 0;
+  hipFree(m.key); hipFree(m.ref); hipFree(m.group); hipFree(m.bucket);
+  m.key = nullptr; m.ref = nullptr; m.group = nullptr; m.bucket = nullptr;
+  m.cap = need;
+  int rc;
+  if ((rc = dalloc(&m.key, m.cap)) || (rc = dalloc(&m.ref, m.cap)) || (rc = dalloc(&m.group, m.cap)) ||
+      (rc = dalloc(&m.bucket, m.cap))) {
+    m.cap = 0;
+    return rc;
+  }
+  return 0;
+}
+
+int pcp_init_reference(pcp_ctx *c, int sp, long macro_amount, double density_left, double density_right,
+                       double temperature_ev, uint64_t seed) {
+  if (c) graph_invalidate(c);
+  if (!c || sp < 0 || sp >= c->cfg.n_species || macro_amount < 0 || macro_amount > 0xFFFFFFFFL ||
+      !(density_left >= 0) || !(density_right >= 0) || !(temperature_ev > 0) || temperature_ev > 3e3)
+    return fail(PCP_E_ARG, "bad argument (temperatures above 3e3 reach the sampler's unsupported branches)");
+  const Geom &g = c->g;
+  // SMB.cpp:125: integer division of the species' macro amount
+  const uint32_t M = (uint32_t)macro_amount / (uint32_t)g.Dr / (uint32_t)g.Dz;
+  if (M == 0) return 0;
+  if (int rc = ensure_capacity(c, sp, (long)M * c->D)) return rc;
+  RadicalPowers pw;
+  for (int k = 0; k < RADICAL_DIGITS; ++k) {
+    // the reference's pow((double)power, (-order)) per digit
+    volatile int order = k + 1;
+    pw.p13[k] = std::pow(13.0, (double)(-order));
+    pw.p11[k] = std::pow(11.0, (double)(-order));
+  }
+  std::vector<uint64_t> gbase(c->mig.ngroups);
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(gbase.data(), c->mig.gbase, gbase.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  std::vector<RefInitDom> doms(c->D);
+  long leavers = 0;
+  for (int d = 0; d < c->D; ++d) {
+    const Dom dm = make_dom(g, d);
+    RefInitDom &P = doms[d];
+    const double dr = g.dr, dz = g.dz;
+    const uint32_t int_cell = (uint32_t)dm.bot_r, ext_cell = (uint32_t)(dm.bot_r + g.nr);
+    const uint32_t left_cell = (uint32_t)dm.left_z, right_cell = (uint32_t)(dm.left_z + g.nz);
+    // SMB.cpp:128-130: local densities of the z ramp
+    const double drho_by_dz = (density_right - density_left) / g.gsize_z;
+    P.dens0 = density_left + drho_by_dz * left_cell * g.dz;
+    P.dens1 = density_left + drho_by_dz * right_cell * g.dz;
+    // specieP.cpp:138-149: extents, less a cell at the outer and right walls
+    P.r_size = (ext_cell - int_cell) * dr;
+    P.z_size = (right_cell - left_cell) * dz;
+    if (dm.w2) P.r_size -= dr;
+    if (dm.w3) P.z_size -= dz;
+    P.int_cell = int_cell;
+    P.left_cell = left_cell;
+    P.seq_base = gbase[(size_t)d * MAX_SPECIES + sp];
+    // set_weight (specieP.cpp:280-331): the volume sum in particle order
+    double v_sum = 0;
+    for (uint32_t k = 0; k < M; ++k) {
+      double r, z;
+      ref_init_position(dr, dz, P, pw, M, k, r, z);
+      v_sum += ref_init_volume(dr, dz, r);
+      if (std::floor(r / dr) >= (double)ext_cell || std::floor(z / dz) >= (double)right_cell) ++leavers;
+    }
+    P.v_avg = v_sum / M;
+    double top_r_num = (double)ext_cell, bot_r_num = (double)int_cell;
+    if (dm.w2) top_r_num -= 1;
+    const double N_total = (P.dens0 + P.dens1) / 2 * PI * dr * dr * (top_r_num * top_r_num - bot_r_num * bot_r_num) *
+                           P.z_size;
+    P.n_per_macro_avg = N_total / M;
+    gbase[(size_t)d * MAX_SPECIES + sp] += M;
+  }
+  // particles placed past an inner domain edge migrate (with margin for
+  // the FP32 split's rounding at a cell boundary)
+  if (int rc = grow_migrants(c, leavers + leavers / 8 + 1024)) return rc;
+  HIP_TRY(hipMemcpy(c->mig.gbase, gbase.data(), gbase.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+  DevBuf<RefInitDom> d_doms;
+  HIP_TRY(d_doms.alloc(c->D));
+  HIP_TRY(hipMemcpy(d_doms.p, doms.data(), sizeof(RefInitDom) * c->D, hipMemcpyHostToDevice));
+  const long base = c->sp[sp].n;
+  launch_init_reference(c, sp, base, M, d_doms.p, pw, temperature_ev, seed);
+  if (c->s32) launch32_migrant_keys(c);
+  else launch_migrant_keys(c);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->sp[sp].n += (long)M * c->D;
+  if (c->s32) c->s32->sp[sp].n = c->sp[sp].n;
+  return check_state(c);
+}
+
 static float *grid_ptr32(pcp_ctx *c, int which, int *ncomp) {
   *ncomp = 3;
   FieldsT<float> &f = c->s32->f;

@@ -5,8 +5,10 @@
 // launch shape.
 #include <hip/hip_runtime.h>
 
+#include "pcp32.hpp"
 #include "pcp_init.hpp"
 #include "pcp_physics.hpp"
+#include "pcp_refinit.hpp"
 #include "pcp_rng.hpp"
 
 namespace pcp {
@@ -80,6 +82,59 @@ __global__ void k_init_bunch(Geom g, SpeciesDev s, long base, long n_new, double
   }
 }
 
+// The reference's initial distribution of one plasma species: particle
+// k of domain d (M per domain) goes to slot base + d M + k.  A particle that
+// the domain places beyond its own cells (inner domain edges are not walls,
+// src/specieP.cpp:148-149) is a migrant of the first runaway collector: it
+// is appended to the migrant list with its generating domain's key and
+// takes its order key in the owning domain from the migration pass.
+template <bool F32>
+__global__ void k_init_reference(Geom g, SpeciesDev s64, Species32 s32, int sp, long base, uint32_t M,
+                                 const RefInitDom *doms, RadicalPowers pw, double temperature, double mass,
+                                 uint64_t seed, Migrants mg, StepState *st) {
+  const long total = (long)M * g.Dr * g.Dz;
+  int err = 0;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < total; p += (long)gridDim.x * blockDim.x) {
+    const int d = (int)(p / M);
+    const uint32_t k = (uint32_t)(p - (long)d * M);
+    const RefInitDom P = doms[d];
+    double r, z, v[3], w;
+    if (!ref_init_particle(g, P, pw, M, k, d, temperature, mass, seed, r, z, v, w)) err |= ERR_NAN_POSITION;
+    const long q = base + p;
+    const uint64_t seq = P.seq_base + k;
+    if (F32) {
+      store_split(s32, q, g, r, z);
+      s32.vr[0][q] = s32.vr[1][q] = (float)v[0];
+      s32.vphi[0][q] = s32.vphi[1][q] = (float)v[1];
+      s32.vz[0][q] = s32.vz[1][q] = (float)v[2];
+      s32.w[q] = (float)w;
+      s32.seq[q] = seq;
+      s32.id[q] = (uint64_t)q;
+    } else {
+      s64.r[0][q] = r; s64.r[1][q] = r;
+      s64.z[0][q] = z; s64.z[1][q] = z;
+      s64.vr[q] = v[0];
+      s64.vphi[q] = v[1];
+      s64.vz[q] = v[2];
+      s64.w[q] = w;
+      s64.seq[q] = seq;
+      s64.id[q] = (uint64_t)q;
+    }
+    int d_own;
+    if (F32) {
+      const uint32_t cell = s32.cell[0][q];
+      const int cr = (int)g.div_Nz.div(cell);
+      d_own = domain_of(g, cr, (int)(cell - (uint32_t)cr * (uint32_t)g.Nz));
+    } else {
+      d_own = domain_of(g, cell_number(r, g.dr), cell_number(z, g.dz));
+    }
+    if (d_own != d) append_migrant(mg, st, sp, q, d, d_own, seq, g, &err);
+  }
+  if (err) atomicOr(&st->err, err);
+}
+
+__global__ void k_reset_migrants(StepState *st) { st->n_mig = 0; }
+
 __global__ void k_count_live(const double *r, long n, unsigned long long *out) {
   unsigned long long c = 0;
   for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long)gridDim.x * blockDim.x) c += r[p] >= 0;
@@ -128,6 +183,19 @@ void launch_init_bunch(pcp_ctx *c, int sp, long base, long n_new, double sigma_r
   unsigned grid = (unsigned)std::min<long>((n_new + 255) / 256, (long)c->n_cu * 32);
   hipLaunchKernelGGL(k_init_bunch, dim3(grid ? grid : 1), dim3(256), 0, c->stream, c->g, c->sp[sp], base, n_new,
                      sigma_r, sigma_z, z0, vz, weight, seed, c->mig.gbase, sp);
+}
+
+void launch_init_reference(pcp_ctx *c, int sp, long base, uint32_t M, const RefInitDom *dev_doms,
+                           const RadicalPowers &pw, double temperature, uint64_t seed) {
+  const long total = (long)M * c->D;
+  unsigned grid = (unsigned)std::min<long>((total + 255) / 256, (long)c->n_cu * 32);
+  hipLaunchKernelGGL(k_reset_migrants, dim3(1), dim3(1), 0, c->stream, c->st);
+  if (c->s32)
+    hipLaunchKernelGGL(k_init_reference<true>, dim3(grid ? grid : 1), dim3(256), 0, c->stream, c->g, SpeciesDev{},
+                       c->s32->sp[sp], sp, base, M, dev_doms, pw, temperature, c->sp[sp].m, seed, c->mig, c->st);
+  else
+    hipLaunchKernelGGL(k_init_reference<false>, dim3(grid ? grid : 1), dim3(256), 0, c->stream, c->g, c->sp[sp],
+                       Species32{}, sp, base, M, dev_doms, pw, temperature, c->sp[sp].m, seed, c->mig, c->st);
 }
 
 void launch_count_live(pcp_ctx *c, int sp, unsigned long long *dev_out) {

@@ -4,6 +4,7 @@
 #include <algorithm>
 
 #include "pcp_internal.hpp"
+#include "pcp_refinit.hpp"
 
 namespace pcp {
 
@@ -12,6 +13,12 @@ void launch_init_plasma(pcp_ctx *c, int sp, long base, long n_new, const long *d
                         const long *dev_dom_off);
 void launch_init_bunch(pcp_ctx *c, int sp, long base, long n_new, double sigma_r, double sigma_z, double z0,
                        double vz, double weight, uint64_t seed);
+// the reference's flat placement + weights + thermal velocities of one
+// plasma species, M particles per domain (dev_doms: one RefInitDom per
+// domain); particles placed beyond their domain are appended as migrants
+// (the caller runs the migration key pass afterwards)
+void launch_init_reference(pcp_ctx *c, int sp, long base, uint32_t M, const RefInitDom *dev_doms,
+                           const RadicalPowers &pw, double temperature, uint64_t seed);
 void launch_count_live(pcp_ctx *c, int sp, unsigned long long *dev_out);
 // device-to-device copy of `bytes` (read + write counted), iters launches
 int bench_copy(pcp_ctx *c, long bytes, int iters, double *gbps);

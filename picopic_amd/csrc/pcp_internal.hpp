@@ -176,6 +176,25 @@ __host__ __device__ inline double *ehist_row(double *h, unsigned long long step)
   return h + (long)(step % EHIST) * EHIST_COLS;
 }
 
+// domain owning global cell (cr, cz)
+__device__ __forceinline__ int domain_of(const Geom &g, int cr, int cz) { return (cr / g.nr) * g.Dz + (cz / g.nz); }
+
+// Appends particle p of species sp, moving from domain d_old to d_new, to
+// the step's migrant list.  Its key orders the arrivals of a destination as
+// particles_runaway_collector's push_backs do: colour pass of the source
+// (SMB.cpp:211-340 visits domains in 2 x 2 colour order), source domain,
+// position in the source vector.
+__device__ __forceinline__ void append_migrant(Migrants mg, StepState *st, int sp, long p, int d_old, int d_new,
+                                               uint64_t seq, const Geom &g, int *err) {
+  unsigned int slot = atomicAdd(&st->n_mig, 1u);
+  if (slot >= (unsigned long)mg.cap) { *err |= ERR_MIGRANT_OVERFLOW; return; }
+  const int I = (int)g.div_Dz.div((uint32_t)d_old), J = d_old - I * g.Dz;
+  const uint64_t colour = (uint64_t)((I % 2) * 2 + (J % 2));
+  mg.key[slot] = (colour << 47) | ((uint64_t)d_old << 35) | (seq & ((1ull << 35) - 1));
+  mg.ref[slot] = ((uint64_t)sp << 40) | (uint64_t)p;
+  mg.group[slot] = (uint32_t)(d_new * MAX_SPECIES + sp);
+}
+
 struct State32;      // FP32 split-precision state (pcp32.hpp)
 struct CollBuffers;  // collision pass buffers (pcp_collide.hip)
 struct MlpState;     // collision surrogate network (pcp_mlp.hpp)

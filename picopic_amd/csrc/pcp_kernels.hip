@@ -71,9 +71,6 @@ struct MarkSink {
   }
 };
 
-// domain owning global cell (cr, cz)
-__device__ __forceinline__ int domain_of(const Geom &g, int cr, int cz) { return (cr / g.nr) * g.Dz + (cz / g.nz); }
-
 __device__ __forceinline__ uint64_t order_key(int sp, uint64_t seq) { return ((uint64_t)sp << 36) | seq; }
 
 // ---------------------------------------------------------------------------
@@ -320,20 +317,6 @@ __device__ __forceinline__ bool move_one(const Geom &g, int beam, double &r, dou
   if (cr < 0 || cz < 0 || cr >= g.Nr || cz >= g.Nz) return false;
   d_new = domain_of(g, (int)cr, (int)cz);
   return true;
-}
-
-__device__ __forceinline__ int colour_rank(const Geom &g, int d) {
-  int I = d / g.Dz, J = d % g.Dz;
-  return (I % 2) * 2 + (J % 2);
-}
-
-__device__ __forceinline__ void append_migrant(Migrants mg, StepState *st, int sp, long p, int d_old, int d_new,
-                                               uint64_t seq, const Geom &g, int *err) {
-  unsigned int slot = atomicAdd(&st->n_mig, 1u);
-  if (slot >= (unsigned long)mg.cap) { *err |= ERR_MIGRANT_OVERFLOW; return; }
-  mg.key[slot] = ((uint64_t)colour_rank(g, d_old) << 47) | ((uint64_t)d_old << 35) | (seq & ((1ull << 35) - 1));
-  mg.ref[slot] = ((uint64_t)sp << 40) | (uint64_t)p;
-  mg.group[slot] = (uint32_t)(d_new * MAX_SPECIES + sp);
 }
 
 // phase "move": pos[cur] -> pos[cur^1]; velocities in place.  Removal and
