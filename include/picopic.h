@@ -131,6 +131,41 @@ int pcp_init_bunch(pcp_ctx *ctx, int sp, long n, double n_peak, double sigma_r, 
  * + velocity_distribution (include/specieP.hpp:225-229). */
 int pcp_init_reference(pcp_ctx *ctx, int sp, long macro_amount, double density_left,
                        double density_right, double temperature_ev, uint64_t seed);
+/* One probe frame (OutWriter::operator(), src/outWriter.cpp:49-100, as
+ * OutController maps a probe onto the domains, src/outController.cpp:49-309):
+ * component comp (0 r, 1 phi, 2 z) of grid `which` over the global cells
+ * [r0, r1) x [z0, z1), each value read from the domain that owns the cell,
+ * row-major (r outer) into host_out ((r1 - r0) x (z1 - z0) doubles).  The
+ * reference's "rec" probe is this window; "col" z is [0, Nr) x [z, z + 1),
+ * "row" r is [r, r + 1) x [0, Nz), "dot" is one cell.  Probes read
+ * E (PCP_GRID_E), H (PCP_GRID_H_AT_ET, as the reference's writers do) and
+ * J (PCP_GRID_J).  The window is gathered on the device. */
+int pcp_probe(pcp_ctx *ctx, int which, int comp, long r0, long z0, long r1, long z1, double *host_out);
+
+/* A particle beam of bunches injected over time: BeamP::BeamP for beam species
+ * sp in every domain (src/beamP.cpp:24-66).  From then on pcp_step injects at
+ * the start of every step as SMB::inject_beam does (src/SMB.cpp:1308-1322,
+ * BeamP::inject src/beamP.cpp:71-157): while the context time lies in a
+ * bunch's window [start, start + length / velocity), each domain at z = 0
+ * whose radial range starts below `radius` appends ceil(macro_per_step)
+ * particles (macro_per_step = macro_amount / bunches_amount (integer
+ * division) x its share of the radius x velocity dt / length) at r uniform
+ * over its share, z uniform over [dz/2, dz/2 + velocity dt), v = (0, 0,
+ * velocity), weights proportional to r.  The reference's uniform() draws
+ * are Philox draw 0 of counter (domain << 40 | injected count), key seed.
+ * Injected particle ids: 1 << 62 | domain << 40 | count.  The species must
+ * have been created with is_beam = 1 and enough capacity.  Replaces
+ * BeamP's constructor and SpecieP::inject (include/beamP.hpp:43-62). */
+int pcp_set_beam(pcp_ctx *ctx, int sp, long macro_amount, double start_time, double radius,
+                 double density, int bunches_amount, double bunch_length, double bunches_distance,
+                 double velocity, uint64_t seed);
+/* SMB::inject_beam at the current context time (for callers that drive the
+ * phases themselves; pcp_step injects by itself). */
+int pcp_inject(pcp_ctx *ctx);
+/* TimeSim::current of the context: 0 at creation, + dt after every step of
+ * pcp_step (src/PiCoPiC.cpp:343 advances it after each iteration). */
+int pcp_set_time(pcp_ctx *ctx, double t);
+double pcp_time(pcp_ctx *ctx);
 
 /* Whole halo grids, domain-major [domain][component][Nr_d+4][Nz_d+4]
  * (one component for EPS and SIGMA), as Grid<double> stores them

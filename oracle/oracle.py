@@ -135,6 +135,13 @@ def lib():
         L.orc_mj_g_inverse.restype = C.c_double
         L.orc_mj_energy.argtypes = [C.c_double, C.c_uint64, C.c_uint64]
         L.orc_mj_energy.restype = C.c_double
+        L.orc_smb_time.argtypes = [C.c_void_p]
+        L.orc_smb_time.restype = C.c_double
+        L.orc_smb_set_time.argtypes = [C.c_void_p, C.c_double]
+        L.orc_smb_set_beam.argtypes = [C.c_void_p, C.c_int, C.c_long, C.c_double, C.c_double, C.c_double, C.c_long,
+                                       C.c_double, C.c_double, C.c_double, C.c_uint64]
+        L.orc_smb_set_beam.restype = C.c_int
+        L.orc_smb_inject.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -269,6 +276,21 @@ class SMB:
                                           seed, id_base)
         if rc:
             raise RuntimeError("orc_smb_init_reference failed (%d)" % rc)
+
+    def set_beam(self, sp, macro_amount, start_time, radius, density, bunches, length, distance, velocity, seed):
+        """BeamP for species sp: bunches injected at the start of each step."""
+        if lib().orc_smb_set_beam(self.h, sp, int(macro_amount), start_time, radius, density, int(bunches), length,
+                                  distance, velocity, seed):
+            raise ValueError("bad beam")
+
+    def inject(self):
+        lib().orc_smb_inject(self.h)
+
+    def time(self):
+        return lib().orc_smb_time(self.h)
+
+    def set_time(self, t):
+        lib().orc_smb_set_time(self.h, t)
 
     def get_field_e(self, d, r, z):
         out = np.zeros(3)

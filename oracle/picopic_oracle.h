@@ -124,8 +124,19 @@ enum { ORC_PH_YEE_E = 0, ORC_PH_OVERLAY_E, ORC_PH_YEE_H, ORC_PH_OVERLAY_H,
        ORC_PH_PUSH, ORC_PH_COLLIDE, ORC_PH_MOVE, ORC_PH_MIGRATE, ORC_PH_RESET_J,
        ORC_PH_DEPOSIT, ORC_PH_OVERLAY_J, ORC_PH_COUNT };
 void orc_smb_phase(orc_smb *s, int phase);
-/* solve_maxvell + advance_particles + solve_current, n times (no beam injection) */
+/* beam injection + solve_maxvell + advance_particles + solve_current, n
+ * times; the clock (TimeSim::current) advances by dt after each */
 void orc_smb_step(orc_smb *s, int n);
+double orc_smb_time(const orc_smb *s);
+void orc_smb_set_time(orc_smb *s, double t);
+/* BeamP::BeamP for species sp in every domain (src/beamP.cpp:24-66); the
+ * injection (BeamP::inject, src/beamP.cpp:71-157) then runs at the start of
+ * every step.  Injected particle ids: 1 << 62 | domain << 40 | count. */
+int orc_smb_set_beam(orc_smb *s, int sp, long macro_amount, double start_time, double radius, double density,
+                     long bunches_amount, double bunch_length, double bunches_distance, double velocity,
+                     uint64_t seed);
+/* SMB::inject_beam at the current time */
+void orc_smb_inject(orc_smb *s);
 void orc_set_threads(int n);
 
 /* ---- unit-level entry points used by the known-answer tests ---- */

@@ -31,6 +31,7 @@ EXPORTS = [
     "pcp_surrogate_fit", "pcp_surrogate_get_norm", "pcp_surrogate_set_norm", "pcp_surrogate_fallback_cells",
     "pcp_species_moments", "pcp_continuity_residual", "pcp_set_graph", "pcp_set_sort_bin", "pcp_set_launch",
     "pcp_get_tuning", "pcp_autotune", "pcp_surrogate_calibrate", "pcp_init_reference",
+    "pcp_set_beam", "pcp_inject", "pcp_set_time", "pcp_time", "pcp_probe",
 ]
 COLL = {"p12": 0, "ta77s": 1, "sk98": 2, "surrogate": 3}
 
@@ -82,6 +83,13 @@ def load(path=LIB_PATH):
     L.pcp_init_plasma.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64]
     L.pcp_init_bunch.argtypes = [vp, C.c_int, C.c_long] + [C.c_double] * 5 + [C.c_uint64]
     L.pcp_init_reference.argtypes = [vp, C.c_int, C.c_long, C.c_double, C.c_double, C.c_double, C.c_uint64]
+    L.pcp_set_beam.argtypes = [vp, C.c_int, C.c_long, C.c_double, C.c_double, C.c_double, C.c_int, C.c_double,
+                               C.c_double, C.c_double, C.c_uint64]
+    L.pcp_inject.argtypes = [vp]
+    L.pcp_probe.argtypes = [vp, C.c_int, C.c_int, C.c_long, C.c_long, C.c_long, C.c_long, dp]
+    L.pcp_set_time.argtypes = [vp, C.c_double]
+    L.pcp_time.argtypes = [vp]
+    L.pcp_time.restype = C.c_double
     L.pcp_download_grid.argtypes = [vp, C.c_int, dp]
     L.pcp_upload_grid.argtypes = [vp, C.c_int, dp]
     L.pcp_grid_points.argtypes = [vp]
@@ -231,6 +239,26 @@ class Context:
         """The reference's flat placement, r-proportional weights and
         Maxwell-Juettner thermal velocities (macro_amount over all domains)."""
         _check(_lib.pcp_init_reference(self.h, sp, int(macro_amount), density_left, density_right, temperature, seed))
+
+    def set_beam(self, sp, macro_amount, start_time, radius, density, bunches, length, distance, velocity, seed):
+        """BeamP for beam species sp: bunches injected at the start of each step."""
+        _check(_lib.pcp_set_beam(self.h, sp, int(macro_amount), start_time, radius, density, int(bunches), length,
+                                 distance, velocity, seed))
+
+    def inject(self):
+        _check(_lib.pcp_inject(self.h))
+
+    def probe(self, which, comp, r0, z0, r1, z1):
+        """Component comp of grid `which` over global cells [r0, r1) x [z0, z1)."""
+        out = np.zeros((r1 - r0, z1 - z0))
+        _check(_lib.pcp_probe(self.h, GRIDS[which], comp, r0, z0, r1, z1, _dp(out)))
+        return out
+
+    def time(self):
+        return _lib.pcp_time(self.h)
+
+    def set_time(self, t):
+        _check(_lib.pcp_set_time(self.h, t))
 
     def init_bunch(self, sp, n, n_peak, sigma_r, sigma_z, z0, gamma, seed):
         _check(_lib.pcp_init_bunch(self.h, sp, n, n_peak, sigma_r, sigma_z, z0, gamma, seed))

@@ -36,14 +36,16 @@ an optional ``"device"`` object (none of them changes a reference key):
   (src/SMB.cpp:134-136), which the reference's reader sets equal to
   ``density``.
 
-The initial plasma is drawn on the device (``pcp_init_plasma``, seeded):
-the reference's own ``distribute()`` draws with a time seed and is not
-reproducible; the per-cell count is the rounded norm above.  Beam
-injection over time (``BeamP::inject``) has no device counterpart;
-``beams[]`` yields species slots only (``Context.init_bunch`` places a
-Gaussian bunch once).  Collisions pair the plasma species named ``ions``
-with the first other plasma species, as ``Collisions`` sorts them
-(src/collisions.cpp:87-108).
+The initial plasma is the reference's own (``device.init`` = ``"reference"``,
+the default): ``pcp_init_reference`` places each species' macro amount as
+SMB::SMB + Domain::distribute do (flat placement, r-proportional weights,
+Maxwell-Juettner thermal velocities; seeded Philox draws in place of the
+reference's time seed).  ``device.init`` = ``"cells"`` draws the synthetic
+per-cell plasma of ``pcp_init_plasma`` instead (uniform in r^2 inside each
+cell, equal weights, the rounded norm per cell).  ``beams[]`` become beam
+species injected over time (``pcp_set_beam``, BeamP::inject).  Collisions
+pair the plasma species named ``ions`` with the first other plasma species,
+as ``Collisions`` sorts them (src/collisions.cpp:87-108).
 """
 import json
 import math
@@ -144,6 +146,33 @@ def per_cell(cfg):
     return max(1, int(round(macro_amounts(cfg)[0])))
 
 
+def beam_capacity(cfg, spec, amount):
+    """Slots a beam needs for every particle BeamP::inject can place: per
+    domain at z = 0 reached by the radius, ceil of its per-step rate
+    (src/beamP.cpp:40-66) for each step of the bunch windows (only domain
+    (0, 0) passes to later bunches)."""
+    size, grid, domains = geometry(cfg)
+    _, _, dt = time(cfg)
+    dr = size[0] / grid[0]
+    nrd = grid[0] // domains[0]
+    b = spec["bunch"]
+    radius, length, bunches = float(b["radius"]), float(b["length"]), int(b["amount"])
+    velocity = float(spec["velocity"])
+    steps = int(math.ceil(length / velocity / dt)) + 2
+    total = 0
+    for i in range(domains[0]):
+        bot, top = i * nrd * dr, (i + 1) * nrd * dr
+        if radius > top:
+            share = size[0] / domains[0]
+        elif radius > bot:
+            share = radius - bot
+        else:
+            share = 0.0
+        rate = (amount // bunches) * (share / radius) * velocity * dt / length
+        total += int(math.ceil(rate)) * steps * (bunches if i == 0 else 1)
+    return max(1, total)
+
+
 def context(src, seed=1):
     """A device context for the configuration, with the plasma species
     initialised.  Returns (ctx, species)."""
@@ -162,20 +191,35 @@ def context(src, seed=1):
         raise ValueError("at most 8 species")
     ppc = per_cell(cfg)
     _, amounts = macro_amounts(cfg)
+    init = dev.get("init", "reference")
+    if init not in ("reference", "cells"):
+        raise ValueError("unknown device.init %r" % init)
     factor = float(dev.get("capacity_factor", 1.1))
     cells = grid[0] * grid[1]
     descs = []
     for (name, q, m, beam, spec), amount in zip(sp, amounts):
-        cap = max(1, amount) if beam else int(math.ceil(ppc * cells * factor))
+        if beam:
+            cap = beam_capacity(cfg, spec, amount)
+        elif init == "reference":
+            cap = max(1, amount)
+        else:
+            cap = int(math.ceil(ppc * cells * factor))
         descs.append((q, m, beam, cap))
     ctx = capi.Context(size, grid, domains, dt, descs, pusher=pusher, ccs=ccs, pml=pml(cfg),
                        precision=dev.get("precision", "fp64"))
-    for k, (name, q, m, beam, spec) in enumerate(sp):
+    for k, ((name, q, m, beam, spec), amount) in enumerate(zip(sp, amounts)):
         if beam:
+            b = spec["bunch"]
+            ctx.set_beam(k, amount, float(spec["start_time"]), float(b["radius"]), float(b["density"]),
+                         int(b["amount"]), float(b["length"]), float(b["distance"]), float(spec["velocity"]),
+                         seed + k)
             continue
         left = float(spec["density"])
         right = float(spec.get("right_density", left))
-        ctx.init_plasma(k, ppc, left, right, float(spec["temperature"]), seed + k)
+        if init == "reference":
+            ctx.init_reference(k, amount, left, right, float(spec["temperature"]), seed + k)
+        else:
+            ctx.init_plasma(k, ppc, left, right, float(spec["temperature"]), seed + k)
     if "sort_bin" in dev:
         ctx.set_sort_bin(*dev["sort_bin"])
     if dev.get("collisions"):

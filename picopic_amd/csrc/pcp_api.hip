@@ -365,7 +365,7 @@ int pcp_destroy(pcp_ctx *c) {
   mlp_free(c);
   hipFree(c->f.E); hipFree(c->f.H); hipFree(c->f.Hat); hipFree(c->f.J);
   hipFree(c->f.eps); hipFree(c->f.sigma); hipFree(c->st); hipFree(c->marker); hipFree(c->cell_flag);
-  hipFree(c->diag); hipFree(c->ehist);
+  hipFree(c->diag); hipFree(c->ehist); hipFree(c->probe_buf);
   for (int s = 0; s < MAX_SPECIES; ++s) {
     SpeciesDev &sp = c->sp[s];
     hipFree(sp.r[0]); hipFree(sp.r[1]); hipFree(sp.z[0]); hipFree(sp.z[1]);
@@ -758,6 +758,56 @@ int pcp_init_reference(pcp_ctx *c, int sp, long macro_amount, double density_lef
   return check_state(c);
 }
 
+// SMB::inject_beam at the current time (src/SMB.cpp:1308-1322 calls
+// BeamP::inject, src/beamP.cpp:71-157, for every domain).  Returns the
+// number of particles injected, or a negative status.
+static long inject_beams(pcp_ctx *c) {
+  const Geom &g = c->g;
+  long total = 0;
+  for (int sp = 0; sp < c->cfg.n_species; ++sp) {
+    BeamHost &b = c->beam[sp];
+    if (!b.on) continue;
+    const double t = c->t_now;
+    for (int d = 0; d < c->D; ++d) {
+      BeamDomHost &B = b.dom[d];
+      if (B.bunch >= b.bunches_amount) continue;
+      const Dom dm = make_dom(g, d);
+      const double bunch_start_time =
+          b.start_time + (b.bunch_length + b.bunches_distance) / b.velocity * B.bunch;
+      const double bunch_finish_time = bunch_start_time + b.bunch_length / b.velocity;
+      if (dm.bot_r * g.dr < b.radius && t >= bunch_start_time && t < bunch_finish_time) {
+        // the loop `for (unsigned int i = 0; i < macro_per_step_to_inject; ++i)`
+        const long n = B.macro_per_step > 0 ? (long)std::ceil(B.macro_per_step) : 0;
+        if (n > 0) {
+          if (int rc = ensure_capacity(c, sp, n)) return rc;
+          const double dr = g.dr, dz = g.dz;
+          const double dl = b.velocity * g.dt;
+          const double v_macro_avg = PI * b.radius * dr * dz;
+          const double N_total = PI * b.radius * b.radius * b.bunch_length * b.bunches_amount * b.density;
+          const double n_per_macro_avg = N_total / b.macro_amount;
+          // particles past the domain's cells (a bunch wider than the domain)
+          const bool spill = B.domain_radius + dr / 2 + dm.bot_r * dr >= (dm.bot_r + g.nr) * dr ||
+                             dl + dz / 2. >= g.nz * dz;
+          launch_inject(c, sp, c->sp[sp].n, n, d, B.domain_radius, dl, b.velocity, v_macro_avg, n_per_macro_avg,
+                        b.seed, B.injected, spill);
+          if (spill) {
+            if (c->s32) launch32_migrant_keys(c);
+            else launch_migrant_keys(c);
+          }
+          c->sp[sp].n += n;
+          if (c->s32) c->s32->sp[sp].n = c->sp[sp].n;
+          B.injected += (uint64_t)n;
+          total += n;
+        }
+      }
+      // the counter advances in domain (0, 0) only (src/beamP.cpp:146-153)
+      if (t > bunch_finish_time && dm.left_z == 0 && dm.bot_r == 0) ++B.bunch;
+    }
+  }
+  if (total) graph_invalidate(c);
+  return total;
+}
+
 static float *grid_ptr32(pcp_ctx *c, int which, int *ncomp) {
   *ncomp = 3;
   FieldsT<float> &f = c->s32->f;
@@ -789,6 +839,32 @@ int pcp_download_grid(pcp_ctx *c, int which, double *host) {
   if (!p) return fail(PCP_E_ARG, "unknown grid");
   HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipMemcpy(host, p, sizeof(double) * (size_t)c->D * nc * c->g.P, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pcp_probe(pcp_ctx *c, int which, int comp, long r0, long z0, long r1, long z1, double *host) {
+  if (!c || !host) return fail(PCP_E_ARG, "bad argument");
+  const Geom &g = c->g;
+  if (!(r0 >= 0 && z0 >= 0 && r0 <= r1 && z0 <= z1 && r1 <= g.Nr && z1 <= g.Nz))
+    return fail(PCP_E_ARG, "probe window outside the grid");
+  int nc;
+  const void *grid = c->s32 ? (const void *)grid_ptr32(c, which, &nc) : (const void *)grid_ptr(c, which, &nc);
+  if (!grid) return fail(PCP_E_ARG, "unknown grid");
+  if (comp < 0 || comp >= nc) return fail(PCP_E_ARG, "bad component");
+  const long n = (r1 - r0) * (z1 - z0);
+  if (n == 0) return 0;
+  if (n > c->probe_cap) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    hipFree(c->probe_buf);
+    c->probe_buf = nullptr;
+    c->probe_cap = 0;
+    if (int rc = dalloc(&c->probe_buf, (size_t)n)) return rc;
+    c->probe_cap = n;
+  }
+  launch_probe(c, grid, c->s32 != nullptr, nc, comp, r0, z0, r1 - r0, z1 - z0, c->probe_buf);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(host, c->probe_buf, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return 0;
 }
 
@@ -1212,6 +1288,61 @@ int pcp_set_graph(pcp_ctx *c, int on) {
   return 0;
 }
 
+int pcp_set_beam(pcp_ctx *c, int sp, long macro_amount, double start_time, double radius, double density,
+                 int bunches_amount, double bunch_length, double bunches_distance, double velocity, uint64_t seed) {
+  if (!c || sp < 0 || sp >= c->cfg.n_species || macro_amount <= 0 || macro_amount > 0xFFFFFFFFL ||
+      bunches_amount <= 0 || !(velocity > 0) || !(radius > 0) || !(bunch_length > 0) || !(density >= 0))
+    return fail(PCP_E_ARG, "bad argument");
+  if (!c->sp[sp].beam) return fail(PCP_E_ARG, "the species was not created as a beam");
+  const Geom &g = c->g;
+  BeamHost &b = c->beam[sp];
+  b.on = 1;
+  b.macro_amount = (unsigned int)macro_amount;
+  b.bunches_amount = (unsigned int)bunches_amount;
+  b.start_time = start_time;
+  b.radius = radius;
+  b.density = density;
+  b.bunch_length = bunch_length;
+  b.bunches_distance = bunches_distance;
+  b.velocity = velocity;
+  b.seed = seed;
+  b.dom.assign(c->D, BeamDomHost{});
+  // BeamP::BeamP in every domain (src/beamP.cpp:40-66)
+  for (int d = 0; d < c->D; ++d) {
+    const Dom dm = make_dom(g, d);
+    BeamDomHost &B = b.dom[d];
+    const long top_r = dm.bot_r + g.nr;
+    if (dm.left_z > 0)
+      B.domain_radius = 0;
+    else if (radius > top_r * g.dr)
+      B.domain_radius = g.size_r;
+    else if (radius > dm.bot_r * g.dr)
+      B.domain_radius = radius - dm.bot_r * g.dr;
+    else
+      B.domain_radius = 0;
+    const double macro_normalizer = B.domain_radius / radius;
+    const double bunch_macro_amount = (double)(b.macro_amount / b.bunches_amount) * macro_normalizer;
+    B.macro_per_step = bunch_macro_amount * velocity * g.dt / bunch_length;
+  }
+  return 0;
+}
+
+int pcp_inject(pcp_ctx *c) {
+  if (!c) return fail(PCP_E_ARG, "null context");
+  long n = inject_beams(c);
+  if (n < 0) return (int)n;
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int pcp_set_time(pcp_ctx *c, double t) {
+  if (!c) return fail(PCP_E_ARG, "null context");
+  c->t_now = t;
+  return 0;
+}
+
+double pcp_time(pcp_ctx *c) { return c ? c->t_now : 0.0; }
+
 int pcp_step(pcp_ctx *c, int n_steps) {
   if (!c || n_steps < 0) return fail(PCP_E_ARG, "bad argument");
   // optional per-phase device timing: events on the context stream
@@ -1226,6 +1357,8 @@ int pcp_step(pcp_ctx *c, int n_steps) {
     auto mark = [&](int k) {
       if (c->timing) (void)hipEventRecord(ev[(size_t)t * 6 + k], c->stream);
     };
+    // SMB::inject_beam first (src/PiCoPiC.cpp:320), outside any graph
+    if (long ni = inject_beams(c); ni < 0) return (int)ni;
     mark(0);
     // the sorts change the particle buffers, so they run outside any graph
     for (int s = 0; s < c->cfg.n_species; ++s)
@@ -1256,6 +1389,7 @@ int pcp_step(pcp_ctx *c, int n_steps) {
       if (coll && f32) graph_invalidate(c);
     }
     ++c->steps_done;
+    c->t_now += c->g.dt;
   }
   HIP_TRY(hipGetLastError());
   int rc = pcp_sync(c);

@@ -168,4 +168,34 @@ int launch_continuity_residual(pcp_ctx *c, double *dev_out) {
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Probe windows (OutWriter::operator(), src/outWriter.cpp:49-100): node
+// values of one component over global cells [r0, r0 + nr_w) x [z0, z0 + nz_w),
+// each read from the domain that owns the cell (its interior node), written
+// row-major as double.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void k_probe(Geom g, const T *grid, int ncomp, int comp, long r0, long z0, long nr_w, long nz_w,
+                        double *out) {
+  const long n = nr_w * nz_w;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    const long i = r0 + t / nz_w, j = z0 + t % nz_w;
+    const int d = (int)(i / g.nr) * g.Dz + (int)(j / g.nz);
+    const int li = (int)(i - (i / g.nr) * g.nr), lj = (int)(j - (j / g.nz) * g.nz);
+    out[t] = (double)grid[gidx(g, d, comp, ncomp, li, lj)];
+  }
+}
+
+void launch_probe(pcp_ctx *c, const void *grid, bool f32, int ncomp, int comp, long r0, long z0, long nr_w, long nz_w,
+                  double *dev_out) {
+  const long n = nr_w * nz_w;
+  const unsigned grid_n = (unsigned)std::max(1L, std::min((n + 255) / 256, (long)c->n_cu * 8));
+  if (f32)
+    hipLaunchKernelGGL(k_probe<float>, dim3(grid_n), dim3(256), 0, c->stream, c->g, (const float *)grid, ncomp, comp,
+                       r0, z0, nr_w, nz_w, dev_out);
+  else
+    hipLaunchKernelGGL(k_probe<double>, dim3(grid_n), dim3(256), 0, c->stream, c->g, (const double *)grid, ncomp,
+                       comp, r0, z0, nr_w, nz_w, dev_out);
+}
+
 }  // namespace pcp

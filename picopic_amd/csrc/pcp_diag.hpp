@@ -9,4 +9,9 @@ namespace pcp {
 // max |dQ + dt div J| over the nodes, max |dQ|, largest |q w|; synchronises
 int launch_continuity_residual(pcp_ctx *c, double *dev_out);
 
+// probe window: component comp of a [D][ncomp][P] grid (float when f32)
+// over global cells [r0, r0 + nr_w) x [z0, z0 + nz_w), row-major doubles
+void launch_probe(pcp_ctx *c, const void *grid, bool f32, int ncomp, int comp, long r0, long z0, long nr_w, long nz_w,
+                  double *dev_out);
+
 }  // namespace pcp

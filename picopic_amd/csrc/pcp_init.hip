@@ -135,6 +135,68 @@ __global__ void k_init_reference(Geom g, SpeciesDev s64, Species32 s32, int sp, 
 
 __global__ void k_reset_migrants(StepState *st) { st->n_mig = 0; }
 
+// BeamP::inject (src/beamP.cpp:71-157) of n particles in domain d: the
+// reference's two uniforms per particle are Philox draw 0 of counter
+// (d << 40 | ctr0 + i); the particles take order keys after every particle
+// of (d, sp) (appended to the domain's vector); one placed past the domain's
+// cells is a migrant to its owner.
+template <bool F32>
+__global__ void k_inject(Geom g, SpeciesDev s64, Species32 s32, int sp, long base, long n, int d, double domain_radius,
+                         double dl, double velocity, double v_macro_avg, double n_per_macro_avg, uint64_t seed,
+                         uint64_t ctr0, const uint64_t *gbase, Migrants mg, StepState *st) {
+  const Dom dm = make_dom(g, d);
+  const double dr = g.dr, dz = g.dz;
+  const uint64_t seq_base = gbase[(long)d * MAX_SPECIES + sp];
+  int err = 0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double rand_r, rand_z;
+    const uint64_t ctr = ((uint64_t)d << 40) | (ctr0 + (uint64_t)i);
+    uniform2(ctr, 0, seed, rand_r, rand_z);
+    const double pos_r = domain_radius * rand_r + dr / 2 + (double)dm.bot_r * dr;
+    const double pos_z = dl * rand_z + dz / 2.;
+    double norm;
+    if (pos_r > dr / 2)
+      norm = 2 * PI * pos_r * dr * dz / v_macro_avg;
+    else
+      norm = PI * (pos_r * pos_r + pos_r * dr + dr * dr / 4.) * dz / v_macro_avg;
+    const double w = n_per_macro_avg * norm;
+    const long q = base + i;
+    const uint64_t seq = seq_base + (uint64_t)i;
+    const uint64_t id = (1ull << 62) | ctr;
+    if (F32) {
+      store_split(s32, q, g, pos_r, pos_z);
+      s32.vr[0][q] = s32.vr[1][q] = 0.f;
+      s32.vphi[0][q] = s32.vphi[1][q] = 0.f;
+      s32.vz[0][q] = s32.vz[1][q] = (float)velocity;
+      s32.w[q] = (float)w;
+      s32.seq[q] = seq;
+      s32.id[q] = id;
+    } else {
+      s64.r[0][q] = pos_r; s64.r[1][q] = pos_r;
+      s64.z[0][q] = pos_z; s64.z[1][q] = pos_z;
+      s64.vr[q] = 0;
+      s64.vphi[q] = 0;
+      s64.vz[q] = velocity;
+      s64.w[q] = w;
+      s64.seq[q] = seq;
+      s64.id[q] = id;
+    }
+    int d_own;
+    if (F32) {
+      const uint32_t cell = s32.cell[0][q];
+      const int cr = (int)g.div_Nz.div(cell);
+      d_own = domain_of(g, cr, (int)(cell - (uint32_t)cr * (uint32_t)g.Nz));
+    } else {
+      d_own = domain_of(g, cell_number(pos_r, dr), cell_number(pos_z, dz));
+    }
+    if (d_own != d) append_migrant(mg, st, sp, q, d, d_own, seq, g, &err);
+  }
+  if (err) atomicOr(&st->err, err);
+}
+
+// the domain's next order key follows the injected particles
+__global__ void k_advance_gbase(uint64_t *gbase, long idx, long n) { gbase[idx] += (uint64_t)n; }
+
 __global__ void k_count_live(const double *r, long n, unsigned long long *out) {
   unsigned long long c = 0;
   for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long)gridDim.x * blockDim.x) c += r[p] >= 0;
@@ -196,6 +258,21 @@ void launch_init_reference(pcp_ctx *c, int sp, long base, uint32_t M, const RefI
   else
     hipLaunchKernelGGL(k_init_reference<false>, dim3(grid ? grid : 1), dim3(256), 0, c->stream, c->g, c->sp[sp],
                        Species32{}, sp, base, M, dev_doms, pw, temperature, c->sp[sp].m, seed, c->mig, c->st);
+}
+
+void launch_inject(pcp_ctx *c, int sp, long base, long n, int d, double domain_radius, double dl, double velocity,
+                   double v_macro_avg, double n_per_macro_avg, uint64_t seed, uint64_t ctr0, bool migrants) {
+  unsigned grid = (unsigned)std::min<long>((n + 255) / 256, (long)c->n_cu * 8);
+  if (migrants) hipLaunchKernelGGL(k_reset_migrants, dim3(1), dim3(1), 0, c->stream, c->st);
+  if (c->s32)
+    hipLaunchKernelGGL(k_inject<true>, dim3(grid ? grid : 1), dim3(256), 0, c->stream, c->g, SpeciesDev{},
+                       c->s32->sp[sp], sp, base, n, d, domain_radius, dl, velocity, v_macro_avg, n_per_macro_avg, seed,
+                       ctr0, c->mig.gbase, c->mig, c->st);
+  else
+    hipLaunchKernelGGL(k_inject<false>, dim3(grid ? grid : 1), dim3(256), 0, c->stream, c->g, c->sp[sp], Species32{},
+                       sp, base, n, d, domain_radius, dl, velocity, v_macro_avg, n_per_macro_avg, seed, ctr0,
+                       c->mig.gbase, c->mig, c->st);
+  hipLaunchKernelGGL(k_advance_gbase, dim3(1), dim3(1), 0, c->stream, c->mig.gbase, (long)d * MAX_SPECIES + sp, n);
 }
 
 void launch_count_live(pcp_ctx *c, int sp, unsigned long long *dev_out) {

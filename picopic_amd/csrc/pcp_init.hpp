@@ -19,6 +19,11 @@ void launch_init_bunch(pcp_ctx *c, int sp, long base, long n_new, double sigma_r
 // (the caller runs the migration key pass afterwards)
 void launch_init_reference(pcp_ctx *c, int sp, long base, uint32_t M, const RefInitDom *dev_doms,
                            const RadicalPowers &pw, double temperature, uint64_t seed);
+// BeamP::inject of n particles of species sp in domain d (slots base ..);
+// with `migrants`, the particles placed past the domain's cells are
+// appended to a fresh migrant list (the caller runs the key pass)
+void launch_inject(pcp_ctx *c, int sp, long base, long n, int d, double domain_radius, double dl, double velocity,
+                   double v_macro_avg, double n_per_macro_avg, uint64_t seed, uint64_t ctr0, bool migrants);
 void launch_count_live(pcp_ctx *c, int sp, unsigned long long *dev_out);
 // device-to-device copy of `bytes` (read + write counted), iters launches
 int bench_copy(pcp_ctx *c, long bytes, int iters, double *gbps);

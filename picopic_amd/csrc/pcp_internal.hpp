@@ -195,6 +195,22 @@ __device__ __forceinline__ void append_migrant(Migrants mg, StepState *st, int s
   mg.group[slot] = (uint32_t)(d_new * MAX_SPECIES + sp);
 }
 
+// BeamP (src/beamP.cpp:24-66) per species: the configuration and, per
+// domain, the constructor's injection radius and rate and the injection
+// state (bunch counter, particles injected so far)
+struct BeamDomHost {
+  double domain_radius = 0, macro_per_step = 0;
+  unsigned int bunch = 0;
+  uint64_t injected = 0;
+};
+struct BeamHost {
+  int on = 0;
+  unsigned int macro_amount = 0, bunches_amount = 0;
+  double start_time = 0, radius = 0, density = 0, bunch_length = 0, bunches_distance = 0, velocity = 0;
+  uint64_t seed = 0;
+  std::vector<BeamDomHost> dom;
+};
+
 struct State32;      // FP32 split-precision state (pcp32.hpp)
 struct CollBuffers;  // collision pass buffers (pcp_collide.hip)
 struct MlpState;     // collision surrogate network (pcp_mlp.hpp)
@@ -235,7 +251,11 @@ struct pcp_ctx {
   uint64_t coll_seed = 0, coll_calls = 0;
   int n_cu;
   int timing;
+  double t_now = 0;                 // TimeSim::current: advances by dt after every step
+  pcp::BeamHost beam[pcp::MAX_SPECIES];
   double *diag = nullptr;           // scratch of the diagnostic reductions (DIAG_DOUBLES)
+  double *probe_buf = nullptr;      // staging of probe windows (grown on demand)
+  long probe_cap = 0;
   double *ehist = nullptr;          // energy log [EHIST][EHIST_COLS]
   int elog = 0;                     // log energies in every step
   std::vector<std::pair<std::string, double>> ktimes;

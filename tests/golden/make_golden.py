@@ -7,6 +7,10 @@ Sources (data only, no reference code is run here):
     The numbers are transcribed from the survey of the reference.
   * the overlay known answers of test/unit/gridTest.cpp:111-215 (o_s = 3,
     interior/halo 8.3 and 20.3 -> 28.6).
+  * ``functional_true_data.json`` -- the [mean, std, var] triples the
+    reference's functional test compares against
+    (test/functional/true_data/<E|H|J>/<r|phi|z>/rec/0-32_0-128/4.dat), read
+    as numbers from a checkout of the reference when one is present.
 """
 import json
 import os
@@ -38,8 +42,26 @@ grid_kat = {
     "overlay_xy": {"a_corner_eq_b_corner": True, "corner": 13},
 }
 
+
+def functional_true_data(root="/root/reference"):
+    base = os.path.join(root, "test", "functional", "true_data")
+    out = {"source": "test/functional/true_data/<E|H|J>/<r|phi|z>/rec/0-32_0-128/4.dat of cosmonaut-ok/PiCoPiC: "
+                     "[mean, std, var] of the 32 x 128 probe (r 0..32, z 0..128) at slice 4 (schedule 20) of "
+                     "test/functional/picopic.json.tmpl with macro_amount 2.1e5; compared by "
+                     "test/functional/test.py:169-222 (rtol 0.15 on std and var)",
+           "values": {}}
+    for f in ("E", "H", "J"):
+        for c in ("r", "phi", "z"):
+            with open(os.path.join(base, f, c, "rec", "0-32_0-128", "4.dat")) as fh:
+                out["values"][f + "/" + c] = [float(x) for x in fh.read().split()]
+    return out
+
+
 if __name__ == "__main__":
     with open(os.path.join(HERE, "kat_reference.json"), "w") as f:
         json.dump(kat, f, indent=1)
     with open(os.path.join(HERE, "kat_grid.json"), "w") as f:
         json.dump(grid_kat, f, indent=1)
+    if os.path.isdir("/root/reference"):
+        with open(os.path.join(HERE, "functional_true_data.json"), "w") as f:
+            json.dump(functional_true_data(), f, indent=1)
