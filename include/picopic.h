@@ -39,7 +39,10 @@ enum { PCP_CCS_VB = 0, PCP_CCS_ZIGZAG = 1 };
 /* grids: MaxwellSolver::field_e/field_h (include/maxwellSolver.hpp:39-46),
  * MaxwellSolverYee::field_h_at_et, Current::current, epsilon, sigma */
 enum { PCP_GRID_E = 0, PCP_GRID_H = 1, PCP_GRID_H_AT_ET = 2, PCP_GRID_J = 3,
-       PCP_GRID_EPS = 4, PCP_GRID_SIGMA = 5 };
+       PCP_GRID_EPS = 4, PCP_GRID_SIGMA = 5,
+       PCP_GRID_MAP = 6 /* the last map of pcp_species_map (probes only) */ };
+/* species maps of the probes (SpecieP::calc_*, src/specieP.cpp:665-839) */
+enum { PCP_MAP_DENSITY = 0, PCP_MAP_TEMPERATURE = 1, PCP_MAP_ENERGY = 2, PCP_MAP_MOMENTUM = 3 };
 /* binary collision schemes: configure.ac --with-coulomb-collisions
  * (CollisionsP12 is the reference default) */
 enum { PCP_COLL_P12 = 0, PCP_COLL_TA77S = 1, PCP_COLL_SK98 = 2,
@@ -141,6 +144,21 @@ int pcp_init_reference(pcp_ctx *ctx, int sp, long macro_amount, double density_l
  * E (PCP_GRID_E), H (PCP_GRID_H_AT_ET, as the reference's writers do) and
  * J (PCP_GRID_J).  The window is gathered on the device. */
 int pcp_probe(pcp_ctx *ctx, int which, int comp, long r0, long z0, long r1, long z1, double *host_out);
+
+/* A species map as the reference computes it before a probe dump:
+ * weight_cylindrical of every particle into its domain (include/algo/
+ * weighter.hpp:27-132), SpecieP::calc_density / calc_temperature /
+ * calc_energy / calc_momentum per domain (src/specieP.cpp:665-839) and
+ * OutController's overlay (src/outController.cpp:409-491), which sums the
+ * r- and z-neighbours' halo deposits into the density and leaves the other
+ * maps as each domain computed them.  Density in m^-3, temperature and
+ * energy in eV, momentum (3 components) in kg m/s per particle.  As in the
+ * reference, the temperature map uses the species' momentum_abs map as the
+ * last energy map left it (zero before any).  host_out ([D][ncomp][Nr_d+4]
+ * [Nz_d+4], interior nodes defined) may be NULL: the map stays on the device
+ * for pcp_probe(ctx, PCP_GRID_MAP, comp, ...).  Replaces the calc_* calls of
+ * OutController::operator() (src/outController.cpp:380-408). */
+int pcp_species_map(pcp_ctx *ctx, int sp, int kind, double *host_out);
 
 /* A particle beam of bunches injected over time: BeamP::BeamP for beam species
  * sp in every domain (src/beamP.cpp:24-66).  From then on pcp_step injects at

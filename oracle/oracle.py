@@ -142,6 +142,7 @@ def lib():
                                        C.c_double, C.c_double, C.c_double, C.c_uint64]
         L.orc_smb_set_beam.restype = C.c_int
         L.orc_smb_inject.argtypes = [C.c_void_p]
+        L.orc_smb_species_map.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -164,6 +165,9 @@ def overlay(kind, a, b):
 
 def overlay_set(a, v):
     lib().orc_grid_overlay_set(C.byref(a.s), v)
+
+
+MAPS = {"density": 0, "temperature": 1, "energy": 2, "momentum": 3}
 
 
 class SMB:
@@ -285,6 +289,14 @@ class SMB:
 
     def inject(self):
         lib().orc_smb_inject(self.h)
+
+    def species_map(self, sp, kind):
+        """density / temperature / energy / momentum map of species sp after the
+        pre-dump overlay: [domain][ncomp][Nr_d+4][Nz_d+4]."""
+        k = MAPS[kind]
+        out = np.zeros((self.n_domains, 3 if kind == "momentum" else 1, self.nr_d + 4, self.nz_d + 4))
+        lib().orc_smb_species_map(self.h, sp, k, _dp(out))
+        return out
 
     def time(self):
         return lib().orc_smb_time(self.h)

@@ -213,6 +213,13 @@ double orc_mj_g(double x);
 double orc_mj_g_inverse(double t, double x_lo);
 double orc_mj_energy(double temperature, uint64_t seed, uint64_t ctr);
 
+/* species maps of the probes (src/specieP.cpp:665-839, include/algo/weighter.hpp:27-132)
+ * with OutController's pre-dump overlay (src/outController.cpp:409-491);
+ * out: [D][ncomp][Nr_d+4][Nz_d+4], ncomp 3 for the momentum map.  The map
+ * grids of a species belong to the last SMB that asked for them. */
+enum { ORC_MAP_DENSITY_ = 0, ORC_MAP_TEMPERATURE_ = 1, ORC_MAP_ENERGY_ = 2, ORC_MAP_MOMENTUM_ = 3 };
+void orc_smb_species_map(orc_smb *s, int sp, int kind, double *out);
+
 #ifdef __cplusplus
 }
 #endif

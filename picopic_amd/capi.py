@@ -14,7 +14,8 @@ LIB_PATH = os.path.join(HERE, "_lib", "libpicopic_hip.so")
 
 PUSHERS = {"boris": 0, "boris-relativistic": 1, "boris-adaptive": 2, "vay": 3, "hc": 4}
 CCS = {"vb": 0, "zigzag": 1}
-GRIDS = {"E": 0, "H": 1, "H_at_et": 2, "J": 3, "eps": 4, "sigma": 5}
+GRIDS = {"E": 0, "H": 1, "H_at_et": 2, "J": 3, "eps": 4, "sigma": 5, "map": 6}
+MAPS = {"density": 0, "temperature": 1, "energy": 2, "momentum": 3}
 PREC = {"fp64": 0, "fp32": 1}
 
 EL_CHARGE = 1.6e-19
@@ -31,7 +32,7 @@ EXPORTS = [
     "pcp_surrogate_fit", "pcp_surrogate_get_norm", "pcp_surrogate_set_norm", "pcp_surrogate_fallback_cells",
     "pcp_species_moments", "pcp_continuity_residual", "pcp_set_graph", "pcp_set_sort_bin", "pcp_set_launch",
     "pcp_get_tuning", "pcp_autotune", "pcp_surrogate_calibrate", "pcp_init_reference",
-    "pcp_set_beam", "pcp_inject", "pcp_set_time", "pcp_time", "pcp_probe",
+    "pcp_set_beam", "pcp_inject", "pcp_set_time", "pcp_time", "pcp_probe", "pcp_species_map",
 ]
 COLL = {"p12": 0, "ta77s": 1, "sk98": 2, "surrogate": 3}
 
@@ -87,6 +88,7 @@ def load(path=LIB_PATH):
                                C.c_double, C.c_double, C.c_uint64]
     L.pcp_inject.argtypes = [vp]
     L.pcp_probe.argtypes = [vp, C.c_int, C.c_int, C.c_long, C.c_long, C.c_long, C.c_long, dp]
+    L.pcp_species_map.argtypes = [vp, C.c_int, C.c_int, dp]
     L.pcp_set_time.argtypes = [vp, C.c_double]
     L.pcp_time.argtypes = [vp]
     L.pcp_time.restype = C.c_double
@@ -247,6 +249,19 @@ class Context:
 
     def inject(self):
         _check(_lib.pcp_inject(self.h))
+
+    def species_map(self, sp, kind, download=True):
+        """density / temperature / energy / momentum map of species sp as the
+        reference computes it before a dump: [domain][ncomp][Nr_d+4][Nz_d+4]
+        (interior nodes); with download=False it stays on the device for
+        probe("map", ...)."""
+        nc = 3 if kind == "momentum" else 1
+        if not download:
+            _check(_lib.pcp_species_map(self.h, sp, MAPS[kind], None))
+            return None
+        out = np.zeros((self.D, nc, self.nr_d + 4, self.nz_d + 4))
+        _check(_lib.pcp_species_map(self.h, sp, MAPS[kind], _dp(out)))
+        return out
 
     def probe(self, which, comp, r0, z0, r1, z1):
         """Component comp of grid `which` over global cells [r0, r1) x [z0, z1)."""

@@ -366,6 +366,8 @@ int pcp_destroy(pcp_ctx *c) {
   hipFree(c->f.E); hipFree(c->f.H); hipFree(c->f.Hat); hipFree(c->f.J);
   hipFree(c->f.eps); hipFree(c->f.sigma); hipFree(c->st); hipFree(c->marker); hipFree(c->cell_flag);
   hipFree(c->diag); hipFree(c->ehist); hipFree(c->probe_buf);
+  hipFree(c->map_n); hipFree(c->map_pw); hipFree(c->map_out);
+  for (int s = 0; s < MAX_SPECIES; ++s) hipFree(c->map_pabs[s]);
   for (int s = 0; s < MAX_SPECIES; ++s) {
     SpeciesDev &sp = c->sp[s];
     hipFree(sp.r[0]); hipFree(sp.r[1]); hipFree(sp.z[0]); hipFree(sp.z[1]);
@@ -842,13 +844,44 @@ int pcp_download_grid(pcp_ctx *c, int which, double *host) {
   return 0;
 }
 
+int pcp_species_map(pcp_ctx *c, int sp, int kind, double *host) {
+  if (!c || sp < 0 || sp >= c->cfg.n_species || kind < PCP_MAP_DENSITY || kind > PCP_MAP_MOMENTUM)
+    return fail(PCP_E_ARG, "bad argument");
+  const size_t DP = (size_t)c->D * c->g.P;
+  int rc;
+  if (!c->map_n && ((rc = dalloc(&c->map_n, DP)) || (rc = dalloc(&c->map_pw, 3 * DP)) ||
+                    (rc = dalloc(&c->map_out, 3 * DP))))
+    return rc;
+  if (!c->map_pabs[sp]) {
+    if ((rc = dalloc(&c->map_pabs[sp], DP))) return rc;
+    HIP_TRY(hipMemsetAsync(c->map_pabs[sp], 0, DP * sizeof(double), c->stream));
+  }
+  if (launch_species_map(c, sp, kind, c->map_n, c->map_pw, c->map_pabs[sp], c->map_out) < 0)
+    return fail(PCP_E_HIP, "species map launch failed");
+  c->map_ncomp = kind == PCP_MAP_MOMENTUM ? 3 : 1;
+  if (host) {
+    HIP_TRY(hipMemcpyAsync(host, c->map_out, sizeof(double) * DP * c->map_ncomp, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
 int pcp_probe(pcp_ctx *c, int which, int comp, long r0, long z0, long r1, long z1, double *host) {
   if (!c || !host) return fail(PCP_E_ARG, "bad argument");
   const Geom &g = c->g;
   if (!(r0 >= 0 && z0 >= 0 && r0 <= r1 && z0 <= z1 && r1 <= g.Nr && z1 <= g.Nz))
     return fail(PCP_E_ARG, "probe window outside the grid");
   int nc;
-  const void *grid = c->s32 ? (const void *)grid_ptr32(c, which, &nc) : (const void *)grid_ptr(c, which, &nc);
+  bool f32 = c->s32 != nullptr;
+  const void *grid;
+  if (which == PCP_GRID_MAP) {
+    if (!c->map_ncomp) return fail(PCP_E_STATE, "no species map computed yet");
+    grid = c->map_out;
+    nc = c->map_ncomp;
+    f32 = false;
+  } else {
+    grid = f32 ? (const void *)grid_ptr32(c, which, &nc) : (const void *)grid_ptr(c, which, &nc);
+  }
   if (!grid) return fail(PCP_E_ARG, "unknown grid");
   if (comp < 0 || comp >= nc) return fail(PCP_E_ARG, "bad component");
   const long n = (r1 - r0) * (z1 - z0);
@@ -861,7 +894,7 @@ int pcp_probe(pcp_ctx *c, int which, int comp, long r0, long z0, long r1, long z
     if (int rc = dalloc(&c->probe_buf, (size_t)n)) return rc;
     c->probe_cap = n;
   }
-  launch_probe(c, grid, c->s32 != nullptr, nc, comp, r0, z0, r1 - r0, z1 - z0, c->probe_buf);
+  launch_probe(c, grid, f32, nc, comp, r0, z0, r1 - r0, z1 - z0, c->probe_buf);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(host, c->probe_buf, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));

@@ -198,4 +198,236 @@ void launch_probe(pcp_ctx *c, const void *grid, bool f32, int ncomp, int comp, l
                        comp, r0, z0, nr_w, nz_w, dev_out);
 }
 
+// ---------------------------------------------------------------------------
+// Species maps of the probes (src/specieP.cpp:665-839): weight_cylindrical
+// (include/algo/weighter.hpp:27-132) of each particle into its domain's
+// halo grids, with FP64 atomics, then the derived maps per node, and for the
+// density the pre-dump overlay's halo sums (src/outController.cpp:409-491:
+// overlay_x and overlay_y over interior rows / columns, no diagonal; the
+// derived maps' halos are zero, so their overlay leaves the interior as it
+// is).  The momentum factors take gamma when |v| is below REL_LIMIT
+// (src/specieP.cpp:731,763, as written).
+// ---------------------------------------------------------------------------
+namespace {
+
+__device__ __forceinline__ void add_node(double *G, const Geom &g, int d, int li, int lk, double v) {
+  atomicAdd(&G[(long)d * g.P + (long)(li + 2) * g.yr + (lk + 2)], v);
+}
+
+// weighter.hpp:27-132 for a particle at (pos_r, pos_z) of domain d (whose
+// first cells are bot_r, left_z)
+__device__ void weight_cylindrical_dev(double *G, const Geom &g, int d, long bot_r, long left_z, double pos_r,
+                                       double pos_z, double weight_value) {
+  const double dr = g.dr, dz = g.dz;
+  double r1, r2, r3, dz1, dz2, rho_w_value, v_0;
+  unsigned int r_i = (unsigned int)cell_number(pos_r, dr);
+  const unsigned int z_k = (unsigned int)cell_number(pos_z, dz);
+  const int r_i_shift = (int)(r_i - (unsigned int)bot_r);
+  const int z_k_shift = (int)(z_k - (unsigned int)left_z);
+  double v_1 = PCP_CELL_VOLUME(r_i, dr, dz);
+  double v_2 = PCP_CELL_VOLUME(r_i + 1, dr, dz);
+  double a[4];
+  if (pos_r > dr) {
+    r1 = pos_r - 0.5 * dr;
+    r2 = (r_i + 0.5) * dr;
+    r3 = pos_r + 0.5 * dr;
+    v_0 = 2. * PI * dz * dr * pos_r;
+    dz1 = (z_k + 0.5) * dz - (pos_z - 0.5 * dz);
+    dz2 = (pos_z + 0.5 * dz) - (z_k + 0.5) * dz;
+    rho_w_value = weight_value / v_0;
+    a[0] = PCP_CYL_RNG_VOL(dz1, r1, r2) / v_1;
+    a[1] = PCP_CYL_RNG_VOL(dz1, r2, r3) / v_2;
+    a[2] = PCP_CYL_RNG_VOL(dz2, r1, r2) / v_1;
+    a[3] = PCP_CYL_RNG_VOL(dz2, r2, r3) / v_2;
+  } else if (pos_r <= dr / 2.) {
+    r_i = 0;
+    r2 = (r_i + 0.5) * dr;
+    r3 = pos_r + 0.5 * dr;
+    dz1 = (z_k + 0.5) * dz - (pos_z - 0.5 * dz);
+    dz2 = (pos_z + 0.5 * dz) - (z_k + 0.5) * dz;
+    v_0 = PI * dz * (2. * pos_r * pos_r + dr * dr / 2.);
+    rho_w_value = weight_value / v_0;
+    a[0] = PI * dz1 * (dr * dr / 2. - pos_r * dr + pos_r * pos_r) / v_1;
+    a[1] = PCP_CYL_RNG_VOL(dz1, r2, r3) / v_2;
+    a[2] = PI * dz2 * (dr * dr / 2. - pos_r * dr + pos_r * pos_r) / v_1;
+    a[3] = PCP_CYL_RNG_VOL(dz2, r2, r3) / v_2;
+  } else {
+    r1 = pos_r - 0.5 * dr;
+    r2 = (r_i + 0.5) * dr;
+    r3 = pos_r + 0.5 * dr;
+    dz1 = (z_k + 0.5) * dz - (pos_z - 0.5 * dz);
+    dz2 = (pos_z + 0.5 * dz) - (z_k + 0.5) * dz;
+    v_0 = 2. * PI * dz * dr * pos_r;
+    v_1 = PI * (dz) * (dr) * (dr);   // CYL_VOL(dz, dr), geometry.hpp:192
+    v_2 = PCP_CELL_VOLUME(r_i + 1, dr, dz);
+    rho_w_value = weight_value / v_0;
+    a[0] = PCP_CYL_RNG_VOL(dz1, r1, r2) / v_1;
+    a[1] = PCP_CYL_RNG_VOL(dz1, r2, r3) / v_2;
+    a[2] = PCP_CYL_RNG_VOL(dz2, r1, r2) / v_1;
+    a[3] = PCP_CYL_RNG_VOL(dz2, r2, r3) / v_2;
+  }
+  add_node(G, g, d, r_i_shift, z_k_shift, rho_w_value * a[0]);
+  add_node(G, g, d, r_i_shift + 1, z_k_shift, rho_w_value * a[1]);
+  add_node(G, g, d, r_i_shift, z_k_shift + 1, rho_w_value * a[2]);
+  add_node(G, g, d, r_i_shift + 1, z_k_shift + 1, rho_w_value * a[3]);
+}
+
+// live particle p: position and velocity (false when removed)
+struct MapParts64 {
+  SpeciesDev s;
+  int cur;
+  __device__ bool at(long p, double &r, double &z, double v[3], double &w) const {
+    r = s.r[cur][p];
+    if (!(r >= 0)) return false;
+    z = s.z[cur][p];
+    v[0] = s.vr[p]; v[1] = s.vphi[p]; v[2] = s.vz[p];
+    w = s.w[p];
+    return true;
+  }
+};
+struct MapParts32 {
+  Species32 s;
+  int cur;
+  long Nz;
+  double dr, dz;
+  __device__ bool at(long p, double &r, double &z, double v[3], double &w) const {
+    const uint32_t c = s.cell[cur][p];
+    if (c == DEAD_CELL) return false;
+    const long cr = c / (uint32_t)Nz, cz = c - (uint32_t)cr * (uint32_t)Nz;
+    r = ((double)cr + (double)s.fr[cur][p]) * dr;
+    z = ((double)cz + (double)s.fz[cur][p]) * dz;
+    v[0] = s.vr[cur][p]; v[1] = s.vphi[cur][p]; v[2] = s.vz[cur][p];
+    w = s.w[p];
+    return true;
+  }
+};
+
+// kind: 0 density only, 1 density + momentum_w (temperature, momentum),
+// 2 density + momentum_abs_w (energy)
+template <class V>
+__global__ void k_map_deposit(Geom g, V parts, long n, double mass, int kind, double *N, double *PW, double *PABS) {
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long)gridDim.x * blockDim.x) {
+    double r, z, v[3], w;
+    if (!parts.at(p, r, z, v, w)) continue;
+    const int d = domain_of(g, cell_number(r, g.dr), cell_number(z, g.dz));
+    const Dom dm = make_dom(g, d);
+    weight_cylindrical_dev(N, g, d, dm.bot_r, dm.left_z, r, z, w);
+    if (kind == 1) {
+      const double m_weighted = mass * w;
+      double pc[3] = {m_weighted * v[0], m_weighted * v[1], m_weighted * v[2]};
+      const double v_abs_2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+      if (v_abs_2 < REL_LIMIT * REL_LIMIT) {
+        const double gamma = 1 / sqrt(1.0 - v_abs_2 / LIGHT_VEL_POW_2);
+        for (int c = 0; c < 3; ++c) pc[c] *= gamma;
+      }
+      for (int c = 0; c < 3; ++c)
+        weight_cylindrical_dev(PW + (long)c * g.Dr * g.Dz * g.P, g, d, dm.bot_r, dm.left_z, r, z, pc[c]);
+    } else if (kind == 2) {
+      const double v_abs = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+      double p_abs = mass * w * v_abs;
+      if (v_abs < REL_LIMIT) p_abs *= 1 / sqrt(1.0 - v_abs * v_abs / LIGHT_VEL_POW_2);
+      weight_cylindrical_dev(PABS, g, d, dm.bot_r, dm.left_z, r, z, p_abs);
+    }
+  }
+}
+
+__device__ __forceinline__ double energy_m_dev(double mass, double momentum_2) {
+  if (momentum_2 > REL_LIMIT * REL_LIMIT * mass * mass) {
+    const double e_rest = mass * LIGHT_VEL_POW_2;
+    return sqrt(momentum_2 * LIGHT_VEL_POW_2 + e_rest * e_rest) - e_rest;
+  }
+  return 0.5 * momentum_2 / mass;
+}
+
+// map kinds: 0 density, 1 temperature, 2 energy, 3 momentum.  OUT is
+// [D][ncomp][P] (ncomp 3 for the momentum); interior nodes only are written
+// (halos zero).  PW is [3][D][P].
+__global__ void k_map_derive(Geom g, int kind, double mass, const double *N, const double *PW, const double *PABS,
+                             double *OUT) {
+  const long DP = (long)g.Dr * g.Dz * g.P;
+  const long total = (long)g.Dr * g.Dz * g.nr * g.nz;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+    const int d = (int)(t / ((long)g.nr * g.nz));
+    const long lc = t - (long)d * g.nr * g.nz;
+    const int li = (int)(lc / g.nz), lk = (int)(lc - (long)li * g.nz);
+    const long loc = (long)(li + 2) * g.yr + (lk + 2);
+    const long ix = (long)d * g.P + loc;
+    if (kind == 0) {
+      // own deposits plus the neighbours' halo deposits (overlay_x / _y)
+      const Dom dm = make_dom(g, d);
+      double v = N[ix];
+      if (li < 2 && dm.I > 0) v += N[(long)(d - g.Dz) * g.P + (long)(li + g.nr + 2) * g.yr + (lk + 2)];
+      if (li >= g.nr - 2 && dm.I < g.Dr - 1) v += N[(long)(d + g.Dz) * g.P + (long)(li - g.nr + 2) * g.yr + (lk + 2)];
+      if (lk < 2 && dm.J > 0) v += N[(long)(d - 1) * g.P + (long)(li + 2) * g.yr + (lk + g.nz + 2)];
+      if (lk >= g.nz - 2 && dm.J < g.Dz - 1) v += N[(long)(d + 1) * g.P + (long)(li + 2) * g.yr + (lk - g.nz + 2)];
+      OUT[ix] = v;
+      continue;
+    }
+    const double density_local = N[ix];
+    if (kind == 3) {
+      for (int c = 0; c < 3; ++c)
+        OUT[((long)d * 3 + c) * g.P + loc] = density_local != 0 ? PW[c * DP + ix] / density_local : 0.0;
+      continue;
+    }
+    if (density_local == 0) {
+      OUT[ix] = 0;
+      continue;
+    }
+    double p2;
+    const double pa = PABS[ix];
+    if (kind == 1) {
+      const double p_vec_sum_2 = PW[ix] * PW[ix] + PW[DP + ix] * PW[DP + ix] + PW[2 * DP + ix] * PW[2 * DP + ix];
+      p2 = pa * pa - p_vec_sum_2;
+    } else {
+      p2 = pa * pa;
+    }
+    p2 /= density_local * density_local;
+    OUT[ix] = energy_m_dev(mass, p2) * (1. / EL_CHARGE);
+  }
+}
+
+__global__ void k_zero_f64(double *p, long n) {
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) p[t] = 0.0;
+}
+
+}  // namespace
+
+int launch_species_map(pcp_ctx *c, int sp, int kind, double *N, double *PW, double *PABS, double *OUT) {
+  const Geom &g = c->g;
+  const long DP = (long)c->D * g.P;
+  const unsigned gz = (unsigned)std::min<long>((4 * DP + 255) / 256, (long)c->n_cu * 16);
+  hipLaunchKernelGGL(k_zero_f64, dim3(gz), dim3(256), 0, c->stream, N, DP);
+  hipLaunchKernelGGL(k_zero_f64, dim3(gz), dim3(256), 0, c->stream, OUT, 3 * DP);
+  const int dep = kind == 0 ? 0 : kind == 2 ? 2 : 1;
+  if (dep == 1) hipLaunchKernelGGL(k_zero_f64, dim3(gz), dim3(256), 0, c->stream, PW, 3 * DP);
+  if (dep == 2) hipLaunchKernelGGL(k_zero_f64, dim3(gz), dim3(256), 0, c->stream, PABS, DP);
+  long n;
+  double mass;
+  if (c->s32) {
+    Species32 &s = c->s32->sp[sp];
+    n = s.n;
+    mass = s.m;
+    if (n > 0) {
+      MapParts32 v{s, c->cur, g.Nz, g.dr, g.dz};
+      const unsigned gp = (unsigned)std::max(1L, std::min((n + 255) / 256, (long)c->n_cu * 16));
+      hipLaunchKernelGGL(k_map_deposit<MapParts32>, dim3(gp), dim3(256), 0, c->stream, g, v, n, mass, dep, N, PW,
+                         PABS);
+    }
+  } else {
+    SpeciesDev &s = c->sp[sp];
+    n = s.n;
+    mass = s.m;
+    if (n > 0) {
+      MapParts64 v{s, c->cur};
+      const unsigned gp = (unsigned)std::max(1L, std::min((n + 255) / 256, (long)c->n_cu * 16));
+      hipLaunchKernelGGL(k_map_deposit<MapParts64>, dim3(gp), dim3(256), 0, c->stream, g, v, n, mass, dep, N, PW,
+                         PABS);
+    }
+  }
+  const long cells = (long)c->D * g.nr * g.nz;
+  const unsigned gc = (unsigned)std::max(1L, std::min((cells + 255) / 256, (long)c->n_cu * 16));
+  hipLaunchKernelGGL(k_map_derive, dim3(gc), dim3(256), 0, c->stream, g, kind, mass, N, PW, PABS, OUT);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 }  // namespace pcp

@@ -14,4 +14,10 @@ int launch_continuity_residual(pcp_ctx *c, double *dev_out);
 void launch_probe(pcp_ctx *c, const void *grid, bool f32, int ncomp, int comp, long r0, long z0, long nr_w, long nz_w,
                   double *dev_out);
 
+// species map `kind` (0 density, 1 temperature, 2 energy, 3 momentum) of
+// species sp into OUT (ncomp x D P; interior nodes): N is scratch (D P),
+// PW scratch (3 D P), PABS the species' momentum_abs_w map, refreshed only by
+// the energy map (the temperature map reads what it holds)
+int launch_species_map(pcp_ctx *c, int sp, int kind, double *N, double *PW, double *PABS, double *OUT);
+
 }  // namespace pcp

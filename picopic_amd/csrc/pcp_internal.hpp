@@ -256,6 +256,12 @@ struct pcp_ctx {
   double *diag = nullptr;           // scratch of the diagnostic reductions (DIAG_DOUBLES)
   double *probe_buf = nullptr;      // staging of probe windows (grown on demand)
   long probe_cap = 0;
+  // species maps: scratch density and momentum_w, the last map (3 x D P),
+  // and each species' momentum_abs_w map (kept between calls, as the
+  // reference's member is)
+  double *map_n = nullptr, *map_pw = nullptr, *map_out = nullptr;
+  double *map_pabs[pcp::MAX_SPECIES] = {};
+  int map_ncomp = 0;
   double *ehist = nullptr;          // energy log [EHIST][EHIST_COLS]
   int elog = 0;                     // log energies in every step
   std::vector<std::pair<std::string, double>> ktimes;

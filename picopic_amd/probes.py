@@ -20,8 +20,11 @@ file per path (``<data_root>/<path>.npy``, float64, [frames, r, z] for
 ``rec``, [frames, n] for ``col`` / ``row``, [frames, 1] for ``dot``).
 
 H probes read the field at integer time (``field_h_at_et``), as the
-reference's writers do.  Species maps (density, temperature, ...) are not
-probe components here.
+reference's writers do.  Species components (``density``, ``temperature``,
+``energy``, ``momentum/r|phi|z`` with a ``specie`` name; path
+``component/specie/shape/...``) read the species map that the dump computes
+on the device (``pcp_species_map``: weight_cylindrical, SpecieP::calc_*,
+the pre-dump overlay).
 """
 import math
 import os
@@ -32,16 +35,31 @@ GRID_OF = {"E": "E", "H": "H_at_et", "J": "J"}
 COMP_OF = {"r": 0, "phi": 1, "z": 2}
 
 
-class Probe:
-    """One entry of ``data.probes``."""
+SPECIES_MAPS = ("density", "temperature", "energy", "momentum")
 
-    def __init__(self, spec, grid):
+
+class Probe:
+    """One entry of ``data.probes``; `species` maps specie names to indices."""
+
+    def __init__(self, spec, grid, species=None):
         comp = spec["component"]
-        if "/" not in comp or comp.split("/")[0] not in GRID_OF or comp.split("/")[1] not in COMP_OF:
+        head, _, c = comp.partition("/")
+        self.map = None
+        if head in SPECIES_MAPS:
+            if (head == "momentum") != (c in COMP_OF) or (head != "momentum" and c):
+                raise ValueError("unsupported probe component %r" % comp)
+            name = spec.get("specie", "unknown")
+            if species is not None and name not in species:
+                raise ValueError("probe %r names an unknown specie %r" % (comp, name))
+            self.map, self.specie = head, name
+            self.sp = species[name] if species is not None else None
+            self.which = "map"
+            self.comp = COMP_OF[c] if head == "momentum" else 0
+        elif head in GRID_OF and c in COMP_OF:
+            self.which = GRID_OF[head]
+            self.comp = COMP_OF[c]
+        else:
             raise ValueError("unsupported probe component %r" % comp)
-        self.field, c = comp.split("/")
-        self.which = GRID_OF[self.field]
-        self.comp = COMP_OF[c]
         self.schedule = int(spec["schedule"])
         if self.schedule <= 0:
             raise ValueError("probe schedule must be positive")
@@ -70,7 +88,7 @@ class Probe:
             raise ValueError("probe %s outside the simulation area" % tail)
         self.shape = shape
         self.window = (r0, z0, r1, z1)
-        self.path = comp + "/" + tail
+        self.path = comp + "/" + (self.specie + "/" if self.map else "") + tail
 
     def frame_shape(self):
         r0, z0, r1, z1 = self.window
@@ -84,18 +102,20 @@ class Probe:
 
     def read(self, ctx):
         r0, z0, r1, z1 = self.window
+        if self.map:
+            ctx.species_map(self.sp, self.map, download=False)
         return ctx.probe(self.which, self.comp, r0, z0, r1, z1).reshape(self.frame_shape())
 
 
-def probes(cfg, grid):
-    return [Probe(p, grid) for p in cfg.get("data", {}).get("probes", [])]
+def probes(cfg, grid, species=None):
+    return [Probe(p, grid, species) for p in cfg.get("data", {}).get("probes", [])]
 
 
 class Writer:
     """Frames of every probe of a run, one ``.npy`` per probe path."""
 
-    def __init__(self, cfg, grid, n_steps, data_root):
-        self.probes = probes(cfg, grid)
+    def __init__(self, cfg, grid, n_steps, data_root, species=None):
+        self.probes = probes(cfg, grid, species)
         self.root = data_root
         self.files = {}
         for p in self.probes:

@@ -37,7 +37,8 @@ def run(src, data_root=None, seed=1, iterations=None, progress=None):
     if iterations is not None:
         total = min(total, int(iterations))
     root = data_root if data_root is not None else cfg.get("data", {}).get("data_root", ".")
-    writer = probes.Writer(cfg, grid, total, root)
+    names = {name: k for k, (name, _, _, _, _) in enumerate(config.species(cfg))}
+    writer = probes.Writer(cfg, grid, total, root, names)
     for it in range(total):
         t = ctx.time()
         ctx.step(1)

