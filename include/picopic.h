@@ -343,6 +343,15 @@ int pcp_species_moments(pcp_ctx *ctx, int sp, double *out);
  * (cylinder r = (i+1/2) dr) and J_z (annulus at z = (k+1/2) dz, pi dr^2/4 at
  * the axis).  out[3] = max |residual|, max |charge change|, largest |q w|. */
 int pcp_continuity_residual(pcp_ctx *ctx, double *out);
+/* pcp_continuity_residual without allocating: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * per-node charge array lives in a fixed device array of 10 x 2^20 doubles
+ * (80 MiB, one per GPU, held until the process ends), so the grid's
+ * (Nr + 4) x (Nz + 4) nodes may not exceed 10,485,760 (PCP_E_ARG otherwise).
+ * Every call on one GPU shares the array: calls that overlap on one GPU
+ * race.  Same results as pcp_continuity_residual. */
+int pcp_continuity_residual_noalloc(pcp_ctx *ctx, double *out);
 
 /* In-step FP64 energy log (no reference counterpart; the reference sums
  * energies on the host from its output maps, src/specieP.cpp:665-839).

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -237,9 +238,18 @@ int pcp_create(const pcp_config *cfg, pcp_ctx **out) {
   c->cur = 0;
 
   hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, 0) != hipSuccess) {
+  if (hipGetDevice(&c->device) != hipSuccess || hipGetDeviceProperties(&prop, c->device) != hipSuccess) {
     delete c;
     return fail(PCP_E_HIP, "no HIP device");
+  }
+  // the allocation-free entry points' fixed device array on this GPU
+  {
+    static std::mutex m;
+    std::lock_guard<std::mutex> lock(m);
+    if (continuity_scratch_init(c->device) != 0) {
+      delete c;
+      return fail(PCP_E_HIP, "hipGetSymbolAddress of the fixed scratch array failed");
+    }
   }
   c->n_cu = prop.multiProcessorCount;
   if (hipStreamCreate(&c->stream) != hipSuccess) {
@@ -1678,6 +1688,23 @@ int pcp_continuity_residual(pcp_ctx *c, double *out) {
   double *d = c->diag;
   if (launch_continuity_residual(c, d) < 0) return fail(PCP_E_HIP, "continuity residual");
   HIP_TRY(hipMemcpy(out, d, 3 * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pcp_continuity_residual_noalloc(pcp_ctx *c, double *out) {
+  if (!c || !out) return fail(PCP_E_ARG, "bad argument");
+  const int rc = launch_continuity_residual_noalloc(c, c->diag);
+  if (rc == -2) {
+    char buf[200];
+    snprintf(buf, sizeof buf,
+             "grid of %ld x %ld nodes (halo included) exceeds the fixed scratch array of %ld doubles; use "
+             "pcp_continuity_residual",
+             (long)(c->g.Nr + 4), (long)(c->g.Nz + 4), CONTINUITY_SCRATCH_DOUBLES);
+    return fail(PCP_E_ARG, buf);
+  }
+  if (rc == -3) return fail(PCP_E_STATE, "fixed scratch array not set up on this device");
+  if (rc < 0) return fail(PCP_E_HIP, "continuity residual");
+  HIP_TRY(hipMemcpy(out, c->diag, 3 * sizeof(double), hipMemcpyDeviceToHost));
   return 0;
 }
 

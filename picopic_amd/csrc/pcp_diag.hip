@@ -129,17 +129,9 @@ __global__ void k_zero_d(double *p, long n) {
 // out[0] = max |residual|, out[1] = max |charge change| over the nodes,
 // out[2] = largest |q w| of a macro-particle (non-negative doubles compare
 // as their bit patterns, hence the integer maxima)
-int launch_continuity_residual(pcp_ctx *c, double *dev_out) {
+static int continuity_residual_into(pcp_ctx *c, double *Q, double *dev_out) {
   const Geom &g = c->g;
   const long ldq = g.Nz + 4, nq = (g.Nr + 4) * ldq;
-  struct Scratch {
-    double *p = nullptr;
-    ~Scratch() {
-      if (p) (void)hipFree(p);
-    }
-  } q;
-  if (hipMalloc(&q.p, nq * sizeof(double)) != hipSuccess) return -1;
-  double *Q = q.p;
   const unsigned gz = (unsigned)std::min<long>((nq + 255) / 256, (long)c->n_cu * 16);
   hipLaunchKernelGGL(k_zero_d, dim3(gz), dim3(256), 0, c->stream, Q, nq);
   hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, c->stream, dev_out, 3L);
@@ -166,6 +158,46 @@ int launch_continuity_residual(pcp_ctx *c, double *dev_out) {
     hipLaunchKernelGGL(k_cont_residual<double>, dim3(gr), dim3(256), 0, c->stream, g, c->f.J, Q, ldq, dev_out);
   if (hipStreamSynchronize(c->stream) != hipSuccess) return -1;
   return 0;
+}
+
+int launch_continuity_residual(pcp_ctx *c, double *dev_out) {
+  const long nq = (c->g.Nr + 4) * (c->g.Nz + 4);
+  struct Scratch {
+    double *p = nullptr;
+    ~Scratch() {
+      if (p) (void)hipFree(p);
+    }
+  } q;
+  if (hipMalloc(&q.p, nq * sizeof(double)) != hipSuccess) return -1;
+  return continuity_residual_into(c, q.p, dev_out);
+}
+
+// The allocation-free variant's charge array: a fixed device array, one per
+// GPU, held until the process ends.  Host code reaches it only through the
+// address hipGetSymbolAddress returns for the current device (the array's
+// name is a host-side stand-in).  Calls on one GPU share it: calls that
+// overlap on one GPU race.
+static __device__ __align__(256) double scratch[CONTINUITY_SCRATCH_DOUBLES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+double *g_scratch[MAX_DEVICES] = {};
+}  // namespace
+
+int continuity_scratch_init(int device) {
+  if (device < 0 || device >= MAX_DEVICES) return -1;
+  if (g_scratch[device]) return 0;
+  double *ptr = nullptr;
+  if (hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return -1;
+  g_scratch[device] = ptr;
+  return 0;
+}
+
+int launch_continuity_residual_noalloc(pcp_ctx *c, double *dev_out) {
+  const long nq = (c->g.Nr + 4) * (c->g.Nz + 4);
+  if (nq > CONTINUITY_SCRATCH_DOUBLES) return -2;
+  if (c->device < 0 || c->device >= MAX_DEVICES || !g_scratch[c->device]) return -3;
+  return continuity_residual_into(c, g_scratch[c->device], dev_out);
 }
 
 // ---------------------------------------------------------------------------

@@ -9,6 +9,15 @@ namespace pcp {
 // max |dQ + dt div J| over the nodes, max |dQ|, largest |q w|; synchronises
 int launch_continuity_residual(pcp_ctx *c, double *dev_out);
 
+// the same with the charge array in a fixed device array of
+// CONTINUITY_SCRATCH_DOUBLES doubles (80 MiB per GPU): no allocation.
+// Returns -2 when the grid's (Nr + 4) (Nz + 4) nodes exceed it, -3 before
+// continuity_scratch_init ran for the context's device.
+constexpr long CONTINUITY_SCRATCH_DOUBLES = 10L << 20;
+int launch_continuity_residual_noalloc(pcp_ctx *c, double *dev_out);
+// resolves the fixed array's address on `device` (current); once per device
+int continuity_scratch_init(int device);
+
 // probe window: component comp of a [D][ncomp][P] grid (float when f32)
 // over global cells [r0, r0 + nr_w) x [z0, z0 + nz_w), row-major doubles
 void launch_probe(pcp_ctx *c, const void *grid, bool f32, int ncomp, int comp, long r0, long z0, long nr_w, long nz_w,

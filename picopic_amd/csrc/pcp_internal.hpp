@@ -250,6 +250,7 @@ struct pcp_ctx {
   int coll_se = 0, coll_si = 1;
   uint64_t coll_seed = 0, coll_calls = 0;
   int n_cu;
+  int device = -1;                  // the HIP device the context runs on
   int timing;
   double t_now = 0;                 // TimeSim::current: advances by dt after every step
   pcp::BeamHost beam[pcp::MAX_SPECIES];

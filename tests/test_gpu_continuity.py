@@ -20,11 +20,11 @@ from tests import helpers as Hh
 pytestmark = pytest.mark.gpu
 
 
-def make(ccs, precision, domains=(2, 2), n=4000, seed=3, corner_gap=4):
+def make(ccs, precision, domains=(2, 2), n=4000, seed=3, corner_gap=4, cells=(16, 32)):
     """Particles of two species; with corner_gap > 0 none starts within
     corner_gap cells (in r and in z) of an interior domain corner (a step
     moves a particle by at most half a cell)."""
-    size, grid = Hh.geometry(16, 32)
+    size, grid = Hh.geometry(*cells)
     dt = 0.5 * (size[1] / grid[1]) / 3e8
     species = Hh.species_list(False)
     ctx = capi.Context(size, grid, domains, dt, [s + (2 * n,) for s in species], ccs=ccs, precision=precision)
