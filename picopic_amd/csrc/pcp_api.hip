@@ -359,8 +359,9 @@ int pcp_create(const pcp_config *cfg, pcp_ctx **out) {
   m.cap = cfg->max_migrants > 0 ? cfg->max_migrants : std::max<long>(1 << 16, total_cap / 64);
   m.ngroups = c->D * MAX_SPECIES;
   if ((rc = dalloc(&m.key, m.cap)) || (rc = dalloc(&m.ref, m.cap)) || (rc = dalloc(&m.group, m.cap)) ||
-      (rc = dalloc(&m.bucket, m.cap)) || (rc = dalloc(&m.gcount, m.ngroups)) || (rc = dalloc(&m.gstart, m.ngroups)) ||
-      (rc = dalloc(&m.gfill, m.ngroups)) || (rc = dalloc(&m.gbase, m.ngroups))) {
+      (rc = dalloc(&m.bucket, m.cap)) || (rc = dalloc(&m.skey, m.cap)) || (rc = dalloc(&m.gcount, m.ngroups)) ||
+      (rc = dalloc(&m.gstart, m.ngroups)) || (rc = dalloc(&m.gfill, m.ngroups)) ||
+      (rc = dalloc(&m.gtile, m.ngroups + 1)) || (rc = dalloc(&m.gbase, m.ngroups))) {
     pcp_destroy(c);
     return rc;
   }
@@ -390,8 +391,8 @@ int pcp_destroy(pcp_ctx *c) {
   hipFree(c->sort_key); hipFree(c->sort_count); hipFree(c->sort_offset); hipFree(c->sort_fill);
   hipFree(c->sort_temp);
   Migrants &m = c->mig;
-  hipFree(m.key); hipFree(m.ref); hipFree(m.group); hipFree(m.bucket);
-  hipFree(m.gcount); hipFree(m.gstart); hipFree(m.gfill); hipFree(m.gbase);
+  hipFree(m.key); hipFree(m.ref); hipFree(m.group); hipFree(m.bucket); hipFree(m.skey);
+  hipFree(m.gcount); hipFree(m.gstart); hipFree(m.gfill); hipFree(m.gtile); hipFree(m.gbase);
   free32(c);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -683,12 +684,12 @@ int pcp_init_bunch(pcp_ctx *c, int sp, long n, double n_peak, double sigma_r, do
 static int grow_migrants(pcp_ctx *c, long need) {
   Migrants &m = c->mig;
   if (need <= m.cap) return 0;
-  hipFree(m.key); hipFree(m.ref); hipFree(m.group); hipFree(m.bucket);
-  m.key = nullptr; m.ref = nullptr; m.group = nullptr; m.bucket = nullptr;
+  hipFree(m.key); hipFree(m.ref); hipFree(m.group); hipFree(m.bucket); hipFree(m.skey);
+  m.key = nullptr; m.ref = nullptr; m.group = nullptr; m.bucket = nullptr; m.skey = nullptr;
   m.cap = need;
   int rc;
   if ((rc = dalloc(&m.key, m.cap)) || (rc = dalloc(&m.ref, m.cap)) || (rc = dalloc(&m.group, m.cap)) ||
-      (rc = dalloc(&m.bucket, m.cap))) {
+      (rc = dalloc(&m.bucket, m.cap)) || (rc = dalloc(&m.skey, m.cap))) {
     m.cap = 0;
     return rc;
   }

@@ -158,12 +158,17 @@ struct Migrants {
   uint64_t *key;      // colour << 47 | srcdom << 35 | old seq
   uint64_t *ref;      // species << 40 | particle index
   uint32_t *group;    // dst domain * MAX_SPECIES + species
-  uint32_t *bucket;   // migrant indices grouped by group
+  uint32_t *bucket;   // migrant indices grouped by group, each group's tiles sorted by key
+  uint64_t *skey;     // the keys in bucket order
   uint32_t *gcount, *gstart, *gfill;
+  uint32_t *gtile;    // exclusive scan of the groups' MIG_TILE-entry tile counts, total last
   uint64_t *gbase;    // next free seq per (domain, species)
   long cap;
   int ngroups;
 };
+// migrants of one group are ranked by sorting tiles of MIG_TILE keys in LDS
+// and binary-searching each key in the group's other tiles
+constexpr int MIG_TILE_THREADS = 256, MIG_TILE_ITEMS = 8, MIG_TILE = MIG_TILE_THREADS * MIG_TILE_ITEMS;
 
 // In-step energy log (pcp_set_energy_log): a ring of EHIST rows of
 // EHIST_COLS doubles -- step number, field energy, kinetic energy of each

@@ -11,6 +11,7 @@
 //                       strict axial motion through per-node markers
 //  k_mig_*              order keys of particles that changed domain
 #include <hip/hip_runtime.h>
+#include <hipcub/block/block_radix_sort.hpp>
 
 #include <climits>
 
@@ -571,23 +572,41 @@ __global__ void k_mig_count(StepState *st, Migrants mg) {
 }
 
 __global__ void k_mig_scan(StepState *st, Migrants mg) {
-  // one block: exclusive scan over groups (small), then reset fill counters
+  // one block: exclusive scans over the groups (few) of the migrant counts
+  // and of their tile counts, then reset the fill counters
   if (st->n_mig == 0) return;
-  __shared__ uint32_t part[1024];
+  __shared__ uint32_t part[1024], tpart[1024];
   int tid = threadIdx.x, nt = blockDim.x;
   int per = (mg.ngroups + nt - 1) / nt;
   int b = tid * per, e = min(mg.ngroups, b + per);
-  uint32_t s = 0;
-  for (int i = b; i < e; ++i) s += mg.gcount[i];
+  uint32_t s = 0, ts = 0;
+  for (int i = b; i < e; ++i) {
+    s += mg.gcount[i];
+    ts += (mg.gcount[i] + MIG_TILE - 1) / MIG_TILE;
+  }
   part[tid] = s;
+  tpart[tid] = ts;
   __syncthreads();
   if (tid == 0) {
-    uint32_t acc = 0;
-    for (int i = 0; i < nt; ++i) { uint32_t v = part[i]; part[i] = acc; acc += v; }
+    uint32_t acc = 0, tacc = 0;
+    for (int i = 0; i < nt; ++i) {
+      uint32_t v = part[i], w = tpart[i];
+      part[i] = acc;
+      tpart[i] = tacc;
+      acc += v;
+      tacc += w;
+    }
+    mg.gtile[mg.ngroups] = tacc;
   }
   __syncthreads();
-  uint32_t acc = part[tid];
-  for (int i = b; i < e; ++i) { mg.gstart[i] = acc; acc += mg.gcount[i]; mg.gfill[i] = 0; }
+  uint32_t acc = part[tid], tacc = tpart[tid];
+  for (int i = b; i < e; ++i) {
+    mg.gstart[i] = acc;
+    mg.gtile[i] = tacc;
+    acc += mg.gcount[i];
+    tacc += (mg.gcount[i] + MIG_TILE - 1) / MIG_TILE;
+    mg.gfill[i] = 0;
+  }
 }
 
 __global__ void k_mig_scatter(StepState *st, Migrants mg) {
@@ -599,15 +618,73 @@ __global__ void k_mig_scatter(StepState *st, Migrants mg) {
   }
 }
 
+// sorts every MIG_TILE-entry tile of every group by key in LDS (a block per
+// tile); the sorted keys go to skey, the migrant indices back to bucket
+__global__ void __launch_bounds__(MIG_TILE_THREADS) k_mig_sort_tiles(StepState *st, Migrants mg) {
+  using Sort = hipcub::BlockRadixSort<uint64_t, MIG_TILE_THREADS, MIG_TILE_ITEMS, uint32_t>;
+  __shared__ typename Sort::TempStorage tmp;
+  if (st->n_mig == 0) return;
+  const uint32_t ntiles = mg.gtile[mg.ngroups];
+  for (uint32_t vt = blockIdx.x; vt < ntiles; vt += gridDim.x) {
+    // the group of tile vt: gtile[lo] <= vt < gtile[lo + 1]
+    int lo = 0, hi = mg.ngroups;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (mg.gtile[mid] <= vt) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t first = (vt - mg.gtile[lo]) * MIG_TILE;
+    const uint32_t base = mg.gstart[lo] + first;
+    const uint32_t len = min((uint32_t)MIG_TILE, mg.gcount[lo] - first);
+    uint64_t k[MIG_TILE_ITEMS];
+    uint32_t v[MIG_TILE_ITEMS];
+#pragma unroll
+    for (int i = 0; i < MIG_TILE_ITEMS; ++i) {
+      const uint32_t pos = threadIdx.x * MIG_TILE_ITEMS + i;
+      v[i] = pos < len ? mg.bucket[base + pos] : 0xFFFFFFFFu;
+      k[i] = pos < len ? mg.key[v[i]] : ~0ull;   // padding sorts last
+    }
+    Sort(tmp).Sort(k, v);
+#pragma unroll
+    for (int i = 0; i < MIG_TILE_ITEMS; ++i) {
+      const uint32_t pos = threadIdx.x * MIG_TILE_ITEMS + i;
+      if (pos < len) {
+        mg.skey[base + pos] = k[i];
+        mg.bucket[base + pos] = v[i];
+      }
+    }
+    __syncthreads();   // tmp is reused by the next tile
+  }
+}
+
+// rank of a migrant in its group = its place in its own sorted tile plus the
+// keys below it in each other tile of the group (keys are unique: the
+// source domain and its vector position); O(n log MIG_TILE) per tile pair
 __global__ void k_mig_rank(StepState *st, Migrants mg, SeqTable seqs) {
   unsigned int n = min(st->n_mig, (unsigned int)mg.cap);
-  for (unsigned int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
-    uint32_t gi = mg.group[t];
-    uint64_t k = mg.key[t];
-    uint32_t b = mg.gstart[gi], e = b + mg.gcount[gi];
-    uint64_t rank = 0;
-    for (uint32_t u = b; u < e; ++u) rank += mg.key[mg.bucket[u]] < k;
-    uint64_t ref = mg.ref[t];
+  for (unsigned int u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+    const uint32_t m = mg.bucket[u];
+    const uint32_t gi = mg.group[m];
+    const uint64_t k = mg.skey[u];
+    const uint32_t b = mg.gstart[gi], cnt = mg.gcount[gi];
+    const uint32_t off = u - b, own = off / MIG_TILE;
+    uint64_t rank = off - own * MIG_TILE;
+    for (uint32_t t0 = 0, tile = 0; t0 < cnt; t0 += MIG_TILE, ++tile) {
+      if (tile == own) continue;
+      const uint64_t *tk = mg.skey + b + t0;
+      uint32_t lo = 0, len = min((uint32_t)MIG_TILE, cnt - t0);
+      while (len > 0) {   // lower bound of k in the tile
+        const uint32_t half = len >> 1;
+        if (tk[lo + half] < k) {
+          lo += half + 1;
+          len -= half + 1;
+        } else {
+          len = half;
+        }
+      }
+      rank += lo;
+    }
+    uint64_t ref = mg.ref[m];
     int sp = (int)(ref >> 40);
     long p = (long)(ref & ((1ull << 40) - 1));
     seqs.p[sp][p] = mg.gbase[gi] + rank;
@@ -745,6 +822,7 @@ void launch_migrant_keys_t(pcp_ctx *c, SeqTable t) {
   hipLaunchKernelGGL(k_mig_count, dim3(gm), dim3(256), 0, c->stream, c->st, c->mig);
   hipLaunchKernelGGL(k_mig_scan, dim3(1), dim3(1024), 0, c->stream, c->st, c->mig);
   hipLaunchKernelGGL(k_mig_scatter, dim3(gm), dim3(256), 0, c->stream, c->st, c->mig);
+  hipLaunchKernelGGL(k_mig_sort_tiles, dim3(gm), dim3(MIG_TILE_THREADS), 0, c->stream, c->st, c->mig);
   hipLaunchKernelGGL(k_mig_rank, dim3(gm), dim3(256), 0, c->stream, c->st, c->mig, t);
   hipLaunchKernelGGL(k_mig_finish, dim3(grid_for(c->mig.ngroups, 256)), dim3(256), 0, c->stream, c->st, c->mig);
 }

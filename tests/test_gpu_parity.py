@@ -160,6 +160,54 @@ def test_deposit_and_stitch_parity(ccs, domains):
     assert e <= 1e-13
 
 
+def test_crowded_migration_order():
+    """Thousands of electrons cross each interior z boundary of 1 x 4 domains
+    in one step, from both sides, so every destination group holds several
+    sorted tiles of migrants from two source domains: the device's order keys
+    reproduce the reference's push_back order (src/SMB.cpp:211-340) and the
+    particle set is the oracle's."""
+    capi = _cap()
+    size, grid = Hh.geometry(16, 64)
+    dr, dz = size[0] / grid[0], size[1] / grid[1]
+    dt = 0.5 * dz / 3e8
+    domains = (1, 4)
+    species = [(-1, 1, False)]
+    orc = O.SMB(size, grid, domains, dt, species)
+    rng = np.random.default_rng(11)
+    parts = []
+    nz_d = grid[1] // domains[1]
+    for J in range(1, domains[1]):
+        zb = J * nz_d * dz
+        for sign in (1, -1):
+            n = 3000 + 500 * J
+            r = rng.uniform(0.5 * dr, size[0] - 0.5 * dr, n)
+            z = zb - sign * rng.uniform(0.02, 0.4, n) * dz
+            vz = np.full(n, sign * 0.9 * 3e8)
+            vr = rng.normal(0, 1e5, n)
+            vp = rng.normal(0, 1e5, n)
+            parts.append((r, z, vr, vp, vz, np.full(n, 1e6)))
+    r, z, vr, vp, vz, w = (np.concatenate(a) for a in zip(*parts))
+    o = rng.permutation(len(r))
+    r, z, vr, vp, vz, w = (a[o] for a in (r, z, vr, vp, vz, w))
+    ids = np.arange(len(r), dtype=np.uint64)
+    gpu = capi.Context(size, grid, domains, dt, [species[0] + (len(r) + 64,)])
+    orc.add_particles(0, r, z, vr, vp, vz, w, ids)
+    gpu.upload(0, r, z, vr, vp, vz, w, ids)
+    for ph in ("push", "move", "migrate"):
+        orc.phase(ph)
+        gpu.phase(ph)
+    compare_particles(orc, gpu, 1, 1e-13, "migrate")
+    a = orc.particles(0)
+    b = gpu.particles(0)
+    ob = b["id"][np.argsort(b["order"])]
+    z0 = np.empty_like(z)
+    z0[ids.astype(np.int64)] = z
+    crossed = np.floor(a["z"] / (nz_d * dz)) != np.floor(z0[a["id"].astype(np.int64)] / (nz_d * dz))
+    print("particles", len(r), "changed domain", int(crossed.sum()))
+    assert crossed.sum() > 0.9 * len(r)
+    assert np.array_equal(a["id"], ob)
+
+
 def test_strict_axial_reset_order():
     """Particles with v_r = 0 reset J_r nodes (currentVB.cpp:548-549); the
     result depends on the reference's particle order, which the exact path
