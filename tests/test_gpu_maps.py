@@ -3,10 +3,17 @@ restatement (src/specieP.cpp:665-839, include/algo/weighter.hpp:27-132,
 src/outController.cpp:409-491), through the C ABI: density, temperature
 (before any energy map, reading the zero momentum_abs map as the reference
 does, and after one), energy and momentum of the reference's initial plasma
-after a few steps over 2 x 4 domains, on every interior node -- within 1e-12
-of each map's largest value in FP64 (the atomics reorder the sums), 1e-5 in
-FP32 (the particles' positions and velocities are FP32) -- and a map probe
-window equal to the downloaded map's."""
+after a few steps over 2 x 4 domains, on every interior node, within 1e-12
+of each map's largest value (the atomics reorder the sums).
+
+An FP64 context is compared with the oracle run alongside it.  An FP32
+context is compared with an oracle that holds the device's own particles
+(positions rebuilt from cell + offset in FP64 as the map kernel rebuilds
+them): the ratio maps (temperature, energy, momentum) at nodes that only the
+edges of a few clouds reach depend on overlaps of a small fraction of a cell,
+whose relative error from the offsets' 2^-24-cell resolution is then far
+above 2^-24, so the FP32 run's own evolution is checked on the density map
+alone (1e-5).  A map probe window equals the downloaded map's."""
 import numpy as np
 import pytest
 
@@ -30,7 +37,18 @@ def test_maps_match_the_oracle(precision):
     O.set_threads(8)
     orc.step(3)
     dev.step(3)
-    tol = 1e-12 if precision == "fp64" else 1e-5
+    if precision == "fp32":
+        for sp in range(2):
+            a = orc.species_map(sp, "density")[:, :, 2:-2, 2:-2]
+            b = dev.species_map(sp, "density")[:, :, 2:-2, 2:-2]
+            err = np.abs(a - b).max() / np.abs(a).max()
+            print("fp32 evolution", sp, "density max rel err %.2e" % err)
+            assert err <= 1e-5
+        orc = O.SMB(SIZE, GRID, DOMAINS, 1e-13, SPECIES)
+        for sp in range(2):
+            p = dev.particles(sp)
+            orc.add_particles(sp, p["r"], p["z"], p["vr"], p["vphi"], p["vz"], p["w"], p["id"])
+    tol = 1e-12
     for sp in range(2):
         for kind in ("density", "temperature", "energy", "temperature", "momentum"):
             a = orc.species_map(sp, kind)[:, :, 2:-2, 2:-2]

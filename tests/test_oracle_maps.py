@@ -113,6 +113,23 @@ def test_momentum_energy_temperature_closed_forms():
     assert np.allclose(t2[m2], 0.5 * EL_MASS * gamma ** 2 * (v @ v) / EL_CHARGE, rtol=1e-9)
 
 
+def test_each_simulation_keeps_its_own_maps():
+    """The momentum_abs map an energy map leaves behind belongs to its
+    simulation: a simulation created after another one is freed (malloc may
+    hand it the same address) starts from zero maps, so its first
+    temperature map is the stale -|P|^2 form."""
+    v = np.array([3e5, -2e5, 4e5])
+    gamma = 1 / np.sqrt(1 - (v @ v) / C ** 2)
+    for _ in range(3):
+        s = one()
+        uniform_velocity_plasma(s, v)
+        mask = s.species_map(0, "density")[0, 0][2:-2, 2:-2] != 0
+        t = s.species_map(0, "temperature")[0, 0][2:-2, 2:-2]
+        assert np.allclose(t[mask], -0.5 * EL_MASS * gamma ** 2 * (v @ v) / EL_CHARGE, rtol=1e-9)
+        s.species_map(0, "energy")
+        del s
+
+
 def test_overlay_matches_a_single_domain_except_the_corner():
     rng = np.random.default_rng(4)
     n = 6000
