@@ -5,15 +5,19 @@
 // correct_velocities (collisionsTA77S.cpp:419-566).
 //
 // Pipeline per pass (two species: electrons `se`, ions `si`):
-//   k_coll_key      per particle: its (domain, cell) bucket and a slot in it
+//   k_coll_count    particles per (domain, cell) bucket
 //   scan            bucket offsets (hipCUB)
-//   k_coll_scatter  bucket lists of particle indices
-//   k_coll_prep     one wavefront per cell: shuffle each bucket by sorting on
+//   k_coll_fill     bucket lists of particle indices
+//   k_coll_classify cells with both species into lists by size (at most 16,
+//                   64 or 256 per species; a cell missing a species ends its
+//                   r-row as the reference's `break` does)
+//   k_coll_prep     one wavefront per listed cell, LDS staging sized by the
+//                   class: shuffle each bucket by sorting on
 //                   a Philox key of (particle id, pass), per-cell sums in the
 //                   shuffled order, density / temperature / Debye length, and
 //                   the first cell of each r-row whose values are not normal
 //                   (the reference's `break` ends the rest of that row)
-//   k_coll_pairs    one wavefront per cell: the reference's pair schedule
+//   k_coll_pairs    one wavefront per listed cell: the reference's pair schedule
 //                   with lanes on independent pair chains, velocities staged
 //                   in LDS in FP64; pair q of global cell c draws the uniforms
 //                   Philox(seed; c << 24 | q, 2 pass + 1)
@@ -90,34 +94,48 @@ __device__ __forceinline__ uint32_t local_cell_key(const Geom &g, int cr, int cz
 // mask (optional): only particles of cells with mask[key] != 0 are bucketed.
 __device__ __forceinline__ bool gate_closed(const unsigned long long *gate) { return gate && *gate == 0ull; }
 
+// bucket of particle p (sentinel: removed, or its cell not in mask)
 template <class V>
-__global__ void __launch_bounds__(256) k_coll_key(Geom g, V v, uint32_t *key, uint32_t *slot, uint32_t *cnt,
-                                                  uint32_t sentinel, const int *mask,
-                                                  const unsigned long long *gate) {
+__device__ __forceinline__ uint32_t bucket_of(const Geom &g, const V &v, long p, long n, uint32_t sentinel,
+                                              const int *mask) {
+  uint32_t k = sentinel;
+  int cr, cz;
+  if (p < n && v.cell_of(g, p, cr, cz)) k = local_cell_key(g, cr, cz);
+  if (mask && k != sentinel && !mask[k]) k = sentinel;
+  return k;
+}
+
+// pass 1: particles per bucket (one atomic per distinct key of a wavefront)
+template <class V>
+__global__ void __launch_bounds__(256) k_coll_count(Geom g, V v, uint32_t *cnt, uint32_t sentinel, const int *mask,
+                                                    const unsigned long long *gate) {
   if (gate_closed(gate)) return;
   const long n = v.n();
   const long stride = (long)gridDim.x * blockDim.x;
   for (long b0 = (long)blockIdx.x * blockDim.x; b0 < n; b0 += stride) {
     const long p = b0 + threadIdx.x;
-    uint32_t k = sentinel;
-    int cr, cz;
-    if (p < n && v.cell_of(g, p, cr, cz)) k = local_cell_key(g, cr, cz);
-    if (mask && k != sentinel && !mask[k]) k = sentinel;
-    const bool a = p < n && k != sentinel;
-    uint32_t s = wave_reserve(cnt, a ? k : 0, a);
-    if (p < n) {
-      key[p] = k;
-      slot[p] = s;
-    }
+    const uint32_t k = bucket_of(g, v, p, n, sentinel, mask);
+    const bool a = k != sentinel;
+    (void)wave_reserve(cnt, a ? k : 0, a);
   }
 }
 
-__global__ void k_coll_scatter(long n, const uint32_t *key, const uint32_t *slot, const uint32_t *off,
-                               uint32_t *perm, uint32_t sentinel, const unsigned long long *gate) {
+// pass 2: each particle's index at its bucket offset + a slot (fill counts
+// from zero); the order inside a bucket is the atomics' -- the shuffle
+// orders every bucket by its particles' keys
+template <class V>
+__global__ void __launch_bounds__(256) k_coll_fill(Geom g, V v, const uint32_t *off, uint32_t *fill, uint32_t *perm,
+                                                   uint32_t sentinel, const int *mask,
+                                                   const unsigned long long *gate) {
   if (gate_closed(gate)) return;
-  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long)gridDim.x * blockDim.x) {
-    uint32_t k = key[p];
-    if (k != sentinel) perm[off[k] + slot[p]] = (uint32_t)p;
+  const long n = v.n();
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long b0 = (long)blockIdx.x * blockDim.x; b0 < n; b0 += stride) {
+    const long p = b0 + threadIdx.x;
+    const uint32_t k = bucket_of(g, v, p, n, sentinel, mask);
+    const bool a = k != sentinel;
+    const uint32_t slot = wave_reserve(fill, a ? k : 0, a);
+    if (a) perm[off[k] + slot] = (uint32_t)p;
   }
 }
 
@@ -200,14 +218,22 @@ __device__ CellSums cell_sums(const double *vx, const double *vy, const double *
 // per-cell values kept between the two kernels
 constexpr int TOT = 16;   // a_e E_e m_e[3] a_i E_i m_i[3] dens_e dens_i debye
 
-struct CellStage {
-  double vx[2][COLL_MAXC], vy[2][COLL_MAXC], vz[2][COLL_MAXC], w[2][COLL_MAXC];
-  uint32_t idx[2][COLL_MAXC];
+// a cell's particles staged in LDS (FP64 velocities), up to MAXC per species
+template <int MAXC>
+struct CellStageT {
+  double vx[2][MAXC], vy[2][MAXC], vz[2][MAXC], w[2][MAXC];
+  uint32_t idx[2][MAXC];
 };
+using CellStage = CellStageT<COLL_MAXC>;
 
-template <class V>
+// cells are processed by size class: at most 16, 64 or COLL_MAXC particles
+// of either species (smaller stages leave room for more wavefronts per CU)
+constexpr int COLL_CLASSES = 3;
+constexpr int COLL_CLASS_MAXC[COLL_CLASSES] = {8, 64, COLL_MAXC};   // class 0: one lane per cell
+
+template <class V, int MAXC>
 __device__ __forceinline__ void stage_species(const V &v, const uint32_t *perm, uint32_t o, int n, int s,
-                                              CellStage &S) {
+                                              CellStageT<MAXC> &S) {
   for (int k = threadIdx.x; k < n; k += COLL_BLOCK) {
     uint32_t p = perm[o + k];
     double vv[3];
@@ -236,24 +262,83 @@ __device__ __forceinline__ void shuffle_bucket(const V &v, uint32_t *perm, uint3
   __syncthreads();
 }
 
-template <class V>
+// Cells with particles of both species go to the list of their size class;
+// a cell missing a species has zero density and temperature, which ends
+// the rest of its r-row (collisionsP12.cpp:351-354).
+__global__ void k_coll_classify(Geom g, const uint32_t *off_e, const uint32_t *off_i, const int *mask,
+                                uint32_t *lists, uint32_t *counts, int *row_bad, StepState *st,
+                                const unsigned long long *gate) {
+  if (gate_closed(gate)) return;
+  const long ncells = (long)g.Dr * g.Dz * g.nr * g.nz;
+  const long stride = (long)gridDim.x * blockDim.x;
+  int err = 0;
+  for (long c0 = (long)blockIdx.x * blockDim.x; c0 < ncells; c0 += stride) {
+    const long c = c0 + threadIdx.x;
+    int cls = -1;
+    if (c < ncells && (!mask || mask[c])) {
+      const int ne = (int)(off_e[c + 1] - off_e[c]), ni = (int)(off_i[c + 1] - off_i[c]);
+      if (ne == 0 || ni == 0) {
+        const CellPos q = cell_pos(g, c);
+        atomicMin(&row_bad[(long)q.d * g.nr + q.i], q.j);
+      } else {
+        const int m = max(ne, ni);
+        cls = 0;
+        while (cls < COLL_CLASSES && m > COLL_CLASS_MAXC[cls]) ++cls;
+        if (cls == COLL_CLASSES) {
+          err |= ERR_COLL_OVERFLOW;
+          cls = -1;
+        }
+      }
+    }
+    // one counter atomic per class present in the wavefront
+    const bool a = cls >= 0;
+    const uint32_t at = wave_reserve(counts, a ? (uint32_t)cls : 0u, a);
+    if (a) lists[(long)cls * ncells + at] = (uint32_t)c;
+  }
+  if (err) atomicOr(&st->err, err);
+}
+
+// Collisions::collect_weighted_params_tot_grid's per-cell values from the
+// cell's sums in shuffled order: density, temperature, Debye length, and
+// the row break of a cell whose values are not normal
+__device__ __forceinline__ void cell_stats(const Geom &g, long c, const CellSums &te, const CellSums &ti,
+                                           double sum_el, double sum_ion, double *tot, int *row_bad) {
+  const CellPos q = cell_pos(g, c);
+  const double inv_e = 1. / EL_CHARGE;
+  double temperature_el = te.energy / te.amount * inv_e;
+  double temperature_ion = ti.energy / ti.amount * inv_e;
+  // Collisions::geometry_cell_volume (collisions.cpp:162-170) passes
+  // `i+shift` to CELL_VOLUME, which expands to 2.0 * i + shift + 1
+  // (quirk kept: the macro argument is not parenthesised)
+  const double shift = (double)((long)(q.d / g.Dz) * g.nr);
+  const int i = q.i;
+  double cell_volume = PCP_CELL_VOLUME(i+shift, g.dr, g.dz);
+  double density_el = sum_el / cell_volume;
+  double density_ion = sum_ion / cell_volume;
+  double *t = tot + c * TOT;
+  t[0] = te.amount; t[1] = te.energy; t[2] = te.m[0]; t[3] = te.m[1]; t[4] = te.m[2];
+  t[5] = ti.amount; t[6] = ti.energy; t[7] = ti.m[0]; t[8] = ti.m[1]; t[9] = ti.m[2];
+  t[10] = density_el; t[11] = density_ion;
+  bool bad = !isnormal(temperature_ion) || !isnormal(temperature_el) || !isnormal(density_ion) ||
+             !isnormal(density_el);
+  t[12] = bad ? 0.0 : debye_length(density_el, density_ion, temperature_el, temperature_ion);
+  if (bad) atomicMin(&row_bad[(long)q.d * g.nr + q.i], q.j);
+}
+
+template <class V, int MAXC>
 __global__ void __launch_bounds__(COLL_BLOCK) k_coll_prep(Geom g, V ve, V vi, const uint32_t *off_e,
                                                           const uint32_t *off_i, uint32_t *perm_e, uint32_t *perm_i,
                                                           double m_e, double m_i, uint64_t seed, uint64_t step,
-                                                          double *tot, int *row_bad, const int *mask, StepState *st,
+                                                          double *tot, int *row_bad, const uint32_t *list,
+                                                          const uint32_t *count, StepState *st,
                                                           const unsigned long long *gate) {
-  __shared__ uint64_t skey[COLL_MAXC];
-  __shared__ CellStage S;
+  __shared__ uint64_t skey[MAXC];
+  __shared__ CellStageT<MAXC> S;
   if (gate_closed(gate)) return;
-  const long ncells = (long)g.Dr * g.Dz * g.nr * g.nz;
-  int err = 0;
-  for (long c = blockIdx.x; c < ncells; c += gridDim.x) {
-    if (mask && !mask[c]) continue;
+  const uint32_t nlist = *count;
+  for (uint32_t li = blockIdx.x; li < nlist; li += gridDim.x) {
+    const long c = list[li];
     int nn[2] = {(int)(off_e[c + 1] - off_e[c]), (int)(off_i[c + 1] - off_i[c])};
-    if (nn[0] > COLL_MAXC || nn[1] > COLL_MAXC) {
-      err |= ERR_COLL_OVERFLOW;
-      continue;
-    }
     shuffle_bucket(ve, perm_e, off_e[c], nn[0], seed, step, skey, S.idx[0]);
     shuffle_bucket(vi, perm_i, off_i[c], nn[1], seed, step, skey, S.idx[1]);
     stage_species(ve, perm_e, off_e[c], nn[0], 0, S);
@@ -262,39 +347,151 @@ __global__ void __launch_bounds__(COLL_BLOCK) k_coll_prep(Geom g, V ve, V vi, co
     if (threadIdx.x == 0) {
       CellSums te = cell_sums(S.vx[0], S.vy[0], S.vz[0], S.w[0], nn[0], m_e);
       CellSums ti = cell_sums(S.vx[1], S.vy[1], S.vz[1], S.w[1], nn[1], m_i);
-      const CellPos q = cell_pos(g, c);
-      const double inv_e = 1. / EL_CHARGE;
-      double temperature_el = te.energy / te.amount * inv_e;
-      double temperature_ion = ti.energy / ti.amount * inv_e;
-      // Collisions::geometry_cell_volume (collisions.cpp:162-170) passes
-      // `i+shift` to CELL_VOLUME, which expands to 2.0 * i + shift + 1
-      // (quirk kept: the macro argument is not parenthesised)
-      const double shift = (double)((long)(q.d / g.Dz) * g.nr);
-      const int i = q.i;
-      double cell_volume = PCP_CELL_VOLUME(i+shift, g.dr, g.dz);
       double sum_el = 0, sum_ion = 0;
       for (int k = 0; k < nn[0]; ++k) sum_el += S.w[0][k];
       for (int k = 0; k < nn[1]; ++k) sum_ion += S.w[1][k];
-      double density_el = sum_el / cell_volume;
-      double density_ion = sum_ion / cell_volume;
-      double *t = tot + c * TOT;
-      t[0] = te.amount; t[1] = te.energy; t[2] = te.m[0]; t[3] = te.m[1]; t[4] = te.m[2];
-      t[5] = ti.amount; t[6] = ti.energy; t[7] = ti.m[0]; t[8] = ti.m[1]; t[9] = ti.m[2];
-      t[10] = density_el; t[11] = density_ion;
-      bool bad = !isnormal(temperature_ion) || !isnormal(temperature_el) || !isnormal(density_ion) ||
-                 !isnormal(density_el);
-      t[12] = bad ? 0.0 : debye_length(density_el, density_ion, temperature_el, temperature_ion);
-      if (bad) atomicMin(&row_bad[(long)q.d * g.nr + q.i], q.j);
+      cell_stats(g, c, te, ti, sum_el, sum_ion, tot, row_bad);
     }
     __syncthreads();
   }
-  if (err && threadIdx.x == 0) atomicOr(&st->err, err);
 }
 
+// ---------------------------------------------------------------------------
+// Tiny cells (at most COLL_TINY particles of either species, e.g. the sparse
+// rows near the axis of a plasma uniform in r^2): one lane per cell, so a
+// wavefront works on 64 cells.  The shuffle sorts the keys in registers
+// (odd-even transposition network, fully unrolled); the pairs run the same
+// schedule as the wavefront kernels, one pair after another, on velocities
+// staged in FP64 in a global per-lane array laid out [slot][lane] (adjacent
+// lanes, adjacent addresses).  Pairs of one schedule phase touch disjoint
+// particles and draw by ordinal, so the results equal the wavefront path's.
+// ---------------------------------------------------------------------------
+constexpr int COLL_TINY = 8;
+constexpr int COLL_LANE_BLOCK = 256;
+constexpr int COLL_LANE_SLOTS = 4 * 2 * COLL_TINY;   // vx, vy, vz, w of both species
+static_assert(COLL_CLASS_MAXC[0] == COLL_TINY, "class 0 is the lane-per-cell class");
+
+template <class V>
+__device__ __forceinline__ void lane_shuffle(const V &v, uint32_t *perm, uint32_t o, int n, uint64_t seed,
+                                             uint64_t step, uint32_t idx[COLL_TINY]) {
+  uint64_t key[COLL_TINY];
+#pragma unroll
+  for (int k = 0; k < COLL_TINY; ++k) {
+    key[k] = ~0ull;
+    idx[k] = 0xFFFFFFFFu;
+    if (k < n) {
+      const uint32_t p = perm[o + k];
+      const uint64_t id = v.id(p);
+      uint32_t ph[4];
+      philox4x32(id, (uint32_t)(2 * step), seed, ph);
+      key[k] = ((uint64_t)ph[0] << 32) | (id & 0xFFFFFFFFull);
+      idx[k] = p;
+    }
+  }
+  // (key, index) ascending, padding last: the order wave_sort gives
+#pragma unroll
+  for (int r = 0; r < COLL_TINY; ++r)
+#pragma unroll
+    for (int k = r & 1; k + 1 < COLL_TINY; k += 2) {
+      const bool gt = key[k] > key[k + 1] || (key[k] == key[k + 1] && idx[k] > idx[k + 1]);
+      if (gt) {
+        const uint64_t tk = key[k]; key[k] = key[k + 1]; key[k + 1] = tk;
+        const uint32_t ti = idx[k]; idx[k] = idx[k + 1]; idx[k + 1] = ti;
+      }
+    }
+#pragma unroll
+  for (int k = 0; k < COLL_TINY; ++k)
+    if (k < n) perm[o + k] = idx[k];
+}
+
+// cell_sums of the shuffled particles, and their weight sum
+template <class V>
+__device__ __forceinline__ CellSums lane_sums(const V &v, const uint32_t idx[COLL_TINY], int n, double mass,
+                                              double &sum_w) {
+  CellSums t{0, 0, {0, 0, 0}};
+  sum_w = 0;
+#pragma unroll
+  for (int k = 0; k < COLL_TINY; ++k) {
+    if (k >= n) break;
+    double vv[3];
+    v.load(idx[k], vv);
+    const double w = v.w(idx[k]);
+    double vr = vv[0], vphi = vv[1], vzz = vv[2];
+    double v_sq = vr * vr + vphi * vphi + vzz * vzz;
+    double weighted_m = w * mass;
+    t.m[0] += weighted_m * vr;
+    t.m[1] += weighted_m * vphi;
+    t.m[2] += weighted_m * vzz;
+    t.amount += w;
+    t.energy += weighted_m * v_sq / 2;
+    sum_w += w;
+  }
+  return t;
+}
+
+template <class V>
+__global__ void __launch_bounds__(COLL_LANE_BLOCK) k_coll_prep_lane(Geom g, V ve, V vi, const uint32_t *off_e,
+                                                                    const uint32_t *off_i, uint32_t *perm_e,
+                                                                    uint32_t *perm_i, double m_e, double m_i,
+                                                                    uint64_t seed, uint64_t step, double *tot,
+                                                                    int *row_bad, const uint32_t *list,
+                                                                    const uint32_t *count,
+                                                                    const unsigned long long *gate) {
+  if (gate_closed(gate)) return;
+  const uint32_t nlist = *count;
+  for (uint32_t li = blockIdx.x * blockDim.x + threadIdx.x; li < nlist; li += gridDim.x * blockDim.x) {
+    const long c = list[li];
+    const int ne = (int)(off_e[c + 1] - off_e[c]), ni = (int)(off_i[c + 1] - off_i[c]);
+    uint32_t ie[COLL_TINY], ii[COLL_TINY];
+    lane_shuffle(ve, perm_e, off_e[c], ne, seed, step, ie);
+    lane_shuffle(vi, perm_i, off_i[c], ni, seed, step, ii);
+    double sum_el, sum_ion;
+    const CellSums te = lane_sums(ve, ie, ne, m_e, sum_el);
+    const CellSums ti = lane_sums(vi, ii, ni, m_i, sum_ion);
+    cell_stats(g, c, te, ti, sum_el, sum_ion, tot, row_bad);
+  }
+}
+
+// who runs a cell's loops: a wavefront (lanes split the independent pairs
+// of a phase) or a single lane (everything in order)
+struct WaveTeam {
+  static constexpr int step = COLL_BLOCK;
+  __device__ __forceinline__ int first() const { return (int)threadIdx.x; }
+  __device__ __forceinline__ void sync() const { __syncthreads(); }
+};
+struct LaneTeam {
+  static constexpr int step = 1;
+  __device__ __forceinline__ int first() const { return 0; }
+  __device__ __forceinline__ void sync() const {}
+};
+
+// a cell's staged FP64 particles: the wavefront's LDS stage, or a lane's
+// slots of the global array
+template <int MAXC>
+struct WaveStage {
+  CellStageT<MAXC> *S;
+  __device__ __forceinline__ double &vx(int s, int k) const { return S->vx[s][k]; }
+  __device__ __forceinline__ double &vy(int s, int k) const { return S->vy[s][k]; }
+  __device__ __forceinline__ double &vz(int s, int k) const { return S->vz[s][k]; }
+  __device__ __forceinline__ double &w(int s, int k) const { return S->w[s][k]; }
+};
+struct LaneStage {
+  double *b;     // this lane's slot 0
+  long stride;   // lanes of the launch
+  __device__ __forceinline__ double &at(int c, int s, int k) const {
+    return b[(long)((c * 2 + s) * COLL_TINY + k) * stride];
+  }
+  __device__ __forceinline__ double &vx(int s, int k) const { return at(0, s, k); }
+  __device__ __forceinline__ double &vy(int s, int k) const { return at(1, s, k); }
+  __device__ __forceinline__ double &vz(int s, int k) const { return at(2, s, k); }
+  __device__ __forceinline__ double &w(int s, int k) const { return at(3, s, k); }
+};
+
 // the pair schedule of one cell (collisionsP12.cpp:356-528)
-template <int SCHEME>
+template <int SCHEME, class Stage, class Team>
 struct PairRunner {
-  CellStage &S;
+  Stage S;
+  Team T;
   double m[2], q[2];   // electrons, ions
   double dens[2], debye, dt;
   uint64_t seed, step, gcell;
@@ -306,25 +503,25 @@ struct PairRunner {
     uniform2(ctr, draw, seed, R.u[0], R.u[1]);
     uniform2(ctr ^ (1ull << 63), draw, seed, R.u[2], R.u[3]);
     R.pos = 0;
-    double va[3] = {S.vx[sa][ka], S.vy[sa][ka], S.vz[sa][ka]};
-    double vb[3] = {S.vx[sb][kb], S.vy[sb][kb], S.vz[sb][kb]};
-    collide_pair<SCHEME>(m[sa], m[sb], q[sa], q[sb], va, S.w[sa][ka], vb, S.w[sb][kb], dens[sa], dens[sb], debye, dt,
-                         R, err);
-    S.vx[sa][ka] = va[0]; S.vy[sa][ka] = va[1]; S.vz[sa][ka] = va[2];
-    S.vx[sb][kb] = vb[0]; S.vy[sb][kb] = vb[1]; S.vz[sb][kb] = vb[2];
+    double va[3] = {S.vx(sa, ka), S.vy(sa, ka), S.vz(sa, ka)};
+    double vb[3] = {S.vx(sb, kb), S.vy(sb, kb), S.vz(sb, kb)};
+    collide_pair<SCHEME>(m[sa], m[sb], q[sa], q[sb], va, S.w(sa, ka), vb, S.w(sb, kb), dens[sa], dens[sb], debye,
+                         dt, R, err);
+    S.vx(sa, ka) = va[0]; S.vy(sa, ka) = va[1]; S.vz(sa, ka) = va[2];
+    S.vx(sb, kb) = vb[0]; S.vy(sb, kb) = vb[1]; S.vz(sb, kb) = vb[2];
   }
   // like-particle pairs of species s (cases 1a / 1b), ordinals from qb
   __device__ void like(int s, int n, uint32_t qb) {
     if (n % 2 == 0) {
-      for (int k = threadIdx.x; k < n / 2; k += COLL_BLOCK) pair(s, 2 * k, s, 2 * k + 1, qb + (uint32_t)k);
+      for (int k = T.first(); k < n / 2; k += Team::step) pair(s, 2 * k, s, 2 * k + 1, qb + (uint32_t)k);
     } else {
-      if (n >= 3 && threadIdx.x == 0) {
+      if (n >= 3 && T.first() == 0) {
         pair(s, 0, s, 1, qb);
         pair(s, 1, s, 2, qb + 1);
         pair(s, 2, s, 0, qb + 2);
       }
       if (n >= 5)
-        for (int k = threadIdx.x; k < (n - 3) / 2; k += COLL_BLOCK)
+        for (int k = T.first(); k < (n - 3) / 2; k += Team::step)
           pair(s, 3 + 2 * k, s, 4 + 2 * k, qb + 3 + (uint32_t)k);
     }
   }
@@ -337,72 +534,157 @@ __device__ __forceinline__ uint32_t like_pairs(int n) {
 
 __device__ __forceinline__ int grp(unsigned f, float div) { return (int)floorf((float)f / div); }
 
-// TA77S correct_velocities for species s of the staged cell; t = the cell's
-// {amount, energy, moment[3]} from before the collisions
-__device__ void correct_species(CellStage &S, int s, int n, double mass, const double *t) {
-  __shared__ double prm[8];   // ok, alpha, V0[3], dV[3]
-  if (threadIdx.x == 0) {
-    double m_new[3] = {0, 0, 0}, E_new = 0;
-    for (int k = 0; k < n; ++k) {
-      double vr = S.vx[s][k], vphi = S.vy[s][k], vz = S.vz[s][k];
-      double v_sq = vr * vr + vphi * vphi + vz * vz;
-      double mass_m = mass * S.w[s][k];
-      m_new[0] += mass_m * vr;
-      m_new[1] += mass_m * vphi;
-      m_new[2] += mass_m * vz;
-      E_new += mass_m * v_sq / 2;
+// cases 1a/1b then 2a/2b of one cell
+template <class Runner>
+__device__ void cell_pairs(Runner &P, int nel, int nion) {
+  const auto &T = P.T;
+  constexpr int STEP = decltype(P.T)::step;
+  // cases 1a/1b in reference order: ions even, electrons even, ions odd, electrons odd
+  const uint32_t n_ion = like_pairs(nion), n_el = like_pairs(nel);
+  // blocks in reference order: ion pairs if nion is even, electron pairs
+  // if nel is even, ion pairs if nion is odd, electron pairs if nel is odd
+  uint32_t q_ion = 0, q_el = n_ion;
+  if (nion % 2 != 0 && nel % 2 == 0) { q_el = 0; q_ion = n_el; }
+  P.like(1, nion, q_ion);
+  T.sync();
+  P.like(0, nel, q_el);
+  T.sync();
+  const uint32_t qc = n_ion + n_el;
+  // case 2a / 2b
+  if (nion == nel) {
+    for (int k = T.first(); k < nel; k += STEP) P.pair(0, k, 1, k, qc + (uint32_t)k);
+  } else if (nion > nel && nel > 0) {
+    unsigned int c_i = (unsigned int)floorf((float)nion / (float)nel);
+    double c_r = (double)((float)nion / (float)nel - (float)c_i);
+    unsigned int ions_1st = (unsigned int)((c_i + 1) * c_r * nel);
+    unsigned int els_1st = (unsigned int)(c_r * nel);
+    unsigned int ions_2nd = (unsigned int)(c_i * (1 - c_r) * nel);
+    const float d1 = (float)(c_i + 1), d2 = (float)c_i;
+    // 1st group: electron e meets ions fgi with floor(fgi / (c_i + 1)) == e
+    for (int e = T.first(); e < nel; e += STEP) {
+      unsigned lo = (unsigned)e * (c_i + 1);
+      lo = lo > 0 ? lo - 1 : 0;
+      for (unsigned fgi = lo; fgi < ions_1st && fgi <= (unsigned)(e + 1) * (c_i + 1); ++fgi)
+        if (grp(fgi, d1) == e && (int)fgi < nion) P.pair(0, e, 1, (int)fgi, qc + fgi);
     }
-    const double amount = t[0], energy = t[1];
-    double mass_tot = amount * mass;
-    double dV[3], V0[3];
-    for (int k = 0; k < 3; ++k) dV[k] = (m_new[k] - t[2 + k]) / mass_tot;
-    for (int k = 0; k < 3; ++k) V0[k] = t[2 + k] / mass_tot;
-    double delta_E = E_new - energy;
-    double V0_sq = V0[0] * V0[0] + V0[1] * V0[1] + V0[2] * V0[2];
-    double alpha = (energy - mass_tot * V0_sq / 2) /
-                   (energy + delta_E -
-                    mass_tot * (pow(V0[0] + dV[0], 2) + pow(V0[1] + dV[1], 2) + pow(V0[2] + dV[2], 2)) / 2);
-    bool ok = isnormal(alpha);
-    if (ok) {
-      if (alpha < 0) alpha = -alpha;
-      if (alpha == 0) alpha = 1;
-      alpha = sqrt(alpha);
+    T.sync();
+    for (int e = T.first(); e < nel; e += STEP) {
+      if (e < (int)els_1st) continue;
+      int le = e - (int)els_1st;
+      unsigned lo = (unsigned)le * c_i;
+      lo = lo > 0 ? lo - 1 : 0;
+      for (unsigned fgi = lo; fgi < ions_2nd && fgi <= (unsigned)(le + 1) * c_i; ++fgi)
+        if (grp(fgi, d2) == le && (int)(fgi + ions_1st) < nion)
+          P.pair(0, e, 1, (int)(fgi + ions_1st), qc + ions_1st + fgi);
     }
-    prm[0] = ok ? 1.0 : 0.0;
-    prm[1] = alpha;
-    for (int k = 0; k < 3; ++k) { prm[2 + k] = V0[k]; prm[5 + k] = dV[k]; }
+  } else if (nion < nel && nion > 0) {
+    double c_i = floorf((float)nel / (float)nion);
+    double c_r = (double)((float)nel / (float)nion) - c_i;
+    unsigned int els_1st = (unsigned int)((c_i + 1) * c_r * nion);
+    unsigned int ions_1st = (unsigned int)(c_r * nion);
+    unsigned int els_2nd = (unsigned int)(c_i * (1 - c_r) * nion);
+    const float d1 = (float)(c_i + 1), d2 = (float)c_i;
+    const unsigned ci1 = (unsigned)c_i + 1, ci0 = (unsigned)c_i;
+    for (int i = T.first(); i < nion; i += STEP) {
+      unsigned lo = (unsigned)i * ci1;
+      lo = lo > 0 ? lo - 1 : 0;
+      for (unsigned fge = lo; fge < els_1st && fge <= (unsigned)(i + 1) * ci1; ++fge)
+        if (grp(fge, d1) == i && (int)fge < nel) P.pair(0, (int)fge, 1, i, qc + fge);
+    }
+    T.sync();
+    for (int i = T.first(); i < nion; i += STEP) {
+      if (i < (int)ions_1st) continue;
+      int li = i - (int)ions_1st;
+      unsigned lo = (unsigned)li * ci0;
+      lo = lo > 0 ? lo - 1 : 0;
+      for (unsigned fge = lo; fge < els_2nd && fge <= (unsigned)(li + 1) * ci0; ++fge)
+        if (grp(fge, d2) == li && (int)(fge + els_1st) < nel)
+          P.pair(0, (int)(fge + els_1st), 1, i, qc + els_1st + fge);
+    }
   }
-  __syncthreads();
-  if (prm[0] != 0.0) {
-    const double alpha = prm[1];
-    for (int k = threadIdx.x; k < n; k += COLL_BLOCK) {
-      double vr = S.vx[s][k], vphi = S.vy[s][k], vz = S.vz[s][k];
-      S.vx[s][k] = prm[2] + alpha * (vr - prm[2] - prm[5]);
-      S.vy[s][k] = prm[3] + alpha * (vphi - prm[3] - prm[6]);
-      S.vz[s][k] = prm[4] + alpha * (vz - prm[4] - prm[7]);
-    }
-  }
-  __syncthreads();
+  T.sync();
 }
 
-template <int SCHEME, class V>
+// TA77S correct_velocities for species s of the staged cell; t = the cell's
+// {amount, energy, moment[3]} from before the collisions
+struct CorrParams {
+  double ok, alpha, V0[3], dV[3];
+};
+template <class Stage>
+__device__ CorrParams correct_params(const Stage &S, int s, int n, double mass, const double *t) {
+  CorrParams P;
+  double m_new[3] = {0, 0, 0}, E_new = 0;
+  for (int k = 0; k < n; ++k) {
+    double vr = S.vx(s, k), vphi = S.vy(s, k), vz = S.vz(s, k);
+    double v_sq = vr * vr + vphi * vphi + vz * vz;
+    double mass_m = mass * S.w(s, k);
+    m_new[0] += mass_m * vr;
+    m_new[1] += mass_m * vphi;
+    m_new[2] += mass_m * vz;
+    E_new += mass_m * v_sq / 2;
+  }
+  const double amount = t[0], energy = t[1];
+  double mass_tot = amount * mass;
+  double dV[3], V0[3];
+  for (int k = 0; k < 3; ++k) dV[k] = (m_new[k] - t[2 + k]) / mass_tot;
+  for (int k = 0; k < 3; ++k) V0[k] = t[2 + k] / mass_tot;
+  double delta_E = E_new - energy;
+  double V0_sq = V0[0] * V0[0] + V0[1] * V0[1] + V0[2] * V0[2];
+  double alpha = (energy - mass_tot * V0_sq / 2) /
+                 (energy + delta_E -
+                  mass_tot * (pow(V0[0] + dV[0], 2) + pow(V0[1] + dV[1], 2) + pow(V0[2] + dV[2], 2)) / 2);
+  bool ok = isnormal(alpha);
+  if (ok) {
+    if (alpha < 0) alpha = -alpha;
+    if (alpha == 0) alpha = 1;
+    alpha = sqrt(alpha);
+  }
+  P.ok = ok ? 1.0 : 0.0;
+  P.alpha = alpha;
+  for (int k = 0; k < 3; ++k) { P.V0[k] = V0[k]; P.dV[k] = dV[k]; }
+  return P;
+}
+
+template <class Stage, class Team>
+__device__ void correct_species(const Stage &S, const Team &T, int s, int n, double mass, const double *t) {
+  CorrParams P;
+  if constexpr (Team::step == 1) {
+    P = correct_params(S, s, n, mass, t);
+  } else {
+    __shared__ CorrParams shared_p;
+    if (threadIdx.x == 0) shared_p = correct_params(S, s, n, mass, t);
+    __syncthreads();
+    P = shared_p;
+  }
+  if (P.ok != 0.0) {
+    const double alpha = P.alpha;
+    for (int k = T.first(); k < n; k += Team::step) {
+      double vr = S.vx(s, k), vphi = S.vy(s, k), vz = S.vz(s, k);
+      S.vx(s, k) = P.V0[0] + alpha * (vr - P.V0[0] - P.dV[0]);
+      S.vy(s, k) = P.V0[1] + alpha * (vphi - P.V0[1] - P.dV[1]);
+      S.vz(s, k) = P.V0[2] + alpha * (vz - P.V0[2] - P.dV[2]);
+    }
+  }
+  T.sync();
+}
+
+template <int SCHEME, class V, int MAXC>
 __global__ void __launch_bounds__(COLL_BLOCK) k_coll_pairs(Geom g, V ve, V vi, const uint32_t *off_e,
                                                            const uint32_t *off_i, const uint32_t *perm_e,
                                                            const uint32_t *perm_i, double m_e, double m_i,
                                                            double q_e, double q_i, uint64_t seed, uint64_t step,
-                                                           const double *tot, const int *row_bad, const int *mask,
+                                                           const double *tot, const int *row_bad,
+                                                           const uint32_t *list, const uint32_t *count,
                                                            StepState *st, const unsigned long long *gate) {
-  __shared__ CellStage S;
+  __shared__ CellStageT<MAXC> S;
   if (gate_closed(gate)) return;
-  const long ncells = (long)g.Dr * g.Dz * g.nr * g.nz;
+  const uint32_t nlist = *count;
   int err = 0;
-  for (long c = blockIdx.x; c < ncells; c += gridDim.x) {
-    if (mask && !mask[c]) continue;
+  const WaveStage<MAXC> WS{&S};
+  const WaveTeam T;
+  for (uint32_t li = blockIdx.x; li < nlist; li += gridDim.x) {
+    const long c = list[li];
     int nn[2] = {(int)(off_e[c + 1] - off_e[c]), (int)(off_i[c + 1] - off_i[c])};
-    if (nn[0] > COLL_MAXC || nn[1] > COLL_MAXC) {
-      err |= ERR_COLL_OVERFLOW;
-      continue;
-    }
     stage_species(ve, perm_e, off_e[c], nn[0], 0, S);
     stage_species(vi, perm_i, off_i[c], nn[1], 1, S);
     __syncthreads();
@@ -410,78 +692,15 @@ __global__ void __launch_bounds__(COLL_BLOCK) k_coll_pairs(Geom g, V ve, V vi, c
     const double *t = tot + c * TOT;
     const bool active = q.j < row_bad[(long)q.d * g.nr + q.i];
     if (active) {
-      PairRunner<SCHEME> P{S, {m_e, m_i}, {q_e, q_i}, {t[10], t[11]}, t[12], g.dt, seed, step,
-                           (uint64_t)q.cr * (uint64_t)g.Nz + (uint64_t)q.cz, &err};
-      const int nel = nn[0], nion = nn[1];
-      // cases 1a/1b in reference order: ions even, electrons even, ions odd, electrons odd
-      const uint32_t n_ion = like_pairs(nion), n_el = like_pairs(nel);
-      // blocks in reference order: ion pairs if nion is even, electron pairs
-      // if nel is even, ion pairs if nion is odd, electron pairs if nel is odd
-      uint32_t q_ion = 0, q_el = n_ion;
-      if (nion % 2 != 0 && nel % 2 == 0) { q_el = 0; q_ion = n_el; }
-      P.like(1, nion, q_ion);
-      __syncthreads();
-      P.like(0, nel, q_el);
-      __syncthreads();
-      const uint32_t qc = n_ion + n_el;
-      // case 2a / 2b
-      if (nion == nel) {
-        for (int k = threadIdx.x; k < nel; k += COLL_BLOCK) P.pair(0, k, 1, k, qc + (uint32_t)k);
-      } else if (nion > nel && nel > 0) {
-        unsigned int c_i = (unsigned int)floorf((float)nion / (float)nel);
-        double c_r = (double)((float)nion / (float)nel - (float)c_i);
-        unsigned int ions_1st = (unsigned int)((c_i + 1) * c_r * nel);
-        unsigned int els_1st = (unsigned int)(c_r * nel);
-        unsigned int ions_2nd = (unsigned int)(c_i * (1 - c_r) * nel);
-        const float d1 = (float)(c_i + 1), d2 = (float)c_i;
-        // 1st group: electron e meets ions fgi with floor(fgi / (c_i + 1)) == e
-        for (int e = threadIdx.x; e < nel; e += COLL_BLOCK) {
-          unsigned lo = (unsigned)e * (c_i + 1);
-          lo = lo > 0 ? lo - 1 : 0;
-          for (unsigned fgi = lo; fgi < ions_1st && fgi <= (unsigned)(e + 1) * (c_i + 1); ++fgi)
-            if (grp(fgi, d1) == e && (int)fgi < nion) P.pair(0, e, 1, (int)fgi, qc + fgi);
-        }
-        __syncthreads();
-        for (int e = threadIdx.x; e < nel; e += COLL_BLOCK) {
-          if (e < (int)els_1st) continue;
-          int le = e - (int)els_1st;
-          unsigned lo = (unsigned)le * c_i;
-          lo = lo > 0 ? lo - 1 : 0;
-          for (unsigned fgi = lo; fgi < ions_2nd && fgi <= (unsigned)(le + 1) * c_i; ++fgi)
-            if (grp(fgi, d2) == le && (int)(fgi + ions_1st) < nion)
-              P.pair(0, e, 1, (int)(fgi + ions_1st), qc + ions_1st + fgi);
-        }
-      } else if (nion < nel && nion > 0) {
-        double c_i = floorf((float)nel / (float)nion);
-        double c_r = (double)((float)nel / (float)nion) - c_i;
-        unsigned int els_1st = (unsigned int)((c_i + 1) * c_r * nion);
-        unsigned int ions_1st = (unsigned int)(c_r * nion);
-        unsigned int els_2nd = (unsigned int)(c_i * (1 - c_r) * nion);
-        const float d1 = (float)(c_i + 1), d2 = (float)c_i;
-        const unsigned ci1 = (unsigned)c_i + 1, ci0 = (unsigned)c_i;
-        for (int i = threadIdx.x; i < nion; i += COLL_BLOCK) {
-          unsigned lo = (unsigned)i * ci1;
-          lo = lo > 0 ? lo - 1 : 0;
-          for (unsigned fge = lo; fge < els_1st && fge <= (unsigned)(i + 1) * ci1; ++fge)
-            if (grp(fge, d1) == i && (int)fge < nel) P.pair(0, (int)fge, 1, i, qc + fge);
-        }
-        __syncthreads();
-        for (int i = threadIdx.x; i < nion; i += COLL_BLOCK) {
-          if (i < (int)ions_1st) continue;
-          int li = i - (int)ions_1st;
-          unsigned lo = (unsigned)li * ci0;
-          lo = lo > 0 ? lo - 1 : 0;
-          for (unsigned fge = lo; fge < els_2nd && fge <= (unsigned)(li + 1) * ci0; ++fge)
-            if (grp(fge, d2) == li && (int)(fge + els_1st) < nel)
-              P.pair(0, (int)(fge + els_1st), 1, i, qc + els_1st + fge);
-        }
-      }
-      __syncthreads();
+      PairRunner<SCHEME, WaveStage<MAXC>, WaveTeam> P{WS, T, {m_e, m_i}, {q_e, q_i}, {t[10], t[11]}, t[12], g.dt,
+                                                      seed, step, (uint64_t)q.cr * (uint64_t)g.Nz + (uint64_t)q.cz,
+                                                      &err};
+      cell_pairs(P, nn[0], nn[1]);
     }
     if (SCHEME == COLL_TA77S) {
       // correct_velocities runs over every cell, ions first
-      correct_species(S, 1, nn[1], m_i, t + 5);
-      correct_species(S, 0, nn[0], m_e, t + 0);
+      correct_species(WS, T, 1, nn[1], m_i, t + 5);
+      correct_species(WS, T, 0, nn[0], m_e, t + 0);
     }
     for (int k = threadIdx.x; k < nn[0]; k += COLL_BLOCK) {
       double v3[3] = {S.vx[0][k], S.vy[0][k], S.vz[0][k]};
@@ -494,6 +713,62 @@ __global__ void __launch_bounds__(COLL_BLOCK) k_coll_pairs(Geom g, V ve, V vi, c
     __syncthreads();
   }
   if (err && threadIdx.x == 0) atomicOr(&st->err, err);
+}
+
+// the pairs of tiny cells, one lane per cell; `stage` holds
+// COLL_LANE_SLOTS doubles per lane of the launch
+template <int SCHEME, class V>
+__global__ void __launch_bounds__(COLL_LANE_BLOCK) k_coll_pairs_lane(
+    Geom g, V ve, V vi, const uint32_t *off_e, const uint32_t *off_i, const uint32_t *perm_e, const uint32_t *perm_i,
+    double m_e, double m_i, double q_e, double q_i, uint64_t seed, uint64_t step, const double *tot,
+    const int *row_bad, const uint32_t *list, const uint32_t *count, double *stage, StepState *st,
+    const unsigned long long *gate) {
+  if (gate_closed(gate)) return;
+  const uint32_t nlist = *count;
+  const long lanes = (long)gridDim.x * blockDim.x;
+  const long lane = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const LaneStage S{stage + lane, lanes};
+  const LaneTeam T;
+  int err = 0;
+  for (long li = lane; li < (long)nlist; li += lanes) {
+    const long c = list[li];
+    const int nn[2] = {(int)(off_e[c + 1] - off_e[c]), (int)(off_i[c + 1] - off_i[c])};
+    const uint32_t o[2] = {off_e[c], off_i[c]};
+    for (int k = 0; k < nn[0]; ++k) {
+      const uint32_t p = perm_e[o[0] + k];
+      double vv[3];
+      ve.load(p, vv);
+      S.vx(0, k) = vv[0]; S.vy(0, k) = vv[1]; S.vz(0, k) = vv[2];
+      S.w(0, k) = ve.w(p);
+    }
+    for (int k = 0; k < nn[1]; ++k) {
+      const uint32_t p = perm_i[o[1] + k];
+      double vv[3];
+      vi.load(p, vv);
+      S.vx(1, k) = vv[0]; S.vy(1, k) = vv[1]; S.vz(1, k) = vv[2];
+      S.w(1, k) = vi.w(p);
+    }
+    const CellPos q = cell_pos(g, c);
+    const double *t = tot + c * TOT;
+    if (q.j < row_bad[(long)q.d * g.nr + q.i]) {
+      PairRunner<SCHEME, LaneStage, LaneTeam> P{S, T, {m_e, m_i}, {q_e, q_i}, {t[10], t[11]}, t[12], g.dt,
+                                                seed, step, (uint64_t)q.cr * (uint64_t)g.Nz + (uint64_t)q.cz, &err};
+      cell_pairs(P, nn[0], nn[1]);
+    }
+    if (SCHEME == COLL_TA77S) {
+      correct_species(S, T, 1, nn[1], m_i, t + 5);
+      correct_species(S, T, 0, nn[0], m_e, t + 0);
+    }
+    for (int k = 0; k < nn[0]; ++k) {
+      double v3[3] = {S.vx(0, k), S.vy(0, k), S.vz(0, k)};
+      ve.store(perm_e[o[0] + k], v3);
+    }
+    for (int k = 0; k < nn[1]; ++k) {
+      double v3[3] = {S.vx(1, k), S.vy(1, k), S.vz(1, k)};
+      vi.store(perm_i[o[1] + k], v3);
+    }
+  }
+  if (err) atomicOr(&st->err, err);
 }
 
 // ---------------------------------------------------------------------------
@@ -809,7 +1084,11 @@ __global__ void k_fill_zero_f64(double *p, long n) {
 
 struct CollBuffers {
   uint32_t *cnt[2] = {nullptr, nullptr}, *off[2] = {nullptr, nullptr};
-  uint32_t *key[2] = {nullptr, nullptr}, *slot[2] = {nullptr, nullptr}, *perm[2] = {nullptr, nullptr};
+  uint32_t *fill[2] = {nullptr, nullptr}, *perm[2] = {nullptr, nullptr};
+  uint32_t *lists = nullptr, *lcount = nullptr;   // cells per size class
+  double *lane_stage = nullptr;                   // per-lane FP64 stage of the tiny-cell pairs
+  long lane_blocks = 0;
+
   long cap[2] = {0, 0};
   double *tot = nullptr;
   int *row_bad = nullptr;
@@ -832,9 +1111,10 @@ void coll_free(pcp_ctx *c) {
   CollBuffers *b = c->coll;
   if (!b) return;
   for (int s = 0; s < 2; ++s) {
-    (void)hipFree(b->cnt[s]); (void)hipFree(b->off[s]); (void)hipFree(b->key[s]);
-    (void)hipFree(b->slot[s]); (void)hipFree(b->perm[s]);
+    (void)hipFree(b->cnt[s]); (void)hipFree(b->off[s]); (void)hipFree(b->fill[s]);
+    (void)hipFree(b->perm[s]);
   }
+  (void)hipFree(b->lists); (void)hipFree(b->lcount); (void)hipFree(b->lane_stage);
   (void)hipFree(b->tot); (void)hipFree(b->row_bad); (void)hipFree(b->scan_tmp);
   (void)hipFree(b->mom); (void)hipFree(b->mom2); (void)hipFree(b->feat); (void)hipFree(b->pred);
   (void)hipFree(b->ok); (void)hipFree(b->fallback); (void)hipFree(b->counter);
@@ -856,11 +1136,17 @@ static int coll_alloc(pcp_ctx *c, long cap_e, long cap_i) {
     b->cap[s] = caps[s];
     if (hipMalloc(&b->cnt[s], (ncells + 1) * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&b->off[s], (ncells + 1) * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc(&b->key[s], caps[s] * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc(&b->slot[s], caps[s] * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&b->fill[s], (ncells + 1) * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&b->perm[s], caps[s] * sizeof(uint32_t)) != hipSuccess)
       return -1;
   }
+  // the lane-per-cell kernels: four workgroups per CU, COLL_LANE_SLOTS doubles per lane
+  b->lane_blocks = (long)c->n_cu * 4;
+  if (hipMalloc(&b->lists, (size_t)COLL_CLASSES * ncells * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&b->lcount, COLL_CLASSES * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&b->lane_stage, (size_t)COLL_LANE_SLOTS * b->lane_blocks * COLL_LANE_BLOCK * sizeof(double)) !=
+          hipSuccess)
+    return -1;
   if (hipMalloc(&b->tot, ncells * TOT * sizeof(double)) != hipSuccess ||
       hipMalloc(&b->row_bad, (long)c->D * c->g.nr * sizeof(int)) != hipSuccess ||
       hipMalloc(&b->mom, ncells * SST * sizeof(double)) != hipSuccess ||
@@ -891,26 +1177,73 @@ static int coll_buckets(pcp_ctx *c, V ve, V vi, long n_e, long n_i, const int *m
   unsigned gz = (unsigned)std::min<long>((ncells + 256) / 256, (long)c->n_cu * 16);
   for (int s = 0; s < 2; ++s) {
     hipLaunchKernelGGL(k_fill_u32, dim3(gz), dim3(256), 0, c->stream, b->cnt[s], ncells + 1);
-    if (nn[s] > 0) {
-      unsigned gp = (unsigned)std::min<long>((nn[s] + 255) / 256, (long)c->n_cu * 16);
-      hipLaunchKernelGGL(k_coll_key<V>, dim3(gp), dim3(256), 0, c->stream, g, s == 0 ? ve : vi, b->key[s],
-                         b->slot[s], b->cnt[s], sentinel, mask, gate);
-    }
+    hipLaunchKernelGGL(k_fill_u32, dim3(gz), dim3(256), 0, c->stream, b->fill[s], ncells + 1);
+    unsigned gp = (unsigned)std::max(1L, std::min<long>((nn[s] + 255) / 256, (long)c->n_cu * 16));
+    if (nn[s] > 0)
+      hipLaunchKernelGGL(k_coll_count<V>, dim3(gp), dim3(256), 0, c->stream, g, s == 0 ? ve : vi, b->cnt[s],
+                         sentinel, mask, gate);
     size_t bytes = b->scan_bytes;
     if (hipcub::DeviceScan::ExclusiveSum(b->scan_tmp, bytes, b->cnt[s], b->off[s], (int)(ncells + 1), c->stream) !=
         hipSuccess)
       return -1;
-    if (nn[s] > 0) {
-      unsigned gp = (unsigned)std::min<long>((nn[s] + 255) / 256, (long)c->n_cu * 16);
-      hipLaunchKernelGGL(k_coll_scatter, dim3(gp), dim3(256), 0, c->stream, nn[s], b->key[s], b->slot[s], b->off[s],
-                         b->perm[s], sentinel, gate);
-    }
+    if (nn[s] > 0)
+      hipLaunchKernelGGL(k_coll_fill<V>, dim3(gp), dim3(256), 0, c->stream, g, s == 0 ? ve : vi, b->off[s],
+                         b->fill[s], b->perm[s], sentinel, mask, gate);
   }
   return 0;
 }
 
 // the binary pass (shuffle + statistics, then pairs) on the cells of `mask`
 // (all cells when null)
+template <class V, int CLS>
+static void coll_class(pcp_ctx *c, int scheme, V ve, V vi, double m_e, double m_i, double q_e, double q_i,
+                       uint64_t seed, uint64_t step, bool pairs, const unsigned long long *gate) {
+  CollBuffers *b = c->coll;
+  const Geom &g = c->g;
+  constexpr int MAXC = COLL_CLASS_MAXC[CLS];
+  const uint32_t *list = b->lists + (size_t)CLS * b->ncells, *count = b->lcount + CLS;
+  const unsigned gc = (unsigned)std::min<long>(b->ncells, (long)c->n_cu * 32);
+  if constexpr (CLS == 0) {
+    const unsigned gl = (unsigned)b->lane_blocks;
+    if (!pairs)
+      hipLaunchKernelGGL((k_coll_prep_lane<V>), dim3(gl), dim3(COLL_LANE_BLOCK), 0, c->stream, g, ve, vi, b->off[0],
+                         b->off[1], b->perm[0], b->perm[1], m_e, m_i, seed, step, b->tot, b->row_bad, list, count,
+                         gate);
+    else if (scheme == COLL_TA77S)
+      hipLaunchKernelGGL((k_coll_pairs_lane<COLL_TA77S, V>), dim3(gl), dim3(COLL_LANE_BLOCK), 0, c->stream, g, ve,
+                         vi, b->off[0], b->off[1], b->perm[0], b->perm[1], m_e, m_i, q_e, q_i, seed, step, b->tot,
+                         b->row_bad, list, count, b->lane_stage, c->st, gate);
+    else if (scheme == COLL_SK98)
+      hipLaunchKernelGGL((k_coll_pairs_lane<COLL_SK98, V>), dim3(gl), dim3(COLL_LANE_BLOCK), 0, c->stream, g, ve,
+                         vi, b->off[0], b->off[1], b->perm[0], b->perm[1], m_e, m_i, q_e, q_i, seed, step, b->tot,
+                         b->row_bad, list, count, b->lane_stage, c->st, gate);
+    else
+      hipLaunchKernelGGL((k_coll_pairs_lane<COLL_P12, V>), dim3(gl), dim3(COLL_LANE_BLOCK), 0, c->stream, g, ve,
+                         vi, b->off[0], b->off[1], b->perm[0], b->perm[1], m_e, m_i, q_e, q_i, seed, step, b->tot,
+                         b->row_bad, list, count, b->lane_stage, c->st, gate);
+  } else {
+  if (!pairs)
+    hipLaunchKernelGGL((k_coll_prep<V, MAXC>), dim3(gc), dim3(COLL_BLOCK), 0, c->stream, g, ve, vi, b->off[0],
+                       b->off[1], b->perm[0], b->perm[1], m_e, m_i, seed, step, b->tot, b->row_bad, list, count, c->st,
+                       gate);
+  else if (scheme == COLL_TA77S)
+    hipLaunchKernelGGL((k_coll_pairs<COLL_TA77S, V, MAXC>), dim3(gc), dim3(COLL_BLOCK), 0, c->stream, g, ve, vi,
+                       b->off[0], b->off[1], b->perm[0], b->perm[1], m_e, m_i, q_e, q_i, seed, step, b->tot,
+                       b->row_bad, list, count, c->st, gate);
+  else if (scheme == COLL_SK98)
+    hipLaunchKernelGGL((k_coll_pairs<COLL_SK98, V, MAXC>), dim3(gc), dim3(COLL_BLOCK), 0, c->stream, g, ve, vi,
+                       b->off[0], b->off[1], b->perm[0], b->perm[1], m_e, m_i, q_e, q_i, seed, step, b->tot,
+                       b->row_bad, list, count, c->st, gate);
+  else
+    hipLaunchKernelGGL((k_coll_pairs<COLL_P12, V, MAXC>), dim3(gc), dim3(COLL_BLOCK), 0, c->stream, g, ve, vi,
+                       b->off[0], b->off[1], b->perm[0], b->perm[1], m_e, m_i, q_e, q_i, seed, step, b->tot,
+                       b->row_bad, list, count, c->st, gate);
+  }
+}
+
+// the binary pass on the cells of `mask` (all cells when null): cells
+// classified by size, then shuffle + statistics of every cell (the row
+// breaks are final only after all of them), then the pairs
 template <class V>
 static void coll_binary(pcp_ctx *c, int scheme, V ve, V vi, double m_e, double m_i, double q_e, double q_i,
                         uint64_t seed, uint64_t step, const int *mask, const unsigned long long *gate = nullptr) {
@@ -920,21 +1253,16 @@ static void coll_binary(pcp_ctx *c, int scheme, V ve, V vi, double m_e, double m
   const long rows = (long)c->D * g.nr;
   hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)std::max<long>(1, std::min<long>((rows + 255) / 256, 1024))),
                      dim3(256), 0, c->stream, b->row_bad, rows, INT_MAX);
-  unsigned gc = (unsigned)std::min<long>(ncells, (long)c->n_cu * 32);
-  hipLaunchKernelGGL(k_coll_prep<V>, dim3(gc), dim3(COLL_BLOCK), 0, c->stream, g, ve, vi, b->off[0], b->off[1],
-                     b->perm[0], b->perm[1], m_e, m_i, seed, step, b->tot, b->row_bad, mask, c->st, gate);
-  if (scheme == COLL_TA77S)
-    hipLaunchKernelGGL((k_coll_pairs<COLL_TA77S, V>), dim3(gc), dim3(COLL_BLOCK), 0, c->stream, g, ve, vi, b->off[0],
-                       b->off[1], b->perm[0], b->perm[1], m_e, m_i, q_e, q_i, seed, step, b->tot, b->row_bad, mask,
-                       c->st, gate);
-  else if (scheme == COLL_SK98)
-    hipLaunchKernelGGL((k_coll_pairs<COLL_SK98, V>), dim3(gc), dim3(COLL_BLOCK), 0, c->stream, g, ve, vi, b->off[0],
-                       b->off[1], b->perm[0], b->perm[1], m_e, m_i, q_e, q_i, seed, step, b->tot, b->row_bad, mask,
-                       c->st, gate);
-  else
-    hipLaunchKernelGGL((k_coll_pairs<COLL_P12, V>), dim3(gc), dim3(COLL_BLOCK), 0, c->stream, g, ve, vi, b->off[0],
-                       b->off[1], b->perm[0], b->perm[1], m_e, m_i, q_e, q_i, seed, step, b->tot, b->row_bad, mask,
-                       c->st, gate);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, b->lcount, (long)COLL_CLASSES);
+  const unsigned gz = (unsigned)std::max(1L, std::min<long>((ncells + 255) / 256, (long)c->n_cu * 16));
+  hipLaunchKernelGGL(k_coll_classify, dim3(gz), dim3(256), 0, c->stream, g, b->off[0], b->off[1], mask, b->lists,
+                     b->lcount, b->row_bad, c->st, gate);
+  coll_class<V, 0>(c, scheme, ve, vi, m_e, m_i, q_e, q_i, seed, step, false, gate);
+  coll_class<V, 1>(c, scheme, ve, vi, m_e, m_i, q_e, q_i, seed, step, false, gate);
+  coll_class<V, 2>(c, scheme, ve, vi, m_e, m_i, q_e, q_i, seed, step, false, gate);
+  coll_class<V, 0>(c, scheme, ve, vi, m_e, m_i, q_e, q_i, seed, step, true, gate);
+  coll_class<V, 1>(c, scheme, ve, vi, m_e, m_i, q_e, q_i, seed, step, true, gate);
+  coll_class<V, 2>(c, scheme, ve, vi, m_e, m_i, q_e, q_i, seed, step, true, gate);
 }
 
 template <class V>
