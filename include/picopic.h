@@ -224,8 +224,9 @@ int pcp_step(pcp_ctx *ctx, int n_steps);
  * velocities.  random_sort is an ordering of each cell's particles by a
  * Philox key of (particle id, pass) -- the reference shuffles with a
  * time-seeded std::random_shuffle -- and pair q of global cell c takes the
- * uniforms Philox(seed; c << 24 | q, 2 pass + 1).  At most 256 macro-particles
- * of a species per cell (PCP_E_STATE beyond). */
+ * uniforms Philox(seed; c << 24 | q, 2 pass + 1).  Cells of any size: those
+ * over 256 macro-particles of a species are staged in global memory, at most
+ * 2^22 macro-particles in all such cells of one pass (PCP_E_STATE beyond). */
 int pcp_collide(pcp_ctx *ctx, int scheme, int sp_el, int sp_ion, uint64_t seed, uint64_t pass);
 /* Enable (scheme >= 0) or disable (scheme < 0) the collision phase of
  * pcp_step, run between the push and the move like Domain::collide in

@@ -132,7 +132,8 @@ int check_state(pcp_ctx *c) {
     char buf[256];
     snprintf(buf, sizeof buf,
              "device reported error bits 0x%x (1 NaN position, 2 beta>1, 4 NaN field, 8 migrant overflow, "
-             "32 more than 256 macro-particles of a species in a collision cell)",
+             "32 crowded collision cells (over 256 macro-particles of a species) holding more than 2^22 "
+             "macro-particles in one pass)",
              h.err);
     int zero = 0;
     hipMemcpy(&c->st->err, &zero, sizeof zero, hipMemcpyHostToDevice);

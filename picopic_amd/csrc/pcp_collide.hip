@@ -8,19 +8,22 @@
 //   k_coll_count    particles per (domain, cell) bucket
 //   scan            bucket offsets (hipCUB)
 //   k_coll_fill     bucket lists of particle indices
-//   k_coll_classify cells with both species into lists by size (at most 16,
-//                   64 or 256 per species; a cell missing a species ends its
-//                   r-row as the reference's `break` does)
-//   k_coll_prep     one wavefront per listed cell, LDS staging sized by the
-//                   class: shuffle each bucket by sorting on
+//   k_coll_classify cells with both species into lists by size (at most 8,
+//                   64, 256 or more particles of either species; a cell
+//                   missing a species ends its r-row as the reference's
+//                   `break` does)
+//   k_coll_prep     per listed cell (one lane per cell of at most 8, one
+//                   wavefront with an LDS stage per cell of at most 64 or
+//                   256, one workgroup with a global stage per larger cell):
+//                   shuffle each bucket by sorting on
 //                   a Philox key of (particle id, pass), per-cell sums in the
 //                   shuffled order, density / temperature / Debye length, and
 //                   the first cell of each r-row whose values are not normal
 //                   (the reference's `break` ends the rest of that row)
-//   k_coll_pairs    one wavefront per listed cell: the reference's pair schedule
-//                   with lanes on independent pair chains, velocities staged
-//                   in LDS in FP64; pair q of global cell c draws the uniforms
-//                   Philox(seed; c << 24 | q, 2 pass + 1)
+//   k_coll_pairs    per listed cell, grouped the same way: the reference's
+//                   pair schedule with lanes on independent pair chains,
+//                   velocities staged in FP64; pair q of global cell c draws
+//                   the uniforms Philox(seed; c << 24 | q, 2 pass + 1)
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -228,8 +231,15 @@ using CellStage = CellStageT<COLL_MAXC>;
 
 // cells are processed by size class: at most 16, 64 or COLL_MAXC particles
 // of either species (smaller stages leave room for more wavefronts per CU)
-constexpr int COLL_CLASSES = 3;
-constexpr int COLL_CLASS_MAXC[COLL_CLASSES] = {8, 64, COLL_MAXC};   // class 0: one lane per cell
+constexpr int COLL_CLASSES = 4;
+// class 0: one lane per cell; 1, 2: a wavefront per cell, LDS stage; 3
+// (crowded): a workgroup per cell, stage in global memory
+constexpr int COLL_CLASS_MAXC[COLL_CLASSES] = {8, 64, COLL_MAXC, INT_MAX};
+// particles of all crowded cells of one pass that the global stage holds;
+// beyond it a pass raises ERR_COLL_OVERFLOW
+constexpr long COLL_CROWD_CAP = 1L << 22;
+constexpr long COLL_CROWD_CELLS = COLL_CROWD_CAP / (COLL_MAXC + 1) + 1;
+constexpr int COLL_CROWD_BLOCK = 256;
 
 template <class V, int MAXC>
 __device__ __forceinline__ void stage_species(const V &v, const uint32_t *perm, uint32_t o, int n, int s,
@@ -266,7 +276,7 @@ __device__ __forceinline__ void shuffle_bucket(const V &v, uint32_t *perm, uint3
 // a cell missing a species has zero density and temperature, which ends
 // the rest of its r-row (collisionsP12.cpp:351-354).
 __global__ void k_coll_classify(Geom g, const uint32_t *off_e, const uint32_t *off_i, const int *mask,
-                                uint32_t *lists, uint32_t *counts, int *row_bad, StepState *st,
+                                uint32_t *lists, uint32_t *counts, uint32_t *crowd_off, int *row_bad, StepState *st,
                                 const unsigned long long *gate) {
   if (gate_closed(gate)) return;
   const long ncells = (long)g.Dr * g.Dz * g.nr * g.nz;
@@ -275,6 +285,7 @@ __global__ void k_coll_classify(Geom g, const uint32_t *off_e, const uint32_t *o
   for (long c0 = (long)blockIdx.x * blockDim.x; c0 < ncells; c0 += stride) {
     const long c = c0 + threadIdx.x;
     int cls = -1;
+    uint32_t crowd_at = 0;
     if (c < ncells && (!mask || mask[c])) {
       const int ne = (int)(off_e[c + 1] - off_e[c]), ni = (int)(off_i[c + 1] - off_i[c]);
       if (ne == 0 || ni == 0) {
@@ -283,10 +294,15 @@ __global__ void k_coll_classify(Geom g, const uint32_t *off_e, const uint32_t *o
       } else {
         const int m = max(ne, ni);
         cls = 0;
-        while (cls < COLL_CLASSES && m > COLL_CLASS_MAXC[cls]) ++cls;
-        if (cls == COLL_CLASSES) {
-          err |= ERR_COLL_OVERFLOW;
-          cls = -1;
+        while (m > COLL_CLASS_MAXC[cls]) ++cls;
+        if (cls == COLL_CLASSES - 1) {
+          // a range of the crowded stage (counts[COLL_CLASSES] particles used)
+          const uint32_t need = (uint32_t)(ne + ni);
+          crowd_at = atomicAdd(&counts[COLL_CLASSES], need);
+          if ((long)crowd_at + need > COLL_CROWD_CAP) {
+            err |= ERR_COLL_OVERFLOW;
+            cls = -1;
+          }
         }
       }
     }
@@ -294,6 +310,7 @@ __global__ void k_coll_classify(Geom g, const uint32_t *off_e, const uint32_t *o
     const bool a = cls >= 0;
     const uint32_t at = wave_reserve(counts, a ? (uint32_t)cls : 0u, a);
     if (a) lists[(long)cls * ncells + at] = (uint32_t)c;
+    if (cls == COLL_CLASSES - 1) crowd_off[at] = crowd_at;
   }
   if (err) atomicOr(&st->err, err);
 }
@@ -454,11 +471,14 @@ __global__ void __launch_bounds__(COLL_LANE_BLOCK) k_coll_prep_lane(Geom g, V ve
 
 // who runs a cell's loops: a wavefront (lanes split the independent pairs
 // of a phase) or a single lane (everything in order)
-struct WaveTeam {
-  static constexpr int step = COLL_BLOCK;
+template <int N>
+struct GroupTeam {
+  static constexpr int step = N;
   __device__ __forceinline__ int first() const { return (int)threadIdx.x; }
   __device__ __forceinline__ void sync() const { __syncthreads(); }
 };
+using WaveTeam = GroupTeam<COLL_BLOCK>;
+using BlockTeam = GroupTeam<COLL_CROWD_BLOCK>;
 struct LaneTeam {
   static constexpr int step = 1;
   __device__ __forceinline__ int first() const { return 0; }
@@ -481,6 +501,15 @@ struct LaneStage {
   __device__ __forceinline__ double &at(int c, int s, int k) const {
     return b[(long)((c * 2 + s) * COLL_TINY + k) * stride];
   }
+  __device__ __forceinline__ double &vx(int s, int k) const { return at(0, s, k); }
+  __device__ __forceinline__ double &vy(int s, int k) const { return at(1, s, k); }
+  __device__ __forceinline__ double &vz(int s, int k) const { return at(2, s, k); }
+  __device__ __forceinline__ double &w(int s, int k) const { return at(3, s, k); }
+};
+struct CrowdStage {
+  double *b;     // the cell's range of the crowded stage
+  int n0;        // electrons first, then ions
+  __device__ __forceinline__ double &at(int c, int s, int k) const { return b[c * COLL_CROWD_CAP + s * n0 + k]; }
   __device__ __forceinline__ double &vx(int s, int k) const { return at(0, s, k); }
   __device__ __forceinline__ double &vy(int s, int k) const { return at(1, s, k); }
   __device__ __forceinline__ double &vz(int s, int k) const { return at(2, s, k); }
@@ -767,6 +796,144 @@ __global__ void __launch_bounds__(COLL_LANE_BLOCK) k_coll_pairs_lane(
       double v3[3] = {S.vx(1, k), S.vy(1, k), S.vz(1, k)};
       vi.store(perm_i[o[1] + k], v3);
     }
+  }
+  if (err) atomicOr(&st->err, err);
+}
+
+// ---------------------------------------------------------------------------
+// Crowded cells (more than COLL_MAXC particles of a species): a workgroup per
+// cell.  The shuffle sorts the cell's keys in the global key array with a
+// bitonic network whose comparators all run ascending (the first merge step
+// of each stage mirrors), so entries past n act as +inf and are skipped; the
+// pairs run the same schedule on an FP64 stage in global memory.
+// ---------------------------------------------------------------------------
+template <class V>
+__device__ void crowd_shuffle(const V &v, uint32_t *perm, uint32_t o, int n, uint64_t seed, uint64_t step,
+                              uint64_t *key, uint32_t *idx) {
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const uint32_t p = perm[o + k];
+    const uint64_t id = v.id(p);
+    uint32_t ph[4];
+    philox4x32(id, (uint32_t)(2 * step), seed, ph);
+    key[k] = ((uint64_t)ph[0] << 32) | (id & 0xFFFFFFFFull);
+    idx[k] = p;
+  }
+  __syncthreads();
+  int P = 1;
+  while (P < n) P <<= 1;
+  for (int kk = 2; kk <= P; kk <<= 1)
+    for (int j = kk >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < P; i += blockDim.x) {
+        const int l = j == (kk >> 1) ? (i ^ (kk - 1)) : (i ^ j);
+        if (l > i && l < n) {
+          const uint64_t a = key[i], b = key[l];
+          if (a > b || (a == b && idx[i] > idx[l])) {
+            key[i] = b;
+            key[l] = a;
+            const uint32_t x = idx[i];
+            idx[i] = idx[l];
+            idx[l] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  for (int k = threadIdx.x; k < n; k += blockDim.x) perm[o + k] = idx[k];
+  __syncthreads();
+}
+
+template <class V>
+__device__ __forceinline__ void crowd_stage_species(const V &v, const uint32_t *perm, uint32_t o, int n, int s,
+                                                    const CrowdStage &S) {
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const uint32_t p = perm[o + k];
+    double vv[3];
+    v.load(p, vv);
+    S.vx(s, k) = vv[0]; S.vy(s, k) = vv[1]; S.vz(s, k) = vv[2];
+    S.w(s, k) = v.w(p);
+  }
+}
+
+template <class V>
+__global__ void __launch_bounds__(COLL_CROWD_BLOCK) k_coll_prep_crowd(
+    Geom g, V ve, V vi, const uint32_t *off_e, const uint32_t *off_i, uint32_t *perm_e, uint32_t *perm_i, double m_e,
+    double m_i, uint64_t seed, uint64_t step, double *tot, int *row_bad, const uint32_t *list, const uint32_t *count,
+    const uint32_t *crowd_off, uint64_t *ckey, uint32_t *cidx, double *cstage, const unsigned long long *gate) {
+  if (gate_closed(gate)) return;
+  const uint32_t nlist = *count;
+  for (uint32_t li = blockIdx.x; li < nlist; li += gridDim.x) {
+    const long c = list[li];
+    const int nn[2] = {(int)(off_e[c + 1] - off_e[c]), (int)(off_i[c + 1] - off_i[c])};
+    const uint32_t base = crowd_off[li];
+    crowd_shuffle(ve, perm_e, off_e[c], nn[0], seed, step, ckey + base, cidx + base);
+    crowd_shuffle(vi, perm_i, off_i[c], nn[1], seed, step, ckey + base + nn[0], cidx + base + nn[0]);
+    const CrowdStage S{cstage + base, nn[0]};
+    crowd_stage_species(ve, perm_e, off_e[c], nn[0], 0, S);
+    crowd_stage_species(vi, perm_i, off_i[c], nn[1], 1, S);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      // cell_sums in the shuffled order, from the stage
+      CellSums t[2];
+      double sum_w[2];
+      const double mass[2] = {m_e, m_i};
+      for (int sp = 0; sp < 2; ++sp) {
+        t[sp] = CellSums{0, 0, {0, 0, 0}};
+        sum_w[sp] = 0;
+        for (int k = 0; k < nn[sp]; ++k) {
+          double vr = S.vx(sp, k), vphi = S.vy(sp, k), vzz = S.vz(sp, k), w = S.w(sp, k);
+          double v_sq = vr * vr + vphi * vphi + vzz * vzz;
+          double weighted_m = w * mass[sp];
+          t[sp].m[0] += weighted_m * vr;
+          t[sp].m[1] += weighted_m * vphi;
+          t[sp].m[2] += weighted_m * vzz;
+          t[sp].amount += w;
+          t[sp].energy += weighted_m * v_sq / 2;
+        }
+        for (int k = 0; k < nn[sp]; ++k) sum_w[sp] += S.w(sp, k);
+      }
+      cell_stats(g, c, t[0], t[1], sum_w[0], sum_w[1], tot, row_bad);
+    }
+    __syncthreads();
+  }
+}
+
+template <int SCHEME, class V>
+__global__ void __launch_bounds__(COLL_CROWD_BLOCK) k_coll_pairs_crowd(
+    Geom g, V ve, V vi, const uint32_t *off_e, const uint32_t *off_i, const uint32_t *perm_e, const uint32_t *perm_i,
+    double m_e, double m_i, double q_e, double q_i, uint64_t seed, uint64_t step, const double *tot,
+    const int *row_bad, const uint32_t *list, const uint32_t *count, const uint32_t *crowd_off, double *cstage,
+    StepState *st, const unsigned long long *gate) {
+  if (gate_closed(gate)) return;
+  const uint32_t nlist = *count;
+  const BlockTeam T;
+  int err = 0;
+  for (uint32_t li = blockIdx.x; li < nlist; li += gridDim.x) {
+    const long c = list[li];
+    const int nn[2] = {(int)(off_e[c + 1] - off_e[c]), (int)(off_i[c + 1] - off_i[c])};
+    const CrowdStage S{cstage + crowd_off[li], nn[0]};
+    crowd_stage_species(ve, perm_e, off_e[c], nn[0], 0, S);
+    crowd_stage_species(vi, perm_i, off_i[c], nn[1], 1, S);
+    __syncthreads();
+    const CellPos q = cell_pos(g, c);
+    const double *t = tot + c * TOT;
+    if (q.j < row_bad[(long)q.d * g.nr + q.i]) {
+      PairRunner<SCHEME, CrowdStage, BlockTeam> P{S, T, {m_e, m_i}, {q_e, q_i}, {t[10], t[11]}, t[12], g.dt, seed,
+                                                  step, (uint64_t)q.cr * (uint64_t)g.Nz + (uint64_t)q.cz, &err};
+      cell_pairs(P, nn[0], nn[1]);
+    }
+    if (SCHEME == COLL_TA77S) {
+      correct_species(S, T, 1, nn[1], m_i, t + 5);
+      correct_species(S, T, 0, nn[0], m_e, t + 0);
+    }
+    for (int k = threadIdx.x; k < nn[0]; k += COLL_CROWD_BLOCK) {
+      double v3[3] = {S.vx(0, k), S.vy(0, k), S.vz(0, k)};
+      ve.store(perm_e[off_e[c] + k], v3);
+    }
+    for (int k = threadIdx.x; k < nn[1]; k += COLL_CROWD_BLOCK) {
+      double v3[3] = {S.vx(1, k), S.vy(1, k), S.vz(1, k)};
+      vi.store(perm_i[off_i[c] + k], v3);
+    }
+    __syncthreads();
   }
   if (err) atomicOr(&st->err, err);
 }
@@ -1087,6 +1254,10 @@ struct CollBuffers {
   uint32_t *fill[2] = {nullptr, nullptr}, *perm[2] = {nullptr, nullptr};
   uint32_t *lists = nullptr, *lcount = nullptr;   // cells per size class
   double *lane_stage = nullptr;                   // per-lane FP64 stage of the tiny-cell pairs
+  uint32_t *crowd_off = nullptr;                  // crowded cells: range of each listed cell
+  uint64_t *crowd_key = nullptr;                  // shuffle keys, indices and FP64 stage of
+  uint32_t *crowd_idx = nullptr;                  // the crowded cells (COLL_CROWD_CAP particles)
+  double *crowd_stage = nullptr;
   long lane_blocks = 0;
 
   long cap[2] = {0, 0};
@@ -1115,6 +1286,7 @@ void coll_free(pcp_ctx *c) {
     (void)hipFree(b->perm[s]);
   }
   (void)hipFree(b->lists); (void)hipFree(b->lcount); (void)hipFree(b->lane_stage);
+  (void)hipFree(b->crowd_off); (void)hipFree(b->crowd_key); (void)hipFree(b->crowd_idx); (void)hipFree(b->crowd_stage);
   (void)hipFree(b->tot); (void)hipFree(b->row_bad); (void)hipFree(b->scan_tmp);
   (void)hipFree(b->mom); (void)hipFree(b->mom2); (void)hipFree(b->feat); (void)hipFree(b->pred);
   (void)hipFree(b->ok); (void)hipFree(b->fallback); (void)hipFree(b->counter);
@@ -1143,7 +1315,11 @@ static int coll_alloc(pcp_ctx *c, long cap_e, long cap_i) {
   // the lane-per-cell kernels: four workgroups per CU, COLL_LANE_SLOTS doubles per lane
   b->lane_blocks = (long)c->n_cu * 4;
   if (hipMalloc(&b->lists, (size_t)COLL_CLASSES * ncells * sizeof(uint32_t)) != hipSuccess ||
-      hipMalloc(&b->lcount, COLL_CLASSES * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&b->lcount, (COLL_CLASSES + 1) * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&b->crowd_off, COLL_CROWD_CELLS * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&b->crowd_key, COLL_CROWD_CAP * sizeof(uint64_t)) != hipSuccess ||
+      hipMalloc(&b->crowd_idx, COLL_CROWD_CAP * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&b->crowd_stage, 4 * COLL_CROWD_CAP * sizeof(double)) != hipSuccess ||
       hipMalloc(&b->lane_stage, (size_t)COLL_LANE_SLOTS * b->lane_blocks * COLL_LANE_BLOCK * sizeof(double)) !=
           hipSuccess)
     return -1;
@@ -1221,6 +1397,24 @@ static void coll_class(pcp_ctx *c, int scheme, V ve, V vi, double m_e, double m_
       hipLaunchKernelGGL((k_coll_pairs_lane<COLL_P12, V>), dim3(gl), dim3(COLL_LANE_BLOCK), 0, c->stream, g, ve,
                          vi, b->off[0], b->off[1], b->perm[0], b->perm[1], m_e, m_i, q_e, q_i, seed, step, b->tot,
                          b->row_bad, list, count, b->lane_stage, c->st, gate);
+  } else if constexpr (CLS == COLL_CLASSES - 1) {
+    const unsigned gw = (unsigned)(c->n_cu * 2);
+    if (!pairs)
+      hipLaunchKernelGGL((k_coll_prep_crowd<V>), dim3(gw), dim3(COLL_CROWD_BLOCK), 0, c->stream, g, ve, vi,
+                         b->off[0], b->off[1], b->perm[0], b->perm[1], m_e, m_i, seed, step, b->tot, b->row_bad,
+                         list, count, b->crowd_off, b->crowd_key, b->crowd_idx, b->crowd_stage, gate);
+    else if (scheme == COLL_TA77S)
+      hipLaunchKernelGGL((k_coll_pairs_crowd<COLL_TA77S, V>), dim3(gw), dim3(COLL_CROWD_BLOCK), 0, c->stream, g, ve,
+                         vi, b->off[0], b->off[1], b->perm[0], b->perm[1], m_e, m_i, q_e, q_i, seed, step, b->tot,
+                         b->row_bad, list, count, b->crowd_off, b->crowd_stage, c->st, gate);
+    else if (scheme == COLL_SK98)
+      hipLaunchKernelGGL((k_coll_pairs_crowd<COLL_SK98, V>), dim3(gw), dim3(COLL_CROWD_BLOCK), 0, c->stream, g, ve,
+                         vi, b->off[0], b->off[1], b->perm[0], b->perm[1], m_e, m_i, q_e, q_i, seed, step, b->tot,
+                         b->row_bad, list, count, b->crowd_off, b->crowd_stage, c->st, gate);
+    else
+      hipLaunchKernelGGL((k_coll_pairs_crowd<COLL_P12, V>), dim3(gw), dim3(COLL_CROWD_BLOCK), 0, c->stream, g, ve,
+                         vi, b->off[0], b->off[1], b->perm[0], b->perm[1], m_e, m_i, q_e, q_i, seed, step, b->tot,
+                         b->row_bad, list, count, b->crowd_off, b->crowd_stage, c->st, gate);
   } else {
   if (!pairs)
     hipLaunchKernelGGL((k_coll_prep<V, MAXC>), dim3(gc), dim3(COLL_BLOCK), 0, c->stream, g, ve, vi, b->off[0],
@@ -1253,16 +1447,18 @@ static void coll_binary(pcp_ctx *c, int scheme, V ve, V vi, double m_e, double m
   const long rows = (long)c->D * g.nr;
   hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)std::max<long>(1, std::min<long>((rows + 255) / 256, 1024))),
                      dim3(256), 0, c->stream, b->row_bad, rows, INT_MAX);
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, b->lcount, (long)COLL_CLASSES);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, b->lcount, (long)COLL_CLASSES + 1);
   const unsigned gz = (unsigned)std::max(1L, std::min<long>((ncells + 255) / 256, (long)c->n_cu * 16));
   hipLaunchKernelGGL(k_coll_classify, dim3(gz), dim3(256), 0, c->stream, g, b->off[0], b->off[1], mask, b->lists,
-                     b->lcount, b->row_bad, c->st, gate);
+                     b->lcount, b->crowd_off, b->row_bad, c->st, gate);
   coll_class<V, 0>(c, scheme, ve, vi, m_e, m_i, q_e, q_i, seed, step, false, gate);
   coll_class<V, 1>(c, scheme, ve, vi, m_e, m_i, q_e, q_i, seed, step, false, gate);
   coll_class<V, 2>(c, scheme, ve, vi, m_e, m_i, q_e, q_i, seed, step, false, gate);
+  coll_class<V, 3>(c, scheme, ve, vi, m_e, m_i, q_e, q_i, seed, step, false, gate);
   coll_class<V, 0>(c, scheme, ve, vi, m_e, m_i, q_e, q_i, seed, step, true, gate);
   coll_class<V, 1>(c, scheme, ve, vi, m_e, m_i, q_e, q_i, seed, step, true, gate);
   coll_class<V, 2>(c, scheme, ve, vi, m_e, m_i, q_e, q_i, seed, step, true, gate);
+  coll_class<V, 3>(c, scheme, ve, vi, m_e, m_i, q_e, q_i, seed, step, true, gate);
 }
 
 template <class V>

@@ -7,8 +7,8 @@
 namespace pcp {
 
 // largest number of macro-particles of one species in one cell that a
-// collision pass accepts (the cell is staged in LDS); more raises
-// ERR_COLL_OVERFLOW
+// collision pass stages in LDS; larger cells take the crowded path (a
+// global stage of 2^22 particles per pass; more raises ERR_COLL_OVERFLOW)
 constexpr int COLL_MAXC = 256;
 
 struct CollBuffers;

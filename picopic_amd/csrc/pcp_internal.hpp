@@ -41,7 +41,7 @@ enum : int {
   ERR_NAN_FIELD = 4,      // maxwellSolverYee.cpp:253-255
   ERR_MIGRANT_OVERFLOW = 8,
   ERR_MOVED_TOO_FAR = 16, // currentVB.cpp:199-204 (LOG_S(ERROR), not fatal)
-  ERR_COLL_OVERFLOW = 32, // a cell holds more macro-particles than a collision pass stages
+  ERR_COLL_OVERFLOW = 32, // crowded collision cells hold more macro-particles than a pass stages
 };
 
 // Geometry shared by every domain of the SMB (domains are equal).

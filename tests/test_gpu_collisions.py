@@ -143,16 +143,50 @@ def test_collision_pass_fp32(scheme):
     print(scheme, compare(orc, gpu, 1e-6))
 
 
-def test_collision_cell_overflow_is_reported():
+@pytest.mark.parametrize("scheme", ["p12", "ta77s", "sk98"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_crowded_cells_match_the_oracle(scheme, precision):
+    """Cells of more than 256 macro-particles of a species (the reference's
+    buckets have no size limit, src/collisions.cpp:87-108) take the crowded
+    path -- a workgroup per cell, keys sorted and velocities staged in
+    global memory -- next to cells of every other size class, and match the
+    oracle's pass at the tolerances above."""
+    rng = np.random.default_rng(35)
+    grid = (3, 4)
+    size = (grid[0] * 1e-4, grid[1] * 1e-4)
+    ce = rng.integers(2, 12, grid)
+    ci = rng.integers(2, 12, grid)
+    ce[0, 1], ci[0, 1] = 300, 257        # both crowded, nion < nel
+    ce[1, 2], ci[1, 2] = 40, 700         # ions crowded, nion > nel
+    ce[2, 3], ci[2, 3] = 1000, 1000      # nion == nel
+    ce[2, 0], ci[2, 0] = 100, 30         # a wavefront cell of the 256 class
+    fp32 = precision == "fp32"
+    data = plasma(rng, grid, size, ce, ci, weights="equal" if scheme != "ta77s" else "mixed", fp32=fp32)
+    orc, gpu = build(grid, (1, 1), data, precision)
+    print("collide crowded", scheme, precision, flush=True)
+    orc.collide(scheme, 0, 1, seed=91, step=2)
+    gpu.collide(scheme, 0, 1, 91, 2)
+    tol = 1e-6 if fp32 else (1e-12 if scheme == "ta77s" else 1e-8)
+    err = compare(orc, gpu, tol)
+    print(scheme, precision, "crowded max rel err", err)
+
+
+def test_crowded_stage_overflow_is_reported():
+    """A pass whose crowded cells hold more than 2^22 macro-particles in all
+    exceeds the global stage: the device reports it and nothing hangs."""
     rng = np.random.default_rng(33)
     grid = (2, 2)
     size = (grid[0] * 1e-4, grid[1] * 1e-4)
-    ce = np.array([[300, 4], [4, 4]])
+    ce = np.array([[2 ** 22 + 1, 4], [4, 4]])
     ci = np.array([[4, 4], [4, 4]])
     data = plasma(rng, grid, size, ce, ci)
-    _, gpu = build(grid, (1, 1), data, "fp64")
+    cap = max(len(d[0]) for d in data) + 16
+    species = [(-1, 1, False), (1, 1836, False)]
+    gpu = capi.Context(size, grid, (1, 1), 1e-13, [s + (cap,) for s in species])
+    for sp, d in enumerate(data):
+        gpu.upload(sp, *d)
     print("collide overflow", flush=True)
-    with pytest.raises(capi.PcpError, match="collision cell"):
+    with pytest.raises(capi.PcpError, match="crowded collision cells"):
         gpu.collide("p12", 0, 1, 1, 0)
 
 
