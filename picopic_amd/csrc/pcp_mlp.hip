@@ -1,8 +1,11 @@
 // BF16 MFMA kernels of the per-cell collision surrogate network.
 //
-// One wavefront owns a tile of 32 samples (two 16-row MFMA tiles) and runs
-// all three layers on it, staging activations in LDS between layers so that
-// each layer reads its A operand in the MFMA row layout.  The instruction is
+// Inference keeps the weights in registers as MFMA A operands and the
+// samples as the N dimension, so the activations pass from layer to layer
+// in registers (k_mlp_forward).  Training: one wavefront owns a tile of 32
+// samples (two 16-row MFMA tiles) and runs all three layers on it, staging
+// activations in LDS between layers so that each layer reads its A operand
+// in the MFMA row layout.  The instruction is
 // v_mfma_f32_16x16x32_bf16: per lane l, A holds row (l & 15), k = 8 (l >> 4)
 // + 0..7; B holds column (l & 15) with the same k; D holds column (l & 15),
 // rows 4 (l >> 4) + 0..3.  Biases enter through the FP32 accumulator.
@@ -164,22 +167,92 @@ __device__ __forceinline__ void load_x(MlpTile &T, const float *X, const long *i
   __syncthreads();
 }
 
-__global__ void __launch_bounds__(WAVE) k_mlp_forward(const float *P, const float *X, long n, float *Y) {
-  __shared__ MlpTile T;
-  const int l = threadIdx.x;
-  for (long base = (long)blockIdx.x * MLP_TILE; base < n; base += (long)gridDim.x * MLP_TILE) {
-    load_x(T, X, nullptr, base, n);
-    f32x4 y[2];
-    tile_forward(P, T, y);
-    const int o = l & 15;
+// Inference, weights resident: each wavefront loads the three weight
+// matrices once, as MFMA A operands (rows = a layer's outputs), and runs
+// tiles of 16 samples as the N dimension, Z^T = W X^T + b.  A layer's D tiles
+// (lane l: rows 4 g + 0..3 of each 16-row tile, g = l >> 4, column = sample
+// l & 15) are the next layer's B operand without leaving the registers: K
+// position 8 g + i of K-step ks stands for hidden unit hperm(ks, g, i), and
+// the next layer's weight columns are loaded in that order.
+__device__ __forceinline__ int hperm(int ks, int g, int i) { return 16 * (2 * ks + (i >> 2)) + 4 * g + (i & 3); }
+
+// tanh(x) = 1 - 2 / (exp(2 x) + 1) with the hardware exponential and
+// reciprocal: exact limits, absolute error about 1e-7 (below the BF16
+// rounding the next layer applies)
+__device__ __forceinline__ float tanh_fast(float x) {
+  return 1.f - 2.f * __builtin_amdgcn_rcpf(__expf(2.f * x) + 1.f);
+}
+
+constexpr int MLP_FWD_BLOCK = 256;
+
+__global__ void __launch_bounds__(MLP_FWD_BLOCK) k_mlp_forward(const float *P, const float *X, long n, float *Y) {
+  const int l = threadIdx.x & 63, g = l >> 4, r = l & 15;
+  const float *W1 = P + MLP_OFF_W1, *b1 = P + MLP_OFF_B1;
+  const float *W2 = P + MLP_OFF_W2, *b2 = P + MLP_OFF_B2;
+  const float *W3 = P + MLP_OFF_W3, *b3 = P + MLP_OFF_B3;
+  bf16x8 a1[4], a2[4][2], a3[2];
+  f32x4 c1[4], c2[4], c3;
 #pragma unroll
-    for (int mt = 0; mt < 2; ++mt)
+  for (int mt = 0; mt < 4; ++mt) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        long s = base + 16 * mt + 4 * (l >> 4) + i;
-        if (o < MLP_OUT && s < n) Y[s * MLP_YSTRIDE + o] = y[mt][i];
-      }
-    __syncthreads();
+    for (int i = 0; i < 8; ++i) a1[mt][i] = (__bf16)W1[(16 * mt + r) * MLP_INP + 8 * g + i];
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+      for (int i = 0; i < 8; ++i) a2[mt][ks][i] = (__bf16)W2[(16 * mt + r) * MLP_HID + hperm(ks, g, i)];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      c1[mt][i] = b1[16 * mt + 4 * g + i];
+      c2[mt][i] = b2[16 * mt + 4 * g + i];
+    }
+  }
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a3[ks][i] = (__bf16)W3[r * MLP_HID + hperm(ks, g, i)];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) c3[i] = b3[4 * g + i];
+
+  const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long nwaves = ((long)gridDim.x * blockDim.x) >> 6;
+  for (long t0 = wave * 16; t0 < n; t0 += nwaves * 16) {
+    const long s = t0 + r;
+    // X^T: lanes of group 0 hold the sample's inputs (padded with zeros)
+    bf16x8 xb;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      float v = 0.f;
+      if (g == 0 && i < MLP_IN && s < n) v = X[s * MLP_XSTRIDE + i];
+      xb[i] = (__bf16)v;
+    }
+    f32x4 h[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      h[mt] = mfma(a1[mt], xb, c1[mt]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) h[mt][i] = tanh_fast(h[mt][i]);
+    }
+    bf16x8 hb[2];
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+      for (int i = 0; i < 8; ++i) hb[ks][i] = (__bf16)h[2 * ks + (i >> 2)][i & 3];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      h[mt] = mfma(a2[mt][1], hb[1], mfma(a2[mt][0], hb[0], c2[mt]));
+#pragma unroll
+      for (int i = 0; i < 4; ++i) h[mt][i] = tanh_fast(h[mt][i]);
+    }
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+      for (int i = 0; i < 8; ++i) hb[ks][i] = (__bf16)h[2 * ks + (i >> 2)][i & 3];
+    const f32x4 y = mfma(a3[1], hb[1], mfma(a3[0], hb[0], c3));
+    // outputs 0..3 are rows 0..3: lane group 0
+    if (g == 0 && s < n) {
+#pragma unroll
+      for (int i = 0; i < MLP_OUT; ++i) Y[s * MLP_YSTRIDE + i] = y[i];
+    }
   }
 }
 
@@ -398,9 +471,12 @@ void mlp_free(pcp_ctx *c) {
 
 void mlp_forward(pcp_ctx *c, const float *X, long n, float *Y) {
   if (n <= 0) return;
-  long tiles = (n + MLP_TILE - 1) / MLP_TILE;
-  unsigned grid = (unsigned)std::min<long>(tiles, (long)c->n_cu * 8);
-  hipLaunchKernelGGL(k_mlp_forward, dim3(grid), dim3(WAVE), 0, c->stream, c->mlp->params, X, n, Y);
+  // 16 samples per wavefront trip; enough wavefronts to fill every SIMD
+  // several times over, each loading the weights once
+  const long waves = (n + 15) / 16;
+  const long per_block = MLP_FWD_BLOCK / WAVE;
+  unsigned grid = (unsigned)std::max<long>(1, std::min<long>((waves + per_block - 1) / per_block, (long)c->n_cu * 16));
+  hipLaunchKernelGGL(k_mlp_forward, dim3(grid), dim3(MLP_FWD_BLOCK), 0, c->stream, c->mlp->params, X, n, Y);
 }
 
 void mlp_grad_idx(pcp_ctx *c, const float *X, const float *T, const long *idx, long n, float *loss_dev) {
