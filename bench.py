@@ -120,14 +120,31 @@ def run_gpu(w, steps, warmup, prec, tune, net=None):
         tuning = ctx.autotune(steps=2) if tune else ctx.tuning()
     ctx.step(warmup)
     L = capi.load()
+    # the timed steps replay captured step graphs (one graph launch per step
+    # after the sorts); the warm-up above ran eagerly, two more steps
+    # capture the graphs of the buffer states the timed window meets first
+    ctx.set_graph(True)
+    ctx.step(2)
+    launches0, captures0 = ctx.graph_stats()
     L.pcp_set_timing(ctx.h, 1)
     ctx.sync()
     t0 = time.perf_counter()
     ctx.step(steps)
     ctx.sync()
     wall = time.perf_counter() - t0
+    launches1, captures1 = ctx.graph_stats()
+    step_ms_graph = L.pcp_kernel_ms(ctx.h, b"step")
+    # the phases (and the particle launch of the roofline figure) from HIP
+    # events around eager launches of the same kernels in the steps that
+    # follow
+    ctx.set_graph(False)
+    eager_steps = min(steps, 20)
+    ctx.step(eager_steps)
     phases = {k: L.pcp_kernel_ms(ctx.h, k.encode())
               for k in ("fields", "collide", "particles", "fixup", "stitch_j", "step")}
+    graph = {"launches_per_step": (launches1 - launches0) / steps, "captures_in_window": captures1 - captures0,
+             "step_ms_events": step_ms_graph, "eager_steps_for_phases": eager_steps,
+             "eager_ms_per_step": phases["step"]}
     n_end = sum(ctx.count(s) for s in range(len(species)))
     log = ctx.energy_log(steps)
     energies = {"step": int(log["step"][-1]), "field_J": float(log["field"][-1]),
@@ -137,7 +154,7 @@ def run_gpu(w, steps, warmup, prec, tune, net=None):
     ctx.close()
     return dict(n_particles=n_part, n_particles_end=n_end, wall=wall, phases=phases, cells=grid[0] * grid[1],
                 copy_gbs=copy_gbs, n_species=len(species), tuning=tuning, collisions=scheme, energies=energies,
-                fallback_cells=fallback)
+                fallback_cells=fallback, graph=graph)
 
 
 def host_cpu():
@@ -324,7 +341,7 @@ def main():
     config = {"workload": w["name"], "particles": n, "cells": g["cells"], "precision": main_prec,
               "deposition": w["ccs"], "pusher": w["pusher"], "domains": list(w["domains"]),
               "collisions": g["collisions"], "phases_ms": g["phases"], "tuning": g["tuning"],
-              "energies": g["energies"]}
+              "energies": g["energies"], "graph": g["graph"]}
     if g["fallback_cells"] is not None:
         config["surrogate"] = {"fallback_cells_last_step": g["fallback_cells"], "fit_loss": net["loss"],
                                "train_seconds": net["seconds"]}
@@ -333,7 +350,7 @@ def main():
         v64 = g64["n_particles"] * args.steps / g64["wall"]
         config["fp64_plain"] = {"value": v64, "ms_per_step": g64["wall"] * 1e3 / args.steps,
                                 "collisions": g64["collisions"], "phases_ms": g64["phases"],
-                                "energies": g64["energies"],
+                                "energies": g64["energies"], "graph": g64["graph"],
                                 "achieved_gbs": 88 * g64["n_particles"] / (g64["phases"]["particles"] * 1e-3) / 1e9}
         config["speedup_fp32_vs_fp64"] = value / v64
     line = {

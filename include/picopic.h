@@ -283,11 +283,24 @@ int pcp_surrogate_fallback_cells(pcp_ctx *ctx, long *cells);
  * The plasma must keep T_par > T_perp for the electrons over the passes. */
 int pcp_surrogate_calibrate(pcp_ctx *ctx, int sp_el, int sp_ion, uint64_t seed, int passes, double *factor);
 
-/* Replay each step from a captured hipGraph (no reference counterpart).
- * Steps with a collision phase, timed steps (pcp_set_timing) and the sorts
- * run as ordinary launches; graphs are recaptured after anything that
- * changes the particle buffers (upload, init, sort). */
+/* Replay each step from a captured hipGraph (no reference counterpart):
+ * one graph launch per step holds the field update, the push, the
+ * collision phase (its pass number lives on the device and advances inside
+ * the graph), the move and deposit, the migration keys, the fix-up and the
+ * current overlay.  The beam injection and the sorts run before it as
+ * ordinary launches.  Graphs are cached by buffer state (the particle
+ * buffers, counts and buffer parity), so the sorts' buffer exchanges cycle
+ * through a few captured graphs; uploads, initialisation, tuning and
+ * collision or network changes drop them.  With pcp_set_timing, a step's
+ * time is its sorts plus its graph launch, and the phases inside the graph
+ * read as zero. */
 int pcp_set_graph(pcp_ctx *ctx, int on);
+/* graph launches and captures since the context was created (no reference
+ * counterpart) */
+int pcp_graph_stats(pcp_ctx *ctx, long *launches, long *captures);
+/* collision passes the step has run since pcp_set_collisions: the device
+ * counter that numbers the passes' draws (no reference counterpart) */
+int pcp_collision_passes(pcp_ctx *ctx, uint64_t *passes);
 
 /* Counting sort of species sp by (domain, sort bin, cell), dropping removed particles
  * (no reference counterpart: the reference keeps per-domain vectors).  The

@@ -30,7 +30,7 @@ EXPORTS = [
     "pcp_collide", "pcp_set_collisions", "pcp_mlp_nparams", "pcp_mlp_init", "pcp_mlp_get_params",
     "pcp_mlp_set_params", "pcp_mlp_forward", "pcp_mlp_grad", "pcp_mlp_adam", "pcp_surrogate_collect",
     "pcp_surrogate_fit", "pcp_surrogate_get_norm", "pcp_surrogate_set_norm", "pcp_surrogate_fallback_cells",
-    "pcp_species_moments", "pcp_continuity_residual", "pcp_set_graph", "pcp_set_sort_bin", "pcp_set_launch",
+    "pcp_species_moments", "pcp_continuity_residual", "pcp_set_graph", "pcp_graph_stats", "pcp_collision_passes", "pcp_set_sort_bin", "pcp_set_launch",
     "pcp_get_tuning", "pcp_autotune", "pcp_surrogate_calibrate", "pcp_init_reference",
     "pcp_set_beam", "pcp_inject", "pcp_set_time", "pcp_time", "pcp_probe", "pcp_species_map",
 ]
@@ -131,6 +131,8 @@ def load(path=LIB_PATH):
     L.pcp_species_moments.argtypes = [vp, C.c_int, dp]
     L.pcp_continuity_residual.argtypes = [vp, dp]
     L.pcp_set_graph.argtypes = [vp, C.c_int]
+    L.pcp_graph_stats.argtypes = [vp, C.POINTER(C.c_long), C.POINTER(C.c_long)]
+    L.pcp_collision_passes.argtypes = [vp, C.POINTER(C.c_uint64)]
     L.pcp_set_sort_bin.argtypes = [vp, C.c_int, C.c_int]
     L.pcp_set_launch.argtypes = [vp, C.c_int, C.c_int]
     L.pcp_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
@@ -338,6 +340,18 @@ class Context:
 
     def set_graph(self, on=True):
         _check(_lib.pcp_set_graph(self.h, 1 if on else 0))
+
+    def collision_passes(self):
+        """Collision passes the step has run since set_collisions (device counter)."""
+        n = C.c_uint64()
+        _check(_lib.pcp_collision_passes(self.h, C.byref(n)))
+        return n.value
+
+    def graph_stats(self):
+        """(graph launches, graph captures) since the context was created."""
+        a, b = C.c_long(), C.c_long()
+        _check(_lib.pcp_graph_stats(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def continuity_residual(self):
         """(max |residual|, max |charge change|, max |q w|) of the last step."""

@@ -263,7 +263,7 @@ int pcp_create(const pcp_config *cfg, pcp_ctx **out) {
   int rc = 0;
   if ((rc = dalloc(&c->f.E, gridn)) || (rc = dalloc(&c->f.H, gridn)) || (rc = dalloc(&c->f.Hat, gridn)) ||
       (rc = dalloc(&c->f.J, gridn)) || (rc = dalloc(&c->f.eps, gp)) || (rc = dalloc(&c->f.sigma, gp)) ||
-      (rc = dalloc(&c->st, 1)) || (rc = dalloc(&c->diag, DIAG_DOUBLES)) ||
+      (rc = dalloc(&c->st, 1)) || (rc = dalloc(&c->diag, DIAG_DOUBLES)) || (rc = dalloc(&c->coll_pass, 1)) ||
       (rc = dalloc(&c->marker, (size_t)c->D * g.P)) ||
       (rc = dalloc(&c->cell_flag, (size_t)g.Nr * g.Nz))) {
     pcp_destroy(c);
@@ -377,7 +377,7 @@ int pcp_destroy(pcp_ctx *c) {
   mlp_free(c);
   hipFree(c->f.E); hipFree(c->f.H); hipFree(c->f.Hat); hipFree(c->f.J);
   hipFree(c->f.eps); hipFree(c->f.sigma); hipFree(c->st); hipFree(c->marker); hipFree(c->cell_flag);
-  hipFree(c->diag); hipFree(c->ehist); hipFree(c->probe_buf);
+  hipFree(c->diag); hipFree(c->ehist); hipFree(c->probe_buf); hipFree(c->coll_pass);
   hipFree(c->map_n); hipFree(c->map_pw); hipFree(c->map_out);
   for (int s = 0; s < MAX_SPECIES; ++s) hipFree(c->map_pabs[s]);
   for (int s = 0; s < MAX_SPECIES; ++s) {
@@ -685,6 +685,7 @@ int pcp_init_bunch(pcp_ctx *c, int sp, long n, double n_peak, double sigma_r, do
 static int grow_migrants(pcp_ctx *c, long need) {
   Migrants &m = c->mig;
   if (need <= m.cap) return 0;
+  graph_invalidate(c);   // captured steps hold the old buffers
   hipFree(m.key); hipFree(m.ref); hipFree(m.group); hipFree(m.bucket); hipFree(m.skey);
   m.key = nullptr; m.ref = nullptr; m.group = nullptr; m.bucket = nullptr; m.skey = nullptr;
   m.cap = need;
@@ -1008,7 +1009,8 @@ static int check_coll_args(pcp_ctx *c, int scheme, int se, int si) {
 
 int pcp_collide(pcp_ctx *c, int scheme, int se, int si, uint64_t seed, uint64_t pass) {
   if (int rc = check_coll_args(c, scheme, se, si)) return rc;
-  if (launch_collide(c, scheme, se, si, seed, pass, c->cur) < 0) return fail(PCP_E_HIP, "collision pass failed");
+  if (launch_collide(c, scheme, se, si, seed, PassNo{nullptr, pass}, c->cur) < 0)
+    return fail(PCP_E_HIP, "collision pass failed");
   HIP_TRY(hipGetLastError());
   return pcp_sync(c);
 }
@@ -1022,6 +1024,7 @@ static int need_mlp(pcp_ctx *c) {
 int pcp_mlp_nparams(void) { return MLP_NPARAM; }
 
 int pcp_mlp_init(pcp_ctx *c, uint64_t seed) {
+  if (c) graph_invalidate(c);   // captured steps hold the normalisation by value
   if (int rc = need_mlp(c)) return rc;
   if (mlp_init(c, seed) < 0) return fail(PCP_E_HIP, "network initialisation");
   c->mlp->trained = 0;
@@ -1037,6 +1040,7 @@ int pcp_mlp_get_params(pcp_ctx *c, float *host) {
 }
 
 int pcp_mlp_set_params(pcp_ctx *c, const float *host) {
+  if (c) graph_invalidate(c);   // captured steps hold the normalisation by value
   if (int rc = need_mlp(c)) return rc;
   if (!host) return fail(PCP_E_ARG, "null buffer");
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1104,6 +1108,7 @@ int pcp_surrogate_collect(pcp_ctx *c, int se, int si, uint64_t seed, uint64_t pa
 }
 
 int pcp_surrogate_fit(pcp_ctx *c, int epochs, long batch, float lr, uint64_t seed, float *loss) {
+  if (c) graph_invalidate(c);   // captured steps hold the normalisation by value
   if (int rc = need_mlp(c)) return rc;
   if (epochs < 1 || batch < 1) return fail(PCP_E_ARG, "epochs and batch must be positive");
   int rc = mlp_fit(c, epochs, batch, lr, seed, loss);
@@ -1122,6 +1127,7 @@ int pcp_surrogate_get_norm(pcp_ctx *c, float *out) {
 }
 
 int pcp_surrogate_set_norm(pcp_ctx *c, const float *in) {
+  if (c) graph_invalidate(c);   // captured steps hold the normalisation by value
   if (int rc = need_mlp(c)) return rc;
   MlpState *m = c->mlp;
   for (int k = 0; k < MLP_IN; ++k) {
@@ -1189,6 +1195,7 @@ static int scratch_rate(pcp_ctx *c, const SpeciesCopy cp[2], int scheme, int se,
 }
 
 int pcp_surrogate_calibrate(pcp_ctx *c, int se, int si, uint64_t seed, int passes, double *factor) {
+  if (c) graph_invalidate(c);   // captured steps hold the normalisation by value
   if (int rc = check_coll_args(c, PCP_COLL_SURROGATE, se, si)) return rc;
   if (passes < 2) return fail(PCP_E_ARG, "calibration needs at least 2 passes");
   // copies of the two species as they are now
@@ -1233,18 +1240,23 @@ int pcp_surrogate_fallback_cells(pcp_ctx *c, long *cells) {
 
 int pcp_set_collisions(pcp_ctx *c, int scheme, int se, int si, uint64_t seed) {
   if (!c) return fail(PCP_E_ARG, "null context");
+  graph_invalidate(c);
   if (scheme < 0) {
     c->coll_scheme = -1;
     return 0;
   }
   if (int rc = check_coll_args(c, scheme, se, si)) return rc;
+  // the phase's buffers exist before any step is captured
+  if (coll_prepare(c, se, si) < 0) return fail(PCP_E_NOMEM, "collision buffers");
   if (scheme == COLL_SURROGATE && c->s32 && fsurr_prepare(c, se, si) < 0)
     return fail(PCP_E_NOMEM, "surrogate collision buffers");
   c->coll_scheme = scheme;
   c->coll_se = se;
   c->coll_si = si;
   c->coll_seed = seed;
-  c->coll_calls = 0;
+  // passes count from 0 on the device
+  launch_set_u64(c, c->coll_pass, 0);
+  HIP_TRY(hipGetLastError());
   return 0;
 }
 
@@ -1279,16 +1291,19 @@ static int step_body(pcp_ctx *c, Mark mark) {
     double *mom = fsurr_mom(c);
     for (int s = 0; s < c->cfg.n_species; ++s)
       launch32_push_mom(c, s, mom, s == c->coll_se ? 0 : s == c->coll_si ? 1 : -1);
-    if (fsurr_cells(c, c->coll_se, c->coll_si, c->coll_seed, c->coll_calls++) < 0)
+    if (fsurr_cells(c, c->coll_se, c->coll_si, c->coll_seed, PassNo{c->coll_pass, 0}) < 0)
       return fail(PCP_E_HIP, "surrogate collision phase failed");
+    launch_inc_u64(c, c->coll_pass);
   } else if (coll) {
     // SMB::advance_particles: push, collide (before the move), move
     for (int s = 0; s < c->cfg.n_species; ++s) {
       if (f32) launch32_push(c, s);
       else launch_push(c, s);
     }
-    if (launch_collide(c, c->coll_scheme, c->coll_se, c->coll_si, c->coll_seed, c->coll_calls++, c->cur ^ 1) < 0)
+    if (launch_collide(c, c->coll_scheme, c->coll_se, c->coll_si, c->coll_seed, PassNo{c->coll_pass, 0},
+                       c->cur ^ 1) < 0)
       return fail(PCP_E_HIP, "collision pass failed");
+    launch_inc_u64(c, c->coll_pass);
   }
   mark(2);
   if (f32) launch32_step_batch(c, !coll, cpar, c->coll_se, c->coll_si);
@@ -1317,14 +1332,102 @@ static int step_body(pcp_ctx *c, Mark mark) {
 }
 
 static void graph_invalidate(pcp_ctx *c) {
-  for (auto &e : c->gexec)
-    if (e) {
-      (void)hipGraphExecDestroy(e);
-      e = nullptr;
+  for (auto &e : c->graphs)
+    if (e.exec) (void)hipGraphExecDestroy(e.exec);
+  c->graphs.clear();
+}
+
+constexpr size_t GRAPH_CACHE = 8;
+
+// what a captured step depends on besides the device state: `cur`, and
+// each species' count and buffer pointers (exchanged by the sorts and by
+// the velocity-slot exchange of collision steps)
+static std::vector<uintptr_t> step_key(const pcp_ctx *c) {
+  std::vector<uintptr_t> k;
+  k.push_back((uintptr_t)c->cur);
+  for (int s = 0; s < c->cfg.n_species; ++s) {
+    if (c->s32) {
+      const Species32 &q = c->s32->sp[s];
+      const void *p[] = {q.cell[0], q.cell[1], q.fr[0], q.fr[1], q.fz[0], q.fz[1], q.vr[0], q.vr[1],
+                         q.vphi[0], q.vphi[1], q.vz[0], q.vz[1], q.w, q.seq, q.id};
+      k.push_back((uintptr_t)q.n);
+      for (const void *x : p) k.push_back((uintptr_t)x);
+    } else {
+      const SpeciesDev &q = c->sp[s];
+      const void *p[] = {q.r[0], q.r[1], q.z[0], q.z[1], q.vr, q.vphi, q.vz, q.w, q.seq, q.id};
+      k.push_back((uintptr_t)q.n);
+      for (const void *x : p) k.push_back((uintptr_t)x);
     }
+  }
+  return k;
+}
+
+// the host-side effects of step_body, which a graph replay does not run:
+// the velocity-slot exchange of FP32 collision steps and the flip of `cur`
+static void step_host_effects(pcp_ctx *c) {
+  if (c->s32 && c->coll_scheme >= 0)
+    for (int s = 0; s < c->cfg.n_species; ++s) {
+      Species32 &q = c->s32->sp[s];
+      std::swap(q.vr[0], q.vr[1]);
+      std::swap(q.vphi[0], q.vphi[1]);
+      std::swap(q.vz[0], q.vz[1]);
+    }
+  c->cur ^= 1;
+}
+
+// the executable graph of the current state: a cached one, or a new capture
+// of step_body (the host state it changes is restored, the caller applies
+// step_host_effects after launching)
+static int step_graph(pcp_ctx *c, hipGraphExec_t *out) {
+  std::vector<uintptr_t> key = step_key(c);
+  for (auto &e : c->graphs)
+    if (e.key == key) {
+      e.used = ++c->graph_tick;
+      *out = e.exec;
+      return 0;
+    }
+  const int cur = c->cur;
+  std::vector<Species32> saved32;
+  if (c->s32) saved32.assign(c->s32->sp, c->s32->sp + c->cfg.n_species);
+  hipGraph_t graph;
+  HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+  int rc = step_body(c, [](int) {});
+  hipError_t e = hipStreamEndCapture(c->stream, &graph);
+  c->cur = cur;   // nothing ran during the capture
+  if (c->s32) std::copy(saved32.begin(), saved32.end(), c->s32->sp);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(PCP_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(PCP_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+  if (c->graphs.size() >= GRAPH_CACHE) {
+    auto lru = std::min_element(c->graphs.begin(), c->graphs.end(),
+                                [](const pcp_ctx::StepGraph &a, const pcp_ctx::StepGraph &b) { return a.used < b.used; });
+    (void)hipGraphExecDestroy(lru->exec);
+    c->graphs.erase(lru);
+  }
+  c->graphs.push_back(pcp_ctx::StepGraph{std::move(key), exec, ++c->graph_tick});
+  ++c->graph_captures;
+  *out = exec;
+  return 0;
 }
 
 extern "C" {
+
+int pcp_collision_passes(pcp_ctx *c, uint64_t *passes) {
+  if (!c || !passes) return fail(PCP_E_ARG, "bad argument");
+  HIP_TRY(hipMemcpyAsync(passes, c->coll_pass, sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int pcp_graph_stats(pcp_ctx *c, long *launches, long *captures) {
+  if (!c) return fail(PCP_E_ARG, "null context");
+  if (launches) *launches = c->graph_launches;
+  if (captures) *captures = c->graph_captures;
+  return 0;
+}
 
 int pcp_set_graph(pcp_ctx *c, int on) {
   if (!c) return fail(PCP_E_ARG, "null context");
@@ -1405,33 +1508,23 @@ int pcp_step(pcp_ctx *c, int n_steps) {
     // SMB::inject_beam first (src/PiCoPiC.cpp:320), outside any graph
     if (long ni = inject_beams(c); ni < 0) return (int)ni;
     mark(0);
-    // the sorts change the particle buffers, so they run outside any graph
+    // the sorts exchange particle buffers; they run outside the step graph,
+    // whose cache is keyed by the buffers
     for (int s = 0; s < c->cfg.n_species; ++s)
-      if (c->sort_every[s] > 0 && c->steps_done % c->sort_every[s] == 0) {
+      if (c->sort_every[s] > 0 && c->steps_done % c->sort_every[s] == 0)
         if ((f32 ? launch32_sort(c, s) : launch_sort(c, s)) < 0) return fail(PCP_E_HIP, "sort failed");
-        graph_invalidate(c);
-      }
-    const bool coll = c->coll_scheme >= 0;
-    if (c->graph_on && !c->timing && !coll) {
-      // one graph per parity of `cur` (the kernels' buffer arguments differ)
-      const int par = c->cur;
-      if (!c->gexec[par]) {
-        hipGraph_t graph;
-        HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
-        int rc = step_body(c, [](int) {});
-        hipError_t e = hipStreamEndCapture(c->stream, &graph);
-        c->cur = par;   // nothing ran during the capture
-        if (rc) return rc;
-        if (e != hipSuccess) return fail(PCP_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-        HIP_TRY(hipGraphInstantiate(&c->gexec[par], graph, nullptr, nullptr, 0));
-        (void)hipGraphDestroy(graph);
-      }
-      HIP_TRY(hipGraphLaunch(c->gexec[par], c->stream));
-      c->cur ^= 1;
+    if (c->graph_on) {
+      // one graph launch for the step; with timing on, events 0 and 5
+      // bracket the sorts and the launch, and the phases inside the graph
+      // are not separated (they read as zero)
+      hipGraphExec_t exec = nullptr;
+      if (int rc = step_graph(c, &exec)) return rc;
+      HIP_TRY(hipGraphLaunch(exec, c->stream));
+      mark(5);
+      ++c->graph_launches;
+      step_host_effects(c);
     } else {
       if (int rc = step_body(c, mark)) return rc;
-      // an FP32 step with collisions exchanges velocity buffers
-      if (coll && f32) graph_invalidate(c);
     }
     ++c->steps_done;
     c->t_now += c->g.dt;
@@ -1439,19 +1532,22 @@ int pcp_step(pcp_ctx *c, int n_steps) {
   HIP_TRY(hipGetLastError());
   int rc = pcp_sync(c);
   if (c->timing && n_steps > 0) {
-    double acc[5] = {0, 0, 0, 0, 0};
-    for (int t = 0; t < n_steps; ++t)
+    double acc[5] = {0, 0, 0, 0, 0}, tot = 0;
+    for (int t = 0; t < n_steps; ++t) {
+      float ms = 0;
+      if (c->graph_on) {
+        (void)hipEventElapsedTime(&ms, ev[(size_t)t * 6], ev[(size_t)t * 6 + 5]);
+        tot += ms;
+        continue;
+      }
       for (int k = 0; k < 5; ++k) {
-        float ms = 0;
         (void)hipEventElapsedTime(&ms, ev[(size_t)t * 6 + k], ev[(size_t)t * 6 + k + 1]);
         acc[k] += ms;
+        tot += ms;
       }
-    c->ktimes.clear();
-    double tot = 0;
-    for (int k = 0; k < 5; ++k) {
-      c->ktimes.emplace_back(names[k], acc[k] / n_steps);
-      tot += acc[k];
     }
+    c->ktimes.clear();
+    for (int k = 0; k < 5; ++k) c->ktimes.emplace_back(names[k], acc[k] / n_steps);
     c->ktimes.emplace_back("step", tot / n_steps);
   }
   return rc;

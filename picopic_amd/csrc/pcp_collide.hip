@@ -72,7 +72,7 @@ struct View32 {
   __device__ __forceinline__ bool cell_of(const Geom &g, long p, int &cr, int &cz) const {
     uint32_t c = s.cell[cur][p];
     if (c == DEAD_CELL) return false;
-    cr = (int)(c / (uint32_t)g.Nz);
+    cr = (int)g.div_Nz.div(c);
     cz = (int)(c - (uint32_t)cr * (uint32_t)g.Nz);
     return true;
   }
@@ -87,10 +87,12 @@ struct View32 {
   __device__ __forceinline__ long n() const { return s.n; }
 };
 
+// domain-major key of global cell (cr, cz): domain d = I Dz + J, then the
+// cell's row-major index inside the domain (multiply-high divisions)
 __device__ __forceinline__ uint32_t local_cell_key(const Geom &g, int cr, int cz) {
-  int d = (cr / g.nr) * g.Dz + cz / g.nz;
-  long lr = cr - (long)(d / g.Dz) * g.nr, lz = cz - (long)(d % g.Dz) * g.nz;
-  return (uint32_t)((long)d * g.nr * g.nz + lr * g.nz + lz);
+  const int I = (int)g.div_nr.div((uint32_t)cr), J = (int)g.div_nz.div((uint32_t)cz);
+  const int lr = cr - I * g.nr, lz = cz - J * g.nz;
+  return (uint32_t)(((long)(I * g.Dz + J) * g.nr + lr) * g.nz + lz);
 }
 
 // gate (optional): a device counter; the pass does nothing while it is 0.
@@ -345,10 +347,11 @@ __device__ __forceinline__ void cell_stats(const Geom &g, long c, const CellSums
 template <class V, int MAXC>
 __global__ void __launch_bounds__(COLL_BLOCK) k_coll_prep(Geom g, V ve, V vi, const uint32_t *off_e,
                                                           const uint32_t *off_i, uint32_t *perm_e, uint32_t *perm_i,
-                                                          double m_e, double m_i, uint64_t seed, uint64_t step,
+                                                          double m_e, double m_i, uint64_t seed, PassNo pass,
                                                           double *tot, int *row_bad, const uint32_t *list,
                                                           const uint32_t *count, StepState *st,
                                                           const unsigned long long *gate) {
+  const uint64_t step = pass.get();
   __shared__ uint64_t skey[MAXC];
   __shared__ CellStageT<MAXC> S;
   if (gate_closed(gate)) return;
@@ -450,10 +453,11 @@ template <class V>
 __global__ void __launch_bounds__(COLL_LANE_BLOCK) k_coll_prep_lane(Geom g, V ve, V vi, const uint32_t *off_e,
                                                                     const uint32_t *off_i, uint32_t *perm_e,
                                                                     uint32_t *perm_i, double m_e, double m_i,
-                                                                    uint64_t seed, uint64_t step, double *tot,
+                                                                    uint64_t seed, PassNo pass, double *tot,
                                                                     int *row_bad, const uint32_t *list,
                                                                     const uint32_t *count,
                                                                     const unsigned long long *gate) {
+  const uint64_t step = pass.get();
   if (gate_closed(gate)) return;
   const uint32_t nlist = *count;
   for (uint32_t li = blockIdx.x * blockDim.x + threadIdx.x; li < nlist; li += gridDim.x * blockDim.x) {
@@ -701,10 +705,11 @@ template <int SCHEME, class V, int MAXC>
 __global__ void __launch_bounds__(COLL_BLOCK) k_coll_pairs(Geom g, V ve, V vi, const uint32_t *off_e,
                                                            const uint32_t *off_i, const uint32_t *perm_e,
                                                            const uint32_t *perm_i, double m_e, double m_i,
-                                                           double q_e, double q_i, uint64_t seed, uint64_t step,
+                                                           double q_e, double q_i, uint64_t seed, PassNo pass,
                                                            const double *tot, const int *row_bad,
                                                            const uint32_t *list, const uint32_t *count,
                                                            StepState *st, const unsigned long long *gate) {
+  const uint64_t step = pass.get();
   __shared__ CellStageT<MAXC> S;
   if (gate_closed(gate)) return;
   const uint32_t nlist = *count;
@@ -749,9 +754,10 @@ __global__ void __launch_bounds__(COLL_BLOCK) k_coll_pairs(Geom g, V ve, V vi, c
 template <int SCHEME, class V>
 __global__ void __launch_bounds__(COLL_LANE_BLOCK) k_coll_pairs_lane(
     Geom g, V ve, V vi, const uint32_t *off_e, const uint32_t *off_i, const uint32_t *perm_e, const uint32_t *perm_i,
-    double m_e, double m_i, double q_e, double q_i, uint64_t seed, uint64_t step, const double *tot,
+    double m_e, double m_i, double q_e, double q_i, uint64_t seed, PassNo pass, const double *tot,
     const int *row_bad, const uint32_t *list, const uint32_t *count, double *stage, StepState *st,
     const unsigned long long *gate) {
+  const uint64_t step = pass.get();
   if (gate_closed(gate)) return;
   const uint32_t nlist = *count;
   const long lanes = (long)gridDim.x * blockDim.x;
@@ -857,8 +863,9 @@ __device__ __forceinline__ void crowd_stage_species(const V &v, const uint32_t *
 template <class V>
 __global__ void __launch_bounds__(COLL_CROWD_BLOCK) k_coll_prep_crowd(
     Geom g, V ve, V vi, const uint32_t *off_e, const uint32_t *off_i, uint32_t *perm_e, uint32_t *perm_i, double m_e,
-    double m_i, uint64_t seed, uint64_t step, double *tot, int *row_bad, const uint32_t *list, const uint32_t *count,
+    double m_i, uint64_t seed, PassNo pass, double *tot, int *row_bad, const uint32_t *list, const uint32_t *count,
     const uint32_t *crowd_off, uint64_t *ckey, uint32_t *cidx, double *cstage, const unsigned long long *gate) {
+  const uint64_t step = pass.get();
   if (gate_closed(gate)) return;
   const uint32_t nlist = *count;
   for (uint32_t li = blockIdx.x; li < nlist; li += gridDim.x) {
@@ -900,9 +907,10 @@ __global__ void __launch_bounds__(COLL_CROWD_BLOCK) k_coll_prep_crowd(
 template <int SCHEME, class V>
 __global__ void __launch_bounds__(COLL_CROWD_BLOCK) k_coll_pairs_crowd(
     Geom g, V ve, V vi, const uint32_t *off_e, const uint32_t *off_i, const uint32_t *perm_e, const uint32_t *perm_i,
-    double m_e, double m_i, double q_e, double q_i, uint64_t seed, uint64_t step, const double *tot,
+    double m_e, double m_i, double q_e, double q_i, uint64_t seed, PassNo pass, const double *tot,
     const int *row_bad, const uint32_t *list, const uint32_t *count, const uint32_t *crowd_off, double *cstage,
     StepState *st, const unsigned long long *gate) {
+  const uint64_t step = pass.get();
   if (gate_closed(gate)) return;
   const uint32_t nlist = *count;
   const BlockTeam T;
@@ -1193,9 +1201,14 @@ __global__ void k_fsurr_features(Geom g, const double *mom, double m_e, double m
   }
 }
 
+// also writes the P12 fallback's bucket counts (local cell keys) from the
+// moments' particle counts: the particles are still in the cells the push
+// counted them in
 __global__ void k_fsurr_params(Geom g, double *mom, double m_e, double m_i, const float *Y, SurrNorm norm,
-                               const int *ok, float *par, int *fallback, unsigned long long *n_fallback) {
+                               const int *ok, float *par, int *fallback, unsigned long long *n_fallback,
+                               uint32_t *cnt_e, uint32_t *cnt_i) {
   const long ncells = g.Nr * g.Nz;
+  if (blockIdx.x == 0 && threadIdx.x == 0) cnt_e[ncells] = cnt_i[ncells] = 0;
   for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += (long)gridDim.x * blockDim.x) {
     double *m = mom + c * SURR_PAR_STRIDE;
     float *P = par + c * SURR_PAR_STRIDE;
@@ -1229,6 +1242,7 @@ __global__ void k_fsurr_params(Geom g, double *mom, double m_e, double m_i, cons
         P[8 * s + 4] = (float)(beta * az[s]);
       }
       fallback[lk] = 0;
+      cnt_e[lk] = cnt_i[lk] = 0;
     } else {
       for (int s = 0; s < 2; ++s) {
         P[8 * s + 0] = P[8 * s + 1] = P[8 * s + 2] = 0.f;
@@ -1236,8 +1250,13 @@ __global__ void k_fsurr_params(Geom g, double *mom, double m_e, double m_i, cons
       }
       const bool any = m[0] > 0 || m[8] > 0;
       fallback[lk] = any ? 1 : 0;
-      if (any) atomicAdd(n_fallback, 1ull);
+      cnt_e[lk] = any ? (uint32_t)m[0] : 0u;
+      cnt_i[lk] = any ? (uint32_t)m[8] : 0u;
     }
+    // fallback cells counted with one atomic per wavefront
+    const bool fb = !ok[c] && (m[0] > 0 || m[8] > 0);
+    const unsigned long long bal = __ballot(fb);
+    if (bal && __lane_id() == __ffsll((long long)__ballot(1)) - 1) atomicAdd(n_fallback, (unsigned long long)__popcll(bal));
     // the next step's push accumulates into zeroed moments
     for (int k = 0; k < SURR_PAR_STRIDE; ++k) m[k] = 0.0;
   }
@@ -1342,9 +1361,11 @@ static int coll_alloc(pcp_ctx *c, long cap_e, long cap_i) {
 }
 
 // bucket lists of both species (sort_to_cells)
+// `counted`: the bucket counts are already in cnt (the fused surrogate phase
+// writes them from the push's per-cell particle counts)
 template <class V>
 static int coll_buckets(pcp_ctx *c, V ve, V vi, long n_e, long n_i, const int *mask = nullptr,
-                        const unsigned long long *gate = nullptr) {
+                        const unsigned long long *gate = nullptr, bool counted = false) {
   CollBuffers *b = c->coll;
   const Geom &g = c->g;
   const long ncells = b->ncells;
@@ -1352,12 +1373,14 @@ static int coll_buckets(pcp_ctx *c, V ve, V vi, long n_e, long n_i, const int *m
   const long nn[2] = {n_e, n_i};
   unsigned gz = (unsigned)std::min<long>((ncells + 256) / 256, (long)c->n_cu * 16);
   for (int s = 0; s < 2; ++s) {
-    hipLaunchKernelGGL(k_fill_u32, dim3(gz), dim3(256), 0, c->stream, b->cnt[s], ncells + 1);
     hipLaunchKernelGGL(k_fill_u32, dim3(gz), dim3(256), 0, c->stream, b->fill[s], ncells + 1);
     unsigned gp = (unsigned)std::max(1L, std::min<long>((nn[s] + 255) / 256, (long)c->n_cu * 16));
-    if (nn[s] > 0)
-      hipLaunchKernelGGL(k_coll_count<V>, dim3(gp), dim3(256), 0, c->stream, g, s == 0 ? ve : vi, b->cnt[s],
-                         sentinel, mask, gate);
+    if (!counted) {
+      hipLaunchKernelGGL(k_fill_u32, dim3(gz), dim3(256), 0, c->stream, b->cnt[s], ncells + 1);
+      if (nn[s] > 0)
+        hipLaunchKernelGGL(k_coll_count<V>, dim3(gp), dim3(256), 0, c->stream, g, s == 0 ? ve : vi, b->cnt[s],
+                           sentinel, mask, gate);
+    }
     size_t bytes = b->scan_bytes;
     if (hipcub::DeviceScan::ExclusiveSum(b->scan_tmp, bytes, b->cnt[s], b->off[s], (int)(ncells + 1), c->stream) !=
         hipSuccess)
@@ -1373,7 +1396,7 @@ static int coll_buckets(pcp_ctx *c, V ve, V vi, long n_e, long n_i, const int *m
 // (all cells when null)
 template <class V, int CLS>
 static void coll_class(pcp_ctx *c, int scheme, V ve, V vi, double m_e, double m_i, double q_e, double q_i,
-                       uint64_t seed, uint64_t step, bool pairs, const unsigned long long *gate) {
+                       uint64_t seed, PassNo step, bool pairs, const unsigned long long *gate) {
   CollBuffers *b = c->coll;
   const Geom &g = c->g;
   constexpr int MAXC = COLL_CLASS_MAXC[CLS];
@@ -1440,7 +1463,7 @@ static void coll_class(pcp_ctx *c, int scheme, V ve, V vi, double m_e, double m_
 // breaks are final only after all of them), then the pairs
 template <class V>
 static void coll_binary(pcp_ctx *c, int scheme, V ve, V vi, double m_e, double m_i, double q_e, double q_i,
-                        uint64_t seed, uint64_t step, const int *mask, const unsigned long long *gate = nullptr) {
+                        uint64_t seed, PassNo step, const int *mask, const unsigned long long *gate = nullptr) {
   CollBuffers *b = c->coll;
   const Geom &g = c->g;
   const long ncells = b->ncells;
@@ -1482,7 +1505,7 @@ static SurrNorm surr_norm(const pcp_ctx *c) {
 
 template <class V>
 static int coll_run(pcp_ctx *c, int scheme, V ve, V vi, long n_e, long n_i, double m_e, double m_i, double q_e,
-                    double q_i, uint64_t seed, uint64_t step) {
+                    double q_i, uint64_t seed, PassNo step) {
   CollBuffers *b = c->coll;
   if (coll_buckets(c, ve, vi, n_e, n_i) < 0) return -1;
   if (scheme != COLL_SURROGATE) {
@@ -1510,7 +1533,7 @@ static int coll_run(pcp_ctx *c, int scheme, V ve, V vi, long n_e, long n_i, doub
 // rows of the cells it collided to the network's training set
 template <class V>
 static int coll_collect(pcp_ctx *c, V ve, V vi, long n_e, long n_i, double m_e, double m_i, double q_e, double q_i,
-                        uint64_t seed, uint64_t step) {
+                        uint64_t seed, PassNo step) {
   CollBuffers *b = c->coll;
   MlpState *m = c->mlp;
   if (coll_buckets(c, ve, vi, n_e, n_i) < 0) return -1;
@@ -1541,7 +1564,12 @@ static int coll_collect(pcp_ctx *c, V ve, V vi, long n_e, long n_i, double m_e, 
   return 0;
 }
 
-int launch_collide(pcp_ctx *c, int scheme, int se, int si, uint64_t seed, uint64_t step, int vslot) {
+int coll_prepare(pcp_ctx *c, int se, int si) {
+  if (c->s32) return coll_alloc(c, c->s32->sp[se].cap, c->s32->sp[si].cap);
+  return coll_alloc(c, c->sp[se].cap, c->sp[si].cap);
+}
+
+int launch_collide(pcp_ctx *c, int scheme, int se, int si, uint64_t seed, PassNo step, int vslot) {
   if (c->s32) {
     Species32 &e = c->s32->sp[se], &i = c->s32->sp[si];
     if (coll_alloc(c, e.cap, i.cap) < 0) return -1;
@@ -1575,7 +1603,7 @@ int fsurr_prepare(pcp_ctx *c, int se, int si) {
 double *fsurr_mom(pcp_ctx *c) { return c->coll ? c->coll->fmom : nullptr; }
 const float *fsurr_par(pcp_ctx *c) { return c->coll ? c->coll->fpar : nullptr; }
 
-int fsurr_cells(pcp_ctx *c, int se, int si, uint64_t seed, uint64_t step) {
+int fsurr_cells(pcp_ctx *c, int se, int si, uint64_t seed, PassNo step) {
   CollBuffers *b = c->coll;
   if (!b || !b->fmom || !c->mlp || !c->mlp->trained) return -2;
   Species32 &e = c->s32->sp[se], &i = c->s32->sp[si];
@@ -1586,16 +1614,17 @@ int fsurr_cells(pcp_ctx *c, int se, int si, uint64_t seed, uint64_t step) {
   mlp_forward(c, b->fX, nc, b->fY);
   hipLaunchKernelGGL(k_fill_zero_ull, dim3(1), dim3(1), 0, c->stream, b->counter);
   hipLaunchKernelGGL(k_fsurr_params, dim3(gt), dim3(256), 0, c->stream, c->g, b->fmom, e.m, i.m, b->fY,
-                     surr_norm(c), b->fok, b->fpar, b->fallback, b->counter);
+                     surr_norm(c), b->fok, b->fpar, b->fallback, b->counter, b->cnt[0], b->cnt[1]);
   // P12 on the cells the network does not cover, on the pushed velocities
   // (every kernel returns at once while no cell is flagged)
   View32 ve{e, c->cur, c->cur ^ 1}, vi{i, c->cur, c->cur ^ 1};
-  if (coll_buckets(c, ve, vi, e.n, i.n, b->fallback, b->counter) < 0) return -1;
+  if (coll_buckets(c, ve, vi, e.n, i.n, b->fallback, b->counter, true) < 0) return -1;
   coll_binary(c, COLL_P12, ve, vi, e.m, i.m, e.q, i.q, seed, step, b->fallback, b->counter);
   return 0;
 }
 
-int surrogate_collect(pcp_ctx *c, int se, int si, uint64_t seed, uint64_t step) {
+int surrogate_collect(pcp_ctx *c, int se, int si, uint64_t seed, uint64_t pass) {
+  const PassNo step{nullptr, pass};
   if (c->s32) {
     Species32 &e = c->s32->sp[se], &i = c->s32->sp[si];
     if (coll_alloc(c, e.cap, i.cap) < 0) return -1;

@@ -13,6 +13,15 @@ constexpr int COLL_MAXC = 256;
 
 struct CollBuffers;
 
+// the pass number a collision pass draws with: a value, or (p set) a device
+// counter that the step advances after its collision phase, so that a
+// captured step graph draws new numbers at every replay
+struct PassNo {
+  const uint64_t *p;
+  uint64_t v;
+  __host__ __device__ __forceinline__ uint64_t get() const { return p ? *p : v; }
+};
+
 // the collision surrogate: network on the cells it covers, P12 on the rest
 constexpr int COLL_SURROGATE = 3;
 // a cell goes to the network only with at least this many macro-particles
@@ -28,7 +37,9 @@ constexpr double SURR_MIN_FACTOR = 0.25;
 // and ion species si, enqueued on the context stream.  `vslot` selects the
 // FP32 velocity slot (ignored for FP64 contexts).  Returns -1 on allocation
 // or launch failure.
-int launch_collide(pcp_ctx *c, int scheme, int se, int si, uint64_t seed, uint64_t step, int vslot);
+int launch_collide(pcp_ctx *c, int scheme, int se, int si, uint64_t seed, PassNo pass, int vslot);
+// allocates the pass's buffers for species se, si (at their capacities)
+int coll_prepare(pcp_ctx *c, int se, int si);
 void coll_free(pcp_ctx *c);
 // one P12 pass whose collided cells append training rows (raw features,
 // relative temperature changes) to the network's training set
@@ -44,6 +55,6 @@ long surrogate_fallback_cells(pcp_ctx *c);
 int fsurr_prepare(pcp_ctx *c, int se, int si);
 double *fsurr_mom(pcp_ctx *c);
 const float *fsurr_par(pcp_ctx *c);
-int fsurr_cells(pcp_ctx *c, int se, int si, uint64_t seed, uint64_t step);
+int fsurr_cells(pcp_ctx *c, int se, int si, uint64_t seed, PassNo pass);
 
 }  // namespace pcp

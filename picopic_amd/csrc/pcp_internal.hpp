@@ -250,10 +250,21 @@ struct pcp_ctx {
     int blocks_per_cu = 0;          // resident blocks per CU (0: occupancy)
   } tune;
   int graph_on = 0;                 // replay captured step graphs
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};   // per parity of `cur`
+  // captured step graphs, one per buffer state (the particle buffer
+  // pointers, counts and `cur` that the step's kernels take); the sorts and
+  // the velocity-slot exchange cycle through a few states
+  struct StepGraph {
+    std::vector<uintptr_t> key;
+    hipGraphExec_t exec = nullptr;
+    unsigned long long used = 0;
+  };
+  std::vector<StepGraph> graphs;
+  unsigned long long graph_tick = 0;
+  long graph_launches = 0, graph_captures = 0;
   int coll_scheme = -1;            // collision phase of pcp_step (-1: off)
   int coll_se = 0, coll_si = 1;
-  uint64_t coll_seed = 0, coll_calls = 0;
+  uint64_t coll_seed = 0;
+  uint64_t *coll_pass = nullptr;    // device: the step's collision pass number
   int n_cu;
   int device = -1;                  // the HIP device the context runs on
   int timing;

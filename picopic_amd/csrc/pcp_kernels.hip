@@ -558,6 +558,9 @@ __global__ void __launch_bounds__(256) k_fix(Geom g, Fields f, SpeciesDev s, int
 }
 
 __global__ void k_stamp(StepState *st) { st->step += 1; }
+// device scalars a captured step reads and advances (no memset nodes)
+__global__ void k_set_u64(uint64_t *p, uint64_t v) { *p = v; }
+__global__ void k_inc_u64(uint64_t *p) { *p += 1; }
 
 // ---------------------------------------------------------------------------
 // migrant order keys: arrivals follow every particle already in the
@@ -765,6 +768,10 @@ void launch_step_begin(pcp_ctx *c) {
   hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, c->stream, c->st, c->elog ? c->ehist : nullptr);
 }
 void launch_stamp(pcp_ctx *c) { hipLaunchKernelGGL(k_stamp, dim3(1), dim3(1), 0, c->stream, c->st); }
+void launch_set_u64(pcp_ctx *c, uint64_t *p, uint64_t v) {
+  hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, c->stream, p, v);
+}
+void launch_inc_u64(pcp_ctx *c, uint64_t *p) { hipLaunchKernelGGL(k_inc_u64, dim3(1), dim3(1), 0, c->stream, p); }
 void launch_push(pcp_ctx *c, int sp) {
   SpeciesDev &s = c->sp[sp];
   if (!s.n) return;

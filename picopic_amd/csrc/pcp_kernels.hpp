@@ -22,6 +22,9 @@ void launch_yee_h(pcp_ctx *c);
 void launch_stitch(pcp_ctx *c, int which);
 void launch_fill(pcp_ctx *c, double *p, long n, double v);
 void launch_step_begin(pcp_ctx *c);
+// one-thread kernels on a device scalar (set / add one)
+void launch_set_u64(pcp_ctx *c, uint64_t *p, uint64_t v);
+void launch_inc_u64(pcp_ctx *c, uint64_t *p);
 void launch_stamp(pcp_ctx *c);
 void launch_push(pcp_ctx *c, int sp);
 void launch_move(pcp_ctx *c, int sp);
