@@ -322,6 +322,12 @@ def main():
         if prec == "fp32" and args.tuning:
             v = [int(x) for x in args.tuning.split(",")]
             tune = dict(zip(("bin_r", "bin_z", "block", "blocks_per_cu", "sort_every"), v))
+        if prec == "fp64" and "fp32" in runs:
+            # the FP64 leg sorts with the FP32 leg's bin and plasma interval,
+            # so that both timed windows hold the same sorts
+            t = runs["fp32"]["tuning"]
+            tune = {"bin_r": t["bin_r"], "bin_z": t["bin_z"], "block": 0, "blocks_per_cu": 0,
+                    "sort_every": t["sort_every"]}
         runs[prec] = run_gpu(w, args.steps, args.warmup, prec, tune=tune, net=net)
     main_prec = "fp32" if "fp32" in runs else "fp64"
     g = runs[main_prec]
