@@ -121,10 +121,11 @@ def run_gpu(w, steps, warmup, prec, tune, net=None):
     ctx.step(warmup)
     L = capi.load()
     # the timed steps replay captured step graphs (one graph launch per step
-    # after the sorts); the warm-up above ran eagerly, two more steps
-    # capture the graphs of the buffer states the timed window meets first
+    # after the sorts); the warm-up above ran eagerly, and two plasma sort
+    # intervals with the graph on capture the graphs of every buffer state
+    # the sorts cycle through, so that the timed window replays only
     ctx.set_graph(True)
-    ctx.step(2)
+    ctx.step(min(2 * int(tuning.get("sort_every", 1)) + 2, 300))
     launches0, captures0 = ctx.graph_stats()
     L.pcp_set_timing(ctx.h, 1)
     ctx.sync()

@@ -563,7 +563,8 @@ __device__ __forceinline__ void deposit32(bool live, const Geom &g, const Dom &d
   } else {
     const double r = rd, z = zd;
     const double ro = ((double)o.cr + (double)o.fr) * g.dr, zo = ((double)o.cz + (double)o.fz) * g.dz;
-    AzTerms A = vb_azimuthal_terms(g.nr, g.nz, g.dr, g.dz, dn.bot_r, dn.left_z, r, z, ro, zo, (double)vphi, p_charge);
+    AzTerms A = vb_azimuthal_terms_local(g.nr, g.nz, g.dr, g.dz, dn.bot_r, dn.left_z, r, z, ro, zo, (double)vphi,
+                                         p_charge);
     SegRecorder R;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
