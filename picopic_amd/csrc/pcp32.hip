@@ -550,8 +550,10 @@ __device__ __forceinline__ int deposit_domain32(const Geom &g, double rd, double
 }
 
 // the deposit of one lane's particle (ZigZag in FP32 relative to the old
-// cell; Villasenor-Buneman terms in FP64 from the old split position and
-// the unrounded end point); every lane of the wavefront calls it
+// cell; Villasenor-Buneman segments split in FP64 from the old split
+// position and the unrounded end point, their terms in the FP32 cell-local
+// form except on the axis cell and for long steps); every lane of the
+// wavefront calls it
 template <int CCS>
 __device__ __forceinline__ void deposit32(bool live, const Geom &g, const Dom &dn, const Split &o,
                                           const Split &n, float vphi, float qw, float inv_dt, double rd, double zd,

@@ -8,9 +8,10 @@
 // unrounded FP64 end point of the move (old split position + FP32
 // displacement), so that a displacement below the offset's resolution still
 // deposits, as in FP64.  FP64 is used for the near-axis
-// radial weights, for wall reflections, for the Villasenor-Buneman terms
-// (their logarithms cancel catastrophically in FP32) and for the energy
-// reductions.
+// radial weights, for wall reflections, for the Villasenor-Buneman terms of
+// the axis cell and of steps longer than 1/32 of the radius (elsewhere they
+// are evaluated in FP32 in a cell-local form whose size-i parts cancel
+// analytically) and for the energy reductions.
 #pragma once
 
 #include "pcp_internal.hpp"
