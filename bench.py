@@ -79,7 +79,7 @@ def train_network(w):
     return dict(params=params, norm=norm, loss=loss, seconds=time.perf_counter() - t0)
 
 
-def run_gpu(w, steps, warmup, prec, tune, net=None):
+def run_gpu(w, steps, warmup, prec, tune, net=None, compensated=False):
     from picopic_amd import capi, surrogate
     dr = dz = 1e-3
     grid = w["grid"]
@@ -106,6 +106,8 @@ def run_gpu(w, steps, warmup, prec, tune, net=None):
             scheme = "p12"
         ctx.set_collisions(scheme, 0, 1, seed=21)
     ctx.set_energy_log(True)
+    if compensated:
+        ctx.set_compensated_current(True)
     # the autotuner times sort bins and launch shapes on the live state (its
     # steps come before the warm-up and are not timed)
     if isinstance(tune, dict):
@@ -307,6 +309,8 @@ def main():
     ap.add_argument("--tuning", default=None,
                     help="FP32: bin_r,bin_z,block,blocks_per_cu,sort_every instead of autotuning (e.g. to profile "
                          "the configuration an earlier run chose)")
+    ap.add_argument("--compensated", action="store_true",
+                    help="FP32: deposit the current with compensated run sums and atomics")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: picopic_amd runs the particle loop on a single MI355X "
@@ -329,7 +333,8 @@ def main():
             t = runs["fp32"]["tuning"]
             tune = {"bin_r": t["bin_r"], "bin_z": t["bin_z"], "block": 0, "blocks_per_cu": 0,
                     "sort_every": t["sort_every"]}
-        runs[prec] = run_gpu(w, args.steps, args.warmup, prec, tune=tune, net=net)
+        runs[prec] = run_gpu(w, args.steps, args.warmup, prec, tune=tune, net=net,
+                             compensated=args.compensated and prec == "fp32")
     main_prec = "fp32" if "fp32" in runs else "fp64"
     g = runs[main_prec]
     n = g["n_particles"]
@@ -349,6 +354,8 @@ def main():
               "deposition": w["ccs"], "pusher": w["pusher"], "domains": list(w["domains"]),
               "collisions": g["collisions"], "phases_ms": g["phases"], "tuning": g["tuning"],
               "energies": g["energies"], "graph": g["graph"]}
+    if args.compensated:
+        config["compensated_current"] = True
     if g["fallback_cells"] is not None:
         config["surrogate"] = {"fallback_cells_last_step": g["fallback_cells"], "fit_loss": net["loss"],
                                "train_seconds": net["seconds"]}

@@ -295,6 +295,19 @@ int pcp_surrogate_calibrate(pcp_ctx *ctx, int sp_el, int sp_ion, uint64_t seed, 
  * time is its sorts plus its graph launch, and the phases inside the graph
  * read as zero. */
 int pcp_set_graph(pcp_ctx *ctx, int on);
+
+/* Compensated current deposition of an FP32 split context (no reference
+ * counterpart; the reference sums J in FP64, currentVB.cpp:731-735,
+ * currentZigZag.cpp:22-136).  On: each wavefront's run sums carry a
+ * Kahan-style compensation term, and each FP32 atomic on a J node returns
+ * the node's previous value, from which the exact rounding error of that
+ * atomic is recovered and added, with the run's compensation, to a
+ * companion array of J's layout; the step folds the companion into J before
+ * the current overlay.  The deposited J then carries one FP32 rounding per
+ * node instead of one per atomic and scan step.  Costs a second array of
+ * J's size and about twice the deposit's run-sum arithmetic.  Returns
+ * PCP_E_ARG for FP64 contexts. */
+int pcp_set_compensated_current(pcp_ctx *ctx, int on);
 /* graph launches and captures since the context was created (no reference
  * counterpart) */
 int pcp_graph_stats(pcp_ctx *ctx, long *launches, long *captures);

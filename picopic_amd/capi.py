@@ -30,7 +30,7 @@ EXPORTS = [
     "pcp_collide", "pcp_set_collisions", "pcp_mlp_nparams", "pcp_mlp_init", "pcp_mlp_get_params",
     "pcp_mlp_set_params", "pcp_mlp_forward", "pcp_mlp_grad", "pcp_mlp_adam", "pcp_surrogate_collect",
     "pcp_surrogate_fit", "pcp_surrogate_get_norm", "pcp_surrogate_set_norm", "pcp_surrogate_fallback_cells",
-    "pcp_species_moments", "pcp_continuity_residual", "pcp_set_graph", "pcp_graph_stats", "pcp_collision_passes", "pcp_set_sort_bin", "pcp_set_launch",
+    "pcp_species_moments", "pcp_continuity_residual", "pcp_set_graph", "pcp_set_compensated_current", "pcp_graph_stats", "pcp_collision_passes", "pcp_set_sort_bin", "pcp_set_launch",
     "pcp_get_tuning", "pcp_autotune", "pcp_surrogate_calibrate", "pcp_init_reference",
     "pcp_set_beam", "pcp_inject", "pcp_set_time", "pcp_time", "pcp_probe", "pcp_species_map",
 ]
@@ -131,6 +131,7 @@ def load(path=LIB_PATH):
     L.pcp_species_moments.argtypes = [vp, C.c_int, dp]
     L.pcp_continuity_residual.argtypes = [vp, dp]
     L.pcp_set_graph.argtypes = [vp, C.c_int]
+    L.pcp_set_compensated_current.argtypes = [vp, C.c_int]
     L.pcp_graph_stats.argtypes = [vp, C.POINTER(C.c_long), C.POINTER(C.c_long)]
     L.pcp_collision_passes.argtypes = [vp, C.POINTER(C.c_uint64)]
     L.pcp_set_sort_bin.argtypes = [vp, C.c_int, C.c_int]
@@ -340,6 +341,10 @@ class Context:
 
     def set_graph(self, on=True):
         _check(_lib.pcp_set_graph(self.h, 1 if on else 0))
+
+    def set_compensated_current(self, on=True):
+        """FP32 split contexts: deposit J with compensated run sums and atomics."""
+        _check(_lib.pcp_set_compensated_current(self.h, 1 if on else 0))
 
     def collision_passes(self):
         """Collision passes the step has run since set_collisions (device counter)."""

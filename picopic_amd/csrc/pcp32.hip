@@ -553,15 +553,17 @@ __device__ __forceinline__ int deposit_domain32(const Geom &g, double rd, double
 // cell; Villasenor-Buneman segments split in FP64 from the old split
 // position and the unrounded end point, their terms in the FP32 cell-local
 // form except on the axis cell and for long steps); every lane of the
-// wavefront calls it
-template <int CCS>
+// wavefront calls it.  COMP: compensated run sums and atomics, the
+// rounding errors into the companion array at element offset jc from J.
+template <int CCS, bool COMP>
 __device__ __forceinline__ void deposit32(bool live, const Geom &g, const Dom &dn, const Split &o,
                                           const Split &n, float vphi, float qw, float inv_dt, double rd, double zd,
                                           double p_charge, const FieldsT<float> &f, int d_dep, StepState *st,
                                           int *err, uint64_t *marker, uint32_t *cflag, const uint64_t *seq, long p,
-                                          int sp) {
+                                          int sp, long jc) {
   if (CCS == PCP_CCS_ZIGZAG) {
-    zigzag_runs(f.J, (long)d_dep * 3 * g.P, g.P, g.yr, g.nr, g.nz, zigzag32(g, dn, o, n, vphi, qw, inv_dt), live, err);
+    zigzag_runs<COMP>(f.J, (long)d_dep * 3 * g.P, g.P, g.yr, g.nr, g.nz, zigzag32(g, dn, o, n, vphi, qw, inv_dt), live,
+                      err, jc);
   } else {
     const double r = rd, z = zd;
     const double ro = ((double)o.cr + (double)o.fr) * g.dr, zo = ((double)o.cz + (double)o.fz) * g.dz;
@@ -577,10 +579,10 @@ __device__ __forceinline__ void deposit32(bool live, const Geom &g, const Dom &d
     if (live && R.is_strict) {
       const ResetMark M{marker + (long)d_dep * g.P, cflag, ((st->step & 0xFFFFFF) << 40) | order_key32(sp, seq[p]),
                         (uint32_t)st->step, dn.bot_r, dn.left_z, g.Nr, g.Nz, g.yr, g.nr, g.nz, &st->need_exact};
-      ResetSink<float> A1{f.J + (long)d_dep * 3 * g.P, g.P, g.yr, g.nr, g.nz, err, M};
+      ResetSink<float> A1{f.J + (long)d_dep * 3 * g.P, g.P, g.yr, g.nr, g.nz, err, M, COMP ? jc : 0};
       vb_strict(A1, g.dr, g.dz, g.dt, dn.bot_r, dn.left_z, r, z, ro, zo, p_charge);
     }
-    vb_runs(f.J, (long)d_dep * 3 * g.P, g.P, g.yr, g, dn, live, A, R, p_charge, err);
+    vb_runs<COMP>(f.J, (long)d_dep * 3 * g.P, g.P, g.yr, g, dn, live, A, R, p_charge, err, jc);
   }
 }
 
@@ -593,11 +595,11 @@ __device__ __forceinline__ void deposit32(bool live, const Geom &g, const Dom &d
 // Returns the lane's kinetic energy w m v^2 / (s (1 + s)), s = sqrt(1 - v^2/c^2)
 // (= w m c^2 (gamma - 1) without cancellation) after the move when `ke` is
 // set, else 0.
-template <int CCS, bool PUSH, bool D1>
+template <int CCS, bool PUSH, bool D1, bool COMP>
 __device__ __forceinline__ float step_trip32(const Geom &g, const FieldsT<float> &f, const Species32 &s, int sp,
                                              const PushConst &K, long p, int cur, StepState *st, const Migrants &mg,
                                              uint64_t *marker, uint32_t *cflag, int &err, const float *cpar,
-                                             int cslot, bool ke) {
+                                             int cslot, bool ke, long jc) {
   const int nxt = cur ^ 1;
   const float dtr = g.dtr_f, dtz = g.dtz_f;
   const float inv_dt = g.inv_dt_f;
@@ -638,8 +640,8 @@ __device__ __forceinline__ float step_trip32(const Geom &g, const FieldsT<float>
   // domain of its FP64 end point
   const int d_dep = CCS == PCP_CCS_ZIGZAG || D1 ? d_new : (live ? deposit_domain32(g, rd, zd) : 0);
   const Dom dn = make_dom_t<D1>(g, d_dep);
-  deposit32<CCS>(live, g, dn, o, n, v[1], qw, inv_dt, rd, zd, s.q * (double)w, f, d_dep, st, &err, marker, cflag,
-                 s.seq, p, sp);
+  deposit32<CCS, COMP>(live, g, dn, o, n, v[1], qw, inv_dt, rd, zd, s.q * (double)w, f, d_dep, st, &err, marker,
+                       cflag, s.seq, p, sp, jc);
   return kin;
 }
 
@@ -659,6 +661,7 @@ struct BatchArgs32 {
   StepState *st;
   Migrants mg;
   double *ehist;   // energy log (null: off); kinetic energies summed per species
+  long jc;         // compensated current: element offset of Jc from J (0: off)
   int cur, chunk;
 };
 typedef __attribute__((address_space(4))) const BatchArgs32 KernargBatch32;
@@ -671,10 +674,11 @@ typedef __attribute__((address_space(4))) const BatchArgs32 KernargBatch32;
 // after, 82 instead of 107 VGPRs, 7 % faster on the wakefield case).  The
 // ZigZag kernel then fits 80 VGPRs without spilling: six wavefronts per
 // SIMD instead of five (4 % faster); the VB kernel fits 128 instead of 130:
-// four instead of three (9 % faster on the batched case).
-template <int CCS, bool PUSH, int BLOCK, bool D1>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(CCS == PCP_CCS_ZIGZAG ? 6 : 4)))
-k32_step_batch(BatchArgs32 args) {
+// four instead of three (9 % faster on the batched case).  The compensated
+// deposit (COMP) carries twice the run-sum state and is held to four.
+template <int CCS, bool PUSH, int BLOCK, bool D1, bool COMP>
+__global__ void __launch_bounds__(BLOCK)
+    __attribute__((amdgpu_waves_per_eu(CCS == PCP_CCS_ZIGZAG && !COMP ? 6 : 4))) k32_step_batch(BatchArgs32 args) {
   (void)args;
   int err = 0;
   KernargBatch32 *A = (KernargBatch32 *)__builtin_amdgcn_kernarg_segment_ptr();
@@ -708,9 +712,9 @@ k32_step_batch(BatchArgs32 args) {
         flush();
         ke_sp = k;
       }
-      const float kin = step_trip32<CCS, PUSH, D1>(x.g, x.f, x.b.s[k], x.b.sp[k], x.b.K[k], (t - first) * 64 + lane,
-                                                   x.cur, x.st, x.mg, x.b.marker, x.b.cflag, err, x.b.cpar,
-                                                   x.b.cslot[k], ke);
+      const float kin = step_trip32<CCS, PUSH, D1, COMP>(x.g, x.f, x.b.s[k], x.b.sp[k], x.b.K[k],
+                                                         (t - first) * 64 + lane, x.cur, x.st, x.mg, x.b.marker,
+                                                         x.b.cflag, err, x.b.cpar, x.b.cslot[k], ke, x.jc);
       if (ke) ke_acc += (double)kin;
     }
   }
@@ -726,7 +730,7 @@ k32_step_batch(BatchArgs32 args) {
 // (four independent cell loads, then four flag loads) and redoes the few
 // that sit in flagged cells.
 __global__ void __launch_bounds__(256) k32_fix(Geom g, FieldsT<float> f, Batch32 b, int cur, StepState *st,
-                                               int do_push) {
+                                               int do_push, long jc) {
   if (!st->need_exact) return;
   const uint64_t stamp = st->step & 0xFFFFFF;
   const uint32_t fval = (uint32_t)st->step;
@@ -764,7 +768,7 @@ __global__ void __launch_bounds__(256) k32_fix(Geom g, FieldsT<float> f, Batch32
       const int d = deposit_domain32(g, r, z);
       const Dom dm = make_dom(g, d);
       FixSink<float> S{f.J + (long)d * 3 * g.P, b.marker + (long)d * g.P, g.yr, g.nr, g.nz, stamp,
-                       order_key32(b.sp[k], s.seq[p])};
+                       order_key32(b.sp[k], s.seq[p]), jc};
       vb_rz(S, g.dr, g.dz, g.dt, dm.bot_r, dm.left_z, r, z, ro, zo, s.q * (double)s.w[p], &err);
     }
   }
@@ -1041,7 +1045,7 @@ int alloc32(pcp_ctx *c) {
 void free32(pcp_ctx *c) {
   State32 *S = c->s32;
   if (!S) return;
-  (void)hipFree(S->f.E); (void)hipFree(S->f.H); (void)hipFree(S->f.Hat); (void)hipFree(S->f.J);
+  (void)hipFree(S->f.E); (void)hipFree(S->f.H); (void)hipFree(S->f.Hat); (void)hipFree(S->f.J); (void)hipFree(S->Jc);
   (void)hipFree(S->f.eps); (void)hipFree(S->f.sigma);
   for (int s = 0; s < MAX_SPECIES; ++s) {
     Species32 &sp = S->sp[s];
@@ -1086,25 +1090,81 @@ void launch32_push_mom(pcp_ctx *c, int sp, double *mom, int slot) {
                      push_const(s, c->g.dt), mom, slot);
 }
 
+// element offset of the current's compensation array from J (0: the
+// deposit is not compensated)
+static long comp_offset(const pcp_ctx *c) { return c->s32->Jc ? (long)(c->s32->Jc - c->s32->f.J) : 0; }
+
+// J += Jc and Jc = 0 over every node: the compensated deposit's rounding
+// errors folded into the current once per step, before the stitch
+__global__ void __launch_bounds__(256) k32_fold_j(float *J, float *Jc, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const float e = Jc[i];
+    if (e != 0.f) {
+      J[i] += e;
+      Jc[i] = 0.f;
+    }
+  }
+}
+
+void launch32_fold_j(pcp_ctx *c) {
+  if (!c->s32->Jc) return;
+  const long n = (long)c->D * 3 * c->g.P;
+  const unsigned grid = (unsigned)std::max(1L, std::min((n + 255) / 256, (long)c->n_cu * 16));
+  hipLaunchKernelGGL(k32_fold_j, dim3(grid), dim3(256), 0, c->stream, c->s32->f.J, c->s32->Jc, n);
+}
+
+int set_compensated32(pcp_ctx *c, bool on) {
+  if (on == (c->s32->Jc != nullptr)) return 0;
+  if (!on) {
+    (void)hipFree(c->s32->Jc);
+    c->s32->Jc = nullptr;
+    return 0;
+  }
+  const size_t bytes = (size_t)c->D * 3 * c->g.P * sizeof(float);
+  float *p = nullptr;
+  if (hipMalloc((void **)&p, bytes) != hipSuccess) return -1;
+  if (hipMemsetAsync(p, 0, bytes, c->stream) != hipSuccess) {
+    (void)hipFree(p);
+    return -1;
+  }
+  c->s32->Jc = p;
+  return 0;
+}
+
 // consecutive wavefront trips per counter increment of the batched step
 constexpr int TRIPS_PER_TICKET = 16;
 
-template <int CCS, bool PUSH, int BLOCK, bool D1>
+template <int CCS, bool PUSH, int BLOCK, bool D1, bool COMP = false>
 static void launch_batch_t(pcp_ctx *c, const Batch32 &b) {
   static int occ = 0;   // resident blocks per CU of this instance
-  if (!occ && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k32_step_batch<CCS, PUSH, BLOCK, D1>, BLOCK, 0) !=
-                   hipSuccess || occ < 1))
+  if (!occ &&
+      (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k32_step_batch<CCS, PUSH, BLOCK, D1, COMP>, BLOCK, 0) !=
+           hipSuccess ||
+       occ < 1))
     occ = 1;
   const int per_cu = c->tune.blocks_per_cu > 0 ? c->tune.blocks_per_cu : occ;
   const int chunk = TRIPS_PER_TICKET;
   const long waves = (b.end[b.nsp - 1] + chunk - 1) / chunk;
   const long grid = std::max(1L, std::min((waves + BLOCK / 64 - 1) / (BLOCK / 64), (long)c->n_cu * per_cu));
-  BatchArgs32 args{c->g, c->s32->f, b, c->st, c->mig, c->elog ? c->ehist : nullptr, c->cur, chunk};
-  hipLaunchKernelGGL((k32_step_batch<CCS, PUSH, BLOCK, D1>), dim3((unsigned)grid), dim3(BLOCK), 0, c->stream, args);
+  BatchArgs32 args{c->g, c->s32->f, b, c->st, c->mig, c->elog ? c->ehist : nullptr, COMP ? comp_offset(c) : 0,
+                   c->cur, chunk};
+  hipLaunchKernelGGL((k32_step_batch<CCS, PUSH, BLOCK, D1, COMP>), dim3((unsigned)grid), dim3(BLOCK), 0, c->stream,
+                     args);
 }
 
 template <int CCS>
 static void launch_batch_ccs(pcp_ctx *c, const Batch32 &b, bool push, int block) {
+  // the compensated deposit runs at one launch shape
+  if (c->s32->Jc) {
+    if (c->D != 1) {
+      if (push) launch_batch_t<CCS, true, 256, false, true>(c, b);
+      else launch_batch_t<CCS, false, 256, false, true>(c, b);
+    } else {
+      if (push) launch_batch_t<CCS, true, 256, true, true>(c, b);
+      else launch_batch_t<CCS, false, 256, true, true>(c, b);
+    }
+    return;
+  }
   if (c->D != 1) {
     if (!push) launch_batch_t<CCS, false, 256, false>(c, b);
     else if (block == 128) launch_batch_t<CCS, true, 128, false>(c, b);
@@ -1175,7 +1235,8 @@ void launch32_fix(pcp_ctx *c, bool push, const float *cpar, int se, int si) {
   batch_coll(c, b, cpar, se, si);
   // 256-particle units, 64 threads per unit
   const unsigned grid = (unsigned)std::max(1L, std::min((acc + 3) / 4, (long)c->n_cu * 16));
-  hipLaunchKernelGGL(k32_fix, dim3(grid), dim3(256), 0, c->stream, c->g, c->s32->f, b, c->cur, c->st, push ? 1 : 0);
+  hipLaunchKernelGGL(k32_fix, dim3(grid), dim3(256), 0, c->stream, c->g, c->s32->f, b, c->cur, c->st, push ? 1 : 0,
+                     comp_offset(c));
 }
 
 void launch32_migrant_keys(pcp_ctx *c) {

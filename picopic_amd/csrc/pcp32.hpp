@@ -65,10 +65,17 @@ struct State32 {
   FieldsT<float> f;
   Species32 sp[MAX_SPECIES];
   SortBuffers32 sb[MAX_SPECIES];
+  // compensated deposit: the rounding errors of J's atomics, same layout as
+  // J, folded into J after the step's deposit (null: not compensated)
+  float *Jc = nullptr;
 };
 
 int alloc32(pcp_ctx *c);
 void free32(pcp_ctx *c);
+// allocates (zeroed) or frees the compensation array; -1 on failure
+int set_compensated32(pcp_ctx *c, bool on);
+// folds the compensation array into J (no-op when not compensated)
+void launch32_fold_j(pcp_ctx *c);
 void launch32_yee_e(pcp_ctx *c, int clear_j);
 void launch32_yee_h(pcp_ctx *c);
 void launch32_stitch(pcp_ctx *c, int which);

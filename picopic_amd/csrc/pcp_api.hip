@@ -1321,6 +1321,7 @@ static int step_body(pcp_ctx *c, Mark mark) {
     else
       for (int s = 0; s < c->cfg.n_species; ++s) launch_fix(c, s);
   }
+  if (f32) launch32_fold_j(c);
   mark(4);
   if (f32) launch32_stitch(c, PCP_GRID_J);
   else launch_stitch(c, PCP_GRID_J);
@@ -1433,6 +1434,14 @@ int pcp_set_graph(pcp_ctx *c, int on) {
   if (!c) return fail(PCP_E_ARG, "null context");
   c->graph_on = on != 0;
   if (!c->graph_on) graph_invalidate(c);
+  return 0;
+}
+
+int pcp_set_compensated_current(pcp_ctx *c, int on) {
+  if (!c) return fail(PCP_E_ARG, "null context");
+  if (!c->s32) return fail(PCP_E_ARG, "compensated deposition is a mode of FP32 split contexts");
+  graph_invalidate(c);
+  if (set_compensated32(c, on != 0)) return fail(PCP_E_HIP, "cannot allocate the compensation array");
   return 0;
 }
 

@@ -111,6 +111,56 @@ __device__ __forceinline__ float run_sum_f(float x, const Runs &R) {
   return x;
 }
 
+// Compensated summation in FP32.  two_sum returns fl(a + b) and sets e to
+// its rounding error, a + b - fl(a + b), exactly (Knuth's branch-free
+// TwoSum; needs round-to-nearest and no reassociation, which the build's
+// flags keep).
+__device__ __forceinline__ float two_sum(float a, float b, float &e) {
+  const float s = a + b;
+  const float bp = s - a;
+  e = (a - (s - bp)) + (b - bp);
+  return s;
+}
+
+template <int CTRL>
+__device__ __forceinline__ void run_step_fc(float &x, float &c, int m) {
+  const float t = __int_as_float(dpp_mov<CTRL>(__float_as_int(x)) & m);
+  const float u = __int_as_float(dpp_mov<CTRL>(__float_as_int(c)) & m);
+  float e;
+  x = two_sum(x, t, e);
+  c = c + (u + e);
+}
+
+// run_sum_f with Kahan-style compensation: each scan step adds the partner's
+// (sum, compensation) pair and keeps the step's rounding error in c, so the
+// last lane of a run holds the run's total as x + c with an error of a few
+// units of c's last place instead of up to 6 x 2^-24 of the run's magnitude
+__device__ __forceinline__ float run_sum_fc(float x, const Runs &R, float &c) {
+  c = 0.f;
+  run_step_fc<0x111>(x, c, R.m[0]);   // row_shr:1
+  run_step_fc<0x112>(x, c, R.m[1]);   // row_shr:2
+  run_step_fc<0x114>(x, c, R.m[2]);   // row_shr:4
+  run_step_fc<0x118>(x, c, R.m[3]);   // row_shr:8
+  run_step_fc<0x142>(x, c, R.m[4]);   // row_bcast:15
+  run_step_fc<0x143>(x, c, R.m[5]);   // row_bcast:31
+  return x;
+}
+
+// Compensated accumulation of (s, c) into the FP32 node *a: the atomic adds
+// s and returns the node's previous value, from which the lane recovers the
+// atomic's own rounding error exactly; that error and c go to the node's
+// companion a[jc] (an array of the same layout), whose contents the step
+// later folds into the node.  Every atomic on *a is exact up to what its
+// companion records, so the final J = fl(J + Jc) carries one rounding plus
+// the companion's own (second-order) error, whatever the number of atomics.
+__device__ __forceinline__ void comp_atomic_add(float *a, long jc, float s, float c) {
+  const float old = atomicAdd(a, s);
+  float e;
+  (void)two_sum(old, s, e);
+  const float r = e + c;
+  if (r != 0.f) unsafeAtomicAdd(a + jc, r);
+}
+
 // partial sums of x along the runs; the last lane of a run holds its total
 __device__ __forceinline__ double run_sum(double x, const Runs &R) {
   x = run_step<0x111>(x, R.m[0]);   // row_shr:1
