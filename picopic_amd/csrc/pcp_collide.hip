@@ -221,7 +221,7 @@ __device__ CellSums cell_sums(const double *vx, const double *vy, const double *
 }
 
 // per-cell values kept between the two kernels
-constexpr int TOT = 16;   // a_e E_e m_e[3] a_i E_i m_i[3] dens_e dens_i debye
+constexpr int TOT = 16;   // a_e E_e m_e[3] a_i E_i m_i[3] dens_e dens_i debye dens_e^(2/3) dens_i^(2/3)
 
 // a cell's particles staged in LDS (FP64 velocities), up to MAXC per species
 template <int MAXC>
@@ -341,6 +341,9 @@ __device__ __forceinline__ void cell_stats(const Geom &g, long c, const CellSums
   bool bad = !isnormal(temperature_ion) || !isnormal(temperature_el) || !isnormal(density_ion) ||
              !isnormal(density_el);
   t[12] = bad ? 0.0 : debye_length(density_el, density_ion, temperature_el, temperature_ion);
+  // P12's s12 cap (collisionsP12.cpp) takes n^(2/3) of both species: once per cell here
+  t[13] = pow(density_el, 2. / 3.);
+  t[14] = pow(density_ion, 2. / 3.);
   if (bad) atomicMin(&row_bad[(long)q.d * g.nr + q.i], q.j);
 }
 
@@ -529,6 +532,7 @@ struct PairRunner {
   double dens[2], debye, dt;
   uint64_t seed, step, gcell;
   int *err;
+  double dens23[2];   // n^(2/3) of electrons and ions
   __device__ __forceinline__ void pair(int sa, int ka, int sb, int kb, uint32_t ord) {
     PairDraws R;
     uint64_t ctr = (gcell << 24) | (uint64_t)(ord & 0xFFFFFFu);
@@ -538,8 +542,8 @@ struct PairRunner {
     R.pos = 0;
     double va[3] = {S.vx(sa, ka), S.vy(sa, ka), S.vz(sa, ka)};
     double vb[3] = {S.vx(sb, kb), S.vy(sb, kb), S.vz(sb, kb)};
-    collide_pair<SCHEME>(m[sa], m[sb], q[sa], q[sb], va, S.w(sa, ka), vb, S.w(sb, kb), dens[sa], dens[sb], debye,
-                         dt, R, err);
+    collide_pair<SCHEME>(m[sa], m[sb], q[sa], q[sb], va, S.w(sa, ka), vb, S.w(sb, kb), dens[sa], dens[sb],
+                         dens23[sa], dens23[sb], debye, dt, R, err);
     S.vx(sa, ka) = va[0]; S.vy(sa, ka) = va[1]; S.vz(sa, ka) = va[2];
     S.vx(sb, kb) = vb[0]; S.vy(sb, kb) = vb[1]; S.vz(sb, kb) = vb[2];
   }
@@ -728,7 +732,7 @@ __global__ void __launch_bounds__(COLL_BLOCK) k_coll_pairs(Geom g, V ve, V vi, c
     if (active) {
       PairRunner<SCHEME, WaveStage<MAXC>, WaveTeam> P{WS, T, {m_e, m_i}, {q_e, q_i}, {t[10], t[11]}, t[12], g.dt,
                                                       seed, step, (uint64_t)q.cr * (uint64_t)g.Nz + (uint64_t)q.cz,
-                                                      &err};
+                                                      &err, {t[13], t[14]}};
       cell_pairs(P, nn[0], nn[1]);
     }
     if (SCHEME == COLL_TA77S) {
@@ -787,7 +791,8 @@ __global__ void __launch_bounds__(COLL_LANE_BLOCK) k_coll_pairs_lane(
     const double *t = tot + c * TOT;
     if (q.j < row_bad[(long)q.d * g.nr + q.i]) {
       PairRunner<SCHEME, LaneStage, LaneTeam> P{S, T, {m_e, m_i}, {q_e, q_i}, {t[10], t[11]}, t[12], g.dt,
-                                                seed, step, (uint64_t)q.cr * (uint64_t)g.Nz + (uint64_t)q.cz, &err};
+                                                seed, step, (uint64_t)q.cr * (uint64_t)g.Nz + (uint64_t)q.cz, &err,
+                                                {t[13], t[14]}};
       cell_pairs(P, nn[0], nn[1]);
     }
     if (SCHEME == COLL_TA77S) {
@@ -926,7 +931,8 @@ __global__ void __launch_bounds__(COLL_CROWD_BLOCK) k_coll_pairs_crowd(
     const double *t = tot + c * TOT;
     if (q.j < row_bad[(long)q.d * g.nr + q.i]) {
       PairRunner<SCHEME, CrowdStage, BlockTeam> P{S, T, {m_e, m_i}, {q_e, q_i}, {t[10], t[11]}, t[12], g.dt, seed,
-                                                  step, (uint64_t)q.cr * (uint64_t)g.Nz + (uint64_t)q.cz, &err};
+                                                  step, (uint64_t)q.cr * (uint64_t)g.Nz + (uint64_t)q.cz, &err,
+                                                  {t[13], t[14]}};
       cell_pairs(P, nn[0], nn[1]);
     }
     if (SCHEME == COLL_TA77S) {
