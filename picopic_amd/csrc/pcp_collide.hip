@@ -8,11 +8,11 @@
 //   k_coll_count    particles per (domain, cell) bucket
 //   scan            bucket offsets (hipCUB)
 //   k_coll_fill     bucket lists of particle indices
-//   k_coll_classify cells with both species into lists by size (at most 8,
+//   k_coll_classify cells with both species into lists by size (at most 16,
 //                   64, 256 or more particles of either species; a cell
 //                   missing a species ends its r-row as the reference's
 //                   `break` does)
-//   k_coll_prep     per listed cell (one lane per cell of at most 8, one
+//   k_coll_prep     per listed cell (one lane per cell of at most 16, one
 //                   wavefront with an LDS stage per cell of at most 64 or
 //                   256, one workgroup with a global stage per larger cell):
 //                   shuffle each bucket by sorting on
@@ -236,7 +236,7 @@ using CellStage = CellStageT<COLL_MAXC>;
 constexpr int COLL_CLASSES = 4;
 // class 0: one lane per cell; 1, 2: a wavefront per cell, LDS stage; 3
 // (crowded): a workgroup per cell, stage in global memory
-constexpr int COLL_CLASS_MAXC[COLL_CLASSES] = {8, 64, COLL_MAXC, INT_MAX};
+constexpr int COLL_CLASS_MAXC[COLL_CLASSES] = {16, 64, COLL_MAXC, INT_MAX};
 // particles of all crowded cells of one pass that the global stage holds;
 // beyond it a pass raises ERR_COLL_OVERFLOW
 constexpr long COLL_CROWD_CAP = 1L << 22;
@@ -389,7 +389,7 @@ __global__ void __launch_bounds__(COLL_BLOCK) k_coll_prep(Geom g, V ve, V vi, co
 // lanes, adjacent addresses).  Pairs of one schedule phase touch disjoint
 // particles and draw by ordinal, so the results equal the wavefront path's.
 // ---------------------------------------------------------------------------
-constexpr int COLL_TINY = 8;
+constexpr int COLL_TINY = 16;
 constexpr int COLL_LANE_BLOCK = 256;
 constexpr int COLL_LANE_SLOTS = 4 * 2 * COLL_TINY;   // vx, vy, vz, w of both species
 static_assert(COLL_CLASS_MAXC[0] == COLL_TINY, "class 0 is the lane-per-cell class");
