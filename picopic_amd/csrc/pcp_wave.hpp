@@ -172,6 +172,30 @@ __device__ __forceinline__ double run_sum(double x, const Runs &R) {
   return x;
 }
 
+// Neighbouring runs.  Cell-ordered lanes put consecutive runs in cells
+// next to each other along z, whose anchors' keys differ by one; the right
+// nodes of one run's cell are then the left nodes of the next run's.  For
+// the last lane of each run: `prev` is the last lane of the run before it
+// (its own lane when there is none), `take` that run's key is this key - 1,
+// `give` the next run's key is this key + 1.  All 64 lanes must call it.
+struct RunLinks {
+  int prev;
+  bool take, give;
+};
+
+__device__ __forceinline__ RunLinks run_links(uint32_t key, const Runs &R) {
+  const int lane = __lane_id();
+  const unsigned long long tails = __ballot(R.tail);
+  const unsigned long long below = tails & ((1ull << lane) - 1ull);
+  RunLinks L;
+  L.prev = below ? 63 - __clzll((long long)below) : lane;
+  const uint32_t kprev = (uint32_t)__shfl((int)key, L.prev, 64);
+  const uint32_t knext = (uint32_t)dpp_mov<0x130>((int)key);   // wave_shl:1 -- key of lane + 1
+  L.take = R.tail && below != 0ull && kprev + 1u == key;
+  L.give = R.tail && lane < 63 && knext == key + 1u;
+  return L;
+}
+
 // one atomic per distinct key in the wavefront; each active lane receives a
 // distinct slot counted from the value the counter held
 __device__ __forceinline__ uint32_t wave_reserve(uint32_t *ctr, uint32_t key, bool active) {
