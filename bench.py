@@ -127,7 +127,18 @@ def run_gpu(w, steps, warmup, prec, tune, net=None, compensated=False):
     # intervals with the graph on capture the graphs of every buffer state
     # the sorts cycle through, so that the timed window replays only
     ctx.set_graph(True)
-    ctx.step(min(2 * int(tuning.get("sort_every", 1)) + 2, 300))
+    every = max(1, int(tuning.get("sort_every", 1)))
+    ctx.step(min(2 * every + 2, 300))
+
+    def to_sort_boundary():
+        # the plasma sorts at steps_done % every == 0; the timed window and
+        # the eager window below both start there and have the same length,
+        # so that their particle launches see the same sort-cycle phases
+        done = int(round(ctx.time() / dt))
+        ctx.step((-done) % every)
+        return (-done) % every
+
+    to_sort_boundary()
     launches0, captures0 = ctx.graph_stats()
     L.pcp_set_timing(ctx.h, 1)
     ctx.sync()
@@ -138,15 +149,17 @@ def run_gpu(w, steps, warmup, prec, tune, net=None, compensated=False):
     launches1, captures1 = ctx.graph_stats()
     step_ms_graph = L.pcp_kernel_ms(ctx.h, b"step")
     # the phases (and the particle launch of the roofline figure) from HIP
-    # events around eager launches of the same kernels in the steps that
-    # follow
+    # events around eager launches of the same kernels, over as many steps
+    # from the same point of the sort cycle
+    gap = to_sort_boundary()
     ctx.set_graph(False)
-    eager_steps = min(steps, 20)
+    eager_steps = steps
     ctx.step(eager_steps)
     phases = {k: L.pcp_kernel_ms(ctx.h, k.encode())
               for k in ("fields", "collide", "particles", "fixup", "stitch_j", "step")}
     graph = {"launches_per_step": (launches1 - launches0) / steps, "captures_in_window": captures1 - captures0,
              "step_ms_events": step_ms_graph, "eager_steps_for_phases": eager_steps,
+             "steps_between_windows": gap,
              "eager_ms_per_step": phases["step"]}
     n_end = sum(ctx.count(s) for s in range(len(species)))
     log = ctx.energy_log(steps)
