@@ -68,6 +68,9 @@ struct State32 {
   // compensated deposit: the rounding errors of J's atomics, same layout as
   // J, folded into J after the step's deposit (null: not compensated)
   float *Jc = nullptr;
+  // the fix-up's list of flagged particles (batch species << 40 | index)
+  uint64_t *fix_list = nullptr;
+  unsigned long long *fix_count = nullptr;
 };
 
 int alloc32(pcp_ctx *c);
