@@ -370,3 +370,44 @@ def test_compensated_current_is_fp32_only():
     with pytest.raises(RuntimeError):
         ctx.set_compensated_current(True)
     ctx.close()
+
+
+def test_fp32_fix_list_overflow_falls_back_to_full_screen():
+    """Every particle in strict axial motion (v_r = v_phi = 0 exactly, quirk
+    Q8 resets in every cell): the fix-up's list of flagged particles (2^20
+    entries) overflows with 2 x 700,000 particles, and the redo screens every
+    particle itself.  Particles and J match the FP64 oracle at the FP32
+    tolerances of the steps test."""
+    capi = _cap()
+    size, grid = Hh.geometry(16, 32)
+    dt = 0.5 * (size[1] / grid[1]) / 3e8
+    species = Hh.species_list(False)
+    n = 700000
+    orc = O.SMB(size, grid, (2, 2), dt, species, pusher="boris-adaptive", ccs="vb")
+    gpu = capi.Context(size, grid, (2, 2), dt, [s + (n + 64,) for s in species], pusher="boris-adaptive", ccs="vb",
+                       precision="fp32")
+    rng = np.random.default_rng(12)
+    dr, dz = size[0] / grid[0], size[1] / grid[1]
+    for sp, (q, m, b) in enumerate(species):
+        r = np.sqrt(rng.uniform((0.5 * dr) ** 2, (size[0] - 0.5 * dr) ** 2, n))
+        z = rng.uniform(0.5 * dz, size[1] - 0.5 * dz, n)
+        ir, iz = np.floor(r / dr), np.floor(z / dz)
+        r = (ir + (r / dr - ir).astype(np.float32).astype(np.float64)) * dr
+        z = (iz + (z / dz - iz).astype(np.float32).astype(np.float64)) * dz
+        vz = rng.normal(0.0, np.sqrt(1.6e-19 / (m * 9.1e-31)), n).astype(np.float32).astype(np.float64)
+        zero = np.zeros(n)
+        w = (rng.uniform(0.5, 1.5, n) * 1e7).astype(np.float32).astype(np.float64)
+        ids = np.arange(n, dtype=np.uint64) + np.uint64(sp * 10 ** 7)
+        orc.add_particles(sp, r, z, zero, zero, vz, w, ids)
+        gpu.upload(sp, r, z, zero, zero, vz, w, ids)
+    orc.step(1)
+    gpu.step(1)
+    for sp in range(len(species)):
+        a, b = by_id(orc.particles(sp)), by_id(gpu.particles(sp))
+        assert np.array_equal(a["id"], b["id"])
+        for k in ("r", "z", "vz"):
+            assert Hh.relerr(b[k], a[k]) <= 1e-5, (sp, k)
+    e = Hh.relerr(interior(gpu.get_grid("J")), interior(orc.get_grid("J")))
+    print("fix-up fallback J error", e)
+    assert e <= 1e-4
+    gpu.close()
