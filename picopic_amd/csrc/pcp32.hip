@@ -532,10 +532,23 @@ __global__ void __launch_bounds__(256) k32_push_mom(Geom g, FieldsT<float> f, Sp
                     w * (v[0] * v[0] + v[1] * v[1])};
 #pragma unroll
       for (int k = 0; k < 7; ++k) q[k] = run_sum_f(q[k], R);
-      if (R.tail && live) {
-        double *m = mom + (size_t)key * SURR_PAR_STRIDE + slot * 8;
+      // one atomic instruction per run: lane k adds moment k of the run's
+      // cell, so the instruction's seven lanes hit one cache line (seven
+      // instructions with one lane per run each measured 2.3 ms more per
+      // full-form step: the L2 takes same-line lanes of an instruction
+      // together)
+      unsigned long long tails = __ballot(R.tail && live);
+      while (tails) {
+        const int t = __ffsll((long long)tails) - 1;
+        tails &= tails - 1ull;
+        const uint32_t kt = (uint32_t)__builtin_amdgcn_readlane((int)key, t);
+        float mine = 0.f;
 #pragma unroll
-        for (int k = 0; k < 7; ++k) atomicAdd(m + k, (double)q[k]);
+        for (int k = 0; k < 7; ++k) {
+          const float qk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q[k]), t));
+          if (lane == k) mine = qk;
+        }
+        if (lane < 7) atomicAdd(mom + (size_t)kt * SURR_PAR_STRIDE + slot * 8 + lane, (double)mine);
       }
     }
   }
