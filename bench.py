@@ -156,7 +156,7 @@ def run_gpu(w, steps, warmup, prec, tune, net=None, compensated=False):
     eager_steps = steps
     ctx.step(eager_steps)
     phases = {k: L.pcp_kernel_ms(ctx.h, k.encode())
-              for k in ("fields", "collide", "particles", "fixup", "stitch_j", "step")}
+              for k in ("sort", "fields", "push", "collide", "particles", "fixup", "stitch_j", "step")}
     graph = {"launches_per_step": (launches1 - launches0) / steps, "captures_in_window": captures1 - captures0,
              "step_ms_events": step_ms_graph, "eager_steps_for_phases": eager_steps,
              "steps_between_windows": gap,

@@ -399,8 +399,12 @@ int pcp_copy_bandwidth(pcp_ctx *ctx, long bytes, int iters, double *gbps);
 
 /* hipStream_t the context enqueues on (as void*) */
 void *pcp_stream(pcp_ctx *ctx);
-/* average device time [ms] of the named kernel over the last pcp_step call
- * when timing is enabled with pcp_set_timing(ctx, 1); -1 if unknown */
+/* average device time [ms] per step of the named phase over the last
+ * pcp_step call when timing is enabled with pcp_set_timing(ctx, 1): "sort",
+ * "fields", "push" (the push of a step with collisions), "collide",
+ * "particles" (the fused move + deposit launch), "fixup", "stitch_j" and
+ * "step" (their sum; with graph mode only "step" is measured); -1 if
+ * unknown */
 int pcp_set_timing(pcp_ctx *ctx, int on);
 double pcp_kernel_ms(pcp_ctx *ctx, const char *name);
 

@@ -1281,7 +1281,7 @@ static int step_body(pcp_ctx *c, Mark mark) {
     launch_stitch(c, PCP_GRID_H);
   }
   if (c->elog) launch_field_energy(c, c->ehist, true);
-  mark(1);
+  mark(2);
   const bool coll = c->coll_scheme >= 0;
   // FP32 with the surrogate: the fused phase (moments in the push, per-cell
   // rescalings applied by the move kernel)
@@ -1291,6 +1291,7 @@ static int step_body(pcp_ctx *c, Mark mark) {
     double *mom = fsurr_mom(c);
     for (int s = 0; s < c->cfg.n_species; ++s)
       launch32_push_mom(c, s, mom, s == c->coll_se ? 0 : s == c->coll_si ? 1 : -1);
+    mark(3);
     if (fsurr_cells(c, c->coll_se, c->coll_si, c->coll_seed, PassNo{c->coll_pass, 0}) < 0)
       return fail(PCP_E_HIP, "surrogate collision phase failed");
     launch_inc_u64(c, c->coll_pass);
@@ -1300,17 +1301,20 @@ static int step_body(pcp_ctx *c, Mark mark) {
       if (f32) launch32_push(c, s);
       else launch_push(c, s);
     }
+    mark(3);
     if (launch_collide(c, c->coll_scheme, c->coll_se, c->coll_si, c->coll_seed, PassNo{c->coll_pass, 0},
                        c->cur ^ 1) < 0)
       return fail(PCP_E_HIP, "collision pass failed");
     launch_inc_u64(c, c->coll_pass);
+  } else {
+    mark(3);   // the push runs in the particle kernel
   }
-  mark(2);
+  mark(4);
   if (f32) launch32_step_batch(c, !coll, cpar, c->coll_se, c->coll_si);
   else
     for (int s = 0; s < c->cfg.n_species; ++s) launch_step_fused(c, s, !coll);
   c->cur ^= 1;
-  mark(3);
+  mark(5);
   if (f32) launch32_migrant_keys(c);
   else launch_migrant_keys(c);
   // strict axial motions reset J_r nodes: take back the contributions of
@@ -1322,13 +1326,13 @@ static int step_body(pcp_ctx *c, Mark mark) {
       for (int s = 0; s < c->cfg.n_species; ++s) launch_fix(c, s);
   }
   if (f32) launch32_fold_j(c);
-  mark(4);
+  mark(6);
   if (f32) launch32_stitch(c, PCP_GRID_J);
   else launch_stitch(c, PCP_GRID_J);
   // FP64 kinetic energies of the log (FP32 sums them in the particle kernel)
   if (c->elog && !f32)
     for (int s = 0; s < c->cfg.n_species; ++s) launch_kinetic_energy_log(c, s);
-  mark(5);
+  mark(7);
   return 0;
 }
 
@@ -1503,16 +1507,18 @@ double pcp_time(pcp_ctx *c) { return c ? c->t_now : 0.0; }
 int pcp_step(pcp_ctx *c, int n_steps) {
   if (!c || n_steps < 0) return fail(PCP_E_ARG, "bad argument");
   // optional per-phase device timing: events on the context stream
-  // fields: Yee + stitches; collide: the push and collision phase of a
-  // step with collisions; particles: the fused particle kernel(s)
-  static const char *names[5] = {"fields", "collide", "particles", "fixup", "stitch_j"};
+  // sort: the step's particle sorts; fields: Yee + stitches; push: the push
+  // of a step with collisions (with the cell moments of the surrogate);
+  // collide: its collision phase; particles: the fused particle kernel(s)
+  constexpr int NPH = 7;
+  static const char *names[NPH] = {"sort", "fields", "push", "collide", "particles", "fixup", "stitch_j"};
   EventSet evs;
-  if (c->timing) HIP_TRY(evs.create((size_t)n_steps * 6));
+  if (c->timing) HIP_TRY(evs.create((size_t)n_steps * (NPH + 1)));
   std::vector<hipEvent_t> &ev = evs.ev;
   const bool f32 = c->s32 != nullptr;
   for (int t = 0; t < n_steps; ++t) {
     auto mark = [&](int k) {
-      if (c->timing) (void)hipEventRecord(ev[(size_t)t * 6 + k], c->stream);
+      if (c->timing) (void)hipEventRecord(ev[(size_t)t * (NPH + 1) + k], c->stream);
     };
     // SMB::inject_beam first (src/PiCoPiC.cpp:320), outside any graph
     if (long ni = inject_beams(c); ni < 0) return (int)ni;
@@ -1522,14 +1528,15 @@ int pcp_step(pcp_ctx *c, int n_steps) {
     for (int s = 0; s < c->cfg.n_species; ++s)
       if (c->sort_every[s] > 0 && c->steps_done % c->sort_every[s] == 0)
         if ((f32 ? launch32_sort(c, s) : launch_sort(c, s)) < 0) return fail(PCP_E_HIP, "sort failed");
+    mark(1);
     if (c->graph_on) {
-      // one graph launch for the step; with timing on, events 0 and 5
+      // one graph launch for the step; with timing on, events 0 and NPH
       // bracket the sorts and the launch, and the phases inside the graph
       // are not separated (they read as zero)
       hipGraphExec_t exec = nullptr;
       if (int rc = step_graph(c, &exec)) return rc;
       HIP_TRY(hipGraphLaunch(exec, c->stream));
-      mark(5);
+      mark(NPH);
       ++c->graph_launches;
       step_host_effects(c);
     } else {
@@ -1541,22 +1548,23 @@ int pcp_step(pcp_ctx *c, int n_steps) {
   HIP_TRY(hipGetLastError());
   int rc = pcp_sync(c);
   if (c->timing && n_steps > 0) {
-    double acc[5] = {0, 0, 0, 0, 0}, tot = 0;
+    double acc[NPH] = {}, tot = 0;
     for (int t = 0; t < n_steps; ++t) {
       float ms = 0;
+      const size_t e0 = (size_t)t * (NPH + 1);
       if (c->graph_on) {
-        (void)hipEventElapsedTime(&ms, ev[(size_t)t * 6], ev[(size_t)t * 6 + 5]);
+        (void)hipEventElapsedTime(&ms, ev[e0], ev[e0 + NPH]);
         tot += ms;
         continue;
       }
-      for (int k = 0; k < 5; ++k) {
-        (void)hipEventElapsedTime(&ms, ev[(size_t)t * 6 + k], ev[(size_t)t * 6 + k + 1]);
+      for (int k = 0; k < NPH; ++k) {
+        (void)hipEventElapsedTime(&ms, ev[e0 + k], ev[e0 + k + 1]);
         acc[k] += ms;
         tot += ms;
       }
     }
     c->ktimes.clear();
-    for (int k = 0; k < 5; ++k) c->ktimes.emplace_back(names[k], acc[k] / n_steps);
+    for (int k = 0; k < NPH; ++k) c->ktimes.emplace_back(names[k], acc[k] / n_steps);
     c->ktimes.emplace_back("step", tot / n_steps);
   }
   return rc;

@@ -142,3 +142,32 @@ def test_graph_replay_with_collisions_matches_eager(precision, scheme):
     launches, captures = graph.graph_stats()
     print("graph launches", launches, "captures", captures)
     assert launches == 12 and captures <= 8
+
+
+@pytest.mark.parametrize("collisions", [False, True])
+def test_step_phase_timing(collisions):
+    """pcp_set_timing: eager steps split into sort, fields, push, collide,
+    particles, fixup and stitch_j phases (HIP events on the context stream)
+    that add up to the step; the push and collision phases are empty without
+    collisions and not with them; with graph mode only the step is timed."""
+    ctx, nsp = make("fp32", "boris-adaptive", "vb", False)
+    if collisions:
+        ctx.set_collisions("p12", 0, 1, seed=3)
+    L = capi.load()
+    L.pcp_set_timing(ctx.h, 1)
+    ctx.step(4)
+    names = ("sort", "fields", "push", "collide", "particles", "fixup", "stitch_j")
+    ph = {k: L.pcp_kernel_ms(ctx.h, k.encode()) for k in names + ("step",)}
+    print("phases", collisions, ph)
+    assert all(ph[k] >= 0 for k in ph)
+    assert abs(sum(ph[k] for k in names) - ph["step"]) <= 1e-3 * ph["step"] + 1e-4
+    assert ph["particles"] > 0 and ph["fields"] > 0 and ph["sort"] > 0
+    if collisions:
+        assert ph["push"] > 0 and ph["collide"] > 0
+    else:
+        assert ph["push"] < 0.01 and ph["collide"] < 0.01
+    ctx.set_graph(True)
+    ctx.step(4)
+    assert L.pcp_kernel_ms(ctx.h, b"step") > 0
+    assert L.pcp_kernel_ms(ctx.h, b"particles") == 0
+    ctx.close()
